@@ -1,0 +1,193 @@
+/*
+ * mgsolve.h -- C-ABI of libmgsolve, the MI355X (gfx950) multigrid solver for
+ * the modified-gravity scalar field of MG-GLAM-type particle-mesh codes.
+ *
+ * Plain pointers and sizes only.  All functions return 0 on success and a
+ * negative MGS_E* code on failure; mgs_last_error() then describes it.
+ * A handle owns a device-resident multigrid hierarchy and one HIP stream;
+ * it is single-threaded (like the reference's module state) and persists
+ * across calls (warm start between time steps).
+ *
+ * Field arrays crossing this boundary are 3-D grids of N^3 values in the
+ * reference's memory order: x fastest, 0-based, i.e. FI(M1,M2,M3) at
+ * offset (M1-1) + N*((M2-1) + N*(M3-1)) (PMP2mod_tools.f90:40-42).
+ *
+ * Reference interfaces replaced (file:line in chzruan/MG-GLAM-Durham):
+ *   scalar_field(i_step)                        PMP2extradof.f90:12
+ *   initialise_extradof                         PMP2extradof.f90:181
+ *   V_cycle(ilevel)                             PMP2extradof.f90:352
+ *   restrict_density(ilevel)                    PMP2extradof.f90:1168
+ *   restrict_scalar_field(ilevel)               PMP2extradof.f90:1262
+ *   correct_solution(ilevel)                    PMP2extradof.f90:1361
+ *   relaxation_iterations_fR(ilevel,redstep)    PMP2MGsolver_fR.f90:20
+ *   calculate_residual_fR(ilevel,res)           PMP2MGsolver_fR.f90:308
+ *   restrict_residual_fR(ilevel)                PMP2MGsolver_fR.f90:569
+ *   calculate_physical_right_hand_side_fR(l)    PMP2MGsolver_fR.f90:789
+ *   relaxation_iterations_DGP(ilevel,redstep)   PMP2MGsolver_DGP.f90:8
+ *   calculate_residual_DGP(ilevel,res)          PMP2MGsolver_DGP.f90:240
+ *   restrict_residual_DGP(ilevel)               PMP2MGsolver_DGP.f90:418
+ *   calculate_physical_right_hand_side_DGP(l)   PMP2MGsolver_DGP.f90:793
+ *   POTENT_FOR_MG (MG potential)                PMP2main.f90:869
+ *   MOVE gradient stencil (2x grad by central difference) PMP2main.f90:565-594
+ *   POWERfft5 (NGP-binned power spectrum)       PMP2mod_power.f90:209
+ */
+#ifndef MGSOLVE_H
+#define MGSOLVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define MGS_OK 0
+#define MGS_E_ARG -1         /* invalid argument (size, level, colour, model, n) */
+#define MGS_E_HIP -2         /* HIP runtime error */
+#define MGS_E_NOMEM -3       /* device allocation failed */
+#define MGS_E_STATE -4       /* call order (e.g. solve before density) */
+#define MGS_E_DOMAIN -5      /* input outside the model's domain (NaN, delta <= -1) */
+#define MGS_E_BUSY -6        /* serving: queue full / admission refused */
+
+/* models (MG_model, PMP2mod_tools.f90:58-62) */
+#define MGS_MODEL_FR 1
+#define MGS_MODEL_DGP 2
+
+/* precision of a solve */
+#define MGS_PREC_FP64 0      /* plain: the reference's method in FP64 throughout */
+#define MGS_PREC_MIXED 1     /* FP32 delta-form FAS inside an FP64 defect-correction loop */
+
+/* element type of host/device arrays crossing the boundary */
+#define MGS_F32 0            /* real(4), the reference's FI/FI2 storage */
+#define MGS_F64 1            /* real(8) */
+
+/* where a pointer lives */
+#define MGS_HOST 0
+#define MGS_DEVICE 1
+
+/* per-level arrays of the FP64 hierarchy (the reference's FI3 sections,
+ * PMP2MGsolver_fR.f90:76-83); level 0 u = FI2, level 0 rho = FI */
+#define MGS_ARR_U 0          /* scalar field u (f(R)) / phi (DGP)      FI3 S4 */
+#define MGS_ARR_RHO 1        /* density (level 0: delta)               FI3 S2 */
+#define MGS_ARR_F 2          /* FAS right-hand side f_l                FI3 S5 */
+#define MGS_ARR_RES 3        /* level residual                        FI3 S6 */
+#define MGS_ARR_RRES 4       /* restricted residual R r               FI3 S1 */
+#define MGS_ARR_RU 5         /* frozen restricted field R u           FI3 S3 */
+
+typedef struct mgs_params {
+    int32_t ngrid;           /* NGRID, cells per side at level 0 (even) */
+    int32_t model;           /* MGS_MODEL_FR or MGS_MODEL_DGP */
+    int32_t fr_n;            /* Hu-Sawicki n in {0,1,2} */
+    int32_t branch;          /* DGP: +1 normal (N_branch), -1 self-accelerating (S_branch) */
+    double box;              /* Box [Mpc/h] */
+    double omega_m;          /* Om */
+    double omega_l;          /* OmL */
+    double aexpn;            /* AEXPN */
+    double fr0;              /* fR0 (< 0 for Hu-Sawicki) */
+    double h0rc;             /* H0rc */
+    int32_t precision;       /* MGS_PREC_FP64 / MGS_PREC_MIXED */
+    int32_t reserved;
+} mgs_params;
+
+typedef struct mgs_solve_opts {
+    int32_t mode;            /* 0: the reference's exits (absolute RMS <= res_conv, 0.5 % stagnation,
+                                iteration cap; PMP2extradof.f90:107-122); 1: relative residual */
+    int32_t max_cycles;      /* iter_count_max (mode 0) / cap on V-cycles or outer iterations (mode 1) */
+    double res_conv;         /* mode 0 threshold on the level-0 RMS residual */
+    double rel_tol;          /* mode 1: stop when RMS(r)/RMS(r0) < rel_tol */
+    int32_t init_guess;      /* 1: start from u = u_bg (f(R)) / phi = 0 (DGP), as initialise_extradof */
+    int32_t inner_cycles;    /* mixed: inner FP32 V-cycles per outer FP64 iteration (default 1) */
+} mgs_solve_opts;
+
+typedef struct mgs_solve_report {
+    int32_t cycles;          /* V-cycles (plain) or inner V-cycles (mixed) performed */
+    int32_t outer;           /* outer FP64 iterations (mixed), = cycles for plain */
+    int32_t converged;
+    int32_t reserved;
+    double res0;             /* RMS residual of the initial field (FP64) */
+    double res_final;        /* RMS residual of the returned field (FP64) */
+    double seconds;          /* device time of the solve (hipEvent), excl. host<->device copies */
+    double history[128];     /* RMS residual after each V-cycle / outer iteration */
+} mgs_solve_report;
+
+typedef struct mgs_handle mgs_handle;
+
+/* ---- lifecycle ------------------------------------------------------ */
+int mgs_create(const mgs_params* params, mgs_handle** out);
+int mgs_destroy(mgs_handle* h);
+const char* mgs_last_error(const mgs_handle* h);   /* also valid for h == NULL (creation errors) */
+int mgs_levels(const mgs_handle* h, int32_t* levelmax, int32_t* levelmin, int32_t* coarsest);
+/* change the cosmology/model scalars of an existing hierarchy (new time step) */
+int mgs_set_params(mgs_handle* h, const mgs_params* params);
+
+/* ---- data in / out -------------------------------------------------- */
+/* density contrast delta (FI) in the reference's layout */
+int mgs_set_density(mgs_handle* h, const void* delta, int32_t dtype, int32_t where);
+/* scalar field u (FI2), un-normalised (the value the kernels work on) */
+int mgs_set_field(mgs_handle* h, const void* u, int32_t dtype, int32_t where);
+/* field out, multiplied by `scale` (1 for the raw field, 1/u_bg for the
+ * normalised field scalar_field leaves in FI2, PMP2extradof.f90:131-146) */
+int mgs_get_field(mgs_handle* h, void* u, int32_t dtype, int32_t where, double scale);
+/* one per-level array (MGS_ARR_*) of the FP64 hierarchy, n_l^3 values */
+int mgs_set_level_array(mgs_handle* h, int32_t level, int32_t which, const void* src, int32_t dtype, int32_t where);
+int mgs_get_level_array(mgs_handle* h, int32_t level, int32_t which, void* dst, int32_t dtype, int32_t where);
+/* density back out (e.g. a generated one), in the reference's layout */
+int mgs_get_density(mgs_handle* h, void* delta, int32_t dtype, int32_t where);
+/* u_bg = |fR_bg|^{1/(n+1)} (f(R)) */
+int mgs_background(const mgs_handle* h, double* fr_bg, double* u_bg);
+
+/* ---- single operators on the FP64 hierarchy (drop-in for the Fortran
+ *      routines above; each call is one or two kernel launches) --------- */
+int mgs_initialise(mgs_handle* h);                                   /* initialise_extradof */
+int mgs_smooth(mgs_handle* h, int32_t level, int32_t red);          /* relaxation_iterations_* */
+int mgs_residual(mgs_handle* h, int32_t level, double* rms);        /* calculate_residual_* (stores r) */
+int mgs_restrict_residual(mgs_handle* h, int32_t level);            /* restrict_residual_* */
+int mgs_restrict_field(mgs_handle* h, int32_t level);               /* restrict_scalar_field */
+int mgs_restrict_density(mgs_handle* h, int32_t level);             /* restrict_density */
+int mgs_rhs(mgs_handle* h, int32_t level);                          /* calculate_physical_right_hand_side_* */
+int mgs_correct(mgs_handle* h, int32_t level);                      /* correct_solution */
+int mgs_vcycle(mgs_handle* h, int32_t level);                       /* V_cycle */
+
+/* ---- whole solve (scalar_field) -------------------------------------- */
+int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* report);
+
+/* ---- fifth force ------------------------------------------------------ */
+/* psi = -1/2 c~^2 fR_bg (u/u_bg)^{n+1} (f(R)) or phi/2 (DGP), the term
+ * POTENT_FOR_MG adds to the potential, and its central-difference gradient
+ * (psi[i+1]-psi[i-1])/2 per axis in grid units (the on-grid form of MOVE's
+ * stencil).  Any output pointer may be NULL. */
+int mgs_fifth_force(mgs_handle* h, void* psi, void* gx, void* gy, void* gz, int32_t dtype, int32_t where);
+/* NGP-binned power spectrum of psi (POWERfft5 binning): nbins = N/2 bins,
+ * k [h/Mpc], P [(Mpc/h)^3], count of modes */
+int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, double* count);
+
+/* ---- synthetic input ------------------------------------------------- */
+/* seeded lognormal density delta = exp(g - sigma_g^2/2) - 1, g Gaussian with
+ * P_g(k) ~ k^slope exp(-k^2/(2 k_c^2)), k_c = kcut_frac k_Nyq (Philox-4x32-10
+ * white noise + FFT filter), written as the handle's density */
+int mgs_make_lognormal_density(mgs_handle* h, uint64_t seed, double sigma_g, double slope, double kcut_frac);
+
+/* ---- profiling ------------------------------------------------------- */
+/* per-kernel-class device timing with hipEvents on the handle's stream */
+#define MGS_KC_SMOOTH 0      /* level-0 smoother colour passes (plain FP64 or mixed FP32) */
+#define MGS_KC_RESIDUAL 1    /* level-0 residual / outer residual */
+#define MGS_KC_RESTRICT 2    /* level-0 -> 1 restrictions */
+#define MGS_KC_CORRECT 3     /* level-1 -> 0 prolongation */
+#define MGS_KC_COARSE 4      /* everything on levels >= 1 */
+#define MGS_KC_UPDATE 5      /* mixed: outer update */
+#define MGS_KC_COUNT 6
+int mgs_profile(mgs_handle* h, int32_t enable);
+/* totals since enable: launches, milliseconds, algorithmic bytes */
+int mgs_profile_get(mgs_handle* h, int32_t kclass, int64_t* launches, double* ms, double* bytes);
+
+/* ---- device plumbing -------------------------------------------------- */
+int mgs_synchronize(mgs_handle* h);
+/* device pointer of the level-0 field in the internal colour-split layout
+ * (for zero-copy consumers on the same device) */
+int mgs_device_field(mgs_handle* h, void** ptr, size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1015 @@
+// mgs_api.cpp -- host side of libmgsolve: the C-ABI of include/mgsolve.h,
+// the device-resident multigrid hierarchy, V-cycle control and the plain
+// FP64 / mixed-precision solve loops.
+//
+// Control flow follows the reference's driver:
+//   scalar_field     PMP2extradof.f90:12-178   -> mgs_solve / solve_plain
+//   V_cycle          PMP2extradof.f90:352-555  -> vcycle_plain
+// The reference keeps every coarse level in one packed FI3 array
+// (PMP2MGsolver_fR.f90:76-83); here each level owns separate colour-split
+// arrays (mgs_common.h).  The level-0 residual the reference computes inside
+// V_cycle is never read (restrict_residual_fR(1) recomputes it from u and
+// delta), so it is not launched; the restricted densities are constant during
+// a solve and are computed once per density instead of once per cycle.
+// Neither change alters any value.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mgsolve.h"
+#include "mgs_internal.h"
+#include "mgs_kernels.h"
+
+using namespace mgs;
+
+static thread_local std::string g_create_error;
+
+#define HIPCHK(h, call)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return set_err((h), MGS_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int set_err(mgs_handle* h, int code, const std::string& msg)
+{
+    if (h)
+        h->err = msg;
+    else
+        g_create_error = msg;
+    return code;
+}
+
+namespace mgs {
+
+// ---------------------------------------------------------------------------
+// coefficients (double, host): PMP2MGsolver_fR.f90:54-71, 333-346, 590-605
+// ---------------------------------------------------------------------------
+static double powi_d(double a, int b)
+{
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return r;
+}
+
+double fr_background(const mgs_params& p, double* ubg)
+{
+    const double a = p.aexpn;
+    const double R_bg = 3.0 * (p.omega_m / (a * a * a) + 4.0 * p.omega_l);
+    const double R0_bg = 3.0 * (p.omega_m + 4.0 * p.omega_l);
+    const double fR_bg = p.fr0 * powi_d(R0_bg / R_bg, p.fr_n + 1);
+    if (ubg) *ubg = std::pow(std::fabs(fR_bg), 1.0 / (1.0 + (double)p.fr_n));
+    return fR_bg;
+}
+
+FrCoef fr_coefs(const mgs_params& p, int level)
+{
+    FrCoef c;
+    const double a = p.aexpn;
+    const double ctilde = 2.99792458e3 * (double)p.ngrid / p.box;
+    const double dx = (double)(1 << level);
+    const double dx2 = dx * dx;
+    const double R_bg = 3.0 * (p.omega_m / (a * a * a) + 4.0 * p.omega_l);
+    const double R0_bg = 3.0 * (p.omega_m + 4.0 * p.omega_l);
+    const double fR_bg = p.fr0 * powi_d(R0_bg / R_bg, p.fr_n + 1);
+    const double sgn = std::copysign(1.0, fR_bg);
+    const double om_a = p.omega_m / a;
+    const double a2 = a * a;
+    const int n = p.fr_n;
+    c.fct2s = dx2 / (ctilde * ctilde * sgn) / 6.0;
+    c.fct1s = -(om_a)*c.fct2s;
+    c.Q2 = R_bg * a2 * c.fct2s / 3.0;
+    c.Q = c.Q2 * std::pow(std::fabs(fR_bg), 1.0 / ((double)n + 1.0));
+    c.Q2onethird = std::pow(-c.Q, 1.0 / 3.0);
+    c.Q4 = 4.0 * c.Q;
+    c.Delta11 = 27.0 * c.Q;
+    c.Delta20 = 12.0 * c.Q;
+    c.fct1r = om_a;
+    c.fct2r = -R_bg * a2 / 3.0;
+    c.fct3r = ctilde * ctilde * sgn / dx2;
+    c.a_fR_rt = std::pow(std::fabs(fR_bg), 1.0 / (double)(n + 1));
+    c.fct3_1 = ctilde * ctilde * sgn / 1.0;
+    c.fct4_1 = c.fct2r * std::pow(std::fabs(fR_bg), 1.0 / (1.0 + (double)n));
+    c.ubg = std::pow(std::fabs(fR_bg), 1.0 / (1.0 + (double)n));
+    return c;
+}
+
+DfCoef df_coefs(const mgs_params& p, int level)
+{
+    const FrCoef c = fr_coefs(p, level);
+    DfCoef d;
+    const double A = c.fct3r * powi_d(c.ubg, p.fr_n + 1);
+    const double C = -c.fct2r;
+    d.A64 = A;
+    d.C64 = C;
+    d.A = (float)A;
+    d.C = (float)C;
+    d.inv6A = (float)(1.0 / (6.0 * A));
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// hierarchy
+// ---------------------------------------------------------------------------
+static int levelmax_of(int N)
+{
+    int l = 0;
+    while ((1 << l) < N) l++;
+    return l;
+}
+
+static int levelmin_of(int N)
+{
+    int odd = N;
+    while (odd % 2 == 0) odd /= 2;
+    int l = 0;
+    while ((1 << l) < odd) l++;
+    return l + 1;
+}
+
+template <typename T>
+static int dalloc(mgs_handle* h, T** p, size_t count)
+{
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) + 256);
+    if (e != hipSuccess) return set_err(h, MGS_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    *p = (T*)q;
+    h->allocs.push_back(q);
+    return 0;
+}
+
+static int alloc_hierarchy(mgs_handle* h)
+{
+    const int N = h->p.ngrid;
+    h->L.assign(h->lc + 1, LevelD());
+    for (int l = 0; l <= h->lc; l++) {
+        LevelD& v = h->L[l];
+        v.n = N >> l;
+        const size_t n3 = (size_t)v.n * v.n * v.n;
+        int rc;
+        if ((rc = dalloc(h, &v.u, n3))) return rc;
+        if ((rc = dalloc(h, &v.rho, n3))) return rc;
+        if (l > 0) {
+            if ((rc = dalloc(h, &v.f, n3))) return rc;
+            if ((rc = dalloc(h, &v.res, n3))) return rc;
+            if ((rc = dalloc(h, &v.rres, n3))) return rc;
+            if ((rc = dalloc(h, &v.ru, n3))) return rc;
+        }
+    }
+    if (h->p.precision == MGS_PREC_MIXED) {
+        h->M.assign(h->lc + 1, LevelF());
+        for (int l = 0; l <= h->lc; l++) {
+            LevelF& v = h->M[l];
+            v.n = N >> l;
+            const size_t n3 = (size_t)v.n * v.n * v.n;
+            int rc;
+            if ((rc = dalloc(h, &v.d, n3))) return rc;
+            if ((rc = dalloc(h, &v.xb, n3))) return rc;
+            if ((rc = dalloc(h, &v.f, n3))) return rc;
+            if (l > 0) {
+                if ((rc = dalloc(h, &v.res, n3))) return rc;
+                if ((rc = dalloc(h, &v.rres, n3))) return rc;
+                if ((rc = dalloc(h, &v.rd, n3))) return rc;
+            }
+        }
+    }
+    // block partials of the largest reduction: 2 n^3 / 2 cells in rows of n/2
+    const size_t cells = (size_t)N * N * N;
+    h->npart = cells / 64 + 1024;
+    int rc;
+    if ((rc = dalloc(h, &h->partials, h->npart))) return rc;
+    if ((rc = dalloc(h, &h->dsum, 16))) return rc;
+    HIPCHK(h, hipHostMalloc((void**)&h->hsum, 16 * sizeof(double), hipHostMallocDefault));
+    return 0;
+}
+
+int ensure_res0(mgs_handle* h)
+{
+    if (h->L[0].res) return 0;
+    const size_t n3 = (size_t)h->p.ngrid * h->p.ngrid * h->p.ngrid;
+    return dalloc(h, &h->L[0].res, n3);
+}
+
+int ensure_staging(mgs_handle* h, size_t bytes)
+{
+    if (h->staging_bytes >= bytes) return 0;
+    if (h->staging) {
+        hipFree(h->staging);
+        h->staging = nullptr;
+        h->staging_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->staging, bytes);
+    if (e != hipSuccess) return set_err(h, MGS_E_NOMEM, std::string("hipMalloc staging: ") + hipGetErrorString(e));
+    h->staging_bytes = bytes;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// profiling
+// ---------------------------------------------------------------------------
+void prof_begin(mgs_handle* h, int kc, double bytes)
+{
+    if (!h->prof) return;
+    ProfRec r;
+    r.kc = kc;
+    r.bytes = bytes;
+    hipEventCreate(&r.a);
+    hipEventCreate(&r.b);
+    hipEventRecord(r.a, h->s);
+    h->recs.push_back(r);
+}
+
+void prof_end(mgs_handle* h)
+{
+    if (!h->prof) return;
+    hipEventRecord(h->recs.back().b, h->s);
+}
+
+static void prof_flush(mgs_handle* h)
+{
+    if (h->recs.empty()) return;
+    hipStreamSynchronize(h->s);
+    for (auto& r : h->recs) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        h->pc[r.kc].launches += 1;
+        h->pc[r.kc].ms += ms;
+        h->pc[r.kc].bytes += r.bytes;
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    h->recs.clear();
+}
+
+// ---------------------------------------------------------------------------
+// plain FP64 operators on the hierarchy
+// ---------------------------------------------------------------------------
+static double n3d(int n) { return (double)n * n * n; }
+
+int op_smooth(mgs_handle* h, int l, int red)
+{
+    LevelD& v = h->L[l];
+    const int colour = red ? 0 : 1;
+    const double s = 8.0;
+    if (l == 0) prof_begin(h, MGS_KC_SMOOTH, 1.5 * n3d(v.n) * s);
+    else prof_begin(h, MGS_KC_COARSE, 2.0 * n3d(v.n) * s);
+    if (h->p.model == MGS_MODEL_FR)
+        launch_fr_smooth64(h->p.fr_n, l, v.u, v.rho, v.f, v.n, colour, fr_coefs(h->p, l), h->s);
+    else
+        launch_dgp_smooth64(l, v.u, v.rho, v.f, h->dgp_tmp, v.n, colour, dgp_coefs(h->p, l), h->s);
+    prof_end(h);
+    return 0;
+}
+
+// level-l residual, stored in res_l when store; rms returned when rms != null
+int op_residual(mgs_handle* h, int l, bool store, double* rms)
+{
+    LevelD& v = h->L[l];
+    if (store && l == 0) {
+        int rc = ensure_res0(h);
+        if (rc) return rc;
+    }
+    double* res = store ? v.res : nullptr;
+    const double s = 8.0;
+    prof_begin(h, l == 0 ? MGS_KC_RESIDUAL : MGS_KC_COARSE, (store ? 3.0 : 2.0) * n3d(v.n) * s);
+    int nb;
+    if (h->p.model == MGS_MODEL_FR)
+        nb = launch_fr_residual64(h->p.fr_n, l, v.u, v.rho, v.f, res, rms ? h->partials : nullptr, v.n,
+                                  fr_coefs(h->p, l), h->s);
+    else
+        nb = launch_dgp_residual64(l, v.u, v.rho, v.f, res, rms ? h->partials : nullptr, v.n, dgp_coefs(h->p, l),
+                                   h->dgp_rms, h->s);
+    prof_end(h);
+    if (rms) {
+        launch_sum_partials(h->partials, nb, h->dsum, h->s);
+        HIPCHK(h, hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s));
+        HIPCHK(h, hipStreamSynchronize(h->s));
+        *rms = std::sqrt(h->hsum[0] / n3d(v.n));
+    }
+    return 0;
+}
+
+int op_restrict_residual(mgs_handle* h, int l)
+{
+    LevelD& c = h->L[l];
+    const LevelD& f = h->L[l - 1];
+    if (l == 1) {
+        prof_begin(h, MGS_KC_RESTRICT, (2.0 * n3d(f.n) + n3d(c.n)) * 8.0);
+        if (h->p.model == MGS_MODEL_FR)
+            launch_fr_restrict_residual1_64(h->p.fr_n, f.u, f.rho, c.rres, f.n, fr_coefs(h->p, 0), h->s);
+        else
+            launch_dgp_restrict_residual1_64(f.u, f.rho, c.rres, f.n, dgp_coefs(h->p, 0), h->s);
+    } else {
+        prof_begin(h, MGS_KC_COARSE, (n3d(f.n) + n3d(c.n)) * 8.0);
+        launch_restrict8_64(f.res, c.rres, nullptr, c.n, h->s);
+    }
+    prof_end(h);
+    return 0;
+}
+
+int op_restrict_field(mgs_handle* h, int l)
+{
+    LevelD& c = h->L[l];
+    const LevelD& f = h->L[l - 1];
+    prof_begin(h, l == 1 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(f.n) + 2.0 * n3d(c.n)) * 8.0);
+    launch_restrict8_64(f.u, c.ru, c.u, c.n, h->s);
+    prof_end(h);
+    return 0;
+}
+
+int op_restrict_density(mgs_handle* h, int l)
+{
+    LevelD& c = h->L[l];
+    const LevelD& f = h->L[l - 1];
+    prof_begin(h, l == 1 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(f.n) + n3d(c.n)) * 8.0);
+    launch_restrict8_64(f.rho, c.rho, nullptr, c.n, h->s);
+    prof_end(h);
+    return 0;
+}
+
+int op_rhs(mgs_handle* h, int l)
+{
+    LevelD& v = h->L[l];
+    prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(v.n) * 8.0);
+    if (h->p.model == MGS_MODEL_FR)
+        launch_fr_rhs64(h->p.fr_n, v.ru, v.rho, v.rres, v.f, v.n, fr_coefs(h->p, l), h->s);
+    else
+        launch_dgp_rhs64(v.ru, v.rho, v.rres, v.f, v.n, dgp_coefs(h->p, l), h->s);
+    prof_end(h);
+    return 0;
+}
+
+int op_correct(mgs_handle* h, int l)
+{
+    LevelD& c = h->L[l];
+    LevelD& f = h->L[l - 1];
+    prof_begin(h, l == 1 ? MGS_KC_CORRECT : MGS_KC_COARSE, (2.0 * n3d(f.n) + 2.0 * n3d(c.n)) * 8.0);
+    launch_correct64(c.u, c.ru, f.u, c.n, h->p.model == MGS_MODEL_FR ? 1 : 0, h->s);
+    prof_end(h);
+    return 0;
+}
+
+static int cache_densities(mgs_handle* h)
+{
+    if (h->rho_cached) return 0;
+    for (int l = 1; l <= h->lc; l++) op_restrict_density(h, l);
+    h->rho_cached = true;
+    return 0;
+}
+
+// V_cycle(ilevel), PMP2extradof.f90:352-555 (Nsmth = 2, :361)
+int vcycle_plain(mgs_handle* h, int l)
+{
+    const int Nsmth = 2;
+    if (l >= h->lc) {
+        for (int s = 0; s < Nsmth; s++) {
+            op_smooth(h, l, 1);
+            op_smooth(h, l, 0);
+        }
+        return 0;
+    }
+    for (int s = 0; s < Nsmth; s++) {
+        op_smooth(h, l, 1);
+        op_smooth(h, l, 0);
+    }
+    if (l > 0) op_residual(h, l, true, nullptr);
+    op_restrict_residual(h, l + 1);
+    op_restrict_field(h, l + 1);
+    op_rhs(h, l + 1);
+    vcycle_plain(h, l + 1);
+    op_correct(h, l + 1);
+    for (int s = 0; s < Nsmth; s++) {
+        op_smooth(h, l, 1);
+        op_smooth(h, l, 0);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision: FP32 delta-form FAS V-cycle (inner) and FP64 outer loop
+// ---------------------------------------------------------------------------
+static int vcycle_inner(mgs_handle* h, int l)
+{
+    const int Nsmth = 2;
+    const int frn = h->p.fr_n;
+    LevelF& v = h->M[l];
+    const DfCoef dc = df_coefs(h->p, l);
+    if (l >= h->lc) {
+        prof_begin(h, MGS_KC_COARSE, 3.0 * n3d(v.n) * 4.0);
+        launch_df_coarsest(frn, v.d, v.xb, v.f, v.n, 8, dc, h->s);
+        prof_end(h);
+        return 0;
+    }
+    const double s = 4.0;
+    for (int it = 0; it < 2 * Nsmth; it++) {
+        if (l == 0) prof_begin(h, MGS_KC_SMOOTH, 2.0 * n3d(v.n) * s);
+        else prof_begin(h, MGS_KC_COARSE, 2.0 * n3d(v.n) * s);
+        launch_df_smooth(frn, v.d, v.xb, v.f, v.n, (it & 1) ? 1 : 0, dc, h->s);
+        prof_end(h);
+    }
+    LevelF& c = h->M[l + 1];
+    if (l == 0) {
+        prof_begin(h, MGS_KC_RESTRICT, (3.0 * n3d(v.n) + n3d(c.n)) * s);
+        launch_df_restrict_residual1(frn, v.d, v.xb, v.f, c.rres, v.n, dc, h->s);
+        prof_end(h);
+    } else {
+        prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(v.n) * s);
+        launch_df_residual(frn, v.d, v.xb, v.f, v.res, v.n, dc, h->s);
+        launch_restrict8_32(v.res, c.rres, nullptr, c.n, h->s);
+        prof_end(h);
+    }
+    prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(v.n) + 2.0 * n3d(c.n)) * s);
+    launch_restrict8_32(v.d, c.rd, c.d, c.n, h->s);
+    prof_end(h);
+    prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(c.n) * s);
+    launch_df_rhs(frn, c.rd, c.xb, c.rres, c.f, c.n, df_coefs(h->p, l + 1), h->s);
+    prof_end(h);
+    vcycle_inner(h, l + 1);
+    prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * s);
+    launch_df_correct(frn, c.d, c.rd, v.d, v.xb, c.n, h->s);
+    prof_end(h);
+    for (int it = 0; it < 2 * Nsmth; it++) {
+        if (l == 0) prof_begin(h, MGS_KC_SMOOTH, 2.0 * n3d(v.n) * s);
+        else prof_begin(h, MGS_KC_COARSE, 2.0 * n3d(v.n) * s);
+        launch_df_smooth(frn, v.d, v.xb, v.f, v.n, (it & 1) ? 1 : 0, dc, h->s);
+        prof_end(h);
+    }
+    return 0;
+}
+
+static int outer_residual(mgs_handle* h, double* rms)
+{
+    LevelD& v = h->L[0];
+    LevelF& m = h->M[0];
+    prof_begin(h, MGS_KC_RESIDUAL, 24.0 * n3d(v.n));
+    const int nb = launch_outer_residual(h->p.fr_n, v.u, v.rho, m.f, m.xb, h->partials, v.n, fr_coefs(h->p, 0), h->s);
+    prof_end(h);
+    launch_sum_partials(h->partials, nb, h->dsum, h->s);
+    HIPCHK(h, hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    *rms = std::sqrt(h->hsum[0] / n3d(v.n));
+    return 0;
+}
+
+// the reference's exits, PMP2extradof.f90:107-122
+static bool reference_exit(const mgs_solve_opts& o, int iter, double res, double res1, double res2, bool* conv)
+{
+    bool can_exit = false;
+    if (res <= o.res_conv) {
+        can_exit = true;
+        *conv = true;
+    }
+    if (iter > 2) {
+        if (std::fabs(res / res1 - 1.0) <= 5.0e-3) can_exit = true;
+        if (std::fabs(res / res1 - 1.0) <= 1.0e-2 && std::fabs(res1 / res2 - 1.0) <= 1.0e-2) can_exit = true;
+    }
+    if (iter >= o.max_cycles) can_exit = true;
+    return can_exit;
+}
+
+static int solve_plain(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep)
+{
+    int rc;
+    if ((rc = cache_densities(h))) return rc;
+    double res0 = 0.0;
+    if ((rc = op_residual(h, 0, false, &res0))) return rc;
+    rep->res0 = res0;
+    double res_old_1 = 1.0e5, res_old_2 = 1.0e10;
+    int iter = 1;
+    bool conv = false;
+    for (;;) {
+        vcycle_plain(h, 0);
+        double res = 0.0;
+        if ((rc = op_residual(h, 0, false, &res))) return rc;
+        if (iter <= 128) rep->history[iter - 1] = res;
+        rep->cycles = iter;
+        rep->outer = iter;
+        rep->res_final = res;
+        if (!std::isfinite(res)) return set_err(h, MGS_E_DOMAIN, "residual is not finite");
+        bool can_exit;
+        if (o.mode == 0) {
+            can_exit = reference_exit(o, iter, res, res_old_1, res_old_2, &conv);
+        } else {
+            conv = res < o.rel_tol * res0;
+            can_exit = conv || iter >= o.max_cycles;
+        }
+        if (can_exit) break;
+        iter++;
+        res_old_2 = res_old_1;
+        res_old_1 = res;
+    }
+    rep->converged = conv ? 1 : 0;
+    return 0;
+}
+
+static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep)
+{
+    int rc;
+    const int frn = h->p.fr_n;
+    const int inner = o.inner_cycles > 0 ? o.inner_cycles : 1;
+    double res_old_1 = 1.0e5, res_old_2 = 1.0e10;
+    bool conv = false;
+    int cycles = 0;
+    for (int it = 1;; it++) {
+        double res = 0.0;
+        if ((rc = outer_residual(h, &res))) return rc;
+        if (!std::isfinite(res)) return set_err(h, MGS_E_DOMAIN, "residual is not finite");
+        if (it == 1) rep->res0 = res;
+        rep->res_final = res;
+        if (it > 1 && it - 1 <= 128) rep->history[it - 2] = res;
+        rep->outer = it - 1;
+        bool can_exit = false;
+        if (it > 1) {
+            if (o.mode == 0) {
+                can_exit = reference_exit(o, it - 1, res, res_old_1, res_old_2, &conv);
+            } else {
+                conv = res < o.rel_tol * rep->res0;
+                can_exit = conv || (it - 1) >= o.max_cycles;
+            }
+            res_old_2 = res_old_1;
+            res_old_1 = res;
+        } else if (o.mode == 1 && res == 0.0) {
+            conv = true;
+            can_exit = true;
+        }
+        if (can_exit) break;
+        // inner problem: base restricted to every level, increment from zero
+        for (int l = 1; l <= h->lc; l++) {
+            prof_begin(h, l == 1 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(h->M[l - 1].n) + n3d(h->M[l].n)) * 4.0);
+            launch_restrict_base(frn, h->M[l - 1].xb, h->M[l].xb, h->M[l].n, h->s);
+            prof_end(h);
+        }
+        launch_fill_f32(h->M[0].d, (size_t)n3d(h->M[0].n), 0.0f, h->s);
+        for (int ic = 0; ic < inner; ic++) {
+            vcycle_inner(h, 0);
+            cycles++;
+        }
+        prof_begin(h, MGS_KC_UPDATE, 24.0 * n3d(h->L[0].n));
+        launch_outer_update(frn, h->L[0].u, h->M[0].d, h->M[0].xb, h->L[0].n, fr_coefs(h->p, 0).ubg, h->s);
+        prof_end(h);
+    }
+    rep->cycles = cycles;
+    rep->converged = conv ? 1 : 0;
+    return 0;
+}
+
+}  // namespace mgs
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+static int check_params(const mgs_params* p, std::string* why)
+{
+    if (!p) {
+        *why = "params is NULL";
+        return MGS_E_ARG;
+    }
+    if (p->ngrid < 2 || (p->ngrid & 1)) {
+        *why = "ngrid must be even and >= 2";
+        return MGS_E_ARG;
+    }
+    if (p->model != MGS_MODEL_FR && p->model != MGS_MODEL_DGP) {
+        *why = "Model currently unsupported (MG_model must be 1 = f(R) or 2 = DGP)";
+        return MGS_E_ARG;
+    }
+    if (p->model == MGS_MODEL_FR && (p->fr_n < 0 || p->fr_n > 2)) {
+        *why = "f(R) n must be 0, 1 or 2";
+        return MGS_E_ARG;
+    }
+    if (p->precision != MGS_PREC_FP64 && p->precision != MGS_PREC_MIXED) {
+        *why = "precision must be MGS_PREC_FP64 or MGS_PREC_MIXED";
+        return MGS_E_ARG;
+    }
+    if (p->precision == MGS_PREC_MIXED && p->model == MGS_MODEL_DGP) {
+        *why = "mixed-precision DGP solve is not available; use MGS_PREC_FP64";
+        return MGS_E_ARG;
+    }
+    if (p->precision == MGS_PREC_MIXED && p->model == MGS_MODEL_FR && !(p->fr0 < 0.0)) {
+        *why = "the mixed-precision f(R) solve requires fR0 < 0";
+        return MGS_E_ARG;
+    }
+    if (!(p->box > 0.0) || !(p->aexpn > 0.0)) {
+        *why = "box and aexpn must be positive";
+        return MGS_E_ARG;
+    }
+    if (p->model == MGS_MODEL_DGP && !(p->h0rc > 0.0)) {
+        *why = "DGP needs H0rc > 0";
+        return MGS_E_ARG;
+    }
+    const int lmax = levelmax_of(p->ngrid), lmin = levelmin_of(p->ngrid);
+    const int lc = lmax - lmin < 0 ? 0 : lmax - lmin;
+    if (((p->ngrid >> lc) & 1) || (p->ngrid >> lc) < 2) {
+        *why = "grid hierarchy: every level must have an even number of cells";
+        return MGS_E_ARG;
+    }
+    return 0;
+}
+
+extern "C" {
+
+int mgs_create(const mgs_params* params, mgs_handle** out)
+{
+    if (!out) return set_err(nullptr, MGS_E_ARG, "out is NULL");
+    *out = nullptr;
+    std::string why;
+    int rc = check_params(params, &why);
+    if (rc) return set_err(nullptr, rc, why);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_err(nullptr, MGS_E_HIP, "no HIP device available: libmgsolve has no CPU fallback");
+    mgs_handle* h = new mgs_handle();
+    h->p = *params;
+    h->lmax = levelmax_of(params->ngrid);
+    h->lmin = levelmin_of(params->ngrid);
+    h->lc = h->lmax - h->lmin < 0 ? 0 : h->lmax - h->lmin;
+    if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return set_err(nullptr, MGS_E_HIP, "hipStreamCreate failed");
+    }
+    rc = alloc_hierarchy(h);
+    if (rc) {
+        g_create_error = h->err;
+        mgs_destroy(h);
+        return rc;
+    }
+    if (h->p.model == MGS_MODEL_DGP) {
+        rc = dgp_alloc(h);
+        if (rc) {
+            g_create_error = h->err;
+            mgs_destroy(h);
+            return rc;
+        }
+    }
+    *out = h;
+    return 0;
+}
+
+int mgs_destroy(mgs_handle* h)
+{
+    if (!h) return 0;
+    if (h->s) hipStreamSynchronize(h->s);
+    prof_flush(h);
+    for (void* p : h->allocs) hipFree(p);
+    if (h->staging) hipFree(h->staging);
+    if (h->hsum) hipHostFree(h->hsum);
+    fft_release(h);
+    if (h->s) hipStreamDestroy(h->s);
+    delete h;
+    return 0;
+}
+
+const char* mgs_last_error(const mgs_handle* h)
+{
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int mgs_levels(const mgs_handle* h, int32_t* levelmax, int32_t* levelmin, int32_t* coarsest)
+{
+    if (!h) return MGS_E_ARG;
+    if (levelmax) *levelmax = h->lmax;
+    if (levelmin) *levelmin = h->lmin;
+    if (coarsest) *coarsest = h->lc;
+    return 0;
+}
+
+int mgs_set_params(mgs_handle* h, const mgs_params* params)
+{
+    if (!h) return MGS_E_ARG;
+    std::string why;
+    int rc = check_params(params, &why);
+    if (rc) return set_err(h, rc, why);
+    if (params->ngrid != h->p.ngrid || params->model != h->p.model || params->precision != h->p.precision)
+        return set_err(h, MGS_E_ARG, "mgs_set_params cannot change ngrid, model or precision");
+    h->p = *params;
+    return 0;
+}
+
+static size_t esize(int32_t dtype) { return dtype == MGS_F32 ? 4 : 8; }
+
+// copy a lexicographic array (host or device) into device memory; returns a
+// device pointer (the caller's own when it is already on the device)
+static int to_device(mgs_handle* h, const void* src, size_t bytes, int32_t where, const void** dev)
+{
+    if (where == MGS_DEVICE) {
+        *dev = src;
+        return 0;
+    }
+    int rc = ensure_staging(h, bytes);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->s));
+    *dev = h->staging;
+    return 0;
+}
+
+int mgs_set_density(mgs_handle* h, const void* delta, int32_t dtype, int32_t where)
+{
+    if (!h || !delta) return set_err(h, MGS_E_ARG, "NULL argument");
+    if (dtype != MGS_F32 && dtype != MGS_F64) return set_err(h, MGS_E_ARG, "dtype");
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    if (where == MGS_HOST) {
+        // reject NaN/Inf before upload (the operators accept any finite delta,
+        // as the reference's do)
+        for (size_t q = 0; q < n3; q++) {
+            const double v = dtype == MGS_F32 ? (double)((const float*)delta)[q] : ((const double*)delta)[q];
+            if (!std::isfinite(v))
+                return set_err(h, MGS_E_DOMAIN, "density contrast must be finite (cell " + std::to_string(q) + ")");
+        }
+    }
+    const void* dev = nullptr;
+    int rc = to_device(h, delta, n3 * esize(dtype), where, &dev);
+    if (rc) return rc;
+    if (dtype == MGS_F32)
+        launch_pack_f64_from_f32((const float*)dev, h->L[0].rho, N, h->s);
+    else
+        launch_pack_f64_from_f64((const double*)dev, h->L[0].rho, N, h->s);
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    h->has_density = true;
+    h->rho_cached = false;
+    return 0;
+}
+
+int mgs_set_field(mgs_handle* h, const void* u, int32_t dtype, int32_t where)
+{
+    if (!h || !u) return set_err(h, MGS_E_ARG, "NULL argument");
+    if (dtype != MGS_F32 && dtype != MGS_F64) return set_err(h, MGS_E_ARG, "dtype");
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    const void* dev = nullptr;
+    int rc = to_device(h, u, n3 * esize(dtype), where, &dev);
+    if (rc) return rc;
+    if (dtype == MGS_F32)
+        launch_pack_f64_from_f32((const float*)dev, h->L[0].u, N, h->s);
+    else
+        launch_pack_f64_from_f64((const double*)dev, h->L[0].u, N, h->s);
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    return 0;
+}
+
+int mgs_get_field(mgs_handle* h, void* u, int32_t dtype, int32_t where, double scale)
+{
+    if (!h || !u) return set_err(h, MGS_E_ARG, "NULL argument");
+    if (dtype != MGS_F32 && dtype != MGS_F64) return set_err(h, MGS_E_ARG, "dtype");
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    const size_t bytes = n3 * esize(dtype);
+    void* dst = u;
+    if (where == MGS_HOST) {
+        int rc = ensure_staging(h, bytes);
+        if (rc) return rc;
+        dst = h->staging;
+    }
+    if (dtype == MGS_F32)
+        launch_unpack_f64_to_f32(h->L[0].u, (float*)dst, N, scale, h->s);
+    else
+        launch_unpack_f64_to_f64(h->L[0].u, (double*)dst, N, scale, h->s);
+    if (where == MGS_HOST) HIPCHK(h, hipMemcpyAsync(u, dst, bytes, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    return 0;
+}
+
+static double* level_array(mgs_handle* h, int level, int which)
+{
+    LevelD& v = h->L[level];
+    switch (which) {
+    case MGS_ARR_U: return v.u;
+    case MGS_ARR_RHO: return v.rho;
+    case MGS_ARR_F: return v.f;
+    case MGS_ARR_RES:
+        if (level == 0 && ensure_res0(h)) return nullptr;
+        return h->L[level].res;
+    case MGS_ARR_RRES: return v.rres;
+    case MGS_ARR_RU: return v.ru;
+    }
+    return nullptr;
+}
+
+int mgs_set_level_array(mgs_handle* h, int32_t level, int32_t which, const void* src, int32_t dtype, int32_t where)
+{
+    if (!h || !src || level < 0 || level > h->lc) return set_err(h, MGS_E_ARG, "level out of range");
+    double* a = level_array(h, level, which);
+    if (!a) return set_err(h, MGS_E_ARG, "no such array on this level");
+    const int n = h->L[level].n;
+    const size_t n3 = (size_t)n * n * n;
+    const void* dev = nullptr;
+    int rc = to_device(h, src, n3 * esize(dtype), where, &dev);
+    if (rc) return rc;
+    if (dtype == MGS_F32)
+        launch_pack_f64_from_f32((const float*)dev, a, n, h->s);
+    else
+        launch_pack_f64_from_f64((const double*)dev, a, n, h->s);
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    if (which == MGS_ARR_RHO) h->rho_cached = false;
+    return 0;
+}
+
+int mgs_get_level_array(mgs_handle* h, int32_t level, int32_t which, void* dst, int32_t dtype, int32_t where)
+{
+    if (!h || !dst || level < 0 || level > h->lc) return set_err(h, MGS_E_ARG, "level out of range");
+    double* a = level_array(h, level, which);
+    if (!a) return set_err(h, MGS_E_ARG, "no such array on this level");
+    const int n = h->L[level].n;
+    const size_t bytes = (size_t)n * n * n * esize(dtype);
+    void* out = dst;
+    if (where == MGS_HOST) {
+        int rc = ensure_staging(h, bytes);
+        if (rc) return rc;
+        out = h->staging;
+    }
+    if (dtype == MGS_F32)
+        launch_unpack_f64_to_f32(a, (float*)out, n, 1.0, h->s);
+    else
+        launch_unpack_f64_to_f64(a, (double*)out, n, 1.0, h->s);
+    if (where == MGS_HOST) HIPCHK(h, hipMemcpyAsync(dst, out, bytes, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    return 0;
+}
+
+int mgs_background(const mgs_handle* h, double* fr_bg, double* u_bg)
+{
+    if (!h) return MGS_E_ARG;
+    double ub = 0.0;
+    const double f = fr_background(h->p, &ub);
+    if (fr_bg) *fr_bg = f;
+    if (u_bg) *u_bg = ub;
+    return 0;
+}
+
+int mgs_initialise(mgs_handle* h)
+{
+    if (!h) return MGS_E_ARG;
+    double ubg = 0.0;
+    fr_background(h->p, &ubg);
+    const size_t n3 = (size_t)h->p.ngrid * h->p.ngrid * h->p.ngrid;
+    launch_fill_f64(h->L[0].u, n3, h->p.model == MGS_MODEL_FR ? ubg : 0.0, h->s);
+    for (int l = 1; l <= h->lc; l++) {
+        const size_t m3 = (size_t)h->L[l].n * h->L[l].n * h->L[l].n;
+        LevelD& v = h->L[l];
+        for (double* a : {v.u, v.f, v.res, v.rres, v.ru}) launch_fill_f64(a, m3, 0.0, h->s);
+    }
+    if (h->L[0].res) launch_fill_f64(h->L[0].res, n3, 0.0, h->s);
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    return 0;
+}
+
+static int check_level(mgs_handle* h, int level, int lo)
+{
+    if (level < lo || level > h->lc) return set_err(h, MGS_E_ARG, "level out of range");
+    return 0;
+}
+
+#define SYNC_RET(h)                                 \
+    do {                                            \
+        HIPCHK(h, hipGetLastError());               \
+        HIPCHK(h, hipStreamSynchronize((h)->s));    \
+        return 0;                                   \
+    } while (0)
+
+int mgs_smooth(mgs_handle* h, int32_t level, int32_t red)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 0)) return MGS_E_ARG;
+    op_smooth(h, level, red);
+    SYNC_RET(h);
+}
+
+int mgs_residual(mgs_handle* h, int32_t level, double* rms)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 0)) return MGS_E_ARG;
+    double r = 0.0;
+    int rc = op_residual(h, level, true, &r);
+    if (rc) return rc;
+    if (rms) *rms = r;
+    SYNC_RET(h);
+}
+
+int mgs_restrict_residual(mgs_handle* h, int32_t level)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 1)) return MGS_E_ARG;
+    op_restrict_residual(h, level);
+    SYNC_RET(h);
+}
+
+int mgs_restrict_field(mgs_handle* h, int32_t level)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 1)) return MGS_E_ARG;
+    op_restrict_field(h, level);
+    SYNC_RET(h);
+}
+
+int mgs_restrict_density(mgs_handle* h, int32_t level)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 1)) return MGS_E_ARG;
+    op_restrict_density(h, level);
+    SYNC_RET(h);
+}
+
+int mgs_rhs(mgs_handle* h, int32_t level)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 1)) return MGS_E_ARG;
+    op_rhs(h, level);
+    SYNC_RET(h);
+}
+
+int mgs_correct(mgs_handle* h, int32_t level)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 1)) return MGS_E_ARG;
+    op_correct(h, level);
+    SYNC_RET(h);
+}
+
+int mgs_vcycle(mgs_handle* h, int32_t level)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 0)) return MGS_E_ARG;
+    if (!h->has_density) return set_err(h, MGS_E_STATE, "no density set");
+    cache_densities(h);
+    vcycle_plain(h, level);
+    SYNC_RET(h);
+}
+
+int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* report)
+{
+    if (!h || !opts || !report) return set_err(h, MGS_E_ARG, "NULL argument");
+    if (!h->has_density) return set_err(h, MGS_E_STATE, "no density set (mgs_set_density)");
+    std::memset(report, 0, sizeof(*report));
+    if (opts->max_cycles < 1) return set_err(h, MGS_E_ARG, "max_cycles must be >= 1");
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    HIPCHK(h, hipEventRecord(e0, h->s));
+    if (opts->init_guess) {
+        double ubg = 0.0;
+        fr_background(h->p, &ubg);
+        launch_fill_f64(h->L[0].u, (size_t)n3d(h->p.ngrid), h->p.model == MGS_MODEL_FR ? ubg : 0.0, h->s);
+    }
+    int rc;
+    if (h->p.precision == MGS_PREC_MIXED && h->p.model == MGS_MODEL_FR)
+        rc = solve_mixed(h, *opts, report);
+    else if (h->p.precision == MGS_PREC_MIXED)
+        rc = solve_mixed_dgp(h, *opts, report);
+    else
+        rc = solve_plain(h, *opts, report);
+    HIPCHK(h, hipEventRecord(e1, h->s));
+    HIPCHK(h, hipEventSynchronize(e1));
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    report->seconds = ms * 1e-3;
+    if (rc) return rc;
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+int mgs_profile(mgs_handle* h, int32_t enable)
+{
+    if (!h) return MGS_E_ARG;
+    prof_flush(h);
+    h->prof = enable != 0;
+    for (auto& c : h->pc) c = ProfClass();
+    return 0;
+}
+
+int mgs_profile_get(mgs_handle* h, int32_t kclass, int64_t* launches, double* ms, double* bytes)
+{
+    if (!h || kclass < 0 || kclass >= MGS_KC_COUNT) return MGS_E_ARG;
+    prof_flush(h);
+    if (launches) *launches = h->pc[kclass].launches;
+    if (ms) *ms = h->pc[kclass].ms;
+    if (bytes) *bytes = h->pc[kclass].bytes;
+    return 0;
+}
+
+int mgs_synchronize(mgs_handle* h)
+{
+    if (!h) return MGS_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    return 0;
+}
+
+int mgs_device_field(mgs_handle* h, void** ptr, size_t* bytes)
+{
+    if (!h || !ptr) return MGS_E_ARG;
+    *ptr = h->L[0].u;
+    if (bytes) *bytes = (size_t)n3d(h->p.ngrid) * sizeof(double);
+    return 0;
+}
+
+}  // extern "C"

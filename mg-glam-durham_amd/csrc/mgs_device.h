@@ -1,0 +1,123 @@
+// mgs_device.h -- launch geometry and device helpers shared by the kernel
+// files of libmgsolve.
+#pragma once
+
+#include "mgs_common.h"
+
+namespace mgs {
+
+// ---------------------------------------------------------------------------
+// launch geometry
+// ---------------------------------------------------------------------------
+struct Launch {
+    dim3 grid, block;
+    int TX, TY, XB;
+    unsigned NB;
+};
+
+// blocks of (TX, TY) threads over `rows` rows of `width` cells each
+inline Launch make_launch(int width, long rows)
+{
+    Launch L;
+    L.TX = width < 256 ? width : 256;
+    if (L.TX < 1) L.TX = 1;
+    int ty = 256 / L.TX;
+    if (ty < 1) ty = 1;
+    if (ty > rows) ty = (int)rows;
+    L.TY = ty;
+    L.XB = (width + L.TX - 1) / L.TX;
+    const long rb = (rows + L.TY - 1) / L.TY;
+    L.NB = (unsigned)(rb * L.XB);
+    const unsigned per = (L.NB + 7) / 8;
+    L.grid = dim3(per * 8);
+    L.block = dim3(L.TX, L.TY);
+    return L;
+}
+
+// XCD-aware mapping: the hardware hands consecutive block ids round-robin to
+// the 8 XCDs; remap so each XCD gets a contiguous run of logical blocks.
+__device__ inline bool map_block(unsigned NB, unsigned& lb)
+{
+    const unsigned b = blockIdx.x;
+    const unsigned per = (NB + 7) >> 3;
+    lb = (b & 7u) * per + (b >> 3);
+    return lb < NB;
+}
+
+// logical (x index, row index) of this thread
+__device__ inline bool map_xy(int width, long rows, int TX, int TY, int XB, unsigned NB, int& x, long& row)
+{
+    unsigned lb;
+    if (!map_block(NB, lb)) return false;
+    const unsigned xb = lb % (unsigned)XB;
+    const unsigned rb = lb / (unsigned)XB;
+    x = (int)(xb * TX + threadIdx.x);
+    row = (long)rb * TY + threadIdx.y;
+    return x < width && row < rows;
+}
+
+// write one block partial; idle blocks of the rounded-up grid write nothing
+__device__ inline void store_partial(double* partials, unsigned NB, double t)
+{
+    unsigned lb;
+    if (threadIdx.x == 0 && threadIdx.y == 0 && partials && map_block(NB, lb)) partials[lb] = t;
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+template <int FRN, typename T>
+__device__ __forceinline__ T wpow(T x)
+{
+    if constexpr (FRN == 0)
+        return x;
+    else if constexpr (FRN == 1)
+        return x * x;
+    else
+        return x * x * x;
+}
+
+// the six colour-split neighbours of a colour-c cell at (m,j,k)
+struct Nb {
+    size_t xp, xm, yp, ym, zp, zm, me;
+};
+
+__device__ __forceinline__ Nb neighbours(const Geo& g, int c, int m, int j, int k)
+{
+    Nb r;
+    const int s = (c + j + k) & 1;
+    int mxm = m + s - 1;
+    if (mxm < 0) mxm += g.h;
+    int mxp = m + s;
+    if (mxp >= g.h) mxp -= g.h;
+    const size_t o = (size_t)(1 - c) * g.half;
+    const size_t r0 = row_base(g, j, k);
+    r.xp = o + r0 + mxp;
+    r.xm = o + r0 + mxm;
+    r.yp = o + row_base(g, wrap_inc(j, g.n), k) + m;
+    r.ym = o + row_base(g, wrap_dec(j, g.n), k) + m;
+    r.zp = o + row_base(g, j, wrap_inc(k, g.n)) + m;
+    r.zm = o + row_base(g, j, wrap_dec(k, g.n)) + m;
+    r.me = (size_t)c * g.half + r0 + m;
+    return r;
+}
+
+// block sum of one double per thread; thread 0 of the block gets the total
+__device__ inline double block_sum(double v)
+{
+    __shared__ double wsum[16];
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nthr = blockDim.x * blockDim.y;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) wsum[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (tid == 0) {
+        const int nw = (nthr + 63) >> 6;
+        for (int w = 0; w < nw; w++) t += wsum[w];
+    }
+    return t;
+}
+
+}  // namespace mgs

@@ -1,0 +1,280 @@
+// mgs_dgp.hip -- nDGP brane-bending multigrid kernels for gfx950.
+//
+// The reference's DGP operators (PMP2MGsolver_DGP.f90) in FP64 on the
+// colour-split layout:
+//   L(phi) = lap(phi)/dx^2 - 3/4 [sgn(alpha) sqrt(alpha^2 + 8/3 Sigma) - alpha] (- f)
+//   Sigma  = fct1 rho + 2/3 (Dxx^2+Dyy^2+Dzz^2) - 2/3 (DxxDyy+DxxDzz+DyyDzz)
+//            + 1/8 (Cxy^2+Cxz^2+Cyz^2),   all / dx^4
+// on the 19-point stencil (faces + 12 edges).  The smoother updates
+// phi <- phi - L / (-6/dx^2) (:226-230).
+//
+// Red-black on a 19-point stencil: a cell's 12 edge neighbours have the
+// cell's own colour.  The reference sweeps a colour sequentially, so a cell
+// sees the already-updated values of the edge neighbours that precede it in
+// memory order.  A parallel colour pass cannot reproduce that order; here
+// every cell of the colour is updated from the values the colour had before
+// the pass (the pass writes into a scratch half-array that is then copied
+// back).  The CPU oracle implements both orders (orc_dgp_smooth, lexi=0/1)
+// and both converge to the same discrete solution.
+
+#include <math.h>
+
+#include "mgs_device.h"
+#include "mgs_internal.h"
+#include "mgs_kernels.h"
+
+namespace mgs {
+
+DgpCoef dgp_coefs(const mgs_params& p, int level)
+{
+    DgpCoef c;
+    const double a = p.aexpn;
+    const double dx = (double)(1 << level);
+    c.dx2 = dx * dx;
+    c.dx4 = c.dx2 * c.dx2;
+    const double Orc = 1.0 / (4.0 * p.h0rc * p.h0rc);
+    const double a3 = a * a * a;
+    if (p.branch >= 0)
+        c.beta = 1.0 + (0.5 * p.omega_m / a3 + p.omega_l) / std::sqrt(Orc * (p.omega_m / a3 + p.omega_l));
+    else
+        c.beta = -(0.5 * p.omega_m / a3 + Orc) / std::sqrt(Orc * (p.omega_m / a3 + Orc));
+    const double Rc_sq = 1.0 / (4.0 * Orc);
+    c.alpha = 3.0 * c.beta * (a * a) / Rc_sq;
+    c.fct1 = c.alpha * p.omega_m / c.beta / a;
+    c.fct2_1 = -0.75 * c.alpha;
+    c.fct3_1 = std::copysign(0.75, c.alpha);
+    return c;
+}
+
+// Sigma's stencil sums and the Laplacian numerator at absolute cell (i,j,k)
+__device__ __forceinline__ void dgp_stencil(const double* __restrict__ u, const Geo& g, int i, int j, int k,
+                                            double& se0, double& se1, double& se2, double& lap, double& c0)
+{
+    const int n = g.n;
+    const int im = wrap_dec(i, n), ip = wrap_inc(i, n);
+    const int jm = wrap_dec(j, n), jp = wrap_inc(j, n);
+    const int km = wrap_dec(k, n), kp = wrap_inc(k, n);
+    c0 = u[split_index(g, i, j, k)];
+    const double xp = u[split_index(g, ip, j, k)], xm = u[split_index(g, im, j, k)];
+    const double yp = u[split_index(g, i, jp, k)], ym = u[split_index(g, i, jm, k)];
+    const double zp = u[split_index(g, i, j, kp)], zm = u[split_index(g, i, j, km)];
+    const double dxx = xp + xm - 2.0 * c0;
+    const double dyy = yp + ym - 2.0 * c0;
+    const double dzz = zp + zm - 2.0 * c0;
+    const double cxy = u[split_index(g, ip, jp, k)] + u[split_index(g, im, jm, k)] - u[split_index(g, ip, jm, k)] -
+                       u[split_index(g, im, jp, k)];
+    const double cxz = u[split_index(g, ip, j, kp)] + u[split_index(g, im, j, km)] - u[split_index(g, ip, j, km)] -
+                       u[split_index(g, im, j, kp)];
+    const double cyz = u[split_index(g, i, jp, kp)] + u[split_index(g, i, jm, km)] - u[split_index(g, i, jp, km)] -
+                       u[split_index(g, i, jm, kp)];
+    se0 = dxx * dxx + dyy * dyy + dzz * dzz;
+    se1 = dxx * dyy + dxx * dzz + dyy * dzz;
+    se2 = cxy * cxy + cxz * cxz + cyz * cyz;
+    const double s6 = xp + xm + yp + ym + zp + zm;
+    lap = s6 - c0 * 6.0;
+}
+
+// L(phi) at a cell (without the FAS right-hand side)
+__device__ __forceinline__ double dgp_L(const double* __restrict__ u, const Geo& g, int i, int j, int k, double rho,
+                                        const DgpCoef& c, double& c0)
+{
+    const double TWOOVERTHREE = 2.0 / 3.0, ONEOVEREIGHT = 0.125, THREEOVERFOUR = 0.75, EIGHTOVERTHREE = 8.0 / 3.0;
+    double se0, se1, se2, lap;
+    dgp_stencil(u, g, i, j, k, se0, se1, se2, lap, c0);
+    double Sigma = c.fct1 * rho;
+    Sigma = Sigma + se0 * TWOOVERTHREE / c.dx4 - se1 * TWOOVERTHREE / c.dx4 + se2 * ONEOVEREIGHT / c.dx4;
+    return lap / c.dx2 - (copysign(1.0, c.alpha) * sqrt(c.alpha * c.alpha + EIGHTOVERTHREE * Sigma) - c.alpha) *
+                             THREEOVERFOUR;
+}
+
+// relaxation_iterations_DGP, one colour, out of place into tmp (one colour half)
+template <bool COARSE>
+__global__ __launch_bounds__(256) void k_dgp_smooth64(const double* __restrict__ u, const double* __restrict__ rho,
+                                                      const double* __restrict__ f, double* __restrict__ tmp, Geo g,
+                                                      int colour, DgpCoef c, int TX, int TY, int XB, unsigned NB)
+{
+    int m;
+    long row;
+    if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const int i = 2 * m + ((colour + j + k) & 1);
+    const size_t me = (size_t)colour * g.half + row_base(g, j, k) + m;
+    double c0;
+    double L = dgp_L(u, g, i, j, k, rho[me], c, c0);
+    if constexpr (COARSE) L = L - f[me];
+    const double dLphi = -6.0 / c.dx2;
+    tmp[row_base(g, j, k) + m] = c0 - L / dLphi;
+}
+
+template <bool COARSE>
+__global__ __launch_bounds__(256) void k_dgp_residual64(const double* __restrict__ u, const double* __restrict__ rho,
+                                                        const double* __restrict__ f, double* __restrict__ res,
+                                                        double* __restrict__ partials, Geo g, DgpCoef c, int TX,
+                                                        int TY, int XB, unsigned NB)
+{
+    int m;
+    long row;
+    double acc = 0.0;
+    if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
+        const long nn = (long)g.n * g.n;
+        const int col = (int)(row / nn);
+        const long r2 = row - col * nn;
+        const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+        const int i = 2 * m + ((col + j + k) & 1);
+        const size_t me = (size_t)col * g.half + row_base(g, j, k) + m;
+        double c0;
+        double OP = dgp_L(u, g, i, j, k, rho[me], c, c0);
+        if constexpr (COARSE) OP = OP - f[me];
+        if (res) res[me] = OP;
+        acc = OP * OP;
+    }
+    store_partial(partials, NB, block_sum(acc));
+}
+
+// restrict_residual_DGP(1), PMP2MGsolver_DGP.f90:459-763 (rearranged Sigma)
+__global__ __launch_bounds__(256) void k_dgp_restrict_residual1_64(const double* __restrict__ u0,
+                                                                   const double* __restrict__ delta,
+                                                                   double* __restrict__ rres1, Geo gf, Geo gc,
+                                                                   DgpCoef c, int TX, int TY, int XB, unsigned NB)
+{
+    int I;
+    long row;
+    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
+    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int N = gf.n;
+    const double ONEOVEREIGHT = 1.0 / 8.0, ONEOVERTHREE = 1.0 / 3.0, EIGHTOVERTHREE = 8.0 / 3.0;
+    const double dx2 = 1.0, dx4 = 1.0;
+    double Q = -8.0 * c.fct2_1;
+    double sig[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int i = 2 * I + (q & 1), j = 2 * J + ((q >> 1) & 1), k = 2 * K + ((q >> 2) & 1);
+        const int im = wrap_dec(i, N), ip = wrap_inc(i, N);
+        const int jm = wrap_dec(j, N), jp = wrap_inc(j, N);
+        const int km = wrap_dec(k, N), kp = wrap_inc(k, N);
+#define U_(x, y, z) u0[split_index(gf, x, y, z)]
+        const double Sx = U_(ip, j, k) + U_(im, j, k);
+        const double Sy = U_(i, jp, k) + U_(i, jm, k);
+        const double Sz = U_(i, j, kp) + U_(i, j, km);
+        const double Dxy = (U_(ip, jp, k) + U_(im, jm, k)) - (U_(ip, jm, k) + U_(im, jp, k));
+        const double Dxz = (U_(ip, j, kp) + U_(im, j, km)) - (U_(ip, j, km) + U_(im, j, kp));
+        const double Dyz = (U_(i, jp, kp) + U_(i, jm, km)) - (U_(i, jp, km) + U_(i, jm, kp));
+#undef U_
+        double s = c.fct1 * delta[split_index(gf, i, j, k)];
+        s = s + Sx * (2.0 * Sx - Sy - Sz) * ONEOVERTHREE / dx4 + Sy * (2.0 * Sy - Sx - Sz) * ONEOVERTHREE / dx4 +
+            Sz * (2.0 * Sz - Sx - Sy) * ONEOVERTHREE / dx4 + (Dxy * Dxy) * ONEOVEREIGHT / dx4 +
+            (Dxz * Dxz) * ONEOVEREIGHT / dx4 + (Dyz * Dyz) * ONEOVEREIGHT / dx4;
+        sig[q] = s;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) Q = Q - c.fct3_1 * sqrt(c.alpha * c.alpha + EIGHTOVERTHREE * sig[q]);
+    const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1, z0 = 2 * K, z1 = 2 * K + 1;
+    const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
+    const int yl = (J == 0) ? N - 1 : 2 * J - 1, yu = (2 * J + 2 == N) ? 0 : 2 * J + 2;
+    const int zl = (K == 0) ? N - 1 : 2 * K - 1, zu = (2 * K + 2 == N) ? 0 : 2 * K + 2;
+    double P = 0.0;
+#define V_(x, y, z) u0[split_index(gf, x, y, z)]
+    P = P - 3.0 * V_(x0, y0, z0);
+    P = P - 3.0 * V_(x1, y0, z0);
+    P = P - 3.0 * V_(x0, y1, z0);
+    P = P - 3.0 * V_(x1, y1, z0);
+    P = P - 3.0 * V_(x0, y0, z1);
+    P = P - 3.0 * V_(x1, y0, z1);
+    P = P - 3.0 * V_(x0, y1, z1);
+    P = P - 3.0 * V_(x1, y1, z1);
+    P = P + V_(xl, y0, z0); P = P + V_(xl, y1, z0); P = P + V_(xl, y0, z1); P = P + V_(xl, y1, z1);
+    P = P + V_(xu, y0, z0); P = P + V_(xu, y1, z0); P = P + V_(xu, y0, z1); P = P + V_(xu, y1, z1);
+    P = P + V_(x0, yl, z0); P = P + V_(x1, yl, z0); P = P + V_(x0, yl, z1); P = P + V_(x1, yl, z1);
+    P = P + V_(x0, yu, z0); P = P + V_(x1, yu, z0); P = P + V_(x0, yu, z1); P = P + V_(x1, yu, z1);
+    P = P + V_(x0, y0, zl); P = P + V_(x1, y0, zl); P = P + V_(x0, y1, zl); P = P + V_(x1, y1, zl);
+    P = P + V_(x0, y0, zu); P = P + V_(x1, y0, zu); P = P + V_(x0, y1, zu); P = P + V_(x1, y1, zu);
+#undef V_
+    P = P / dx2 + Q;
+    rres1[split_index(gc, I, J, K)] = P / 8.0;
+}
+
+// calculate_physical_right_hand_side_DGP, PMP2MGsolver_DGP.f90:793-885
+__global__ __launch_bounds__(256) void k_dgp_rhs64(const double* __restrict__ ru, const double* __restrict__ rho,
+                                                   const double* __restrict__ rres, double* __restrict__ f, Geo g,
+                                                   DgpCoef c, int TX, int TY, int XB, unsigned NB)
+{
+    int m;
+    long row;
+    if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const long nn = (long)g.n * g.n;
+    const int col = (int)(row / nn);
+    const long r2 = row - col * nn;
+    const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+    const int i = 2 * m + ((col + j + k) & 1);
+    const size_t me = (size_t)col * g.half + row_base(g, j, k) + m;
+    double c0;
+    double OP = dgp_L(ru, g, i, j, k, rho[me], c, c0);
+    OP = OP - rres[me];
+    f[me] = OP;
+}
+
+void launch_dgp_smooth64(int level, double* u, const double* rho, const double* f, double* tmp, int n, int colour,
+                         const DgpCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, (long)n * n);
+    if (level == 0)
+        hipLaunchKernelGGL((k_dgp_smooth64<false>), L.grid, L.block, 0, s, u, rho, f, tmp, g, colour, c, L.TX, L.TY,
+                           L.XB, L.NB);
+    else
+        hipLaunchKernelGGL((k_dgp_smooth64<true>), L.grid, L.block, 0, s, u, rho, f, tmp, g, colour, c, L.TX, L.TY,
+                           L.XB, L.NB);
+    launch_copy_f64(tmp, u + (size_t)colour * g.half, g.half, s);
+}
+
+int launch_dgp_residual64(int level, const double* u, const double* rho, const double* f, double* res,
+                          double* partials, int n, const DgpCoef& c, double* scratch, hipStream_t s)
+{
+    (void)scratch;
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+    if (level == 0)
+        hipLaunchKernelGGL((k_dgp_residual64<false>), L.grid, L.block, 0, s, u, rho, f, res, partials, g, c, L.TX,
+                           L.TY, L.XB, L.NB);
+    else
+        hipLaunchKernelGGL((k_dgp_residual64<true>), L.grid, L.block, 0, s, u, rho, f, res, partials, g, c, L.TX,
+                           L.TY, L.XB, L.NB);
+    return (int)L.NB;
+}
+
+void launch_dgp_restrict_residual1_64(const double* u0, const double* delta, double* rres1, int N, const DgpCoef& c0,
+                                      hipStream_t s)
+{
+    const Geo gf = make_geo(N), gc = make_geo(N / 2);
+    const Launch L = make_launch(gc.n, (long)gc.n * gc.n);
+    hipLaunchKernelGGL(k_dgp_restrict_residual1_64, L.grid, L.block, 0, s, u0, delta, rres1, gf, gc, c0, L.TX, L.TY,
+                       L.XB, L.NB);
+}
+
+void launch_dgp_rhs64(const double* ru, const double* rho, const double* rres, double* f, int n, const DgpCoef& c,
+                      hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+    hipLaunchKernelGGL(k_dgp_rhs64, L.grid, L.block, 0, s, ru, rho, rres, f, g, c, L.TX, L.TY, L.XB, L.NB);
+}
+
+int dgp_alloc(mgs_handle* h)
+{
+    const size_t half = (size_t)h->p.ngrid * h->p.ngrid * (h->p.ngrid / 2);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, half * sizeof(double) + 256);
+    if (e != hipSuccess) return set_err(h, MGS_E_NOMEM, "hipMalloc DGP scratch");
+    h->allocs.push_back(q);
+    h->dgp_tmp = (double*)q;
+    return 0;
+}
+
+int solve_mixed_dgp(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep)
+{
+    (void)o;
+    (void)rep;
+    return set_err(h, MGS_E_ARG, "mixed-precision DGP solve is not available; use MGS_PREC_FP64");
+}
+
+}  // namespace mgs

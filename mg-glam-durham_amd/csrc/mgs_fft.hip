@@ -1,0 +1,384 @@
+// mgs_fft.hip -- grid-level pieces around the solve:
+//   * seeded lognormal density (the synthetic input of every benchmark
+//     configuration; no reference counterpart): Philox-4x32-10 white noise,
+//     hipFFT D2Z, k-space filter, Z2D, normalisation to sigma_g, exp map;
+//   * the fifth-force potential psi and its central-difference gradient
+//     (POTENT_FOR_MG, PMP2main.f90:884-950, and MOVE's stencil :565-594);
+//   * the NGP-binned power spectrum with POWERfft5's bins
+//     (PMP2mod_power.f90:323-418).
+
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <vector>
+
+#include "mgs_device.h"
+#include "mgs_internal.h"
+#include "mgs_kernels.h"
+
+namespace mgs {
+
+struct FftState {
+    int N = 0;
+    hipfftHandle fwd = 0, inv = 0;
+    bool have_fwd = false, have_inv = false;
+    double* real = nullptr;          // N^3 lexicographic
+    hipfftDoubleComplex* cplx = nullptr;  // N*N*(N/2+1)
+    double* bins = nullptr;          // 3 * nb: power, count, sum|m|
+};
+
+static FftState* fft_of(mgs_handle* h) { return (FftState*)h->fft; }
+
+static int fft_ensure(mgs_handle* h)
+{
+    if (!h->fft) h->fft = new FftState();
+    FftState* f = fft_of(h);
+    const int N = h->p.ngrid;
+    if (f->N == N && f->real) return 0;
+    f->N = N;
+    const size_t n3 = (size_t)N * N * N;
+    const size_t nc = (size_t)N * N * (N / 2 + 1);
+    if (hipMalloc((void**)&f->real, n3 * sizeof(double)) != hipSuccess) return set_err(h, MGS_E_NOMEM, "FFT buffer");
+    if (hipMalloc((void**)&f->cplx, nc * sizeof(hipfftDoubleComplex)) != hipSuccess)
+        return set_err(h, MGS_E_NOMEM, "FFT buffer");
+    if (hipMalloc((void**)&f->bins, 3 * (size_t)(N + 2) * sizeof(double)) != hipSuccess)
+        return set_err(h, MGS_E_NOMEM, "FFT bins");
+    if (hipfftPlan3d(&f->fwd, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftPlan3d D2Z");
+    f->have_fwd = true;
+    if (hipfftPlan3d(&f->inv, N, N, N, HIPFFT_Z2D) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftPlan3d Z2D");
+    f->have_inv = true;
+    hipfftSetStream(f->fwd, h->s);
+    hipfftSetStream(f->inv, h->s);
+    return 0;
+}
+
+void fft_release(mgs_handle* h)
+{
+    FftState* f = fft_of(h);
+    if (!f) return;
+    if (f->have_fwd) hipfftDestroy(f->fwd);
+    if (f->have_inv) hipfftDestroy(f->inv);
+    if (f->real) hipFree(f->real);
+    if (f->cplx) hipFree(f->cplx);
+    if (f->bins) hipFree(f->bins);
+    delete f;
+    h->fft = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)c[0] * 0xD2511F53ull;
+        const uint64_t p1 = (uint64_t)c[2] * 0xCD9E8D57ull;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n1 = lo1, n2 = hi0 ^ c[3] ^ k1, n3 = lo0;
+        c[0] = n0;
+        c[1] = n1;
+        c[2] = n2;
+        c[3] = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// one standard normal per cell: counter (index lo, index hi, 0, 0), key = seed;
+// Box-Muller on two 53-bit uniforms, cosine branch
+__global__ __launch_bounds__(256) void k_white_noise(double* __restrict__ w, size_t n3, uint32_t k0, uint32_t k1)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+        uint32_t c[4] = {(uint32_t)i, (uint32_t)(i >> 32), 0u, 0u};
+        philox4x32(c, k0, k1);
+        const double u1 = ((double)(c[0] >> 5) * 67108864.0 + (double)(c[1] >> 6) + 1.0) / 9007199254740992.0;
+        const double u2 = ((double)(c[2] >> 5) * 67108864.0 + (double)(c[3] >> 6)) / 9007199254740992.0;
+        w[i] = sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+    }
+}
+
+__device__ __forceinline__ int fold(int a, int N) { return a <= N / 2 ? a : a - N; }
+
+// multiply the half spectrum by k^(slope/2) exp(-k^2 / (4 kc^2)), k = 0 -> 0
+__global__ __launch_bounds__(256) void k_filter(hipfftDoubleComplex* __restrict__ c, int N, double kf, double kc,
+                                                double slope)
+{
+    const int nh = N / 2 + 1;
+    const size_t total = (size_t)N * N * nh;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+        const int i = (int)(q % nh);
+        const size_t r = q / nh;
+        const int j = (int)(r % N), k = (int)(r / N);
+        const double mx = i, my = fold(j, N), mz = fold(k, N);
+        const double kk = kf * sqrt(mx * mx + my * my + mz * mz);
+        double a = 0.0;
+        if (kk > 0.0) a = pow(kk, 0.5 * slope) * exp(-kk * kk / (4.0 * kc * kc));
+        c[q].x *= a;
+        c[q].y *= a;
+    }
+}
+
+// block sums of g and of (g - mu)^2
+__global__ __launch_bounds__(256) void k_moment(const double* __restrict__ g, size_t n3, double mu, int power,
+                                                double* __restrict__ partials)
+{
+    double acc = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+        const double v = g[i] - mu;
+        acc += power == 1 ? v : v * v;
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// delta = exp((g - mu) * scale - sigma^2/2) - 1, written into the split layout
+__global__ __launch_bounds__(256) void k_lognormal(const double* __restrict__ g, double* __restrict__ rho, Geo geo,
+                                                   double mu, double scale, double half_var)
+{
+    const size_t n3 = 2 * geo.half;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n3; q += stride) {
+        const int i = (int)(q % geo.n);
+        const size_t r = q / geo.n;
+        const int j = (int)(r % geo.n), k = (int)(r / geo.n);
+        rho[split_index(geo, i, j, k)] = exp((g[q] - mu) * scale - half_var) - 1.0;
+    }
+}
+
+// fifth-force potential at one cell from the split field
+__device__ __forceinline__ double psi_at(const double* __restrict__ u, const Geo& g, int i, int j, int k, int model,
+                                         int frn, double fct, double inv_ubg)
+{
+    const double v = u[split_index(g, i, j, k)];
+    if (model == MGS_MODEL_DGP) return 0.5 * v;
+    const double x = v * inv_ubg;
+    const double w = frn == 0 ? x : (frn == 1 ? x * x : x * x * x);
+    return -fct * w;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fifth_force(const double* __restrict__ u, T* __restrict__ psi,
+                                                     T* __restrict__ gx, T* __restrict__ gy, T* __restrict__ gz, Geo g,
+                                                     int model, int frn, double fct, double inv_ubg)
+{
+    const size_t n3 = 2 * g.half;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n3; q += stride) {
+        const int i = (int)(q % g.n);
+        const size_t r = q / g.n;
+        const int j = (int)(r % g.n), k = (int)(r / g.n);
+        const int n = g.n;
+        if (psi) psi[q] = (T)psi_at(u, g, i, j, k, model, frn, fct, inv_ubg);
+        if (gx)
+            gx[q] = (T)(0.5 * (psi_at(u, g, wrap_inc(i, n), j, k, model, frn, fct, inv_ubg) -
+                               psi_at(u, g, wrap_dec(i, n), j, k, model, frn, fct, inv_ubg)));
+        if (gy)
+            gy[q] = (T)(0.5 * (psi_at(u, g, i, wrap_inc(j, n), k, model, frn, fct, inv_ubg) -
+                               psi_at(u, g, i, wrap_dec(j, n), k, model, frn, fct, inv_ubg)));
+        if (gz)
+            gz[q] = (T)(0.5 * (psi_at(u, g, i, j, wrap_inc(k, n), model, frn, fct, inv_ubg) -
+                               psi_at(u, g, i, j, wrap_dec(k, n), model, frn, fct, inv_ubg)));
+    }
+}
+
+// power per half-spectrum entry into NGP bins: bins[0..nb) power,
+// [nb..2nb) mode count, [2nb..3nb) sum of |m|
+__global__ __launch_bounds__(256) void k_bin_power(const hipfftDoubleComplex* __restrict__ c, int N, double norm,
+                                                   double* __restrict__ bins, int nb)
+{
+    const int nh = N / 2 + 1;
+    const size_t total = (size_t)N * N * nh;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+        const int i = (int)(q % nh);
+        const size_t r = q / nh;
+        const int j = (int)(r % N), k = (int)(r / N);
+        const double mx = i, my = fold(j, N), mz = fold(k, N);
+        const double rk = sqrt(mx * mx + my * my + mz * mz);
+        if (rk < 0.5) continue;
+        int mp = (int)(rk + 0.5);
+        if (mp < 1) mp = 1;
+        if (mp >= nb) continue;
+        const double w = (i == 0 || 2 * i == N) ? 1.0 : 2.0;  // conjugate half
+        const double pw = (c[q].x * c[q].x + c[q].y * c[q].y) * norm;
+        atomicAdd(&bins[mp], w * pw);
+        atomicAdd(&bins[nb + mp], w);
+        atomicAdd(&bins[2 * nb + mp], w * rk);
+    }
+}
+
+static dim3 grid_for(size_t count)
+{
+    size_t b = (count + 255) / 256;
+    if (b > 16384) b = 16384;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+static int sum_blocks(mgs_handle* h, double* partials, int nblk, double* out)
+{
+    launch_sum_partials(partials, nblk, h->dsum, h->s);
+    if (hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s) != hipSuccess ||
+        hipStreamSynchronize(h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, "reduction copy");
+    *out = h->hsum[0];
+    return 0;
+}
+
+}  // namespace mgs
+
+using namespace mgs;
+
+extern "C" {
+
+int mgs_make_lognormal_density(mgs_handle* h, uint64_t seed, double sigma_g, double slope, double kcut_frac)
+{
+    if (!h || !(sigma_g > 0.0) || !(kcut_frac > 0.0)) return set_err(h, MGS_E_ARG, "lognormal parameters");
+    int rc = fft_ensure(h);
+    if (rc) return rc;
+    FftState* f = fft_of(h);
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    hipLaunchKernelGGL(k_white_noise, grid_for(n3), dim3(256), 0, h->s, f->real, n3, (uint32_t)(seed & 0xFFFFFFFFu),
+                       (uint32_t)(seed >> 32));
+    if (hipfftExecD2Z(f->fwd, f->real, f->cplx) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecD2Z");
+    const double kf = 2.0 * M_PI / h->p.box;
+    const double kc = kcut_frac * M_PI * N / h->p.box;
+    hipLaunchKernelGGL(k_filter, grid_for((size_t)N * N * (N / 2 + 1)), dim3(256), 0, h->s, f->cplx, N, kf, kc, slope);
+    if (hipfftExecZ2D(f->inv, f->cplx, f->real) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecZ2D");
+    const dim3 gb = grid_for(n3);
+    double s1 = 0.0, s2 = 0.0;
+    hipLaunchKernelGGL(k_moment, gb, dim3(256), 0, h->s, f->real, n3, 0.0, 1, h->partials);
+    if ((rc = sum_blocks(h, h->partials, (int)gb.x, &s1))) return rc;
+    const double mu = s1 / (double)n3;
+    hipLaunchKernelGGL(k_moment, gb, dim3(256), 0, h->s, f->real, n3, mu, 2, h->partials);
+    if ((rc = sum_blocks(h, h->partials, (int)gb.x, &s2))) return rc;
+    const double sd = std::sqrt(s2 / (double)n3);
+    if (!(sd > 0.0)) return set_err(h, MGS_E_DOMAIN, "degenerate Gaussian field");
+    hipLaunchKernelGGL(k_lognormal, gb, dim3(256), 0, h->s, f->real, h->L[0].rho, make_geo(N), mu, sigma_g / sd,
+                       0.5 * sigma_g * sigma_g);
+    if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "lognormal density");
+    h->has_density = true;
+    h->rho_cached = false;
+    return 0;
+}
+
+int mgs_get_density(mgs_handle* h, void* dst, int32_t dtype, int32_t where)
+{
+    if (!h || !dst) return set_err(h, MGS_E_ARG, "NULL argument");
+    const int N = h->p.ngrid;
+    const size_t bytes = (size_t)N * N * N * (dtype == MGS_F32 ? 4 : 8);
+    void* out = dst;
+    if (where == MGS_HOST) {
+        int rc = ensure_staging(h, bytes);
+        if (rc) return rc;
+        out = h->staging;
+    }
+    if (dtype == MGS_F32)
+        launch_unpack_f64_to_f32(h->L[0].rho, (float*)out, N, 1.0, h->s);
+    else
+        launch_unpack_f64_to_f64(h->L[0].rho, (double*)out, N, 1.0, h->s);
+    if (where == MGS_HOST && hipMemcpyAsync(dst, out, bytes, hipMemcpyDeviceToHost, h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, "density copy");
+    if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "density copy");
+    return 0;
+}
+
+int mgs_fifth_force(mgs_handle* h, void* psi, void* gx, void* gy, void* gz, int32_t dtype, int32_t where)
+{
+    if (!h) return set_err(h, MGS_E_ARG, "NULL handle");
+    if (dtype != MGS_F32 && dtype != MGS_F64) return set_err(h, MGS_E_ARG, "dtype");
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    const size_t es = dtype == MGS_F32 ? 4 : 8;
+    void* outs[4] = {psi, gx, gy, gz};
+    void* dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<void*> tmp;
+    for (int a = 0; a < 4; a++) {
+        if (!outs[a]) continue;
+        if (where == MGS_DEVICE) {
+            dev[a] = outs[a];
+        } else {
+            void* q = nullptr;
+            if (hipMalloc(&q, n3 * es) != hipSuccess) {
+                for (void* t : tmp) hipFree(t);
+                return set_err(h, MGS_E_NOMEM, "fifth-force staging");
+            }
+            tmp.push_back(q);
+            dev[a] = q;
+        }
+    }
+    double ubg = 1.0;
+    const double fbg = fr_background(h->p, &ubg);
+    const double ctilde = 2.99792458e3 * (double)N / h->p.box;
+    const double fct = ctilde * ctilde * 0.5 * fbg;
+    prof_begin(h, MGS_KC_UPDATE, 0.0);
+    if (dtype == MGS_F32)
+        hipLaunchKernelGGL((k_fifth_force<float>), grid_for(n3), dim3(256), 0, h->s, h->L[0].u, (float*)dev[0],
+                           (float*)dev[1], (float*)dev[2], (float*)dev[3], make_geo(N), h->p.model, h->p.fr_n, fct,
+                           1.0 / ubg);
+    else
+        hipLaunchKernelGGL((k_fifth_force<double>), grid_for(n3), dim3(256), 0, h->s, h->L[0].u, (double*)dev[0],
+                           (double*)dev[1], (double*)dev[2], (double*)dev[3], make_geo(N), h->p.model, h->p.fr_n,
+                           fct, 1.0 / ubg);
+    prof_end(h);
+    int rc = 0;
+    if (where == MGS_HOST)
+        for (int a = 0; a < 4; a++)
+            if (outs[a] && hipMemcpyAsync(outs[a], dev[a], n3 * es, hipMemcpyDeviceToHost, h->s) != hipSuccess)
+                rc = set_err(h, MGS_E_HIP, "fifth-force copy");
+    if (hipStreamSynchronize(h->s) != hipSuccess) rc = set_err(h, MGS_E_HIP, "fifth force");
+    for (void* t : tmp) hipFree(t);
+    return rc;
+}
+
+// which: 0 density delta, 1 field u, 2 fifth-force potential psi
+int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, double* count)
+{
+    if (!h || !k || !pk) return set_err(h, MGS_E_ARG, "NULL argument");
+    if (which < 0 || which > 2) return set_err(h, MGS_E_ARG, "which must be 0 (delta), 1 (u) or 2 (psi)");
+    int rc = fft_ensure(h);
+    if (rc) return rc;
+    FftState* f = fft_of(h);
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    const Geo g = make_geo(N);
+    if (which == 0)
+        launch_unpack_f64_to_f64(h->L[0].rho, f->real, N, 1.0, h->s);
+    else if (which == 1)
+        launch_unpack_f64_to_f64(h->L[0].u, f->real, N, 1.0, h->s);
+    else {
+        double ubg = 1.0;
+        const double fbg = fr_background(h->p, &ubg);
+        const double ctilde = 2.99792458e3 * (double)N / h->p.box;
+        hipLaunchKernelGGL((k_fifth_force<double>), grid_for(n3), dim3(256), 0, h->s, h->L[0].u, f->real,
+                           (double*)nullptr, (double*)nullptr, (double*)nullptr, g, h->p.model, h->p.fr_n,
+                           ctilde * ctilde * 0.5 * fbg, 1.0 / ubg);
+    }
+    if (hipfftExecD2Z(f->fwd, f->real, f->cplx) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecD2Z");
+    const int nb = N + 1;
+    if (hipMemsetAsync(f->bins, 0, 3 * (size_t)nb * sizeof(double), h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, "bins");
+    const double norm = 1.0 / ((double)n3 * (double)n3);
+    hipLaunchKernelGGL(k_bin_power, grid_for((size_t)N * N * (N / 2 + 1)), dim3(256), 0, h->s, f->cplx, N, norm,
+                       f->bins, nb);
+    std::vector<double> hb(3 * (size_t)nb);
+    if (hipMemcpyAsync(hb.data(), f->bins, hb.size() * sizeof(double), hipMemcpyDeviceToHost, h->s) != hipSuccess ||
+        hipStreamSynchronize(h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, "bins copy");
+    const double box = h->p.box;
+    for (int b = 1; b <= N / 2; b++) {
+        const double c = hb[nb + b];
+        k[b - 1] = c > 0 ? hb[2 * nb + b] / c * 2.0 * M_PI / box : 0.0;
+        pk[b - 1] = c > 0 ? box * box * box * hb[b] / c : 0.0;
+        if (count) count[b - 1] = c;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,856 @@
+// mgs_fr.hip -- f(R) multigrid kernels for gfx950 (CDNA4).
+//
+// Plain FP64 path: the reference's operators (PMP2MGsolver_fR.f90,
+// PMP2extradof.f90:1168-1578) evaluated in FP64 with the reference's
+// operand order, on the colour-split layout of mgs_common.h.  Built with
+// -ffp-contract=off so every add/mul rounds as the CPU oracle's does; the
+// transfer operators are bit-identical to the oracle's FP64 variant and the
+// smoother differs only through the device libm (pow/acos/cos).
+//
+// Mixed path: the FP32 delta-form FAS used inside the FP64 defect-correction
+// loop (see mgs_api.cpp, solve_mixed).  Unknown d = x^{n+1} - xb^{n+1} with
+// x = u / u_bg and xb an FP32 copy of the current FP64 iterate.  The inner
+// operator
+//     D(d) = A (sum_nb d_nb - 6 d) + C (1/xb - 1/x)
+// is N(xb + .) - N(xb) written so that every term is proportional to d:
+// FP32 rounding then scales with the correction and the outer FP64 residual
+// converges to FP64 accuracy.  Each cell is solved by the reference's closed
+// form for x^{n+2} + P x + Q = 0 in FP32 (robust far from the solution),
+// followed by two Newton steps on D(d) = rhs in d (restores relative
+// precision near the solution).
+//
+// Memory-bound stencils: one thread per cell, 256-thread blocks, rows of one
+// colour contiguous, and an XCD-aware block order (mgs_map_cell) so that the
+// rows a block's stencil touches are resident in the same XCD's L2.
+
+#include <math.h>
+
+#include "mgs_device.h"
+#include "mgs_kernels.h"
+
+namespace mgs {
+
+// ---------------------------------------------------------------------------
+// closed-form local roots of u^{n+2} + P u + Q = 0,
+// PMP2MGsolver_fR.f90:105-112 (n=0), 159-179 (n=1), 240-256 (n=2)
+// ---------------------------------------------------------------------------
+template <int FRN>
+__device__ __forceinline__ double fr_root64(double P, const FrCoef& c)
+{
+    const double ZERO = 1.0e-20;
+    const double ONEOVERTHREE = 1.0 / 3.0;
+    const double TWOOVERTHREE = 2.0 / 3.0;
+    const double TWOPIOVERTHREE = 2.0 / 3.0 * 3.14159265358979323846;
+    if constexpr (FRN == 0) {
+        return 0.5 * (-P + sqrt(P * P - c.Q4));
+    } else if constexpr (FRN == 1) {
+        const double Delta10 = -3.0 * P;
+        const double CC = c.Delta11 * c.Delta11 - 4.0 * Delta10 * Delta10 * Delta10;
+        if (CC >= 0.0) {
+            const double tmp1 = 0.5 * (sqrt(CC) + c.Delta11);
+            const double tmp2 = tmp1 - c.Delta11;
+            if (fabs(P) < ZERO) return c.Q2onethird;
+            return -(copysign(pow(fabs(tmp1), ONEOVERTHREE), tmp1) - copysign(pow(fabs(tmp2), ONEOVERTHREE), tmp2)) *
+                   ONEOVERTHREE;
+        }
+        const double sd = sqrt(Delta10);
+        const double S = acos(0.5 * c.Delta11 / (sd * sd * sd));
+        return -TWOOVERTHREE * sd * cos(S * ONEOVERTHREE + TWOPIOVERTHREE);
+    } else {
+        const double Delta21 = 27.0 * P * P;
+        const double tmp1 = 0.5 * (sqrt(Delta21 * Delta21 - 4.0 * c.Delta20 * c.Delta20 * c.Delta20) + Delta21);
+        const double cb = copysign(pow(fabs(tmp1), ONEOVERTHREE), tmp1);
+        const double S = sqrt(cb + c.Delta20 / cb) / sqrt(12.0);
+        if (P > ZERO) return -S + 0.5 * sqrt(-4.0 * S * S + P / S);
+        if (P < -ZERO) return S + 0.5 * sqrt(-4.0 * S * S - P / S);
+        return sqrt(sqrt(-c.Q));
+    }
+}
+
+// ===========================================================================
+// Plain FP64 path
+// ===========================================================================
+
+// relaxation_iterations_fR, one colour, PMP2MGsolver_fR.f90:20-301
+template <int FRN, bool COARSE>
+__global__ __launch_bounds__(256) void k_fr_smooth64(double* __restrict__ u, const double* __restrict__ rho,
+                                                     const double* __restrict__ f, Geo g, int colour, FrCoef c, int TX,
+                                                     int TY, int XB, unsigned NB)
+{
+    int m;
+    long row;
+    if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const Nb q = neighbours(g, colour, m, j, k);
+    const double S = wpow<FRN>(u[q.xp]) + wpow<FRN>(u[q.xm]) + wpow<FRN>(u[q.yp]) + wpow<FRN>(u[q.ym]) +
+                     wpow<FRN>(u[q.zp]) + wpow<FRN>(u[q.zm]);
+    double P;
+    if constexpr (!COARSE)
+        P = c.fct1s * rho[q.me] - c.Q2 - S / 6.0;
+    else
+        P = c.fct1s * rho[q.me] - c.Q2 + c.fct2s * f[q.me] - S / 6.0;
+    u[q.me] = fr_root64<FRN>(P, c);
+}
+
+// residual of one cell, calculate_residual_fR, PMP2MGsolver_fR.f90:434-458
+template <int FRN>
+__device__ __forceinline__ double fr_res_cell(const double* __restrict__ u, const Nb& q, double rho, const FrCoef& c)
+{
+    const double uc = u[q.me];
+    const double S = wpow<FRN>(u[q.xp]) + wpow<FRN>(u[q.xm]) + wpow<FRN>(u[q.yp]) + wpow<FRN>(u[q.ym]) +
+                     wpow<FRN>(u[q.zp]) + wpow<FRN>(u[q.zm]);
+    double OP = S - wpow<FRN>(uc) * 6.0;
+    OP = OP * c.fct3r;
+    OP = OP + c.fct1r * rho;
+    OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
+    return OP;
+}
+
+template <int FRN, bool COARSE>
+__global__ __launch_bounds__(256) void k_fr_residual64(const double* __restrict__ u, const double* __restrict__ rho,
+                                                       const double* __restrict__ f, double* __restrict__ res,
+                                                       double* __restrict__ partials, Geo g, FrCoef c, int TX, int TY,
+                                                       int XB, unsigned NB)
+{
+    int m;
+    long row;
+    double acc = 0.0;
+    if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
+        const long nn = (long)g.n * g.n;
+        const int col = (int)(row / nn);
+        const long r2 = row - col * nn;
+        const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+        const Nb q = neighbours(g, col, m, j, k);
+        double OP = fr_res_cell<FRN>(u, q, rho[q.me], c);
+        if constexpr (COARSE) OP = OP - f[q.me];
+        if (res) res[q.me] = OP;
+        acc = OP * OP;
+    }
+    store_partial(partials, NB, block_sum(acc));
+}
+
+// restrict_residual_fR(1): the 8 children's residuals averaged, recomputed
+// from u and delta, PMP2MGsolver_fR.f90:607-749
+template <int FRN>
+__global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* __restrict__ u0,
+                                                                  const double* __restrict__ delta,
+                                                                  double* __restrict__ rres1, Geo gf, Geo gc,
+                                                                  FrCoef c, int TX, int TY, int XB, unsigned NB)
+{
+    int I;
+    long row;
+    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
+    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int N = gf.n;
+    const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1, z0 = 2 * K, z1 = 2 * K + 1;
+    const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
+    const int yl = (J == 0) ? N - 1 : 2 * J - 1, yu = (2 * J + 2 == N) ? 0 : 2 * J + 2;
+    const int zl = (K == 0) ? N - 1 : 2 * K - 1, zu = (2 * K + 2 == N) ? 0 : 2 * K + 2;
+    const size_t ch[8] = {split_index(gf, x0, y0, z0), split_index(gf, x1, y0, z0), split_index(gf, x0, y1, z0),
+                          split_index(gf, x1, y1, z0), split_index(gf, x0, y0, z1), split_index(gf, x1, y0, z1),
+                          split_index(gf, x0, y1, z1), split_index(gf, x1, y1, z1)};
+    double Q = -8.0 * c.fct2r;
+#pragma unroll
+    for (int q = 0; q < 8; q++) Q = Q + c.fct1r * delta[ch[q]] + c.fct4_1 / u0[ch[q]];
+    double s8 = wpow<FRN>(u0[ch[0]]);
+#pragma unroll
+    for (int q = 1; q < 8; q++) s8 = s8 + wpow<FRN>(u0[ch[q]]);
+    double P = -3.0 * s8;
+#define W_(x, y, z) wpow<FRN>(u0[split_index(gf, x, y, z)])
+    P = P + W_(xl, y0, z0) + W_(xl, y1, z0) + W_(xl, y0, z1) + W_(xl, y1, z1);
+    P = P + W_(xu, y0, z0) + W_(xu, y1, z0) + W_(xu, y0, z1) + W_(xu, y1, z1);
+    P = P + W_(x0, yl, z0) + W_(x1, yl, z0) + W_(x0, yl, z1) + W_(x1, yl, z1);
+    P = P + W_(x0, yu, z0) + W_(x1, yu, z0) + W_(x0, yu, z1) + W_(x1, yu, z1);
+    P = P + W_(x0, y0, zl) + W_(x1, y0, zl) + W_(x0, y1, zl) + W_(x1, y1, zl);
+    P = P + W_(x0, y0, zu) + W_(x1, y0, zu) + W_(x0, y1, zu) + W_(x1, y1, zu);
+#undef W_
+    rres1[split_index(gc, I, J, K)] = (P * c.fct3_1 + Q) / 8.0;
+}
+
+// 8-child average (restrict_density, restrict_scalar_field,
+// restrict_residual_fR at levels > 1); out2 optional second copy
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T* __restrict__ out1,
+                                                   T* __restrict__ out2, Geo gf, Geo gc, int TX, int TY, int XB,
+                                                   unsigned NB)
+{
+    int I;
+    long row;
+    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
+    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    // children in the reference's order (x fastest), accumulated in double
+    // for FP64 data and in float for FP32 data
+    using A = T;
+    A P = fine[split_index(gf, 2 * I, 2 * J, 2 * K)];
+    P = P + fine[split_index(gf, 2 * I + 1, 2 * J, 2 * K)];
+    P = P + fine[split_index(gf, 2 * I, 2 * J + 1, 2 * K)];
+    P = P + fine[split_index(gf, 2 * I + 1, 2 * J + 1, 2 * K)];
+    P = P + fine[split_index(gf, 2 * I, 2 * J, 2 * K + 1)];
+    P = P + fine[split_index(gf, 2 * I + 1, 2 * J, 2 * K + 1)];
+    P = P + fine[split_index(gf, 2 * I, 2 * J + 1, 2 * K + 1)];
+    P = P + fine[split_index(gf, 2 * I + 1, 2 * J + 1, 2 * K + 1)];
+    const size_t o = split_index(gc, I, J, K);
+    const T v = P / (A)8.0;
+    out1[o] = v;
+    if (out2) out2[o] = v;
+}
+
+// calculate_physical_right_hand_side_fR, PMP2MGsolver_fR.f90:789-927
+template <int FRN>
+__global__ __launch_bounds__(256) void k_fr_rhs64(const double* __restrict__ ru, const double* __restrict__ rho,
+                                                  const double* __restrict__ rres, double* __restrict__ f, Geo g,
+                                                  FrCoef c, int TX, int TY, int XB, unsigned NB)
+{
+    int m;
+    long row;
+    if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const long nn = (long)g.n * g.n;
+    const int col = (int)(row / nn);
+    const long r2 = row - col * nn;
+    const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+    const Nb q = neighbours(g, col, m, j, k);
+    double OP = fr_res_cell<FRN>(ru, q, rho[q.me], c);
+    OP = OP - rres[q.me];
+    f[q.me] = OP;
+}
+
+// correct_solution, PMP2extradof.f90:1361-1578: u_h += P(u_H - R u_h) with the
+// 27/9/3/1 (/64) weights; guard: keep the old value unless the new one is > 0
+template <typename T, typename A>
+__global__ __launch_bounds__(256) void k_correct(const T* __restrict__ uH, const T* __restrict__ ru,
+                                                 T* __restrict__ uf, Geo gf, Geo gc, int guard, int TX, int TY,
+                                                 int XB, unsigned NB)
+{
+    int I;
+    long row;
+    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
+    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int nc = gc.n;
+    const A a = (A)1.0 / (A)64.0, b = (A)3.0 * a, c9 = (A)9.0 * a, d = (A)27.0 * a;
+    const A wt[8] = {a, b, b, c9, b, c9, c9, d};
+    const int Im = wrap_dec(I, nc), Ip = wrap_inc(I, nc);
+    const int Jm = wrap_dec(J, nc), Jp = wrap_inc(J, nc);
+    const int Km = wrap_dec(K, nc), Kp = wrap_inc(K, nc);
+#pragma unroll
+    for (int sind = 0; sind < 8; sind++) {
+        const int s1 = sind & 1, s2 = (sind >> 1) & 1, s3 = (sind >> 2) & 1;
+        const int nx = s1 ? Ip : Im, ny = s2 ? Jp : Jm, nz = s3 ? Kp : Km;
+        A P = (A)0.0;
+#pragma unroll
+        for (int ind = 0; ind < 8; ind++) {
+            const int X = (ind & 1) ? I : nx;
+            const int Y = (ind & 2) ? J : ny;
+            const int Z = (ind & 4) ? K : nz;
+            const size_t qq = split_index(gc, X, Y, Z);
+            P = P + wt[ind] * (A)uH[qq];
+            P = P - wt[ind] * (A)ru[qq];
+        }
+        const size_t fi = split_index(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3);
+        const A tmpt = (A)uf[fi] + P;
+        if (!guard || tmpt > (A)0.0) uf[fi] = (T)tmpt;
+    }
+}
+
+// deterministic final sum of block partials
+__global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict__ p, int count, double* out)
+{
+    __shared__ double s[1024];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) v += p[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion
+// ---------------------------------------------------------------------------
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ lex, TO* __restrict__ split, Geo g, int TX,
+                                              int TY, int XB, unsigned NB)
+{
+    int i;
+    long row;
+    if (!map_xy(g.n, (long)g.n * g.n, TX, TY, XB, NB, i, row)) return;
+    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    split[split_index(g, i, j, k)] = (TO)lex[(size_t)row * g.n + i];
+}
+
+template <typename TO>
+__global__ __launch_bounds__(256) void k_unpack(const double* __restrict__ split, TO* __restrict__ lex, Geo g,
+                                                double scale, int TX, int TY, int XB, unsigned NB)
+{
+    int i;
+    long row;
+    if (!map_xy(g.n, (long)g.n * g.n, TX, TY, XB, NB, i, row)) return;
+    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const double v = split[split_index(g, i, j, k)];
+    lex[(size_t)row * g.n + i] = (TO)(scale == 1.0 ? v : v * scale);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T* __restrict__ p, size_t count, T v)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy(const T* __restrict__ s, T* __restrict__ d, size_t count)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) d[i] = s[i];
+}
+
+// ===========================================================================
+// Mixed precision: FP32 delta-form FAS
+// ===========================================================================
+
+// closed-form positive root of x^{n+2} + P x + Q = 0 in FP32 (normalised
+// variables; same branch structure as the reference's smoother)
+template <int FRN>
+__device__ __forceinline__ float df_root32(float P, float Q)
+{
+    if constexpr (FRN == 0) {
+        return 0.5f * (-P + sqrtf(fmaxf(P * P - 4.0f * Q, 0.0f)));
+    } else if constexpr (FRN == 1) {
+        const float D0 = -3.0f * P;
+        const float D1 = 27.0f * Q;
+        const float CC = D1 * D1 - 4.0f * D0 * D0 * D0;
+        if (CC >= 0.0f) {
+            const float t1 = 0.5f * (sqrtf(CC) + D1);
+            const float t2 = t1 - D1;
+            return -(cbrtf(t1) - cbrtf(t2)) * (1.0f / 3.0f);
+        }
+        const float sd = sqrtf(D0);
+        const float arg = fminf(fmaxf(0.5f * D1 / (sd * sd * sd), -1.0f), 1.0f);
+        return -(2.0f / 3.0f) * sd * cosf(acosf(arg) * (1.0f / 3.0f) + 2.0943951023931955f);
+    } else {
+        const float D20 = 12.0f * Q;
+        const float D21 = 27.0f * P * P;
+        const float t1 = 0.5f * (sqrtf(fmaxf(D21 * D21 - 4.0f * D20 * D20 * D20, 0.0f)) + D21);
+        const float cb = cbrtf(t1);
+        const float S = sqrtf(cb + D20 / cb) * 0.28867513459481287f;  // 1/sqrt(12)
+        if (P > 0.0f) return -S + 0.5f * sqrtf(fmaxf(-4.0f * S * S + P / S, 0.0f));
+        if (P < 0.0f) return S + 0.5f * sqrtf(fmaxf(-4.0f * S * S - P / S, 0.0f));
+        return sqrtf(sqrtf(-Q));
+    }
+}
+
+// x from w = x^{n+1}
+template <int FRN, typename T>
+__device__ __forceinline__ T xroot(T w)
+{
+    if constexpr (FRN == 0)
+        return w;
+    else if constexpr (FRN == 1)
+        return sqrt(w);
+    else
+        return cbrt(w);
+}
+
+// (x^{n+1} - xb^{n+1}) / (x - xb)
+template <int FRN, typename T>
+__device__ __forceinline__ T dden(T x, T xb)
+{
+    if constexpr (FRN == 0)
+        return (T)1;
+    else if constexpr (FRN == 1)
+        return x + xb;
+    else
+        return x * x + x * xb + xb * xb;
+}
+
+// local term C (1/xb - 1/x) of D for increment d (x consistent with d)
+template <int FRN, typename T>
+__device__ __forceinline__ T dlocal(T d, T x, T xb, T C)
+{
+    return C * (d / dden<FRN>(x, xb)) / (x * xb);
+}
+
+// solve A (S - 6 d) + C (1/xb - 1/x(d)) = r for d
+template <int FRN>
+__device__ __forceinline__ float df_solve(float S, float xb, float r, const DfCoef& c)
+{
+    const float wb = wpow<FRN>(xb);
+    const float K = c.A * S + 6.0f * c.A * wb + c.C / xb - r;
+    float x = df_root32<FRN>(-K * c.inv6A, c.C * c.inv6A);
+    if (!(x > 0.0f) || !isfinite(x)) x = xb;
+    float d = wpow<FRN>(x) - wb;
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        if (it > 0) x = xroot<FRN>(fmaxf(wb + d, 1.0e-30f));
+        const float F = c.A * (S - 6.0f * d) + dlocal<FRN>(d, x, xb, c.C) - r;
+        const float Fp = -6.0f * c.A + c.C / ((float)(FRN + 1) * x * wpow<FRN>(x));
+        d = d - F / Fp;
+    }
+    // stay inside the domain x > 0
+    if (!(wb + d > 0.0f)) d = -0.5f * wb;
+    return d;
+}
+
+template <int FRN>
+__global__ __launch_bounds__(256) void k_df_smooth(float* __restrict__ d, const float* __restrict__ xb,
+                                                   const float* __restrict__ rhs, Geo g, int colour, DfCoef c, int TX,
+                                                   int TY, int XB, unsigned NB)
+{
+    int m;
+    long row;
+    if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const Nb q = neighbours(g, colour, m, j, k);
+    const float S = d[q.xp] + d[q.xm] + d[q.yp] + d[q.ym] + d[q.zp] + d[q.zm];
+    d[q.me] = df_solve<FRN>(S, xb[q.me], rhs[q.me], c);
+}
+
+// D(d) - rhs at one cell
+template <int FRN>
+__device__ __forceinline__ float df_res_cell(const float* __restrict__ d, const Nb& q, float xb, float rhs,
+                                             const DfCoef& c)
+{
+    const float dc = d[q.me];
+    const float S = d[q.xp] + d[q.xm] + d[q.yp] + d[q.ym] + d[q.zp] + d[q.zm];
+    const float x = xroot<FRN>(fmaxf(wpow<FRN>(xb) + dc, 1.0e-30f));
+    return c.A * (S - 6.0f * dc) + dlocal<FRN>(dc, x, xb, c.C) - rhs;
+}
+
+template <int FRN>
+__global__ __launch_bounds__(256) void k_df_residual(const float* __restrict__ d, const float* __restrict__ xb,
+                                                     const float* __restrict__ rhs, float* __restrict__ res, Geo g,
+                                                     DfCoef c, int TX, int TY, int XB, unsigned NB)
+{
+    int m;
+    long row;
+    if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const long nn = (long)g.n * g.n;
+    const int col = (int)(row / nn);
+    const long r2 = row - col * nn;
+    const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+    const Nb q = neighbours(g, col, m, j, k);
+    res[q.me] = df_res_cell<FRN>(d, q, xb[q.me], rhs[q.me], c);
+}
+
+// f = D(rd) - rres  (FAS right-hand side of the inner problem)
+template <int FRN>
+__global__ __launch_bounds__(256) void k_df_rhs(const float* __restrict__ rd, const float* __restrict__ xb,
+                                                const float* __restrict__ rres, float* __restrict__ f, Geo g,
+                                                DfCoef c, int TX, int TY, int XB, unsigned NB)
+{
+    int m;
+    long row;
+    if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const long nn = (long)g.n * g.n;
+    const int col = (int)(row / nn);
+    const long r2 = row - col * nn;
+    const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+    const Nb q = neighbours(g, col, m, j, k);
+    f[q.me] = df_res_cell<FRN>(rd, q, xb[q.me], 0.0f, c) - rres[q.me];
+}
+
+// level-0 inner residual D(d) - g averaged over the 8 children; the
+// children's Laplacians collapse to -3 sum(children) + sum(24 outer faces)
+template <int FRN>
+__global__ __launch_bounds__(256) void k_df_restrict_residual1(const float* __restrict__ d0,
+                                                               const float* __restrict__ xb0,
+                                                               const float* __restrict__ g0, float* __restrict__ rres1,
+                                                               Geo gf, Geo gc, DfCoef c, int TX, int TY, int XB,
+                                                               unsigned NB)
+{
+    int I;
+    long row;
+    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
+    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int N = gf.n;
+    const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1, z0 = 2 * K, z1 = 2 * K + 1;
+    const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
+    const int yl = (J == 0) ? N - 1 : 2 * J - 1, yu = (2 * J + 2 == N) ? 0 : 2 * J + 2;
+    const int zl = (K == 0) ? N - 1 : 2 * K - 1, zu = (2 * K + 2 == N) ? 0 : 2 * K + 2;
+    float loc = 0.0f, s8 = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const size_t o = split_index(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1));
+        const float dc = d0[o], xbc = xb0[o];
+        const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
+        loc += dlocal<FRN>(dc, x, xbc, c.C) - g0[o];
+        s8 += dc;
+    }
+#define D_(x, y, z) d0[split_index(gf, x, y, z)]
+    float outer = D_(xl, y0, z0) + D_(xl, y1, z0) + D_(xl, y0, z1) + D_(xl, y1, z1);
+    outer += D_(xu, y0, z0) + D_(xu, y1, z0) + D_(xu, y0, z1) + D_(xu, y1, z1);
+    outer += D_(x0, yl, z0) + D_(x1, yl, z0) + D_(x0, yl, z1) + D_(x1, yl, z1);
+    outer += D_(x0, yu, z0) + D_(x1, yu, z0) + D_(x0, yu, z1) + D_(x1, yu, z1);
+    outer += D_(x0, y0, zl) + D_(x1, y0, zl) + D_(x0, y1, zl) + D_(x1, y1, zl);
+    outer += D_(x0, y0, zu) + D_(x1, y0, zu) + D_(x0, y1, zu) + D_(x1, y1, zu);
+#undef D_
+    rres1[split_index(gc, I, J, K)] = (c.A * (outer - 3.0f * s8) + loc) * 0.125f;
+}
+
+// restricted base: xb_c = (mean of the children's xb^{n+1})^{1/(n+1)}
+template <int FRN>
+__global__ __launch_bounds__(256) void k_restrict_base(const float* __restrict__ xf, float* __restrict__ xc, Geo gf,
+                                                       Geo gc, int TX, int TY, int XB, unsigned NB)
+{
+    int I;
+    long row;
+    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
+    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    float w = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        w += wpow<FRN>(xf[split_index(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1))]);
+    xc[split_index(gc, I, J, K)] = xroot<FRN>(w * 0.125f);
+}
+
+// inner coarse-grid correction d_h += P(d_H - R d_h), kept only where the
+// corrected x stays positive (the delta-form image of the reference's guard)
+template <int FRN>
+__global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH, const float* __restrict__ rd,
+                                                    float* __restrict__ df, const float* __restrict__ xbf, Geo gf,
+                                                    Geo gc, int TX, int TY, int XB, unsigned NB)
+{
+    int I;
+    long row;
+    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
+    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int nc = gc.n;
+    const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
+    const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
+    const int Im = wrap_dec(I, nc), Ip = wrap_inc(I, nc);
+    const int Jm = wrap_dec(J, nc), Jp = wrap_inc(J, nc);
+    const int Km = wrap_dec(K, nc), Kp = wrap_inc(K, nc);
+    float e[27];
+#pragma unroll
+    for (int z = 0; z < 3; z++)
+#pragma unroll
+        for (int y = 0; y < 3; y++)
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                const int X = x == 0 ? Im : (x == 1 ? I : Ip);
+                const int Y = y == 0 ? Jm : (y == 1 ? J : Jp);
+                const int Z = z == 0 ? Km : (z == 1 ? K : Kp);
+                const size_t qq = split_index(gc, X, Y, Z);
+                e[x + 3 * y + 9 * z] = dH[qq] - rd[qq];
+            }
+#pragma unroll
+    for (int sind = 0; sind < 8; sind++) {
+        const int s1 = sind & 1, s2 = (sind >> 1) & 1, s3 = (sind >> 2) & 1;
+        float P = 0.0f;
+#pragma unroll
+        for (int ind = 0; ind < 8; ind++) {
+            const int x = (ind & 1) ? 1 : (s1 ? 2 : 0);
+            const int y = (ind & 2) ? 1 : (s2 ? 2 : 0);
+            const int z = (ind & 4) ? 1 : (s3 ? 2 : 0);
+            P += wt[ind] * e[x + 3 * y + 9 * z];
+        }
+        const size_t fi = split_index(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3);
+        const float dn = df[fi] + P;
+        if (wpow<FRN>(xbf[fi]) + dn > 0.0f) df[fi] = dn;
+    }
+}
+
+// coarsest level: red-black sweeps of the delta-form smoother in FP64 in one
+// workgroup (the level holds at most a few thousand cells)
+template <int FRN>
+__global__ __launch_bounds__(1024) void k_df_coarsest(float* __restrict__ d, const float* __restrict__ xb,
+                                                      const float* __restrict__ rhs, Geo g, int sweeps, DfCoef c)
+{
+    const int ncell = (int)(2 * g.half);
+    for (int sw = 0; sw < sweeps; sw++) {
+        for (int colour = 0; colour < 2; colour++) {
+            for (int t = threadIdx.x; t < (int)g.half; t += blockDim.x) {
+                const int m = t % g.h;
+                const int row = t / g.h;
+                const int j = row % g.n, k = row / g.n;
+                const Nb q = neighbours(g, colour, m, j, k);
+                const double S = (double)d[q.xp] + d[q.xm] + d[q.yp] + d[q.ym] + d[q.zp] + d[q.zm];
+                const double xbc = xb[q.me];
+                const double wb = wpow<FRN>(xbc);
+                const double r = rhs[q.me];
+                // Newton on the increment in FP64 from the current value
+                double dl = d[q.me];
+                for (int it = 0; it < 30; it++) {
+                    double x = xroot<FRN>(fmax(wb + dl, 1.0e-300));
+                    const double F = c.A64 * (S - 6.0 * dl) + dlocal<FRN>(dl, x, xbc, c.C64) - r;
+                    const double Fp = -6.0 * c.A64 + c.C64 / ((double)(FRN + 1) * x * wpow<FRN>(x));
+                    double dn = dl - F / Fp;
+                    if (!(wb + dn > 0.0)) dn = 0.5 * (dl - wb);  // halve the distance to x = 0
+                    const bool done = fabs(dn - dl) <= 1.0e-15 * (fabs(dn) + wb);
+                    dl = dn;
+                    if (done) break;
+                }
+                d[q.me] = (float)dl;
+            }
+            __syncthreads();
+        }
+    }
+    (void)ncell;
+}
+
+// FP64 outer residual r = N(u) (the plain path's residual), emitting the
+// inner problem's data: g = -r (FP32) and xb = u / u_bg (FP32)
+template <int FRN>
+__global__ __launch_bounds__(256) void k_outer_residual(const double* __restrict__ u, const double* __restrict__ delta,
+                                                        float* __restrict__ g32, float* __restrict__ xb,
+                                                        double* __restrict__ partials, Geo g, FrCoef c, int TX, int TY,
+                                                        int XB, unsigned NB)
+{
+    int m;
+    long row;
+    double acc = 0.0;
+    if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
+        const long nn = (long)g.n * g.n;
+        const int col = (int)(row / nn);
+        const long r2 = row - col * nn;
+        const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+        const Nb q = neighbours(g, col, m, j, k);
+        const double OP = fr_res_cell<FRN>(u, q, delta[q.me], c);
+        g32[q.me] = (float)(-OP);
+        xb[q.me] = (float)(u[q.me] / c.ubg);
+        acc = OP * OP;
+    }
+    store_partial(partials, NB, block_sum(acc));
+}
+
+// u += u_bg (x(d) - xb), evaluated in FP64 as u_bg d / dden(x, xb)
+template <int FRN>
+__global__ __launch_bounds__(256) void k_outer_update(double* __restrict__ u, const float* __restrict__ d,
+                                                      const float* __restrict__ xb, size_t count, double ubg)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const double dl = d[i], xbc = xb[i];
+        const double w = wpow<FRN>(xbc) + dl;
+        if (!(w > 0.0)) continue;
+        const double x = xroot<FRN>(w);
+        u[i] = u[i] + ubg * (dl / dden<FRN>(x, xbc));
+    }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+#define LAUNCH_ARGS L.TX, L.TY, L.XB, L.NB
+
+#define FRN_DISPATCH(frn, KERNEL, ...)                                     \
+    do {                                                                   \
+        if ((frn) == 0)                                                    \
+            hipLaunchKernelGGL((KERNEL<0>), L.grid, L.block, 0, s, __VA_ARGS__); \
+        else if ((frn) == 1)                                               \
+            hipLaunchKernelGGL((KERNEL<1>), L.grid, L.block, 0, s, __VA_ARGS__); \
+        else                                                               \
+            hipLaunchKernelGGL((KERNEL<2>), L.grid, L.block, 0, s, __VA_ARGS__); \
+    } while (0)
+
+void launch_pack_f64_from_f64(const double* lex, double* split, int n, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(n, (long)n * n);
+    hipLaunchKernelGGL((k_pack<double, double>), L.grid, L.block, 0, s, lex, split, g, LAUNCH_ARGS);
+}
+
+void launch_pack_f64_from_f32(const float* lex, double* split, int n, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(n, (long)n * n);
+    hipLaunchKernelGGL((k_pack<float, double>), L.grid, L.block, 0, s, lex, split, g, LAUNCH_ARGS);
+}
+
+void launch_unpack_f64_to_f64(const double* split, double* lex, int n, double scale, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(n, (long)n * n);
+    hipLaunchKernelGGL((k_unpack<double>), L.grid, L.block, 0, s, split, lex, g, scale, LAUNCH_ARGS);
+}
+
+void launch_unpack_f64_to_f32(const double* split, float* lex, int n, double scale, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(n, (long)n * n);
+    hipLaunchKernelGGL((k_unpack<float>), L.grid, L.block, 0, s, split, lex, g, scale, LAUNCH_ARGS);
+}
+
+static dim3 flat_grid(size_t count)
+{
+    size_t b = (count + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+void launch_fill_f64(double* p, size_t count, double v, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_fill<double>), flat_grid(count), dim3(256), 0, s, p, count, v);
+}
+
+void launch_fill_f32(float* p, size_t count, float v, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_fill<float>), flat_grid(count), dim3(256), 0, s, p, count, v);
+}
+
+void launch_copy_f64(const double* src, double* dst, size_t count, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_copy<double>), flat_grid(count), dim3(256), 0, s, src, dst, count);
+}
+
+void launch_fr_smooth64(int frn, int level, double* u, const double* rho, const double* f, int n, int colour,
+                        const FrCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, (long)n * n);
+#define SM(FRN, CO) hipLaunchKernelGGL((k_fr_smooth64<FRN, CO>), L.grid, L.block, 0, s, u, rho, f, g, colour, c, LAUNCH_ARGS)
+    if (level == 0) {
+        if (frn == 0) SM(0, false);
+        else if (frn == 1) SM(1, false);
+        else SM(2, false);
+    } else {
+        if (frn == 0) SM(0, true);
+        else if (frn == 1) SM(1, true);
+        else SM(2, true);
+    }
+#undef SM
+}
+
+int launch_fr_residual64(int frn, int level, const double* u, const double* rho, const double* f, double* res,
+                         double* partials, int n, const FrCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+#define RS(FRN, CO) \
+    hipLaunchKernelGGL((k_fr_residual64<FRN, CO>), L.grid, L.block, 0, s, u, rho, f, res, partials, g, c, LAUNCH_ARGS)
+    if (level == 0) {
+        if (frn == 0) RS(0, false);
+        else if (frn == 1) RS(1, false);
+        else RS(2, false);
+    } else {
+        if (frn == 0) RS(0, true);
+        else if (frn == 1) RS(1, true);
+        else RS(2, true);
+    }
+#undef RS
+    return (int)L.NB;
+}
+
+void launch_fr_restrict_residual1_64(int frn, const double* u0, const double* delta, double* rres1, int N,
+                                     const FrCoef& c0, hipStream_t s)
+{
+    const Geo gf = make_geo(N), gc = make_geo(N / 2);
+    const Launch L = make_launch(gc.n, (long)gc.n * gc.n);
+    FRN_DISPATCH(frn, k_fr_restrict_residual1_64, u0, delta, rres1, gf, gc, c0, LAUNCH_ARGS);
+}
+
+void launch_restrict8_64(const double* fine, double* out1, double* out2, int nc, hipStream_t s)
+{
+    const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
+    const Launch L = make_launch(nc, (long)nc * nc);
+    hipLaunchKernelGGL((k_restrict8<double>), L.grid, L.block, 0, s, fine, out1, out2, gf, gc, LAUNCH_ARGS);
+}
+
+void launch_restrict8_32(const float* fine, float* out1, float* out2, int nc, hipStream_t s)
+{
+    const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
+    const Launch L = make_launch(nc, (long)nc * nc);
+    hipLaunchKernelGGL((k_restrict8<float>), L.grid, L.block, 0, s, fine, out1, out2, gf, gc, LAUNCH_ARGS);
+}
+
+void launch_fr_rhs64(int frn, const double* ru, const double* rho, const double* rres, double* f, int n,
+                     const FrCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+    FRN_DISPATCH(frn, k_fr_rhs64, ru, rho, rres, f, g, c, LAUNCH_ARGS);
+}
+
+void launch_correct64(const double* uH, const double* ru, double* ufine, int nc, int guard, hipStream_t s)
+{
+    const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
+    const Launch L = make_launch(nc, (long)nc * nc);
+    hipLaunchKernelGGL((k_correct<double, double>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, LAUNCH_ARGS);
+}
+
+void launch_sum_partials(const double* partials, int count, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, partials, count, out);
+}
+
+int launch_outer_residual(int frn, const double* u, const double* delta, float* g32, float* xb, double* partials,
+                          int n, const FrCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+    FRN_DISPATCH(frn, k_outer_residual, u, delta, g32, xb, partials, g, c, LAUNCH_ARGS);
+    return (int)L.NB;
+}
+
+void launch_outer_update(int frn, double* u, const float* d, const float* xb, int n, double ubg, hipStream_t s)
+{
+    const size_t count = (size_t)n * n * n;
+    const dim3 grid = flat_grid(count);
+    if (frn == 0) hipLaunchKernelGGL((k_outer_update<0>), grid, dim3(256), 0, s, u, d, xb, count, ubg);
+    else if (frn == 1) hipLaunchKernelGGL((k_outer_update<1>), grid, dim3(256), 0, s, u, d, xb, count, ubg);
+    else hipLaunchKernelGGL((k_outer_update<2>), grid, dim3(256), 0, s, u, d, xb, count, ubg);
+}
+
+void launch_restrict_base(int frn, const float* xb_f, float* xb_c, int nc, hipStream_t s)
+{
+    const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
+    const Launch L = make_launch(nc, (long)nc * nc);
+    FRN_DISPATCH(frn, k_restrict_base, xb_f, xb_c, gf, gc, LAUNCH_ARGS);
+}
+
+void launch_df_smooth(int frn, float* d, const float* xb, const float* rhs, int n, int colour, const DfCoef& c,
+                      hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, (long)n * n);
+    FRN_DISPATCH(frn, k_df_smooth, d, xb, rhs, g, colour, c, LAUNCH_ARGS);
+}
+
+void launch_df_residual(int frn, const float* d, const float* xb, const float* rhs, float* res, int n,
+                        const DfCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+    FRN_DISPATCH(frn, k_df_residual, d, xb, rhs, res, g, c, LAUNCH_ARGS);
+}
+
+void launch_df_restrict_residual1(int frn, const float* d0, const float* xb0, const float* g0, float* rres1, int N,
+                                  const DfCoef& c0, hipStream_t s)
+{
+    const Geo gf = make_geo(N), gc = make_geo(N / 2);
+    const Launch L = make_launch(gc.n, (long)gc.n * gc.n);
+    FRN_DISPATCH(frn, k_df_restrict_residual1, d0, xb0, g0, rres1, gf, gc, c0, LAUNCH_ARGS);
+}
+
+void launch_df_rhs(int frn, const float* rd, const float* xb, const float* rres, float* f, int n, const DfCoef& c,
+                   hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+    FRN_DISPATCH(frn, k_df_rhs, rd, xb, rres, f, g, c, LAUNCH_ARGS);
+}
+
+void launch_df_correct(int frn, const float* dH, const float* rd, float* dfine, const float* xbfine, int nc,
+                       hipStream_t s)
+{
+    const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
+    const Launch L = make_launch(nc, (long)nc * nc);
+    FRN_DISPATCH(frn, k_df_correct, dH, rd, dfine, xbfine, gf, gc, LAUNCH_ARGS);
+}
+
+void launch_df_coarsest(int frn, float* d, const float* xb, const float* rhs, int n, int sweeps, const DfCoef& c,
+                        hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    int threads = (int)g.half;
+    if (threads > 1024) threads = 1024;
+    threads = ((threads + 63) / 64) * 64;
+    if (frn == 0) hipLaunchKernelGGL((k_df_coarsest<0>), dim3(1), dim3(threads), 0, s, d, xb, rhs, g, sweeps, c);
+    else if (frn == 1) hipLaunchKernelGGL((k_df_coarsest<1>), dim3(1), dim3(threads), 0, s, d, xb, rhs, g, sweeps, c);
+    else hipLaunchKernelGGL((k_df_coarsest<2>), dim3(1), dim3(threads), 0, s, d, xb, rhs, g, sweeps, c);
+}
+
+}  // namespace mgs

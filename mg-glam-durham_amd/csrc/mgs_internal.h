@@ -1,0 +1,112 @@
+// mgs_internal.h -- the handle behind include/mgsolve.h and the host-side
+// helpers shared by mgs_api.cpp, mgs_dgp.hip, mgs_fft.hip and mgs_serve.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mgsolve.h"
+#include "mgs_common.h"
+
+namespace mgs {
+
+// one level of the FP64 hierarchy (the reference's FI/FI2/FI3 sections)
+struct LevelD {
+    int n = 0;
+    double* u = nullptr;     // FI2 (level 0) / FI3 S4
+    double* rho = nullptr;   // FI (level 0) / FI3 S2
+    double* f = nullptr;     // FI3 S5
+    double* res = nullptr;   // FI3 S6 (level 0: FI3 itself, allocated on demand)
+    double* rres = nullptr;  // FI3 S1
+    double* ru = nullptr;    // FI3 S3
+};
+
+// one level of the FP32 delta-form hierarchy (mixed precision)
+struct LevelF {
+    int n = 0;
+    float* d = nullptr;      // increment of x^{n+1}
+    float* xb = nullptr;     // base x = u / u_bg
+    float* f = nullptr;      // right-hand side (level 0: -r of the FP64 residual)
+    float* res = nullptr;
+    float* rres = nullptr;
+    float* rd = nullptr;     // frozen R d
+};
+
+struct ProfRec {
+    int kc;
+    double bytes;
+    hipEvent_t a, b;
+};
+
+struct ProfClass {
+    long long launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+// DGP constants of one level, PMP2MGsolver_DGP.f90:36-48
+struct DgpCoef {
+    double dx2, dx4, alpha, beta, fct1;
+    double fct2_1, fct3_1;   // fused level-1 restriction
+};
+
+}  // namespace mgs
+
+struct mgs_handle {
+    mgs_params p{};
+    int lmax = 0, lmin = 0, lc = 0;
+    hipStream_t s = nullptr;
+    std::vector<mgs::LevelD> L;
+    std::vector<mgs::LevelF> M;
+    std::vector<void*> allocs;
+    double* partials = nullptr;
+    size_t npart = 0;
+    double* dsum = nullptr;
+    double* hsum = nullptr;          // pinned
+    void* staging = nullptr;
+    size_t staging_bytes = 0;
+    bool has_density = false;
+    bool rho_cached = false;
+    // DGP: out-of-place colour pass target and per-block partials
+    double* dgp_tmp = nullptr;
+    double* dgp_rms = nullptr;
+    // FFT plans (density generator, power spectrum)
+    void* fft = nullptr;
+    // profiling
+    bool prof = false;
+    std::vector<mgs::ProfRec> recs;
+    mgs::ProfClass pc[MGS_KC_COUNT];
+    std::string err;
+};
+
+namespace mgs {
+// mgs_api.cpp
+FrCoef fr_coefs(const mgs_params& p, int level);
+DfCoef df_coefs(const mgs_params& p, int level);
+double fr_background(const mgs_params& p, double* ubg);
+int ensure_staging(mgs_handle* h, size_t bytes);
+void prof_begin(mgs_handle* h, int kc, double bytes);
+void prof_end(mgs_handle* h);
+int vcycle_plain(mgs_handle* h, int l);
+int op_residual(mgs_handle* h, int l, bool store, double* rms);
+
+// mgs_dgp.hip
+DgpCoef dgp_coefs(const mgs_params& p, int level);
+int dgp_alloc(mgs_handle* h);
+void launch_dgp_smooth64(int level, double* u, const double* rho, const double* f, double* tmp, int n, int colour,
+                         const DgpCoef& c, hipStream_t s);
+int launch_dgp_residual64(int level, const double* u, const double* rho, const double* f, double* res,
+                          double* partials, int n, const DgpCoef& c, double* scratch, hipStream_t s);
+void launch_dgp_restrict_residual1_64(const double* u0, const double* delta, double* rres1, int N, const DgpCoef& c0,
+                                      hipStream_t s);
+void launch_dgp_rhs64(const double* ru, const double* rho, const double* rres, double* f, int n, const DgpCoef& c,
+                      hipStream_t s);
+int solve_mixed_dgp(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep);
+
+// mgs_fft.hip
+void fft_release(mgs_handle* h);
+}  // namespace mgs
+
+int set_err(mgs_handle* h, int code, const std::string& msg);

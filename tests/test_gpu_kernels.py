@@ -1,0 +1,228 @@
+"""GPU parity of every multigrid operator, through libmgsolve's C-ABI, against
+the FP64 oracle (oracle/mgs_oracle.c) on identical inputs.
+
+Each operator is checked in isolation: the device state is set from the
+oracle's arrays before the call, so a difference can only come from that
+operator.  Inputs are the seeded fields of the golden fixtures (promoted to
+FP64), parameters cover n = 0, 1, 2 at a = 1 and a = 1/2, and both DGP
+branches.
+
+Tolerances (FP64):
+  * transfer operators, residuals, right-hand sides, prolongation and the DGP
+    smoother use only +, -, *, / and sqrt in the oracle's operand order with
+    FMA contraction off: bit-identical (0 ulp);
+  * the f(R) smoother evaluates pow/acos/cos with the device math library
+    instead of glibc (<= 1 ulp apart): per cell |du| <= rtol |u| +
+    1e-13 max|u| with rtol = 1e-12 for n = 0, 1.  For n = 2 rtol = 1e-10:
+    the reference's Ferrari form S = sqrt(cb + Delta20/cb)/sqrt(12)
+    (PMP2MGsolver_fR.f90:248) cancels as P -> 0 (cb^2 -> 12|Q|) and scales a
+    1-ulp difference of the cube root by cb^2/|cb^2 - 12|Q|| (observed
+    worst 8.8e-12 on the fixtures).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import mgsolve as M
+
+from conftest import GOLDEN
+
+FR_FIX = [f"kernels_fr_n{n}_{t}" for n in (0, 1, 2) for t in ("a1", "a05")]
+DGP_FIX = [f"kernels_dgp_{b}_{t}" for b in ("N", "S") for t in ("a1", "a05")]
+RT_FR_SMOOTH = {0: 1e-12, 1: 1e-12, 2: 1e-10}
+AT_FR_SMOOTH = 1e-13
+
+
+def cfgs(par, precision=M.PREC_FP64):
+    N, model, frn, br, box, om, oml, a, fr0, h0rc = [float(x) for x in par]
+    op = O.Params(N=int(N), model=int(model), fr_n=int(frn), branch=int(br), box=box, om=om, oml=oml, a=a,
+                  fr0=fr0, h0rc=h0rc)
+    gc = M.Config(N=int(N), model=int(model), fr_n=int(frn), branch=int(br), box=box, omega_m=om, omega_l=oml,
+                  aexpn=a, fr0=fr0, h0rc=h0rc, precision=precision)
+    return op, gc
+
+
+def max_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.maximum(np.abs(b), 1e-300)
+    return float(np.max(np.abs(a - b) / scale))
+
+
+def exact(a, b, what):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nd = int(np.count_nonzero(a != b))
+    assert nd == 0, f"{what}: {nd} cells differ, max rel {max_rel(a, b):.3e}"
+
+
+def run_kernel_chain(name, model):
+    fx = np.load(os.path.join(GOLDEN, name + ".npz"))
+    op, gc = cfgs(fx["params"])
+    print(f"[kernel parity] {name} N={op.N} model={model}", flush=True)
+    delta = fx["delta"].astype(np.float64)
+    u = fx["u_in"].astype(np.float64)
+    S = M.Solver(gc)
+    S.set_density(delta)
+    S.set_field(u)
+    smooth = O.fr_smooth if model == 1 else (lambda p, l, r, uu, rho, f, prec: O.dgp_smooth(p, l, r, uu, rho, f, False, prec))
+    residual = O.fr_residual if model == 1 else O.dgp_residual
+    rr1 = O.fr_restrict_residual1 if model == 1 else O.dgp_restrict_residual1
+    rhs = O.fr_rhs if model == 1 else O.dgp_rhs
+    worst = 0.0
+    # level-0 smoother, both colours
+    for red in (1, 0):
+        S.smooth(0, red)
+        smooth(op, 0, red, u, delta, None, "f64")
+        got = S.get_field()
+        if model == 1:
+            worst = max(worst, max_rel(got, u))
+            assert np.allclose(got, u, rtol=RT_FR_SMOOTH[op.fr_n], atol=AT_FR_SMOOTH * np.max(np.abs(u))), f"smoother red={red}"
+        else:
+            exact(got, u, f"DGP smoother red={red}")
+        S.set_field(u)
+    # level-0 residual (stored) and its RMS
+    res_o = np.zeros_like(u)
+    rms_o = residual(op, 0, u, delta, None, res_o, "f64")
+    rms_g = S.residual(0)
+    exact(S.get_level(0, M.ARR_RES), res_o, "residual l0")
+    assert abs(rms_g - rms_o) <= 1e-13 * rms_o
+    # fused restriction of the level-0 residual
+    S.restrict_residual(1)
+    rres1 = rr1(op, u, delta, "f64")
+    exact(S.get_level(1, M.ARR_RRES), rres1, "restrict_residual l1")
+    S.restrict_field(1)
+    ru1 = O.restrict8(u, "f64")
+    exact(S.get_level(1, M.ARR_RU), ru1, "restrict_field ru")
+    exact(S.get_level(1, M.ARR_U), ru1, "restrict_field u")
+    S.restrict_density(1)
+    rho1 = O.restrict8(delta, "f64")
+    exact(S.get_level(1, M.ARR_RHO), rho1, "restrict_density")
+    S.rhs(1)
+    f1 = rhs(op, 1, ru1, rho1, rres1, "f64")
+    exact(S.get_level(1, M.ARR_F), f1, "rhs l1")
+    # level-1 smoother and residual
+    u1 = ru1.copy()
+    for red in (1, 0):
+        S.smooth(1, red)
+        smooth(op, 1, red, u1, rho1, f1, "f64")
+        got = S.get_level(1, M.ARR_U)
+        if model == 1:
+            worst = max(worst, max_rel(got, u1))
+            assert np.allclose(got, u1, rtol=RT_FR_SMOOTH[op.fr_n], atol=AT_FR_SMOOTH * np.max(np.abs(u1)))
+        else:
+            exact(got, u1, f"DGP smoother l1 red={red}")
+        S.set_level(1, M.ARR_U, u1)
+    res1 = np.zeros_like(u1)
+    residual(op, 1, u1, rho1, f1, res1, "f64")
+    S.residual(1)
+    exact(S.get_level(1, M.ARR_RES), res1, "residual l1")
+    _, _, lc = S.levels()
+    if lc >= 2:
+        S.restrict_residual(2)
+        exact(S.get_level(2, M.ARR_RRES), O.restrict8(res1, "f64"), "restrict_residual l2")
+    # prolongation + correction with a perturbed coarse solution
+    uH = fx["u1_corr_in"].astype(np.float64)
+    S.set_level(1, M.ARR_U, uH)
+    S.correct(1)
+    O.correct(uH, ru1, u, model == 1, "f64")
+    exact(S.get_field(), u, "correct_solution")
+    S.close()
+    return worst
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", FR_FIX)
+def test_fr_operators_fp64(name):
+    worst = run_kernel_chain(name, 1)
+    print(f"  f(R) smoother worst rel diff {worst:.3e}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", DGP_FIX)
+def test_dgp_operators_fp64(name):
+    run_kernel_chain(name, 2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [1, 2])
+def test_vcycles_fp64_vs_oracle(model):
+    """Six V-cycles from the same start: residual history and field."""
+    fx = np.load(os.path.join(GOLDEN, f"vcycles_{'fr' if model == 1 else 'dgp'}.npz"))
+    op, gc = cfgs(fx["params"])
+    print(f"[vcycles] model={model} N={op.N}", flush=True)
+    delta = fx["delta"].astype(np.float64)
+    u0 = fx["u_in"].astype(np.float64)
+    u = u0.copy()
+    hist_o = O.vcycles(op, u, delta, 6, dgp_lexi=False, prec="f64")
+    S = M.Solver(gc)
+    S.set_density(delta)
+    S.set_field(u0)
+    hist_g = []
+    for _ in range(6):
+        S.vcycle(0)
+        hist_g.append(S.residual(0))
+    ug = S.get_field()
+    S.close()
+    hist_g = np.array(hist_g)
+    print(f"  history rel diff {np.max(np.abs(hist_g - hist_o) / hist_o):.3e}, field rel diff {max_rel(ug, u):.3e}")
+    if model == 1:
+        assert np.allclose(hist_g, hist_o, rtol=1e-9, atol=1e-12 * hist_o[0]), (hist_g, hist_o)
+    else:
+        assert np.allclose(hist_g, hist_o, rtol=1e-13, atol=0), (hist_g, hist_o)
+    if model == 1:
+        assert max_rel(ug, u) <= 1e-9
+    else:
+        exact(ug, u, "DGP field after 6 V-cycles")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("frn", [0, 1, 2])
+def test_fp64_solve_matches_oracle(frn):
+    """scalar_field to relative residual 1e-8 on a seeded lognormal field."""
+    N = 32
+    op = O.Params(N=N, model=1, fr_n=frn, box=64.0, om=0.3, oml=0.7, a=1.0, fr0=-1e-5)
+    delta = O.lognormal_density(N, op.box, 1.0, seed=1001)
+    u_o, rep_o = O.solve(op, delta, mode=1, rel_tol=1e-8, max_cycles=60, prec="f64")
+    gc = M.Config(N=N, model=1, fr_n=frn, box=64.0, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=-1e-5)
+    print(f"[solve fp64] n={frn} oracle cycles={rep_o['cycles']}", flush=True)
+    with M.Solver(gc) as S:
+        S.set_density(delta)
+        rep = S.solve(mode=1, rel_tol=1e-8, max_cycles=60)
+        ug = S.get_field()
+    assert rep["converged"] and rep_o["converged"]
+    assert rep["cycles"] == rep_o["cycles"]
+    # residuals far below the operator's terms differ by the terms' rounding:
+    # compare on the scale of the initial residual
+    assert np.allclose(rep["history"], rep_o["history"][: rep["cycles"]], rtol=1e-6, atol=1e-12 * rep["res0"])
+    assert max_rel(ug, u_o) <= 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("frn,fr0,sigma", [(1, -1e-5, 1.0), (1, -1e-6, 1.5), (0, -1e-5, 1.0), (2, -1e-5, 1.2)])
+def test_mixed_solve_reaches_fp64_oracle(frn, fr0, sigma):
+    """FP32 delta-form FAS inside the FP64 defect-correction loop reaches the
+    FP64 oracle's relative residual 1e-8; its field agrees with the FP64
+    oracle's converged field to the accuracy both solves carry."""
+    N = 32
+    box = 64.0
+    op = O.Params(N=N, model=1, fr_n=frn, box=box, om=0.3, oml=0.7, a=1.0, fr0=fr0)
+    delta = O.lognormal_density(N, box, sigma, seed=1002)
+    u_o, rep_o = O.solve(op, delta, mode=1, rel_tol=1e-10, max_cycles=80, prec="f64")
+    gc = M.Config(N=N, model=1, fr_n=frn, box=box, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=fr0,
+                  precision=M.PREC_MIXED)
+    print(f"[solve mixed] n={frn} fR0={fr0} sigma={sigma}", flush=True)
+    with M.Solver(gc) as S:
+        S.set_density(delta)
+        rep = S.solve(mode=1, rel_tol=1e-8, max_cycles=60)
+        ug = S.get_field()
+    assert rep["converged"], rep
+    assert rep["res_final"] < 1e-8 * rep["res0"]
+    # the oracle's FP64 residual of the mixed result
+    r_o = O.fr_residual(op, 0, ug.copy(), delta, None, None, "f64")
+    assert r_o < 1.05e-8 * rep["res0"]
+    err = np.max(np.abs(ug - u_o)) / np.max(np.abs(u_o))
+    print(f"  mixed: outer={rep['outer']} inner={rep['cycles']} rel L_inf vs FP64 oracle {err:.2e}")
+    assert err < 1e-6
