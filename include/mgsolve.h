@@ -87,8 +87,16 @@ typedef struct mgs_params {
     double fr0;              /* fR0 (< 0 for Hu-Sawicki) */
     double h0rc;             /* H0rc */
     int32_t precision;       /* MGS_PREC_FP64 / MGS_PREC_MIXED */
-    int32_t reserved;
+    int32_t flags;           /* MGS_FLAG_* */
 } mgs_params;
+
+/* f(R) FP64 smoother: evaluate the reference's closed-form root literally
+ * (Cardano / trigonometric / Ferrari forms, PMP2MGsolver_fR.f90:105-256).
+ * Default (flag clear): the same root computed from a cancellation-free FP32
+ * closed-form seed and FP64 Newton steps on u^{n+2} + P u + Q = 0, which
+ * keeps full relative precision in dense cells where the literal forms
+ * cancel (there the literal forms floor the solve's relative residual). */
+#define MGS_FLAG_REFERENCE_ROOTS 1
 
 typedef struct mgs_solve_opts {
     int32_t mode;            /* 0: the reference's exits (absolute RMS <= res_conv, 0.5 % stagnation,
@@ -183,6 +191,9 @@ int mgs_profile_get(mgs_handle* h, int32_t kclass, int64_t* launches, double* ms
 
 /* ---- device plumbing -------------------------------------------------- */
 int mgs_synchronize(mgs_handle* h);
+/* launch the coarse levels of each V-cycle from a captured hipGraph (default
+ * 1) or as individual kernels (0); results are identical either way */
+int mgs_use_graphs(mgs_handle* h, int32_t enable);
 /* device pointer of the level-0 field in the internal colour-split layout
  * (for zero-copy consumers on the same device) */
 int mgs_device_field(mgs_handle* h, void** ptr, size_t* bytes);

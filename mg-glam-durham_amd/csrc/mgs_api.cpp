@@ -101,6 +101,8 @@ FrCoef fr_coefs(const mgs_params& p, int level)
     c.fct3_1 = ctilde * ctilde * sgn / 1.0;
     c.fct4_1 = c.fct2r * std::pow(std::fabs(fR_bg), 1.0 / (1.0 + (double)n));
     c.ubg = std::pow(std::fabs(fR_bg), 1.0 / (1.0 + (double)n));
+    c.inv_w = 1.0 / powi_d(c.ubg, n + 1);
+    c.reference_roots = (p.flags & MGS_FLAG_REFERENCE_ROOTS) ? 1 : 0;
     return c;
 }
 
@@ -367,6 +369,41 @@ static int cache_densities(mgs_handle* h)
     return 0;
 }
 
+// Run `body` (launches only, no host synchronisation) from a hipGraph: captured
+// on the first call, replayed afterwards.  The coarse levels of a V-cycle are
+// ~10 small launches per level; one graph launch replaces them.
+template <typename F>
+static int run_captured(mgs_handle* h, hipGraphExec_t* exec, F body)
+{
+    if (!h->use_graphs) return body();
+    if (!*exec) {
+        const bool prof = h->prof;
+        h->prof = false;
+        hipGraph_t g = nullptr;
+        HIPCHK(h, hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
+        const int rc = body();
+        hipError_t e = hipStreamEndCapture(h->s, &g);
+        h->prof = prof;
+        if (rc) return rc;
+        if (e != hipSuccess) return set_err(h, MGS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return set_err(h, MGS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    prof_begin(h, MGS_KC_COARSE, 0.0);
+    HIPCHK(h, hipGraphLaunch(*exec, h->s));
+    prof_end(h);
+    return 0;
+}
+
+void drop_graphs(mgs_handle* h)
+{
+    if (h->g_plain) (void)hipGraphExecDestroy(h->g_plain);
+    if (h->g_inner) (void)hipGraphExecDestroy(h->g_inner);
+    h->g_plain = nullptr;
+    h->g_inner = nullptr;
+}
+
 // V_cycle(ilevel), PMP2extradof.f90:352-555 (Nsmth = 2, :361)
 int vcycle_plain(mgs_handle* h, int l)
 {
@@ -385,8 +422,16 @@ int vcycle_plain(mgs_handle* h, int l)
     if (l > 0) op_residual(h, l, true, nullptr);
     op_restrict_residual(h, l + 1);
     op_restrict_field(h, l + 1);
-    op_rhs(h, l + 1);
-    vcycle_plain(h, l + 1);
+    if (l == 0) {
+        int rc = run_captured(h, &h->g_plain, [h]() {
+            op_rhs(h, 1);
+            return vcycle_plain(h, 1);
+        });
+        if (rc) return rc;
+    } else {
+        op_rhs(h, l + 1);
+        vcycle_plain(h, l + 1);
+    }
     op_correct(h, l + 1);
     for (int s = 0; s < Nsmth; s++) {
         op_smooth(h, l, 1);
@@ -431,10 +476,18 @@ static int vcycle_inner(mgs_handle* h, int l)
     prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(v.n) + 2.0 * n3d(c.n)) * s);
     launch_restrict8_32(v.d, c.rd, c.d, c.n, h->s);
     prof_end(h);
-    prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(c.n) * s);
-    launch_df_rhs(frn, c.rd, c.xb, c.rres, c.f, c.n, df_coefs(h->p, l + 1), h->s);
-    prof_end(h);
-    vcycle_inner(h, l + 1);
+    auto coarse = [h, l, frn, &c]() {
+        prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(c.n) * 4.0);
+        launch_df_rhs(frn, c.rd, c.xb, c.rres, c.f, c.n, df_coefs(h->p, l + 1), h->s);
+        prof_end(h);
+        return vcycle_inner(h, l + 1);
+    };
+    if (l == 0) {
+        int rc = run_captured(h, &h->g_inner, coarse);
+        if (rc) return rc;
+    } else {
+        coarse();
+    }
     prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * s);
     launch_df_correct(frn, c.d, c.rd, v.d, v.xb, c.n, h->s);
     prof_end(h);
@@ -657,13 +710,14 @@ int mgs_create(const mgs_params* params, mgs_handle** out)
 int mgs_destroy(mgs_handle* h)
 {
     if (!h) return 0;
-    if (h->s) hipStreamSynchronize(h->s);
+    if (h->s) (void)hipStreamSynchronize(h->s);
     prof_flush(h);
-    for (void* p : h->allocs) hipFree(p);
-    if (h->staging) hipFree(h->staging);
-    if (h->hsum) hipHostFree(h->hsum);
+    drop_graphs(h);
+    for (void* p : h->allocs) (void)hipFree(p);
+    if (h->staging) (void)hipFree(h->staging);
+    if (h->hsum) (void)hipHostFree(h->hsum);
     fft_release(h);
-    if (h->s) hipStreamDestroy(h->s);
+    if (h->s) (void)hipStreamDestroy(h->s);
     delete h;
     return 0;
 }
@@ -691,6 +745,9 @@ int mgs_set_params(mgs_handle* h, const mgs_params* params)
     if (params->ngrid != h->p.ngrid || params->model != h->p.model || params->precision != h->p.precision)
         return set_err(h, MGS_E_ARG, "mgs_set_params cannot change ngrid, model or precision");
     h->p = *params;
+    // the captured coarse cycles hold the old coefficients
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    drop_graphs(h);
     return 0;
 }
 
@@ -994,6 +1051,15 @@ int mgs_profile_get(mgs_handle* h, int32_t kclass, int64_t* launches, double* ms
     if (launches) *launches = h->pc[kclass].launches;
     if (ms) *ms = h->pc[kclass].ms;
     if (bytes) *bytes = h->pc[kclass].bytes;
+    return 0;
+}
+
+int mgs_use_graphs(mgs_handle* h, int32_t enable)
+{
+    if (!h) return MGS_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    drop_graphs(h);
+    h->use_graphs = enable != 0;
     return 0;
 }
 

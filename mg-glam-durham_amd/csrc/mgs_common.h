@@ -61,6 +61,10 @@ struct FrCoef {
     // fused level-1 restriction (dx = 1)
     double fct3_1, fct4_1;
     double ubg;
+    // normalisation of the local polynomial for the FP32 seed:
+    // x = u / ubg solves x^{n+2} + (P inv_w) x + Q2 inv_w = 0, inv_w = ubg^-(n+1)
+    double inv_w;
+    int reference_roots;  // 1: evaluate the reference's closed form literally
 };
 
 // Delta-form (mixed precision) constants of one level:

@@ -45,15 +45,22 @@ __device__ inline bool map_block(unsigned NB, unsigned& lb)
 }
 
 // logical (x index, row index) of this thread
-__device__ inline bool map_xy(int width, long rows, int TX, int TY, int XB, unsigned NB, int& x, long& row)
+// (rows < 2^31 and 32-bit index math throughout: n^3 <= 2^30 for n <= 1024)
+__device__ inline bool map_xy(int width, long rows, int TX, int TY, int XB, unsigned NB, int& x, unsigned& row)
 {
     unsigned lb;
     if (!map_block(NB, lb)) return false;
-    const unsigned xb = lb % (unsigned)XB;
-    const unsigned rb = lb / (unsigned)XB;
+    unsigned xb, rb;
+    if (XB == 1) {
+        xb = 0;
+        rb = lb;
+    } else {
+        xb = lb % (unsigned)XB;
+        rb = lb / (unsigned)XB;
+    }
     x = (int)(xb * TX + threadIdx.x);
-    row = (long)rb * TY + threadIdx.y;
-    return x < width && row < rows;
+    row = rb * (unsigned)TY + threadIdx.y;
+    return x < width && row < (unsigned)rows;
 }
 
 // write one block partial; idle blocks of the rounded-up grid write nothing
@@ -77,9 +84,10 @@ __device__ __forceinline__ T wpow(T x)
         return x * x * x;
 }
 
-// the six colour-split neighbours of a colour-c cell at (m,j,k)
+// the six colour-split neighbours of a colour-c cell at (m,j,k), as 32-bit
+// offsets (a level holds at most 2^30 cells)
 struct Nb {
-    size_t xp, xm, yp, ym, zp, zm, me;
+    unsigned xp, xm, yp, ym, zp, zm, me;
 };
 
 __device__ __forceinline__ Nb neighbours(const Geo& g, int c, int m, int j, int k)
@@ -90,15 +98,16 @@ __device__ __forceinline__ Nb neighbours(const Geo& g, int c, int m, int j, int 
     if (mxm < 0) mxm += g.h;
     int mxp = m + s;
     if (mxp >= g.h) mxp -= g.h;
-    const size_t o = (size_t)(1 - c) * g.half;
-    const size_t r0 = row_base(g, j, k);
+    const unsigned n = (unsigned)g.n, h = (unsigned)g.h;
+    const unsigned o = (unsigned)(1 - c) * (unsigned)g.half;
+    const unsigned r0 = ((unsigned)k * n + (unsigned)j) * h;
     r.xp = o + r0 + mxp;
     r.xm = o + r0 + mxm;
-    r.yp = o + row_base(g, wrap_inc(j, g.n), k) + m;
-    r.ym = o + row_base(g, wrap_dec(j, g.n), k) + m;
-    r.zp = o + row_base(g, j, wrap_inc(k, g.n)) + m;
-    r.zm = o + row_base(g, j, wrap_dec(k, g.n)) + m;
-    r.me = (size_t)c * g.half + r0 + m;
+    r.yp = o + ((unsigned)k * n + (unsigned)wrap_inc(j, g.n)) * h + m;
+    r.ym = o + ((unsigned)k * n + (unsigned)wrap_dec(j, g.n)) * h + m;
+    r.zp = o + ((unsigned)wrap_inc(k, g.n) * n + (unsigned)j) * h + m;
+    r.zm = o + ((unsigned)wrap_dec(k, g.n) * n + (unsigned)j) * h + m;
+    r.me = (unsigned)c * (unsigned)g.half + r0 + m;
     return r;
 }
 

@@ -94,9 +94,9 @@ __global__ __launch_bounds__(256) void k_dgp_smooth64(const double* __restrict__
                                                       int colour, DgpCoef c, int TX, int TY, int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
     const int i = 2 * m + ((colour + j + k) & 1);
     const size_t me = (size_t)colour * g.half + row_base(g, j, k) + m;
     double c0;
@@ -113,13 +113,13 @@ __global__ __launch_bounds__(256) void k_dgp_residual64(const double* __restrict
                                                         int TY, int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     double acc = 0.0;
     if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
-        const long nn = (long)g.n * g.n;
-        const int col = (int)(row / nn);
-        const long r2 = row - col * nn;
-        const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+        const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+        const int col = row >= nn ? 1 : 0;
+        const unsigned r2 = row - (unsigned)col * nn;
+        const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
         const int i = 2 * m + ((col + j + k) & 1);
         const size_t me = (size_t)col * g.half + row_base(g, j, k) + m;
         double c0;
@@ -138,9 +138,9 @@ __global__ __launch_bounds__(256) void k_dgp_restrict_residual1_64(const double*
                                                                    DgpCoef c, int TX, int TY, int XB, unsigned NB)
 {
     int I;
-    long row;
+    unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
     const int N = gf.n;
     const double ONEOVEREIGHT = 1.0 / 8.0, ONEOVERTHREE = 1.0 / 3.0, EIGHTOVERTHREE = 8.0 / 3.0;
     const double dx2 = 1.0, dx4 = 1.0;
@@ -199,12 +199,12 @@ __global__ __launch_bounds__(256) void k_dgp_rhs64(const double* __restrict__ ru
                                                    DgpCoef c, int TX, int TY, int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const long nn = (long)g.n * g.n;
-    const int col = (int)(row / nn);
-    const long r2 = row - col * nn;
-    const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+    const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+    const int col = row >= nn ? 1 : 0;
+    const unsigned r2 = row - (unsigned)col * nn;
+    const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
     const int i = 2 * m + ((col + j + k) & 1);
     const size_t me = (size_t)col * g.half + row_base(g, j, k) + m;
     double c0;

@@ -67,20 +67,101 @@ __device__ __forceinline__ double fr_root64(double P, const FrCoef& c)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Seed of the positive root of x^{n+2} + P x + Q = 0 (Q < 0, O(1) normalised
+// coefficients) in FP32, written without the cancellations of the literal
+// closed forms:
+//   n=0: the quadratic root, as -2Q/(P + sqrt(P^2-4Q)) when P > 0;
+//   n=1: Cardano with t1 = 54 P^3 / (sqrt(CC) + |27Q|) (= t1 of the literal
+//        form) and, when P > 0, cbrt(t2) - cbrt(t1) = |27Q| / (a^2 + ab + b^2);
+//        the trigonometric form when CC < 0;
+//   n=2: Ferrari for P <= 0; for P > 0 the bound min(|Q|/P, |Q|^{1/4}), which
+//        lies within a factor 2 above the root.
+// Hardware transcendentals: the seed is refined by Newton's method.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fast_cbrt(float t)
+{
+    if (t == 0.0f) return 0.0f;
+    const float r = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(fabsf(t)) * (1.0f / 3.0f));
+    return copysignf(r, t);
+}
+
+template <int FRN>
+__device__ __forceinline__ float seed_root(float P, float Q)
+{
+    if constexpr (FRN == 0) {
+        const float sq = __builtin_sqrtf(fmaxf(P * P - 4.0f * Q, 0.0f));
+        return P > 0.0f ? (-2.0f * Q) / (P + sq) : 0.5f * (sq - P);
+    } else if constexpr (FRN == 1) {
+        const float aD1 = -27.0f * Q;
+        const float CC = aD1 * aD1 + 108.0f * P * P * P;
+        if (CC >= 0.0f) {
+            const float sq = __builtin_sqrtf(CC);
+            const float t2 = 0.5f * (sq + aD1);
+            const float t1 = 54.0f * P * P * P / (sq + aD1);
+            const float a = fast_cbrt(t2), b = fast_cbrt(t1);
+            if (t1 > 0.0f) return aD1 / (3.0f * (a * a + a * b + b * b));
+            return (a - b) * (1.0f / 3.0f);
+        }
+        const float sd = __builtin_sqrtf(-3.0f * P);
+        const float arg = fminf(fmaxf(-0.5f * aD1 / (sd * sd * sd), -1.0f), 1.0f);
+        return -(2.0f / 3.0f) * sd * __cosf(acosf(arg) * (1.0f / 3.0f) + 2.0943951023931955f);
+    } else {
+        if (P > 0.0f) return fminf(-Q / P, __builtin_sqrtf(__builtin_sqrtf(-Q)));
+        if (P == 0.0f) return __builtin_sqrtf(__builtin_sqrtf(-Q));
+        const float D20 = 12.0f * Q;
+        const float D21 = 27.0f * P * P;
+        const float t1 = 0.5f * (__builtin_sqrtf(fmaxf(D21 * D21 - 4.0f * D20 * D20 * D20, 0.0f)) + D21);
+        const float cb = fast_cbrt(t1);
+        const float S = __builtin_sqrtf(fmaxf(cb + D20 / cb, 0.0f)) * 0.28867513459481287f;
+        return S + 0.5f * __builtin_sqrtf(fmaxf(-4.0f * S * S - P / S, 0.0f));
+    }
+}
+
+// 1/x in FP64 from the hardware estimate and one Newton-Raphson correction
+// (relative error ~1e-15: enough for a Newton step's length)
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    const double e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// Newton's method in FP64 on u^{n+2} + P u + Q = 0 from u, to |step| <= 1e-15 u
+template <int FRN>
+__device__ __forceinline__ double newton_root64(double u, double P, double Q)
+{
+#pragma unroll 1
+    for (int it = 0; it < 8; it++) {
+        const double un1 = wpow<FRN>(u);
+        const double f = un1 * u + P * u + Q;
+        const double fp = (double)(FRN + 2) * un1 + P;
+        if (!(fp > 0.0)) break;
+        const double d = f * rcp64(fp);
+        const double u1 = u - d;
+        if (!(u1 > 0.0)) break;
+        u = u1;
+        if (fabs(d) <= 1.0e-15 * u) break;
+    }
+    return u;
+}
+
 // ===========================================================================
 // Plain FP64 path
 // ===========================================================================
 
-// relaxation_iterations_fR, one colour, PMP2MGsolver_fR.f90:20-301
-template <int FRN, bool COARSE>
+// relaxation_iterations_fR, one colour, PMP2MGsolver_fR.f90:20-301.
+// REFROOT: the reference's closed form evaluated literally; otherwise the
+// same root from the FP32 seed refined by FP64 Newton steps.
+template <int FRN, bool COARSE, bool REFROOT>
 __global__ __launch_bounds__(256) void k_fr_smooth64(double* __restrict__ u, const double* __restrict__ rho,
                                                      const double* __restrict__ f, Geo g, int colour, FrCoef c, int TX,
                                                      int TY, int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
     const Nb q = neighbours(g, colour, m, j, k);
     const double S = wpow<FRN>(u[q.xp]) + wpow<FRN>(u[q.xm]) + wpow<FRN>(u[q.yp]) + wpow<FRN>(u[q.ym]) +
                      wpow<FRN>(u[q.zp]) + wpow<FRN>(u[q.zm]);
@@ -89,7 +170,12 @@ __global__ __launch_bounds__(256) void k_fr_smooth64(double* __restrict__ u, con
         P = c.fct1s * rho[q.me] - c.Q2 - S / 6.0;
     else
         P = c.fct1s * rho[q.me] - c.Q2 + c.fct2s * f[q.me] - S / 6.0;
-    u[q.me] = fr_root64<FRN>(P, c);
+    if constexpr (REFROOT) {
+        u[q.me] = fr_root64<FRN>(P, c);
+    } else {
+        const float x = seed_root<FRN>((float)(P * c.inv_w), (float)(c.Q2 * c.inv_w));
+        u[q.me] = newton_root64<FRN>((double)x * c.ubg, P, c.Q);
+    }
 }
 
 // residual of one cell, calculate_residual_fR, PMP2MGsolver_fR.f90:434-458
@@ -113,13 +199,13 @@ __global__ __launch_bounds__(256) void k_fr_residual64(const double* __restrict_
                                                        int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     double acc = 0.0;
     if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
-        const long nn = (long)g.n * g.n;
-        const int col = (int)(row / nn);
-        const long r2 = row - col * nn;
-        const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+        const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+        const int col = row >= nn ? 1 : 0;
+        const unsigned r2 = row - (unsigned)col * nn;
+        const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
         const Nb q = neighbours(g, col, m, j, k);
         double OP = fr_res_cell<FRN>(u, q, rho[q.me], c);
         if constexpr (COARSE) OP = OP - f[q.me];
@@ -138,9 +224,9 @@ __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* 
                                                                   FrCoef c, int TX, int TY, int XB, unsigned NB)
 {
     int I;
-    long row;
+    unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
     const int N = gf.n;
     const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1, z0 = 2 * K, z1 = 2 * K + 1;
     const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
@@ -175,9 +261,9 @@ __global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T
                                                    unsigned NB)
 {
     int I;
-    long row;
+    unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
     // children in the reference's order (x fastest), accumulated in double
     // for FP64 data and in float for FP32 data
     using A = T;
@@ -202,12 +288,12 @@ __global__ __launch_bounds__(256) void k_fr_rhs64(const double* __restrict__ ru,
                                                   FrCoef c, int TX, int TY, int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const long nn = (long)g.n * g.n;
-    const int col = (int)(row / nn);
-    const long r2 = row - col * nn;
-    const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+    const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+    const int col = row >= nn ? 1 : 0;
+    const unsigned r2 = row - (unsigned)col * nn;
+    const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
     const Nb q = neighbours(g, col, m, j, k);
     double OP = fr_res_cell<FRN>(ru, q, rho[q.me], c);
     OP = OP - rres[q.me];
@@ -222,9 +308,9 @@ __global__ __launch_bounds__(256) void k_correct(const T* __restrict__ uH, const
                                                  int XB, unsigned NB)
 {
     int I;
-    long row;
+    unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
     const int nc = gc.n;
     const A a = (A)1.0 / (A)64.0, b = (A)3.0 * a, c9 = (A)9.0 * a, d = (A)27.0 * a;
     const A wt[8] = {a, b, b, c9, b, c9, c9, d};
@@ -274,9 +360,9 @@ __global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ lex, TO* __
                                               int TY, int XB, unsigned NB)
 {
     int i;
-    long row;
+    unsigned row;
     if (!map_xy(g.n, (long)g.n * g.n, TX, TY, XB, NB, i, row)) return;
-    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
     split[split_index(g, i, j, k)] = (TO)lex[(size_t)row * g.n + i];
 }
 
@@ -285,9 +371,9 @@ __global__ __launch_bounds__(256) void k_unpack(const double* __restrict__ split
                                                 double scale, int TX, int TY, int XB, unsigned NB)
 {
     int i;
-    long row;
+    unsigned row;
     if (!map_xy(g.n, (long)g.n * g.n, TX, TY, XB, NB, i, row)) return;
-    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
     const double v = split[split_index(g, i, j, k)];
     lex[(size_t)row * g.n + i] = (TO)(scale == 1.0 ? v : v * scale);
 }
@@ -309,37 +395,6 @@ __global__ __launch_bounds__(256) void k_copy(const T* __restrict__ s, T* __rest
 // ===========================================================================
 // Mixed precision: FP32 delta-form FAS
 // ===========================================================================
-
-// closed-form positive root of x^{n+2} + P x + Q = 0 in FP32 (normalised
-// variables; same branch structure as the reference's smoother)
-template <int FRN>
-__device__ __forceinline__ float df_root32(float P, float Q)
-{
-    if constexpr (FRN == 0) {
-        return 0.5f * (-P + sqrtf(fmaxf(P * P - 4.0f * Q, 0.0f)));
-    } else if constexpr (FRN == 1) {
-        const float D0 = -3.0f * P;
-        const float D1 = 27.0f * Q;
-        const float CC = D1 * D1 - 4.0f * D0 * D0 * D0;
-        if (CC >= 0.0f) {
-            const float t1 = 0.5f * (sqrtf(CC) + D1);
-            const float t2 = t1 - D1;
-            return -(cbrtf(t1) - cbrtf(t2)) * (1.0f / 3.0f);
-        }
-        const float sd = sqrtf(D0);
-        const float arg = fminf(fmaxf(0.5f * D1 / (sd * sd * sd), -1.0f), 1.0f);
-        return -(2.0f / 3.0f) * sd * cosf(acosf(arg) * (1.0f / 3.0f) + 2.0943951023931955f);
-    } else {
-        const float D20 = 12.0f * Q;
-        const float D21 = 27.0f * P * P;
-        const float t1 = 0.5f * (sqrtf(fmaxf(D21 * D21 - 4.0f * D20 * D20 * D20, 0.0f)) + D21);
-        const float cb = cbrtf(t1);
-        const float S = sqrtf(cb + D20 / cb) * 0.28867513459481287f;  // 1/sqrt(12)
-        if (P > 0.0f) return -S + 0.5f * sqrtf(fmaxf(-4.0f * S * S + P / S, 0.0f));
-        if (P < 0.0f) return S + 0.5f * sqrtf(fmaxf(-4.0f * S * S - P / S, 0.0f));
-        return sqrtf(sqrtf(-Q));
-    }
-}
 
 // x from w = x^{n+1}
 template <int FRN, typename T>
@@ -377,16 +432,24 @@ template <int FRN>
 __device__ __forceinline__ float df_solve(float S, float xb, float r, const DfCoef& c)
 {
     const float wb = wpow<FRN>(xb);
-    const float K = c.A * S + 6.0f * c.A * wb + c.C / xb - r;
-    float x = df_root32<FRN>(-K * c.inv6A, c.C * c.inv6A);
+    const float K = c.A * S + 6.0f * c.A * wb + c.C * __builtin_amdgcn_rcpf(xb) - r;
+    float x = seed_root<FRN>(-K * c.inv6A, c.C * c.inv6A);
     if (!(x > 0.0f) || !isfinite(x)) x = xb;
     float d = wpow<FRN>(x) - wb;
+    // One Newton step in d from the cancellation-free seed leaves an error
+    // ~ (F''/F') (seed error)^2 ~ 1e-13 wb; the n=2 seed for P > 0 is only a
+    // bound within a factor 2 of the root and takes four.
+    constexpr int NIT = FRN == 2 ? 4 : 1;
+    const float rxb = __builtin_amdgcn_rcpf(xb);
 #pragma unroll
-    for (int it = 0; it < 2; it++) {
+    for (int it = 0; it < NIT; it++) {
         if (it > 0) x = xroot<FRN>(fmaxf(wb + d, 1.0e-30f));
-        const float F = c.A * (S - 6.0f * d) + dlocal<FRN>(d, x, xb, c.C) - r;
-        const float Fp = -6.0f * c.A + c.C / ((float)(FRN + 1) * x * wpow<FRN>(x));
-        d = d - F / Fp;
+        const float rx = __builtin_amdgcn_rcpf(x);
+        // C (1/xb - 1/x) = C d / (dden(x, xb) x xb)
+        const float loc = c.C * d * __builtin_amdgcn_rcpf(dden<FRN>(x, xb)) * rx * rxb;
+        const float F = c.A * (S - 6.0f * d) + loc - r;
+        const float Fp = -6.0f * c.A + c.C * (1.0f / (float)(FRN + 1)) * rx * __builtin_amdgcn_rcpf(wpow<FRN>(x));
+        d = d - F * __builtin_amdgcn_rcpf(Fp);
     }
     // stay inside the domain x > 0
     if (!(wb + d > 0.0f)) d = -0.5f * wb;
@@ -399,9 +462,9 @@ __global__ __launch_bounds__(256) void k_df_smooth(float* __restrict__ d, const 
                                                    int TY, int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const int j = (int)(row % g.n), k = (int)(row / g.n);
+    const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
     const Nb q = neighbours(g, colour, m, j, k);
     const float S = d[q.xp] + d[q.xm] + d[q.yp] + d[q.ym] + d[q.zp] + d[q.zm];
     d[q.me] = df_solve<FRN>(S, xb[q.me], rhs[q.me], c);
@@ -424,12 +487,12 @@ __global__ __launch_bounds__(256) void k_df_residual(const float* __restrict__ d
                                                      DfCoef c, int TX, int TY, int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const long nn = (long)g.n * g.n;
-    const int col = (int)(row / nn);
-    const long r2 = row - col * nn;
-    const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+    const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+    const int col = row >= nn ? 1 : 0;
+    const unsigned r2 = row - (unsigned)col * nn;
+    const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
     const Nb q = neighbours(g, col, m, j, k);
     res[q.me] = df_res_cell<FRN>(d, q, xb[q.me], rhs[q.me], c);
 }
@@ -441,12 +504,12 @@ __global__ __launch_bounds__(256) void k_df_rhs(const float* __restrict__ rd, co
                                                 DfCoef c, int TX, int TY, int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const long nn = (long)g.n * g.n;
-    const int col = (int)(row / nn);
-    const long r2 = row - col * nn;
-    const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+    const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+    const int col = row >= nn ? 1 : 0;
+    const unsigned r2 = row - (unsigned)col * nn;
+    const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
     const Nb q = neighbours(g, col, m, j, k);
     f[q.me] = df_res_cell<FRN>(rd, q, xb[q.me], 0.0f, c) - rres[q.me];
 }
@@ -461,9 +524,9 @@ __global__ __launch_bounds__(256) void k_df_restrict_residual1(const float* __re
                                                                unsigned NB)
 {
     int I;
-    long row;
+    unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
     const int N = gf.n;
     const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1, z0 = 2 * K, z1 = 2 * K + 1;
     const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
@@ -495,9 +558,9 @@ __global__ __launch_bounds__(256) void k_restrict_base(const float* __restrict__
                                                        Geo gc, int TX, int TY, int XB, unsigned NB)
 {
     int I;
-    long row;
+    unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
     float w = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; q++)
@@ -513,9 +576,9 @@ __global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH
                                                     Geo gc, int TX, int TY, int XB, unsigned NB)
 {
     int I;
-    long row;
+    unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % gc.n), K = (int)(row / gc.n);
+    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
     const int nc = gc.n;
     const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
     const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
@@ -599,13 +662,13 @@ __global__ __launch_bounds__(256) void k_outer_residual(const double* __restrict
                                                         int XB, unsigned NB)
 {
     int m;
-    long row;
+    unsigned row;
     double acc = 0.0;
     if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
-        const long nn = (long)g.n * g.n;
-        const int col = (int)(row / nn);
-        const long r2 = row - col * nn;
-        const int j = (int)(r2 % g.n), k = (int)(r2 / g.n);
+        const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+        const int col = row >= nn ? 1 : 0;
+        const unsigned r2 = row - (unsigned)col * nn;
+        const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
         const Nb q = neighbours(g, col, m, j, k);
         const double OP = fr_res_cell<FRN>(u, q, delta[q.me], c);
         g32[q.me] = (float)(-OP);
@@ -701,16 +764,25 @@ void launch_fr_smooth64(int frn, int level, double* u, const double* rho, const 
 {
     const Geo g = make_geo(n);
     const Launch L = make_launch(g.h, (long)n * n);
-#define SM(FRN, CO) hipLaunchKernelGGL((k_fr_smooth64<FRN, CO>), L.grid, L.block, 0, s, u, rho, f, g, colour, c, LAUNCH_ARGS)
+#define SM(FRN, CO, RR) \
+    hipLaunchKernelGGL((k_fr_smooth64<FRN, CO, RR>), L.grid, L.block, 0, s, u, rho, f, g, colour, c, LAUNCH_ARGS)
+#define SM2(FRN, CO)            \
+    do {                        \
+        if (c.reference_roots)  \
+            SM(FRN, CO, true);  \
+        else                    \
+            SM(FRN, CO, false); \
+    } while (0)
     if (level == 0) {
-        if (frn == 0) SM(0, false);
-        else if (frn == 1) SM(1, false);
-        else SM(2, false);
+        if (frn == 0) SM2(0, false);
+        else if (frn == 1) SM2(1, false);
+        else SM2(2, false);
     } else {
-        if (frn == 0) SM(0, true);
-        else if (frn == 1) SM(1, true);
-        else SM(2, true);
+        if (frn == 0) SM2(0, true);
+        else if (frn == 1) SM2(1, true);
+        else SM2(2, true);
     }
+#undef SM2
 #undef SM
 }
 

@@ -74,6 +74,11 @@ struct mgs_handle {
     double* dgp_rms = nullptr;
     // FFT plans (density generator, power spectrum)
     void* fft = nullptr;
+    // captured coarse-level parts of the V-cycles (levels >= 1), replayed
+    // every cycle; rebuilt when the model scalars change
+    hipGraphExec_t g_plain = nullptr;
+    hipGraphExec_t g_inner = nullptr;
+    bool use_graphs = true;
     // profiling
     bool prof = false;
     std::vector<mgs::ProfRec> recs;

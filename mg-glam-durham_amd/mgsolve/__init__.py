@@ -30,6 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmgsolve.so")
 
 MODEL_FR, MODEL_DGP = 1, 2
+FLAG_REFERENCE_ROOTS = 1
 PREC_FP64, PREC_MIXED = 0, 1
 F32, F64 = 0, 1
 HOST, DEVICE = 0, 1
@@ -46,7 +47,7 @@ class Params(C.Structure):
     _fields_ = [
         ("ngrid", C.c_int32), ("model", C.c_int32), ("fr_n", C.c_int32), ("branch", C.c_int32),
         ("box", C.c_double), ("omega_m", C.c_double), ("omega_l", C.c_double), ("aexpn", C.c_double),
-        ("fr0", C.c_double), ("h0rc", C.c_double), ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("fr0", C.c_double), ("h0rc", C.c_double), ("precision", C.c_int32), ("flags", C.c_int32),
     ]
 
 
@@ -107,6 +108,7 @@ def load():
         "mgs_profile": ([H, i32], C.c_int),
         "mgs_profile_get": ([H, i32, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "mgs_synchronize": ([H], C.c_int),
+        "mgs_use_graphs": ([H, i32], C.c_int),
         "mgs_device_field": ([H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -134,10 +136,11 @@ class Config:
     fr0: float = -1.0e-5
     h0rc: float = 1.0
     precision: int = PREC_FP64
+    flags: int = 0
 
     def c(self):
         return Params(self.N, self.model, self.fr_n, self.branch, self.box, self.omega_m, self.omega_l,
-                      self.aexpn, self.fr0, self.h0rc, self.precision, 0)
+                      self.aexpn, self.fr0, self.h0rc, self.precision, self.flags)
 
 
 class Solver:
@@ -301,6 +304,9 @@ class Solver:
 
     def synchronize(self):
         self._chk(self.L.mgs_synchronize(self.h), "mgs_synchronize")
+
+    def use_graphs(self, enable=True):
+        self._chk(self.L.mgs_use_graphs(self.h, int(bool(enable))), "mgs_use_graphs")
 
 
 def exported_symbols_declared_in_header(header_path):

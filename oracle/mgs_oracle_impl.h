@@ -24,19 +24,26 @@ static inline STOR FN(wpow)(STOR x, int n)
     return x * x * x;
 }
 
-/* One Newton step on f(u) = u^{n+2} + P u + Q from the closed-form root r.
- * The closed forms cancel where u << |P|^{1/(n+1)} (dense, screened cells:
- * Cardano's cbrt(t1) - cbrt(t2) with t1 ~ t2, the quadratic's -P + sqrt(...)
- * with P > 0, Ferrari's -S + sqrt(...)/2); f(r) is evaluated with an absolute
- * error ~ eps |Q| there, so the step restores full relative precision. */
+/* Newton's method on f(u) = u^{n+2} + P u + Q from the closed-form root r,
+ * to convergence (|step| <= 1e-15 u, at most 8 steps).  The closed forms
+ * cancel where u << |P|^{1/(n+1)} (dense, screened cells: Cardano's
+ * cbrt(t1) - cbrt(t2) with t1 ~ t2, the quadratic's -P + sqrt(...) with
+ * P > 0, Ferrari's -S + sqrt(...)/2); f is evaluated with an absolute error
+ * ~ eps |Q| there, so the iteration restores full relative precision. */
 static inline double FN(polish_root)(double r, double P, double Q, int n)
 {
-    const double rn1 = n == 0 ? r : (n == 1 ? r * r : r * r * r);
-    const double f = rn1 * r + P * r + Q;
-    const double fp = (double)(n + 2) * rn1 + P;
-    if (!(fp > 0.0)) return r;
-    const double r1 = r - f / fp;
-    return r1 > 0.0 ? r1 : r;
+    for (int it = 0; it < 8; it++) {
+        const double rn1 = n == 0 ? r : (n == 1 ? r * r : r * r * r);
+        const double f = rn1 * r + P * r + Q;
+        const double fp = (double)(n + 2) * rn1 + P;
+        if (!(fp > 0.0)) return r;
+        const double d = f / fp;
+        const double r1 = r - d;
+        if (!(r1 > 0.0)) return r;
+        r = r1;
+        if (fabs(d) <= 1.0e-15 * r) break;
+    }
+    return r;
 }
 
 /* ---------------------------------------------------------------------- */

@@ -11,13 +11,17 @@ Tolerances (FP64):
   * transfer operators, residuals, right-hand sides, prolongation and the DGP
     smoother use only +, -, *, / and sqrt in the oracle's operand order with
     FMA contraction off: bit-identical (0 ulp);
-  * the f(R) smoother evaluates pow/acos/cos with the device math library
+  * the f(R) smoother with the reference's closed form evaluated literally
+    (MGS_FLAG_REFERENCE_ROOTS) evaluates pow/acos/cos with the device math library
     instead of glibc (<= 1 ulp apart): per cell |du| <= rtol |u| +
     1e-13 max|u| with rtol = 1e-12 for n = 0, 1.  For n = 2 rtol = 1e-10:
     the reference's Ferrari form S = sqrt(cb + Delta20/cb)/sqrt(12)
     (PMP2MGsolver_fR.f90:248) cancels as P -> 0 (cb^2 -> 12|Q|) and scales a
     1-ulp difference of the cube root by cb^2/|cb^2 - 12|Q|| (observed
-    worst 8.8e-12 on the fixtures).
+    worst 8.8e-12 on the fixtures);
+  * the default f(R) smoother (FP32 seed + FP64 Newton) against the oracle's
+    closed form + Newton to convergence: both are the root to within a few
+    ulp, rtol 1e-13 (+ 1e-13 max|u| absolute).
 """
 import os
 
@@ -33,14 +37,20 @@ FR_FIX = [f"kernels_fr_n{n}_{t}" for n in (0, 1, 2) for t in ("a1", "a05")]
 DGP_FIX = [f"kernels_dgp_{b}_{t}" for b in ("N", "S") for t in ("a1", "a05")]
 RT_FR_SMOOTH = {0: 1e-12, 1: 1e-12, 2: 1e-10}
 AT_FR_SMOOTH = 1e-13
+RT_REFINED = 1e-13
 
 
-def cfgs(par, precision=M.PREC_FP64):
+def cfgs(par, precision=M.PREC_FP64, roots="reference"):
+    """Oracle and device parameters.  roots='reference': the reference's
+    closed form evaluated literally on both sides; 'refined': the device's
+    default (FP32 seed + FP64 Newton) against the oracle's closed form
+    followed by Newton steps to convergence."""
     N, model, frn, br, box, om, oml, a, fr0, h0rc = [float(x) for x in par]
+    ref = roots == "reference"
     op = O.Params(N=int(N), model=int(model), fr_n=int(frn), branch=int(br), box=box, om=om, oml=oml, a=a,
-                  fr0=fr0, h0rc=h0rc)
+                  fr0=fr0, h0rc=h0rc, polish=0 if ref else 1)
     gc = M.Config(N=int(N), model=int(model), fr_n=int(frn), branch=int(br), box=box, omega_m=om, omega_l=oml,
-                  aexpn=a, fr0=fr0, h0rc=h0rc, precision=precision)
+                  aexpn=a, fr0=fr0, h0rc=h0rc, precision=precision, flags=M.FLAG_REFERENCE_ROOTS if ref else 0)
     return op, gc
 
 
@@ -58,9 +68,10 @@ def exact(a, b, what):
     assert nd == 0, f"{what}: {nd} cells differ, max rel {max_rel(a, b):.3e}"
 
 
-def run_kernel_chain(name, model):
+def run_kernel_chain(name, model, roots="reference"):
     fx = np.load(os.path.join(GOLDEN, name + ".npz"))
-    op, gc = cfgs(fx["params"])
+    op, gc = cfgs(fx["params"], roots=roots)
+    rt = RT_FR_SMOOTH[op.fr_n] if roots == "reference" else RT_REFINED
     print(f"[kernel parity] {name} N={op.N} model={model}", flush=True)
     delta = fx["delta"].astype(np.float64)
     u = fx["u_in"].astype(np.float64)
@@ -79,7 +90,7 @@ def run_kernel_chain(name, model):
         got = S.get_field()
         if model == 1:
             worst = max(worst, max_rel(got, u))
-            assert np.allclose(got, u, rtol=RT_FR_SMOOTH[op.fr_n], atol=AT_FR_SMOOTH * np.max(np.abs(u))), f"smoother red={red}"
+            assert np.allclose(got, u, rtol=rt, atol=AT_FR_SMOOTH * np.max(np.abs(u))), f"smoother red={red}"
         else:
             exact(got, u, f"DGP smoother red={red}")
         S.set_field(u)
@@ -111,7 +122,7 @@ def run_kernel_chain(name, model):
         got = S.get_level(1, M.ARR_U)
         if model == 1:
             worst = max(worst, max_rel(got, u1))
-            assert np.allclose(got, u1, rtol=RT_FR_SMOOTH[op.fr_n], atol=AT_FR_SMOOTH * np.max(np.abs(u1)))
+            assert np.allclose(got, u1, rtol=rt, atol=AT_FR_SMOOTH * np.max(np.abs(u1)))
         else:
             exact(got, u1, f"DGP smoother l1 red={red}")
         S.set_level(1, M.ARR_U, u1)
@@ -134,10 +145,11 @@ def run_kernel_chain(name, model):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("roots", ["reference", "refined"])
 @pytest.mark.parametrize("name", FR_FIX)
-def test_fr_operators_fp64(name):
-    worst = run_kernel_chain(name, 1)
-    print(f"  f(R) smoother worst rel diff {worst:.3e}")
+def test_fr_operators_fp64(name, roots):
+    worst = run_kernel_chain(name, 1, roots)
+    print(f"  f(R) smoother ({roots} roots) worst rel diff {worst:.3e}")
 
 
 @pytest.mark.gpu
@@ -183,7 +195,7 @@ def test_vcycles_fp64_vs_oracle(model):
 def test_fp64_solve_matches_oracle(frn):
     """scalar_field to relative residual 1e-8 on a seeded lognormal field."""
     N = 32
-    op = O.Params(N=N, model=1, fr_n=frn, box=64.0, om=0.3, oml=0.7, a=1.0, fr0=-1e-5)
+    op = O.Params(N=N, model=1, fr_n=frn, box=64.0, om=0.3, oml=0.7, a=1.0, fr0=-1e-5, polish=1)
     delta = O.lognormal_density(N, op.box, 1.0, seed=1001)
     u_o, rep_o = O.solve(op, delta, mode=1, rel_tol=1e-8, max_cycles=60, prec="f64")
     gc = M.Config(N=N, model=1, fr_n=frn, box=64.0, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=-1e-5)
@@ -208,7 +220,7 @@ def test_mixed_solve_reaches_fp64_oracle(frn, fr0, sigma):
     oracle's converged field to the accuracy both solves carry."""
     N = 32
     box = 64.0
-    op = O.Params(N=N, model=1, fr_n=frn, box=box, om=0.3, oml=0.7, a=1.0, fr0=fr0)
+    op = O.Params(N=N, model=1, fr_n=frn, box=box, om=0.3, oml=0.7, a=1.0, fr0=fr0, polish=1)
     delta = O.lognormal_density(N, box, sigma, seed=1002)
     u_o, rep_o = O.solve(op, delta, mode=1, rel_tol=1e-10, max_cycles=80, prec="f64")
     gc = M.Config(N=N, model=1, fr_n=frn, box=box, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=fr0,

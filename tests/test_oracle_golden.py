@@ -1,0 +1,130 @@
+"""The oracle's FP32-storage variant against the reference's own outputs.
+
+The golden vectors (tests/golden/*.npz, written by make_golden.py from the
+reference compiled with flang) hold the reference's inputs and outputs of
+every multigrid routine.  The oracle's f32 variant restates the reference
+with the reference's operand order and Fortran typing, so it must reproduce
+every stored field bit for bit; RMS norms are summed in a different order
+(per-row partials) and agree to 1e-13.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN
+
+FR_FIX = [f"kernels_fr_n{n}_{t}" for n in (0, 1, 2) for t in ("a1", "a05")]
+DGP_FIX = [f"kernels_dgp_{b}_{t}" for b in ("N", "S") for t in ("a1", "a05")]
+
+
+def params_of(fx, **kw):
+    N, model, frn, br, box, om, oml, a, fr0, h0rc = [float(x) for x in fx["params"]]
+    return O.Params(N=int(N), model=int(model), fr_n=int(frn), branch=int(br), box=box, om=om, oml=oml, a=a,
+                    fr0=fr0, h0rc=h0rc, **kw)
+
+
+def same(a, b, what):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    nd = int(np.count_nonzero(a.view(np.int32) != b.view(np.int32)))
+    assert nd == 0, f"{what}: {nd} cells differ from the reference"
+
+
+def replay(fx, model):
+    p = params_of(fx)
+    delta = fx["delta"].copy()
+    u = fx["u_in"].copy()
+    if model == 1:
+        smooth = lambda l, r, uu, rho, f: O.fr_smooth(p, l, r, uu, rho, f, prec="f32")
+        residual, rr1, rhs = O.fr_residual, O.fr_restrict_residual1, O.fr_rhs
+    else:
+        smooth = lambda l, r, uu, rho, f: O.dgp_smooth(p, l, r, uu, rho, f, lexi=True, prec="f32")
+        residual, rr1, rhs = O.dgp_residual, O.dgp_restrict_residual1, O.dgp_rhs
+    smooth(0, 1, u, delta, None)
+    same(u, fx["u_red"], "smoother red, level 0")
+    smooth(0, 0, u, delta, None)
+    same(u, fx["u_black"], "smoother black, level 0")
+    res0 = np.zeros_like(u)
+    rms = residual(p, 0, u, delta, None, res0, "f32")
+    same(res0, fx["res0"], "residual, level 0")
+    assert abs(rms - float(fx["res0_rms"])) <= 1e-13 * float(fx["res0_rms"])
+    rres1 = rr1(p, u, delta, "f32")
+    same(rres1, fx["rres1"], "restrict_residual(1)")
+    ru1 = O.restrict8(u, "f32")
+    same(ru1, fx["ru1"], "restrict_scalar_field(1) R u")
+    same(ru1, fx["u1_init"], "restrict_scalar_field(1) u_H")
+    rho1 = O.restrict8(delta, "f32")
+    same(rho1, fx["rho1"], "restrict_density(1)")
+    f1 = rhs(p, 1, ru1, rho1, rres1, "f32")
+    same(f1, fx["f1"], "right-hand side, level 1")
+    u1 = ru1.copy()
+    smooth(1, 1, u1, rho1, f1)
+    same(u1, fx["u1_red"], "smoother red, level 1")
+    smooth(1, 0, u1, rho1, f1)
+    same(u1, fx["u1_black"], "smoother black, level 1")
+    res1 = np.zeros_like(u1)
+    residual(p, 1, u1, rho1, f1, res1, "f32")
+    same(res1, fx["res1"], "residual, level 1")
+    same(O.restrict8(res1, "f32"), fx["rres2"], "restrict_residual(2)")
+    same(O.restrict8(u1, "f32"), fx["ru2"], "restrict_scalar_field(2)")
+    same(O.restrict8(rho1, "f32"), fx["rho2"], "restrict_density(2)")
+    uf = fx["u_before_corr"].copy()
+    O.correct(fx["u1_corr_in"].copy(), ru1, uf, model == 1, "f32")
+    same(uf, fx["u_after_corr"], "correct_solution(1)")
+
+
+@pytest.mark.parametrize("name", FR_FIX)
+def test_fr_routines_match_reference(name):
+    replay(np.load(os.path.join(GOLDEN, name + ".npz")), 1)
+
+
+@pytest.mark.parametrize("name", DGP_FIX)
+def test_dgp_routines_match_reference(name):
+    replay(np.load(os.path.join(GOLDEN, name + ".npz")), 2)
+
+
+@pytest.mark.parametrize("name,model", [("vcycles_fr", 1), ("vcycles_dgp", 2)])
+def test_vcycles_match_reference(name, model):
+    fx = np.load(os.path.join(GOLDEN, name + ".npz"))
+    p = params_of(fx)
+    u = fx["u_in"].copy()
+    hist = O.vcycles(p, u, fx["delta"].copy(), len(fx["hist"]), dgp_lexi=True, prec="f32")
+    assert np.allclose(hist, fx["hist"], rtol=1e-12, atol=0)
+    same(u, fx["u_out"], "field after 6 V-cycles")
+
+
+def test_scalar_field_matches_reference():
+    """scalar_field(1) with the reference's exits and iter_count_max = 8,
+    including the final division by u_bg (PMP2extradof.f90:131-146)."""
+    fx = np.load(os.path.join(GOLDEN, "scalar_field_fr.npz"))
+    p = params_of(fx)
+    u, rep = O.solve(p, fx["delta"].copy(), mode=0, res_conv=1e-8, max_cycles=8, normalize=True, prec="f32")
+    same(u, fx["u_out"], "scalar_field output")
+
+
+def test_levels_follow_reference_rule():
+    """levelmax/levelmin (PMP2extradof.f90:240-253) and the coarsest level
+    the V-cycle stops at (:368)."""
+    expect = {128: (7, 1, 6), 512: (9, 1, 8), 1024: (10, 1, 9), 64: (6, 1, 5), 192: (8, 3, 5), 256: (8, 1, 7)}
+    for N, (lmax, lmin, lc) in expect.items():
+        assert O.levelmax(N) == lmax and O.levelmin(N) == lmin and O.coarsest_level(N) == lc
+        assert (N >> lc) % 2 == 0
+    assert 192 >> O.coarsest_level(192) == 6
+
+
+def test_fixture_branch_coverage():
+    """The n=1 and n=2 fixtures exercise both closed-form branches."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(GOLDEN, "make_golden.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    for n, keys in ((1, ("cardano", "trig")), (2, ("P>0", "P<0"))):
+        fr = {k: 0.0 for k in keys}
+        for t in ("a1", "a05"):
+            fx = np.load(os.path.join(GOLDEN, f"kernels_fr_n{n}_{t}.npz"))
+            st = mk.branch_stats(params_of(fx), fx["u_in"], fx["delta"])
+            for k in keys:
+                fr[k] = max(fr[k], st[k])
+        assert all(v > 0.05 for v in fr.values()), fr
