@@ -114,9 +114,18 @@ __device__ __forceinline__ Nb neighbours(const Geo& g, int c, int m, int j, int 
 // block sum of one double per thread; thread 0 of the block gets the total
 __device__ inline double block_sum(double v)
 {
-    __shared__ double wsum[16];
+    __shared__ double wsum[64];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int nthr = blockDim.x * blockDim.y;
+    if (nthr < 64) {
+        // a partial wave: lanes beyond the block do not exist, so no shuffles
+        wsum[tid] = v;
+        __syncthreads();
+        double t = 0.0;
+        if (tid == 0)
+            for (int w = 0; w < nthr; w++) t += wsum[w];
+        return t;
+    }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int wave = tid >> 6, lane = tid & 63;
     if (lane == 0) wsum[wave] = v;

@@ -1,38 +1,39 @@
 // mgs_fr.hip -- f(R) multigrid kernels for gfx950 (CDNA4).
 //
 // Plain FP64 path: the reference's operators (PMP2MGsolver_fR.f90,
-// PMP2extradof.f90:1168-1578) evaluated in FP64 with the reference's
-// operand order, on the colour-split layout of mgs_common.h.  Built with
-// -ffp-contract=off so every add/mul rounds as the CPU oracle's does; the
-// transfer operators are bit-identical to the oracle's FP64 variant and the
-// smoother differs only through the device libm (pow/acos/cos).
+// PMP2extradof.f90:1168-1578) evaluated in FP64 with the reference's operand
+// order, on the colour-split layout of mgs_common.h.  Built with
+// -ffp-contract=off, so the residual, the restrictions, the right-hand side
+// and the prolongation round every operation as the CPU oracle does and are
+// bit-identical to it.  The smoother solves the same local polynomial
+// u^{n+2} + P u + Q = 0; by default its root comes from a cancellation-free
+// FP32 seed refined by FP64 Newton steps (MGS_FLAG_REFERENCE_ROOTS selects
+// the reference's closed forms evaluated literally with the device libm).
 //
 // Mixed path: the FP32 delta-form FAS used inside the FP64 defect-correction
-// loop (see mgs_api.cpp, solve_mixed).  Unknown d = x^{n+1} - xb^{n+1} with
+// loop (mgs_api.cpp, solve_mixed).  Unknown d = x^{n+1} - xb^{n+1} with
 // x = u / u_bg and xb an FP32 copy of the current FP64 iterate.  The inner
 // operator
 //     D(d) = A (sum_nb d_nb - 6 d) + C (1/xb - 1/x)
-// is N(xb + .) - N(xb) written so that every term is proportional to d:
-// FP32 rounding then scales with the correction and the outer FP64 residual
-// converges to FP64 accuracy.  Each cell is solved by the reference's closed
-// form for x^{n+2} + P x + Q = 0 in FP32 (robust far from the solution),
-// followed by two Newton steps on D(d) = rhs in d (restores relative
-// precision near the solution).
+// is N(xb + .) - N(xb) written so that every term is proportional to d: FP32
+// rounding then scales with the correction and the outer FP64 residual
+// converges to FP64 accuracy.  Each cell is solved from the FP32 closed-form
+// seed followed by a Newton step on D(d) = rhs in d.
 //
-// Memory-bound stencils: one thread per cell, 256-thread blocks, rows of one
-// colour contiguous, and an XCD-aware block order (mgs_map_cell) so that the
-// rows a block's stencil touches are resident in the same XCD's L2.
+// Stencil kernels march over z (mgs_stencil.h); transfer kernels map one
+// thread to one coarse cell; all are memory-bound, with 32-bit index math.
 
 #include <math.h>
 
 #include "mgs_device.h"
 #include "mgs_kernels.h"
+#include "mgs_stencil.h"
 
 namespace mgs {
 
 // ---------------------------------------------------------------------------
-// closed-form local roots of u^{n+2} + P u + Q = 0,
-// PMP2MGsolver_fR.f90:105-112 (n=0), 159-179 (n=1), 240-256 (n=2)
+// closed-form local roots of u^{n+2} + P u + Q = 0 as the reference writes
+// them, PMP2MGsolver_fR.f90:105-112 (n=0), 159-179 (n=1), 240-256 (n=2)
 // ---------------------------------------------------------------------------
 template <int FRN>
 __device__ __forceinline__ double fr_root64(double P, const FrCoef& c)
@@ -146,6 +147,13 @@ __device__ __forceinline__ double newton_root64(double u, double P, double Q)
     return u;
 }
 
+// 32-bit colour-split offset of cell (i,j,k)
+__device__ __forceinline__ unsigned sidx(const Geo& g, int i, int j, int k)
+{
+    return (unsigned)((i + j + k) & 1) * (unsigned)g.half + ((unsigned)k * (unsigned)g.n + (unsigned)j) * (unsigned)g.h +
+           (unsigned)(i >> 1);
+}
+
 // ===========================================================================
 // Plain FP64 path
 // ===========================================================================
@@ -154,66 +162,84 @@ __device__ __forceinline__ double newton_root64(double u, double P, double Q)
 // REFROOT: the reference's closed form evaluated literally; otherwise the
 // same root from the FP32 seed refined by FP64 Newton steps.
 template <int FRN, bool COARSE, bool REFROOT>
-__global__ __launch_bounds__(256) void k_fr_smooth64(double* __restrict__ u, const double* __restrict__ rho,
-                                                     const double* __restrict__ f, Geo g, int colour, FrCoef c, int TX,
-                                                     int TY, int XB, unsigned NB)
-{
-    int m;
-    unsigned row;
-    if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
-    const Nb q = neighbours(g, colour, m, j, k);
-    const double S = wpow<FRN>(u[q.xp]) + wpow<FRN>(u[q.xm]) + wpow<FRN>(u[q.yp]) + wpow<FRN>(u[q.ym]) +
-                     wpow<FRN>(u[q.zp]) + wpow<FRN>(u[q.zm]);
-    double P;
-    if constexpr (!COARSE)
-        P = c.fct1s * rho[q.me] - c.Q2 - S / 6.0;
-    else
-        P = c.fct1s * rho[q.me] - c.Q2 + c.fct2s * f[q.me] - S / 6.0;
-    if constexpr (REFROOT) {
-        u[q.me] = fr_root64<FRN>(P, c);
-    } else {
-        const float x = seed_root<FRN>((float)(P * c.inv_w), (float)(c.Q2 * c.inv_w));
-        u[q.me] = newton_root64<FRN>((double)x * c.ubg, P, c.Q);
+struct FrSmoothOp {
+    using T = double;
+    double* u;
+    const double* rho;
+    const double* f;
+    FrCoef c;
+    __device__ double* field() const { return u; }
+    __device__ __forceinline__ void update(unsigned o, double xp, double xm, double yp, double ym, double zp,
+                                           double zm) const
+    {
+        const double S =
+            wpow<FRN>(xp) + wpow<FRN>(xm) + wpow<FRN>(yp) + wpow<FRN>(ym) + wpow<FRN>(zp) + wpow<FRN>(zm);
+        double P;
+        if constexpr (!COARSE)
+            P = c.fct1s * rho[o] - c.Q2 - S / 6.0;
+        else
+            P = c.fct1s * rho[o] - c.Q2 + c.fct2s * f[o] - S / 6.0;
+        if constexpr (REFROOT) {
+            u[o] = fr_root64<FRN>(P, c);
+        } else {
+            const float x = seed_root<FRN>((float)(P * c.inv_w), (float)(c.Q2 * c.inv_w));
+            u[o] = newton_root64<FRN>((double)x * c.ubg, P, c.Q);
+        }
     }
-}
+};
 
-// residual of one cell, calculate_residual_fR, PMP2MGsolver_fR.f90:434-458
-template <int FRN>
-__device__ __forceinline__ double fr_res_cell(const double* __restrict__ u, const Nb& q, double rho, const FrCoef& c)
-{
-    const double uc = u[q.me];
-    const double S = wpow<FRN>(u[q.xp]) + wpow<FRN>(u[q.xm]) + wpow<FRN>(u[q.yp]) + wpow<FRN>(u[q.ym]) +
-                     wpow<FRN>(u[q.zp]) + wpow<FRN>(u[q.zm]);
-    double OP = S - wpow<FRN>(uc) * 6.0;
-    OP = OP * c.fct3r;
-    OP = OP + c.fct1r * rho;
-    OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
-    return OP;
-}
-
+// residual of one cell, calculate_residual_fR, PMP2MGsolver_fR.f90:434-458;
+// COARSE subtracts f (:482-488).  Also the FAS right-hand side (:836-923)
+// with (u, f, out) = (R u, R r, f_l).
 template <int FRN, bool COARSE>
-__global__ __launch_bounds__(256) void k_fr_residual64(const double* __restrict__ u, const double* __restrict__ rho,
-                                                       const double* __restrict__ f, double* __restrict__ res,
-                                                       double* __restrict__ partials, Geo g, FrCoef c, int TX, int TY,
-                                                       int XB, unsigned NB)
-{
-    int m;
-    unsigned row;
-    double acc = 0.0;
-    if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
-        const unsigned nn = (unsigned)g.n * (unsigned)g.n;
-        const int col = row >= nn ? 1 : 0;
-        const unsigned r2 = row - (unsigned)col * nn;
-        const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
-        const Nb q = neighbours(g, col, m, j, k);
-        double OP = fr_res_cell<FRN>(u, q, rho[q.me], c);
-        if constexpr (COARSE) OP = OP - f[q.me];
-        if (res) res[q.me] = OP;
-        acc = OP * OP;
+struct FrResOp {
+    using T = double;
+    const double* u;
+    const double* rho;
+    const double* f;
+    double* out;
+    FrCoef c;
+    __device__ const double* field() const { return u; }
+    __device__ __forceinline__ double cell(unsigned o, double uc, double xp, double xm, double yp, double ym,
+                                           double zp, double zm) const
+    {
+        const double S =
+            wpow<FRN>(xp) + wpow<FRN>(xm) + wpow<FRN>(yp) + wpow<FRN>(ym) + wpow<FRN>(zp) + wpow<FRN>(zm);
+        double OP = S - wpow<FRN>(uc) * 6.0;
+        OP = OP * c.fct3r;
+        OP = OP + c.fct1r * rho[o];
+        OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
+        if constexpr (COARSE) OP = OP - f[o];
+        if (out) out[o] = OP;
+        return OP;
     }
-    store_partial(partials, NB, block_sum(acc));
-}
+};
+
+// FP64 outer residual r = N(u) of the mixed loop, emitting the inner
+// problem's data: g = -r (FP32) and xb = u / u_bg (FP32)
+template <int FRN>
+struct FrOuterOp {
+    using T = double;
+    const double* u;
+    const double* delta;
+    float* g;
+    float* xb;
+    FrCoef c;
+    __device__ const double* field() const { return u; }
+    __device__ __forceinline__ double cell(unsigned o, double uc, double xp, double xm, double yp, double ym,
+                                           double zp, double zm) const
+    {
+        const double S =
+            wpow<FRN>(xp) + wpow<FRN>(xm) + wpow<FRN>(yp) + wpow<FRN>(ym) + wpow<FRN>(zp) + wpow<FRN>(zm);
+        double OP = S - wpow<FRN>(uc) * 6.0;
+        OP = OP * c.fct3r;
+        OP = OP + c.fct1r * delta[o];
+        OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
+        g[o] = (float)(-OP);
+        xb[o] = (float)(uc / c.ubg);
+        return OP;
+    }
+};
 
 // restrict_residual_fR(1): the 8 children's residuals averaged, recomputed
 // from u and delta, PMP2MGsolver_fR.f90:607-749
@@ -232,17 +258,19 @@ __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* 
     const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
     const int yl = (J == 0) ? N - 1 : 2 * J - 1, yu = (2 * J + 2 == N) ? 0 : 2 * J + 2;
     const int zl = (K == 0) ? N - 1 : 2 * K - 1, zu = (2 * K + 2 == N) ? 0 : 2 * K + 2;
-    const size_t ch[8] = {split_index(gf, x0, y0, z0), split_index(gf, x1, y0, z0), split_index(gf, x0, y1, z0),
-                          split_index(gf, x1, y1, z0), split_index(gf, x0, y0, z1), split_index(gf, x1, y0, z1),
-                          split_index(gf, x0, y1, z1), split_index(gf, x1, y1, z1)};
+    const unsigned ch[8] = {sidx(gf, x0, y0, z0), sidx(gf, x1, y0, z0), sidx(gf, x0, y1, z0), sidx(gf, x1, y1, z0),
+                            sidx(gf, x0, y0, z1), sidx(gf, x1, y0, z1), sidx(gf, x0, y1, z1), sidx(gf, x1, y1, z1)};
+    double uc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) uc[q] = u0[ch[q]];
     double Q = -8.0 * c.fct2r;
 #pragma unroll
-    for (int q = 0; q < 8; q++) Q = Q + c.fct1r * delta[ch[q]] + c.fct4_1 / u0[ch[q]];
-    double s8 = wpow<FRN>(u0[ch[0]]);
+    for (int q = 0; q < 8; q++) Q = Q + c.fct1r * delta[ch[q]] + c.fct4_1 / uc[q];
+    double s8 = wpow<FRN>(uc[0]);
 #pragma unroll
-    for (int q = 1; q < 8; q++) s8 = s8 + wpow<FRN>(u0[ch[q]]);
+    for (int q = 1; q < 8; q++) s8 = s8 + wpow<FRN>(uc[q]);
     double P = -3.0 * s8;
-#define W_(x, y, z) wpow<FRN>(u0[split_index(gf, x, y, z)])
+#define W_(x, y, z) wpow<FRN>(u0[sidx(gf, x, y, z)])
     P = P + W_(xl, y0, z0) + W_(xl, y1, z0) + W_(xl, y0, z1) + W_(xl, y1, z1);
     P = P + W_(xu, y0, z0) + W_(xu, y1, z0) + W_(xu, y0, z1) + W_(xu, y1, z1);
     P = P + W_(x0, yl, z0) + W_(x1, yl, z0) + W_(x0, yl, z1) + W_(x1, yl, z1);
@@ -250,11 +278,12 @@ __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* 
     P = P + W_(x0, y0, zl) + W_(x1, y0, zl) + W_(x0, y1, zl) + W_(x1, y1, zl);
     P = P + W_(x0, y0, zu) + W_(x1, y0, zu) + W_(x0, y1, zu) + W_(x1, y1, zu);
 #undef W_
-    rres1[split_index(gc, I, J, K)] = (P * c.fct3_1 + Q) / 8.0;
+    rres1[sidx(gc, I, J, K)] = (P * c.fct3_1 + Q) / 8.0;
 }
 
 // 8-child average (restrict_density, restrict_scalar_field,
-// restrict_residual_fR at levels > 1); out2 optional second copy
+// restrict_residual_fR at levels > 1, PMP2extradof.f90:1198-1247,
+// 1293-1345, PMP2MGsolver_fR.f90:750-779); out2: optional second copy
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T* __restrict__ out1,
                                                    T* __restrict__ out2, Geo gf, Geo gc, int TX, int TY, int XB,
@@ -264,92 +293,85 @@ __global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T
     unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
     const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
-    // children in the reference's order (x fastest), accumulated in double
-    // for FP64 data and in float for FP32 data
-    using A = T;
-    A P = fine[split_index(gf, 2 * I, 2 * J, 2 * K)];
-    P = P + fine[split_index(gf, 2 * I + 1, 2 * J, 2 * K)];
-    P = P + fine[split_index(gf, 2 * I, 2 * J + 1, 2 * K)];
-    P = P + fine[split_index(gf, 2 * I + 1, 2 * J + 1, 2 * K)];
-    P = P + fine[split_index(gf, 2 * I, 2 * J, 2 * K + 1)];
-    P = P + fine[split_index(gf, 2 * I + 1, 2 * J, 2 * K + 1)];
-    P = P + fine[split_index(gf, 2 * I, 2 * J + 1, 2 * K + 1)];
-    P = P + fine[split_index(gf, 2 * I + 1, 2 * J + 1, 2 * K + 1)];
-    const size_t o = split_index(gc, I, J, K);
-    const T v = P / (A)8.0;
+    // children in the reference's order (x fastest), accumulated in T
+    T P = fine[sidx(gf, 2 * I, 2 * J, 2 * K)];
+    P = P + fine[sidx(gf, 2 * I + 1, 2 * J, 2 * K)];
+    P = P + fine[sidx(gf, 2 * I, 2 * J + 1, 2 * K)];
+    P = P + fine[sidx(gf, 2 * I + 1, 2 * J + 1, 2 * K)];
+    P = P + fine[sidx(gf, 2 * I, 2 * J, 2 * K + 1)];
+    P = P + fine[sidx(gf, 2 * I + 1, 2 * J, 2 * K + 1)];
+    P = P + fine[sidx(gf, 2 * I, 2 * J + 1, 2 * K + 1)];
+    P = P + fine[sidx(gf, 2 * I + 1, 2 * J + 1, 2 * K + 1)];
+    const unsigned o = sidx(gc, I, J, K);
+    const T v = P / (T)8.0;
     out1[o] = v;
     if (out2) out2[o] = v;
 }
 
-// calculate_physical_right_hand_side_fR, PMP2MGsolver_fR.f90:789-927
-template <int FRN>
-__global__ __launch_bounds__(256) void k_fr_rhs64(const double* __restrict__ ru, const double* __restrict__ rho,
-                                                  const double* __restrict__ rres, double* __restrict__ f, Geo g,
-                                                  FrCoef c, int TX, int TY, int XB, unsigned NB)
-{
-    int m;
-    unsigned row;
-    if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const unsigned nn = (unsigned)g.n * (unsigned)g.n;
-    const int col = row >= nn ? 1 : 0;
-    const unsigned r2 = row - (unsigned)col * nn;
-    const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
-    const Nb q = neighbours(g, col, m, j, k);
-    double OP = fr_res_cell<FRN>(ru, q, rho[q.me], c);
-    OP = OP - rres[q.me];
-    f[q.me] = OP;
-}
-
 // correct_solution, PMP2extradof.f90:1361-1578: u_h += P(u_H - R u_h) with the
-// 27/9/3/1 (/64) weights; guard: keep the old value unless the new one is > 0
-template <typename T, typename A>
-__global__ __launch_bounds__(256) void k_correct(const T* __restrict__ uH, const T* __restrict__ ru,
-                                                 T* __restrict__ uf, Geo gf, Geo gc, int guard, int TX, int TY,
-                                                 int XB, unsigned NB)
+// 27/9/3/1 (/64) weights; guard: keep the old value unless the new one is > 0.
+// Each coarse value of the 3x3x3 block is loaded once; the sum keeps the
+// reference's order P = P + w uH - w Ru over the 8 contributors of a son.
+__global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH, const double* __restrict__ ru,
+                                                   double* __restrict__ uf, Geo gf, Geo gc, int guard, int TX, int TY,
+                                                   int XB, unsigned NB)
 {
     int I;
     unsigned row;
     if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
     const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
     const int nc = gc.n;
-    const A a = (A)1.0 / (A)64.0, b = (A)3.0 * a, c9 = (A)9.0 * a, d = (A)27.0 * a;
-    const A wt[8] = {a, b, b, c9, b, c9, c9, d};
-    const int Im = wrap_dec(I, nc), Ip = wrap_inc(I, nc);
-    const int Jm = wrap_dec(J, nc), Jp = wrap_inc(J, nc);
-    const int Km = wrap_dec(K, nc), Kp = wrap_inc(K, nc);
+    const double a = 1.0 / 64.0, b = 3.0 * a, c9 = 9.0 * a, d = 27.0 * a;
+    const double wt[8] = {a, b, b, c9, b, c9, c9, d};
+    const int Xs[3] = {wrap_dec(I, nc), I, wrap_inc(I, nc)};
+    const int Ys[3] = {wrap_dec(J, nc), J, wrap_inc(J, nc)};
+    const int Zs[3] = {wrap_dec(K, nc), K, wrap_inc(K, nc)};
+    double vh[27], vr[27];
+#pragma unroll
+    for (int z = 0; z < 3; z++)
+#pragma unroll
+        for (int y = 0; y < 3; y++)
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                const unsigned q = sidx(gc, Xs[x], Ys[y], Zs[z]);
+                vh[x + 3 * y + 9 * z] = uH[q];
+                vr[x + 3 * y + 9 * z] = ru[q];
+            }
 #pragma unroll
     for (int sind = 0; sind < 8; sind++) {
         const int s1 = sind & 1, s2 = (sind >> 1) & 1, s3 = (sind >> 2) & 1;
-        const int nx = s1 ? Ip : Im, ny = s2 ? Jp : Jm, nz = s3 ? Kp : Km;
-        A P = (A)0.0;
+        double P = 0.0;
 #pragma unroll
         for (int ind = 0; ind < 8; ind++) {
-            const int X = (ind & 1) ? I : nx;
-            const int Y = (ind & 2) ? J : ny;
-            const int Z = (ind & 4) ? K : nz;
-            const size_t qq = split_index(gc, X, Y, Z);
-            P = P + wt[ind] * (A)uH[qq];
-            P = P - wt[ind] * (A)ru[qq];
+            const int x = (ind & 1) ? 1 : (s1 ? 2 : 0);
+            const int y = (ind & 2) ? 1 : (s2 ? 2 : 0);
+            const int z = (ind & 4) ? 1 : (s3 ? 2 : 0);
+            P = P + wt[ind] * vh[x + 3 * y + 9 * z];
+            P = P - wt[ind] * vr[x + 3 * y + 9 * z];
         }
-        const size_t fi = split_index(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3);
-        const A tmpt = (A)uf[fi] + P;
-        if (!guard || tmpt > (A)0.0) uf[fi] = (T)tmpt;
+        const unsigned fi = sidx(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3);
+        const double tmpt = uf[fi] + P;
+        if (!guard || tmpt > 0.0) uf[fi] = tmpt;
     }
 }
 
-// deterministic final sum of block partials
-__global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict__ p, int count, double* out)
+// deterministic sum of block partials: stage 1 folds contiguous chunks, stage
+// 2 (one block) the chunk sums; the order depends only on the count
+__global__ __launch_bounds__(1024) void k_sum_chunks(const double* __restrict__ p, int count, int chunk,
+                                                     double* __restrict__ out)
 {
     __shared__ double s[1024];
+    const int lo = blockIdx.x * chunk;
+    const int hi = lo + chunk < count ? lo + chunk : count;
     double v = 0.0;
-    for (int i = threadIdx.x; i < count; i += blockDim.x) v += p[i];
+    for (int i = lo + (int)threadIdx.x; i < hi; i += blockDim.x) v += p[i];
     s[threadIdx.x] = v;
     __syncthreads();
     for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = s[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -363,7 +385,7 @@ __global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ lex, TO* __
     unsigned row;
     if (!map_xy(g.n, (long)g.n * g.n, TX, TY, XB, NB, i, row)) return;
     const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
-    split[split_index(g, i, j, k)] = (TO)lex[(size_t)row * g.n + i];
+    split[sidx(g, i, j, k)] = (TO)lex[(size_t)row * g.n + i];
 }
 
 template <typename TO>
@@ -374,7 +396,7 @@ __global__ __launch_bounds__(256) void k_unpack(const double* __restrict__ split
     unsigned row;
     if (!map_xy(g.n, (long)g.n * g.n, TX, TY, XB, NB, i, row)) return;
     const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
-    const double v = split[split_index(g, i, j, k)];
+    const double v = split[sidx(g, i, j, k)];
     lex[(size_t)row * g.n + i] = (TO)(scale == 1.0 ? v : v * scale);
 }
 
@@ -432,7 +454,8 @@ template <int FRN>
 __device__ __forceinline__ float df_solve(float S, float xb, float r, const DfCoef& c)
 {
     const float wb = wpow<FRN>(xb);
-    const float K = c.A * S + 6.0f * c.A * wb + c.C * __builtin_amdgcn_rcpf(xb) - r;
+    const float rxb = __builtin_amdgcn_rcpf(xb);
+    const float K = c.A * S + 6.0f * c.A * wb + c.C * rxb - r;
     float x = seed_root<FRN>(-K * c.inv6A, c.C * c.inv6A);
     if (!(x > 0.0f) || !isfinite(x)) x = xb;
     float d = wpow<FRN>(x) - wb;
@@ -440,7 +463,6 @@ __device__ __forceinline__ float df_solve(float S, float xb, float r, const DfCo
     // ~ (F''/F') (seed error)^2 ~ 1e-13 wb; the n=2 seed for P > 0 is only a
     // bound within a factor 2 of the root and takes four.
     constexpr int NIT = FRN == 2 ? 4 : 1;
-    const float rxb = __builtin_amdgcn_rcpf(xb);
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
         if (it > 0) x = xroot<FRN>(fmaxf(wb + d, 1.0e-30f));
@@ -457,62 +479,42 @@ __device__ __forceinline__ float df_solve(float S, float xb, float r, const DfCo
 }
 
 template <int FRN>
-__global__ __launch_bounds__(256) void k_df_smooth(float* __restrict__ d, const float* __restrict__ xb,
-                                                   const float* __restrict__ rhs, Geo g, int colour, DfCoef c, int TX,
-                                                   int TY, int XB, unsigned NB)
-{
-    int m;
-    unsigned row;
-    if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
-    const Nb q = neighbours(g, colour, m, j, k);
-    const float S = d[q.xp] + d[q.xm] + d[q.yp] + d[q.ym] + d[q.zp] + d[q.zm];
-    d[q.me] = df_solve<FRN>(S, xb[q.me], rhs[q.me], c);
-}
+struct DfSmoothOp {
+    using T = float;
+    float* d;
+    const float* xb;
+    const float* rhs;
+    DfCoef c;
+    __device__ float* field() const { return d; }
+    __device__ __forceinline__ void update(unsigned o, float xp, float xm, float yp, float ym, float zp, float zm) const
+    {
+        const float S = xp + xm + yp + ym + zp + zm;
+        d[o] = df_solve<FRN>(S, xb[o], rhs[o], c);
+    }
+};
 
-// D(d) - rhs at one cell
+// D(d) - rhs at a cell; with (d, rhs, out) = (R d, R r, f) the inner FAS
+// right-hand side f = D(R d) - R r
 template <int FRN>
-__device__ __forceinline__ float df_res_cell(const float* __restrict__ d, const Nb& q, float xb, float rhs,
-                                             const DfCoef& c)
-{
-    const float dc = d[q.me];
-    const float S = d[q.xp] + d[q.xm] + d[q.yp] + d[q.ym] + d[q.zp] + d[q.zm];
-    const float x = xroot<FRN>(fmaxf(wpow<FRN>(xb) + dc, 1.0e-30f));
-    return c.A * (S - 6.0f * dc) + dlocal<FRN>(dc, x, xb, c.C) - rhs;
-}
-
-template <int FRN>
-__global__ __launch_bounds__(256) void k_df_residual(const float* __restrict__ d, const float* __restrict__ xb,
-                                                     const float* __restrict__ rhs, float* __restrict__ res, Geo g,
-                                                     DfCoef c, int TX, int TY, int XB, unsigned NB)
-{
-    int m;
-    unsigned row;
-    if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const unsigned nn = (unsigned)g.n * (unsigned)g.n;
-    const int col = row >= nn ? 1 : 0;
-    const unsigned r2 = row - (unsigned)col * nn;
-    const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
-    const Nb q = neighbours(g, col, m, j, k);
-    res[q.me] = df_res_cell<FRN>(d, q, xb[q.me], rhs[q.me], c);
-}
-
-// f = D(rd) - rres  (FAS right-hand side of the inner problem)
-template <int FRN>
-__global__ __launch_bounds__(256) void k_df_rhs(const float* __restrict__ rd, const float* __restrict__ xb,
-                                                const float* __restrict__ rres, float* __restrict__ f, Geo g,
-                                                DfCoef c, int TX, int TY, int XB, unsigned NB)
-{
-    int m;
-    unsigned row;
-    if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
-    const unsigned nn = (unsigned)g.n * (unsigned)g.n;
-    const int col = row >= nn ? 1 : 0;
-    const unsigned r2 = row - (unsigned)col * nn;
-    const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
-    const Nb q = neighbours(g, col, m, j, k);
-    f[q.me] = df_res_cell<FRN>(rd, q, xb[q.me], 0.0f, c) - rres[q.me];
-}
+struct DfResOp {
+    using T = float;
+    const float* d;
+    const float* xb;
+    const float* rhs;
+    float* out;
+    DfCoef c;
+    __device__ const float* field() const { return d; }
+    __device__ __forceinline__ double cell(unsigned o, float dc, float xp, float xm, float yp, float ym, float zp,
+                                           float zm) const
+    {
+        const float S = xp + xm + yp + ym + zp + zm;
+        const float xbc = xb[o];
+        const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
+        const float r = c.A * (S - 6.0f * dc) + dlocal<FRN>(dc, x, xbc, c.C) - rhs[o];
+        out[o] = r;
+        return 0.0;
+    }
+};
 
 // level-0 inner residual D(d) - g averaged over the 8 children; the
 // children's Laplacians collapse to -3 sum(children) + sum(24 outer faces)
@@ -535,13 +537,13 @@ __global__ __launch_bounds__(256) void k_df_restrict_residual1(const float* __re
     float loc = 0.0f, s8 = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        const size_t o = split_index(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1));
+        const unsigned o = sidx(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1));
         const float dc = d0[o], xbc = xb0[o];
         const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
         loc += dlocal<FRN>(dc, x, xbc, c.C) - g0[o];
         s8 += dc;
     }
-#define D_(x, y, z) d0[split_index(gf, x, y, z)]
+#define D_(x, y, z) d0[sidx(gf, x, y, z)]
     float outer = D_(xl, y0, z0) + D_(xl, y1, z0) + D_(xl, y0, z1) + D_(xl, y1, z1);
     outer += D_(xu, y0, z0) + D_(xu, y1, z0) + D_(xu, y0, z1) + D_(xu, y1, z1);
     outer += D_(x0, yl, z0) + D_(x1, yl, z0) + D_(x0, yl, z1) + D_(x1, yl, z1);
@@ -549,7 +551,7 @@ __global__ __launch_bounds__(256) void k_df_restrict_residual1(const float* __re
     outer += D_(x0, y0, zl) + D_(x1, y0, zl) + D_(x0, y1, zl) + D_(x1, y1, zl);
     outer += D_(x0, y0, zu) + D_(x1, y0, zu) + D_(x0, y1, zu) + D_(x1, y1, zu);
 #undef D_
-    rres1[split_index(gc, I, J, K)] = (c.A * (outer - 3.0f * s8) + loc) * 0.125f;
+    rres1[sidx(gc, I, J, K)] = (c.A * (outer - 3.0f * s8) + loc) * 0.125f;
 }
 
 // restricted base: xb_c = (mean of the children's xb^{n+1})^{1/(n+1)}
@@ -564,8 +566,8 @@ __global__ __launch_bounds__(256) void k_restrict_base(const float* __restrict__
     float w = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; q++)
-        w += wpow<FRN>(xf[split_index(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1))]);
-    xc[split_index(gc, I, J, K)] = xroot<FRN>(w * 0.125f);
+        w += wpow<FRN>(xf[sidx(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1))]);
+    xc[sidx(gc, I, J, K)] = xroot<FRN>(w * 0.125f);
 }
 
 // inner coarse-grid correction d_h += P(d_H - R d_h), kept only where the
@@ -582,9 +584,9 @@ __global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH
     const int nc = gc.n;
     const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
     const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
-    const int Im = wrap_dec(I, nc), Ip = wrap_inc(I, nc);
-    const int Jm = wrap_dec(J, nc), Jp = wrap_inc(J, nc);
-    const int Km = wrap_dec(K, nc), Kp = wrap_inc(K, nc);
+    const int Xs[3] = {wrap_dec(I, nc), I, wrap_inc(I, nc)};
+    const int Ys[3] = {wrap_dec(J, nc), J, wrap_inc(J, nc)};
+    const int Zs[3] = {wrap_dec(K, nc), K, wrap_inc(K, nc)};
     float e[27];
 #pragma unroll
     for (int z = 0; z < 3; z++)
@@ -592,10 +594,7 @@ __global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH
         for (int y = 0; y < 3; y++)
 #pragma unroll
             for (int x = 0; x < 3; x++) {
-                const int X = x == 0 ? Im : (x == 1 ? I : Ip);
-                const int Y = y == 0 ? Jm : (y == 1 ? J : Jp);
-                const int Z = z == 0 ? Km : (z == 1 ? K : Kp);
-                const size_t qq = split_index(gc, X, Y, Z);
+                const unsigned qq = sidx(gc, Xs[x], Ys[y], Zs[z]);
                 e[x + 3 * y + 9 * z] = dH[qq] - rd[qq];
             }
 #pragma unroll
@@ -609,7 +608,7 @@ __global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH
             const int z = (ind & 4) ? 1 : (s3 ? 2 : 0);
             P += wt[ind] * e[x + 3 * y + 9 * z];
         }
-        const size_t fi = split_index(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3);
+        const unsigned fi = sidx(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3);
         const float dn = df[fi] + P;
         if (wpow<FRN>(xbf[fi]) + dn > 0.0f) df[fi] = dn;
     }
@@ -621,7 +620,6 @@ template <int FRN>
 __global__ __launch_bounds__(1024) void k_df_coarsest(float* __restrict__ d, const float* __restrict__ xb,
                                                       const float* __restrict__ rhs, Geo g, int sweeps, DfCoef c)
 {
-    const int ncell = (int)(2 * g.half);
     for (int sw = 0; sw < sweeps; sw++) {
         for (int colour = 0; colour < 2; colour++) {
             for (int t = threadIdx.x; t < (int)g.half; t += blockDim.x) {
@@ -650,32 +648,6 @@ __global__ __launch_bounds__(1024) void k_df_coarsest(float* __restrict__ d, con
             __syncthreads();
         }
     }
-    (void)ncell;
-}
-
-// FP64 outer residual r = N(u) (the plain path's residual), emitting the
-// inner problem's data: g = -r (FP32) and xb = u / u_bg (FP32)
-template <int FRN>
-__global__ __launch_bounds__(256) void k_outer_residual(const double* __restrict__ u, const double* __restrict__ delta,
-                                                        float* __restrict__ g32, float* __restrict__ xb,
-                                                        double* __restrict__ partials, Geo g, FrCoef c, int TX, int TY,
-                                                        int XB, unsigned NB)
-{
-    int m;
-    unsigned row;
-    double acc = 0.0;
-    if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
-        const unsigned nn = (unsigned)g.n * (unsigned)g.n;
-        const int col = row >= nn ? 1 : 0;
-        const unsigned r2 = row - (unsigned)col * nn;
-        const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
-        const Nb q = neighbours(g, col, m, j, k);
-        const double OP = fr_res_cell<FRN>(u, q, delta[q.me], c);
-        g32[q.me] = (float)(-OP);
-        xb[q.me] = (float)(u[q.me] / c.ubg);
-        acc = OP * OP;
-    }
-    store_partial(partials, NB, block_sum(acc));
 }
 
 // u += u_bg (x(d) - xb), evaluated in FP64 as u_bg d / dden(x, xb)
@@ -697,16 +669,32 @@ __global__ __launch_bounds__(256) void k_outer_update(double* __restrict__ u, co
 // launchers
 // ===========================================================================
 #define LAUNCH_ARGS L.TX, L.TY, L.XB, L.NB
+#define ZARGS L.TX, L.TY, L.XB, L.JB, L.KC, L.NB
 
-#define FRN_DISPATCH(frn, KERNEL, ...)                                     \
-    do {                                                                   \
-        if ((frn) == 0)                                                    \
-            hipLaunchKernelGGL((KERNEL<0>), L.grid, L.block, 0, s, __VA_ARGS__); \
-        else if ((frn) == 1)                                               \
-            hipLaunchKernelGGL((KERNEL<1>), L.grid, L.block, 0, s, __VA_ARGS__); \
-        else                                                               \
-            hipLaunchKernelGGL((KERNEL<2>), L.grid, L.block, 0, s, __VA_ARGS__); \
+#define FRN_DISPATCH(frn, KERNEL, ...)                                                   \
+    do {                                                                                 \
+        if ((frn) == 0)                                                                  \
+            hipLaunchKernelGGL((KERNEL<0>), L.grid, L.block, 0, s, __VA_ARGS__);         \
+        else if ((frn) == 1)                                                             \
+            hipLaunchKernelGGL((KERNEL<1>), L.grid, L.block, 0, s, __VA_ARGS__);         \
+        else                                                                             \
+            hipLaunchKernelGGL((KERNEL<2>), L.grid, L.block, 0, s, __VA_ARGS__);         \
     } while (0)
+
+template <class OP>
+static void colour_pass(const OP& op, const Geo& g, int colour, hipStream_t s)
+{
+    const ZLaunch L = make_zlaunch(g);
+    hipLaunchKernelGGL((k_colour_pass<OP>), L.grid, L.block, 0, s, op, g, colour, ZARGS);
+}
+
+template <class OP>
+static int both_colours(const OP& op, double* partials, const Geo& g, hipStream_t s)
+{
+    const ZLaunch L = make_zlaunch(g);
+    hipLaunchKernelGGL((k_both_colours<OP>), L.grid, L.block, 0, s, op, partials, g, ZARGS);
+    return (int)L.NB;
+}
 
 void launch_pack_f64_from_f64(const double* lex, double* split, int n, hipStream_t s)
 {
@@ -763,9 +751,7 @@ void launch_fr_smooth64(int frn, int level, double* u, const double* rho, const 
                         const FrCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    const Launch L = make_launch(g.h, (long)n * n);
-#define SM(FRN, CO, RR) \
-    hipLaunchKernelGGL((k_fr_smooth64<FRN, CO, RR>), L.grid, L.block, 0, s, u, rho, f, g, colour, c, LAUNCH_ARGS)
+#define SM(FRN, CO, RR) colour_pass(FrSmoothOp<FRN, CO, RR>{u, rho, f, c}, g, colour, s)
 #define SM2(FRN, CO)            \
     do {                        \
         if (c.reference_roots)  \
@@ -790,20 +776,16 @@ int launch_fr_residual64(int frn, int level, const double* u, const double* rho,
                          double* partials, int n, const FrCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    const Launch L = make_launch(g.h, 2L * n * n);
-#define RS(FRN, CO) \
-    hipLaunchKernelGGL((k_fr_residual64<FRN, CO>), L.grid, L.block, 0, s, u, rho, f, res, partials, g, c, LAUNCH_ARGS)
+#define RS(FRN, CO) both_colours(FrResOp<FRN, CO>{u, rho, f, res, c}, partials, g, s)
     if (level == 0) {
-        if (frn == 0) RS(0, false);
-        else if (frn == 1) RS(1, false);
-        else RS(2, false);
-    } else {
-        if (frn == 0) RS(0, true);
-        else if (frn == 1) RS(1, true);
-        else RS(2, true);
+        if (frn == 0) return RS(0, false);
+        if (frn == 1) return RS(1, false);
+        return RS(2, false);
     }
+    if (frn == 0) return RS(0, true);
+    if (frn == 1) return RS(1, true);
+    return RS(2, true);
 #undef RS
-    return (int)L.NB;
 }
 
 void launch_fr_restrict_residual1_64(int frn, const double* u0, const double* delta, double* rres1, int N,
@@ -832,29 +814,40 @@ void launch_fr_rhs64(int frn, const double* ru, const double* rho, const double*
                      const FrCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    const Launch L = make_launch(g.h, 2L * n * n);
-    FRN_DISPATCH(frn, k_fr_rhs64, ru, rho, rres, f, g, c, LAUNCH_ARGS);
+    if (frn == 0) both_colours(FrResOp<0, true>{ru, rho, rres, f, c}, nullptr, g, s);
+    else if (frn == 1) both_colours(FrResOp<1, true>{ru, rho, rres, f, c}, nullptr, g, s);
+    else both_colours(FrResOp<2, true>{ru, rho, rres, f, c}, nullptr, g, s);
 }
 
 void launch_correct64(const double* uH, const double* ru, double* ufine, int nc, int guard, hipStream_t s)
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
     const Launch L = make_launch(nc, (long)nc * nc);
-    hipLaunchKernelGGL((k_correct<double, double>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, LAUNCH_ARGS);
+    hipLaunchKernelGGL(k_correct64, L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, LAUNCH_ARGS);
 }
 
+// partials[0..count) -> out[0]; stage 1 writes its chunk sums after the
+// partials (the buffer holds count + 1024 doubles)
 void launch_sum_partials(const double* partials, int count, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, partials, count, out);
+    if (count <= 8192) {
+        hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(1024), 0, s, partials, count, count, out);
+        return;
+    }
+    const int nchunk = 1024;
+    const int chunk = (count + nchunk - 1) / nchunk;
+    double* mid = const_cast<double*>(partials) + count;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(nchunk), dim3(1024), 0, s, partials, count, chunk, mid);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(1024), 0, s, (const double*)mid, nchunk, nchunk, out);
 }
 
 int launch_outer_residual(int frn, const double* u, const double* delta, float* g32, float* xb, double* partials,
                           int n, const FrCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    const Launch L = make_launch(g.h, 2L * n * n);
-    FRN_DISPATCH(frn, k_outer_residual, u, delta, g32, xb, partials, g, c, LAUNCH_ARGS);
-    return (int)L.NB;
+    if (frn == 0) return both_colours(FrOuterOp<0>{u, delta, g32, xb, c}, partials, g, s);
+    if (frn == 1) return both_colours(FrOuterOp<1>{u, delta, g32, xb, c}, partials, g, s);
+    return both_colours(FrOuterOp<2>{u, delta, g32, xb, c}, partials, g, s);
 }
 
 void launch_outer_update(int frn, double* u, const float* d, const float* xb, int n, double ubg, hipStream_t s)
@@ -877,16 +870,18 @@ void launch_df_smooth(int frn, float* d, const float* xb, const float* rhs, int 
                       hipStream_t s)
 {
     const Geo g = make_geo(n);
-    const Launch L = make_launch(g.h, (long)n * n);
-    FRN_DISPATCH(frn, k_df_smooth, d, xb, rhs, g, colour, c, LAUNCH_ARGS);
+    if (frn == 0) colour_pass(DfSmoothOp<0>{d, xb, rhs, c}, g, colour, s);
+    else if (frn == 1) colour_pass(DfSmoothOp<1>{d, xb, rhs, c}, g, colour, s);
+    else colour_pass(DfSmoothOp<2>{d, xb, rhs, c}, g, colour, s);
 }
 
 void launch_df_residual(int frn, const float* d, const float* xb, const float* rhs, float* res, int n,
                         const DfCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    const Launch L = make_launch(g.h, 2L * n * n);
-    FRN_DISPATCH(frn, k_df_residual, d, xb, rhs, res, g, c, LAUNCH_ARGS);
+    if (frn == 0) both_colours(DfResOp<0>{d, xb, rhs, res, c}, nullptr, g, s);
+    else if (frn == 1) both_colours(DfResOp<1>{d, xb, rhs, res, c}, nullptr, g, s);
+    else both_colours(DfResOp<2>{d, xb, rhs, res, c}, nullptr, g, s);
 }
 
 void launch_df_restrict_residual1(int frn, const float* d0, const float* xb0, const float* g0, float* rres1, int N,
@@ -900,9 +895,7 @@ void launch_df_restrict_residual1(int frn, const float* d0, const float* xb0, co
 void launch_df_rhs(int frn, const float* rd, const float* xb, const float* rres, float* f, int n, const DfCoef& c,
                    hipStream_t s)
 {
-    const Geo g = make_geo(n);
-    const Launch L = make_launch(g.h, 2L * n * n);
-    FRN_DISPATCH(frn, k_df_rhs, rd, xb, rres, f, g, c, LAUNCH_ARGS);
+    launch_df_residual(frn, rd, xb, rres, f, n, c, s);
 }
 
 void launch_df_correct(int frn, const float* dH, const float* rd, float* dfine, const float* xbfine, int nc,
