@@ -131,6 +131,13 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    # The JSON line goes to the original stdout; everything else written to
+    # fd 1 (the Fortran runtime of the CPU baseline prints there and flushes
+    # at exit) is sent to stderr so that the JSON line stays last.
+    json_out = os.fdopen(os.dup(1), "w")
+    sys.stdout.flush()
+    os.dup2(2, 1)
+
     N, box, fr0, sigma, seed = WORKLOADS[args.workload]
     log(f"[bench] workload {args.workload}: N={N} box={box} fR0={fr0} sigma_g={sigma} seed={seed}")
     Smix = make_solver(N, box, fr0, M.PREC_MIXED)
@@ -212,7 +219,7 @@ def main():
         "fp64": fp64,
         "speedup_mixed_vs_fp64": (fp64["ms_median"] / ms_step) if fp64 else None,
         "roofline": {
-            "kernel": "level-0 smoother colour pass (mixed, FP32 delta form: k_df_smooth<1>)",
+            "kernel": "level-0 smoother colour pass (mixed, FP32 delta form: k_colour_pass<DfSmoothOp<1>>)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -225,7 +232,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    json_out.write(json.dumps(out) + "\n")
+    json_out.flush()
 
 
 if __name__ == "__main__":

@@ -443,12 +443,17 @@ int vcycle_plain(mgs_handle* h, int l)
 // ---------------------------------------------------------------------------
 // mixed precision: FP32 delta-form FAS V-cycle (inner) and FP64 outer loop
 // ---------------------------------------------------------------------------
-static int vcycle_inner(mgs_handle* h, int l)
+// One inner V(2,2) cycle at level l.  On level 0, `first` means d was set by
+// the fused outer residual (colour 0 already relaxed once from zero: that
+// pass is skipped) and `last` fuses the final colour-1 pass with the outer
+// update of u.
+static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = false)
 {
     const int Nsmth = 2;
     const int frn = h->p.fr_n;
     LevelF& v = h->M[l];
     const DfCoef dc = df_coefs(h->p, l);
+    const int cls = l == 0 ? MGS_KC_SMOOTH : MGS_KC_COARSE;
     if (l >= h->lc) {
         prof_begin(h, MGS_KC_COARSE, 3.0 * n3d(v.n) * 4.0);
         launch_df_coarsest(frn, v.d, v.xb, v.f, v.n, 8, dc, h->s);
@@ -456,23 +461,15 @@ static int vcycle_inner(mgs_handle* h, int l)
         return 0;
     }
     const double s = 4.0;
-    for (int it = 0; it < 2 * Nsmth; it++) {
-        if (l == 0) prof_begin(h, MGS_KC_SMOOTH, 2.0 * n3d(v.n) * s);
-        else prof_begin(h, MGS_KC_COARSE, 2.0 * n3d(v.n) * s);
+    for (int it = (l == 0 && first) ? 1 : 0; it < 2 * Nsmth; it++) {
+        prof_begin(h, cls, 2.0 * n3d(v.n) * s);
         launch_df_smooth(frn, v.d, v.xb, v.f, v.n, (it & 1) ? 1 : 0, dc, h->s);
         prof_end(h);
     }
     LevelF& c = h->M[l + 1];
-    if (l == 0) {
-        prof_begin(h, MGS_KC_RESTRICT, (3.0 * n3d(v.n) + n3d(c.n)) * s);
-        launch_df_restrict_residual1(frn, v.d, v.xb, v.f, c.rres, v.n, dc, h->s);
-        prof_end(h);
-    } else {
-        prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(v.n) * s);
-        launch_df_residual(frn, v.d, v.xb, v.f, v.res, v.n, dc, h->s);
-        launch_restrict8_32(v.res, c.rres, nullptr, c.n, h->s);
-        prof_end(h);
-    }
+    prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (3.0 * n3d(v.n) + n3d(c.n)) * s);
+    launch_df_restrict_residual(frn, v.d, v.xb, v.f, c.rres, v.n, dc, h->s);
+    prof_end(h);
     prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(v.n) + 2.0 * n3d(c.n)) * s);
     launch_restrict8_32(v.d, c.rd, c.d, c.n, h->s);
     prof_end(h);
@@ -491,21 +488,30 @@ static int vcycle_inner(mgs_handle* h, int l)
     prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * s);
     launch_df_correct(frn, c.d, c.rd, v.d, v.xb, c.n, h->s);
     prof_end(h);
-    for (int it = 0; it < 2 * Nsmth; it++) {
-        if (l == 0) prof_begin(h, MGS_KC_SMOOTH, 2.0 * n3d(v.n) * s);
-        else prof_begin(h, MGS_KC_COARSE, 2.0 * n3d(v.n) * s);
+    const int npost = (l == 0 && last) ? 2 * Nsmth - 1 : 2 * Nsmth;
+    for (int it = 0; it < npost; it++) {
+        prof_begin(h, cls, 2.0 * n3d(v.n) * s);
         launch_df_smooth(frn, v.d, v.xb, v.f, v.n, (it & 1) ? 1 : 0, dc, h->s);
+        prof_end(h);
+    }
+    if (l == 0 && last) {
+        // reads d (colour 0), xb, g (colour 1), reads and writes u: 24 B/cell
+        prof_begin(h, MGS_KC_UPDATE, 24.0 * n3d(v.n));
+        launch_df_smooth_update(frn, v.d, v.xb, v.f, h->L[0].u, v.n, dc, fr_coefs(h->p, 0).ubg, h->s);
         prof_end(h);
     }
     return 0;
 }
 
+// FP64 outer residual; also writes the inner problem's g, xb and d after its
+// first colour-0 pass (reads u, delta; writes g, xb, d: 28 B/cell)
 static int outer_residual(mgs_handle* h, double* rms)
 {
     LevelD& v = h->L[0];
     LevelF& m = h->M[0];
-    prof_begin(h, MGS_KC_RESIDUAL, 24.0 * n3d(v.n));
-    const int nb = launch_outer_residual(h->p.fr_n, v.u, v.rho, m.f, m.xb, h->partials, v.n, fr_coefs(h->p, 0), h->s);
+    prof_begin(h, MGS_KC_RESIDUAL, 28.0 * n3d(v.n));
+    const int nb = launch_outer_residual(h->p.fr_n, v.u, v.rho, m.f, m.xb, m.d, h->partials, v.n, fr_coefs(h->p, 0),
+                                         df_coefs(h->p, 0), h->s);
     prof_end(h);
     launch_sum_partials(h->partials, nb, h->dsum, h->s);
     HIPCHK(h, hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s));
@@ -602,14 +608,12 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
             launch_restrict_base(frn, h->M[l - 1].xb, h->M[l].xb, h->M[l].n, h->s);
             prof_end(h);
         }
-        launch_fill_f32(h->M[0].d, (size_t)n3d(h->M[0].n), 0.0f, h->s);
+        // d = 0 with its first colour-0 pass came from the outer residual;
+        // the last inner pass also applies u += u_bg (x(d) - xb)
         for (int ic = 0; ic < inner; ic++) {
-            vcycle_inner(h, 0);
+            vcycle_inner(h, 0, ic == 0, ic == inner - 1);
             cycles++;
         }
-        prof_begin(h, MGS_KC_UPDATE, 24.0 * n3d(h->L[0].n));
-        launch_outer_update(frn, h->L[0].u, h->M[0].d, h->M[0].xb, h->L[0].n, fr_coefs(h->p, 0).ubg, h->s);
-        prof_end(h);
     }
     rep->cycles = cycles;
     rep->converged = conv ? 1 : 0;

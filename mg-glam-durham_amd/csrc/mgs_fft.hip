@@ -25,7 +25,11 @@ struct FftState {
     double* real = nullptr;          // N^3 lexicographic
     hipfftDoubleComplex* cplx = nullptr;  // N*N*(N/2+1)
     double* bins = nullptr;          // 3 * nb: power, count, sum|m|
+    double* binpart = nullptr;       // BIN_BLOCKS * 3 * nb per-block histograms
 };
+
+// blocks of the binning kernel: each accumulates a private histogram in LDS
+static const int BIN_BLOCKS = 512;
 
 static FftState* fft_of(mgs_handle* h) { return (FftState*)h->fft; }
 
@@ -42,6 +46,8 @@ static int fft_ensure(mgs_handle* h)
     if (hipMalloc((void**)&f->cplx, nc * sizeof(hipfftDoubleComplex)) != hipSuccess)
         return set_err(h, MGS_E_NOMEM, "FFT buffer");
     if (hipMalloc((void**)&f->bins, 3 * (size_t)(N + 2) * sizeof(double)) != hipSuccess)
+        return set_err(h, MGS_E_NOMEM, "FFT bins");
+    if (hipMalloc((void**)&f->binpart, (size_t)BIN_BLOCKS * 3 * (N + 2) * sizeof(double)) != hipSuccess)
         return set_err(h, MGS_E_NOMEM, "FFT bins");
     if (hipfftPlan3d(&f->fwd, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftPlan3d D2Z");
     f->have_fwd = true;
@@ -61,6 +67,7 @@ void fft_release(mgs_handle* h)
     if (f->real) hipFree(f->real);
     if (f->cplx) hipFree(f->cplx);
     if (f->bins) hipFree(f->bins);
+    if (f->binpart) hipFree(f->binpart);
     delete f;
     h->fft = nullptr;
 }
@@ -186,11 +193,14 @@ __global__ __launch_bounds__(256) void k_fifth_force(const double* __restrict__ 
     }
 }
 
-// power per half-spectrum entry into NGP bins: bins[0..nb) power,
-// [nb..2nb) mode count, [2nb..3nb) sum of |m|
+// power per half-spectrum entry into NGP bins, one private LDS histogram per
+// block (3 nb doubles: power, mode count, sum of |m|) written to part[block]
 __global__ __launch_bounds__(256) void k_bin_power(const hipfftDoubleComplex* __restrict__ c, int N, double norm,
-                                                   double* __restrict__ bins, int nb)
+                                                   double* __restrict__ part, int nb)
 {
+    extern __shared__ double hist[];
+    for (int t = threadIdx.x; t < 3 * nb; t += blockDim.x) hist[t] = 0.0;
+    __syncthreads();
     const int nh = N / 2 + 1;
     const size_t total = (size_t)N * N * nh;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -206,10 +216,24 @@ __global__ __launch_bounds__(256) void k_bin_power(const hipfftDoubleComplex* __
         if (mp >= nb) continue;
         const double w = (i == 0 || 2 * i == N) ? 1.0 : 2.0;  // conjugate half
         const double pw = (c[q].x * c[q].x + c[q].y * c[q].y) * norm;
-        atomicAdd(&bins[mp], w * pw);
-        atomicAdd(&bins[nb + mp], w);
-        atomicAdd(&bins[2 * nb + mp], w * rk);
+        atomicAdd(&hist[mp], w * pw);
+        atomicAdd(&hist[nb + mp], w);
+        atomicAdd(&hist[2 * nb + mp], w * rk);
     }
+    __syncthreads();
+    double* out = part + (size_t)blockIdx.x * 3 * nb;
+    for (int t = threadIdx.x; t < 3 * nb; t += blockDim.x) out[t] = hist[t];
+}
+
+// bins[t] = sum over blocks of part[block][t], in block order
+__global__ __launch_bounds__(256) void k_bin_reduce(const double* __restrict__ part, int nblk, int len,
+                                                    double* __restrict__ bins)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= len) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(size_t)b * len + t];
+    bins[t] = s;
 }
 
 static dim3 grid_for(size_t count)
@@ -362,11 +386,11 @@ int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, doub
     }
     if (hipfftExecD2Z(f->fwd, f->real, f->cplx) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecD2Z");
     const int nb = N + 1;
-    if (hipMemsetAsync(f->bins, 0, 3 * (size_t)nb * sizeof(double), h->s) != hipSuccess)
-        return set_err(h, MGS_E_HIP, "bins");
     const double norm = 1.0 / ((double)n3 * (double)n3);
-    hipLaunchKernelGGL(k_bin_power, grid_for((size_t)N * N * (N / 2 + 1)), dim3(256), 0, h->s, f->cplx, N, norm,
-                       f->bins, nb);
+    hipLaunchKernelGGL(k_bin_power, dim3(BIN_BLOCKS), dim3(256), 3 * (size_t)nb * sizeof(double), h->s, f->cplx, N,
+                       norm, f->binpart, nb);
+    hipLaunchKernelGGL(k_bin_reduce, dim3((3 * nb + 255) / 256), dim3(256), 0, h->s, (const double*)f->binpart,
+                       BIN_BLOCKS, 3 * nb, f->bins);
     std::vector<double> hb(3 * (size_t)nb);
     if (hipMemcpyAsync(hb.data(), f->bins, hb.size() * sizeof(double), hipMemcpyDeviceToHost, h->s) != hipSuccess ||
         hipStreamSynchronize(h->s) != hipSuccess)

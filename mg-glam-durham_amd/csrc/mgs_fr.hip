@@ -87,35 +87,41 @@ __device__ __forceinline__ float fast_cbrt(float t)
     return copysignf(r, t);
 }
 
+// hardware square root and reciprocal (about 1 ulp): the seed only has to
+// be close, the Newton steps that follow set the accuracy
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <int FRN>
 __device__ __forceinline__ float seed_root(float P, float Q)
 {
     if constexpr (FRN == 0) {
-        const float sq = __builtin_sqrtf(fmaxf(P * P - 4.0f * Q, 0.0f));
-        return P > 0.0f ? (-2.0f * Q) / (P + sq) : 0.5f * (sq - P);
+        const float sq = hw_sqrt(fmaxf(P * P - 4.0f * Q, 0.0f));
+        return P > 0.0f ? (-2.0f * Q) * hw_rcp(P + sq) : 0.5f * (sq - P);
     } else if constexpr (FRN == 1) {
         const float aD1 = -27.0f * Q;
         const float CC = aD1 * aD1 + 108.0f * P * P * P;
         if (CC >= 0.0f) {
-            const float sq = __builtin_sqrtf(CC);
+            const float sq = hw_sqrt(CC);
             const float t2 = 0.5f * (sq + aD1);
-            const float t1 = 54.0f * P * P * P / (sq + aD1);
+            const float t1 = 54.0f * P * P * P * hw_rcp(sq + aD1);
             const float a = fast_cbrt(t2), b = fast_cbrt(t1);
-            if (t1 > 0.0f) return aD1 / (3.0f * (a * a + a * b + b * b));
+            if (t1 > 0.0f) return aD1 * hw_rcp(3.0f * (a * a + a * b + b * b));
             return (a - b) * (1.0f / 3.0f);
         }
-        const float sd = __builtin_sqrtf(-3.0f * P);
-        const float arg = fminf(fmaxf(-0.5f * aD1 / (sd * sd * sd), -1.0f), 1.0f);
+        const float sd = hw_sqrt(-3.0f * P);
+        const float arg = fminf(fmaxf(-0.5f * aD1 * hw_rcp(sd * sd * sd), -1.0f), 1.0f);
         return -(2.0f / 3.0f) * sd * __cosf(acosf(arg) * (1.0f / 3.0f) + 2.0943951023931955f);
     } else {
-        if (P > 0.0f) return fminf(-Q / P, __builtin_sqrtf(__builtin_sqrtf(-Q)));
-        if (P == 0.0f) return __builtin_sqrtf(__builtin_sqrtf(-Q));
+        const float q4 = hw_sqrt(hw_sqrt(-Q));
+        if (P > 0.0f) return fminf(-Q * hw_rcp(P), q4);
+        if (P == 0.0f) return q4;
         const float D20 = 12.0f * Q;
         const float D21 = 27.0f * P * P;
-        const float t1 = 0.5f * (__builtin_sqrtf(fmaxf(D21 * D21 - 4.0f * D20 * D20 * D20, 0.0f)) + D21);
+        const float t1 = 0.5f * (hw_sqrt(fmaxf(D21 * D21 - 4.0f * D20 * D20 * D20, 0.0f)) + D21);
         const float cb = fast_cbrt(t1);
-        const float S = __builtin_sqrtf(fmaxf(cb + D20 / cb, 0.0f)) * 0.28867513459481287f;
-        return S + 0.5f * __builtin_sqrtf(fmaxf(-4.0f * S * S - P / S, 0.0f));
+        const float S = hw_sqrt(fmaxf(cb + D20 * hw_rcp(cb), 0.0f)) * 0.28867513459481287f;
+        return S + 0.5f * hw_sqrt(fmaxf(-4.0f * S * S - P * hw_rcp(S), 0.0f));
     }
 }
 
@@ -128,7 +134,11 @@ __device__ __forceinline__ double rcp64(double x)
     return __builtin_fma(r, e, r);
 }
 
-// Newton's method in FP64 on u^{n+2} + P u + Q = 0 from u, to |step| <= 1e-15 u
+// Newton's method in FP64 on u^{n+2} + P u + Q = 0 from u.  With Q < 0 the
+// function is convex for u > 0 and negative at 0, so its derivative at the
+// positive root is at least -Q/u > 0 and the iteration converges
+// quadratically with an O(1) constant in the normalised variable: once a
+// step is below 1e-11 u the error left is ~1e-22 u, so it stops there.
 template <int FRN>
 __device__ __forceinline__ double newton_root64(double u, double P, double Q)
 {
@@ -142,7 +152,7 @@ __device__ __forceinline__ double newton_root64(double u, double P, double Q)
         const double u1 = u - d;
         if (!(u1 > 0.0)) break;
         u = u1;
-        if (fabs(d) <= 1.0e-15 * u) break;
+        if (fabs(d) <= 1.0e-11 * u) break;
     }
     return u;
 }
@@ -164,25 +174,36 @@ __device__ __forceinline__ unsigned sidx(const Geo& g, int i, int j, int k)
 template <int FRN, bool COARSE, bool REFROOT>
 struct FrSmoothOp {
     using T = double;
+    struct Own {
+        double rho, f;
+    };
     double* u;
     const double* rho;
     const double* f;
     FrCoef c;
     __device__ double* field() const { return u; }
-    __device__ __forceinline__ void update(unsigned o, double xp, double xm, double yp, double ym, double zp,
-                                           double zm) const
+    __device__ __forceinline__ Own load(unsigned o, unsigned) const
+    {
+        Own w;
+        w.rho = rho[o];
+        w.f = COARSE ? f[o] : 0.0;
+        return w;
+    }
+    __device__ __forceinline__ void update(unsigned o, unsigned, const Own& w, double, double xp, double xm,
+                                           double yp, double ym, double zp, double zm) const
     {
         const double S =
             wpow<FRN>(xp) + wpow<FRN>(xm) + wpow<FRN>(yp) + wpow<FRN>(ym) + wpow<FRN>(zp) + wpow<FRN>(zm);
         double P;
         if constexpr (!COARSE)
-            P = c.fct1s * rho[o] - c.Q2 - S / 6.0;
+            P = c.fct1s * w.rho - c.Q2 - S / 6.0;
         else
-            P = c.fct1s * rho[o] - c.Q2 + c.fct2s * f[o] - S / 6.0;
+            P = c.fct1s * w.rho - c.Q2 + c.fct2s * w.f - S / 6.0;
         if constexpr (REFROOT) {
             u[o] = fr_root64<FRN>(P, c);
         } else {
-            const float x = seed_root<FRN>((float)(P * c.inv_w), (float)(c.Q2 * c.inv_w));
+            float x = seed_root<FRN>((float)(P * c.inv_w), (float)(c.Q2 * c.inv_w));
+            if (!(x > 0.0f) || !(x < 3.0e38f)) x = 1.0f;  // Newton from u_bg
             u[o] = newton_root64<FRN>((double)x * c.ubg, P, c.Q);
         }
     }
@@ -194,91 +215,135 @@ struct FrSmoothOp {
 template <int FRN, bool COARSE>
 struct FrResOp {
     using T = double;
+    struct Own {
+        double rho, f;
+    };
     const double* u;
     const double* rho;
     const double* f;
     double* out;
     FrCoef c;
     __device__ const double* field() const { return u; }
-    __device__ __forceinline__ double cell(unsigned o, double uc, double xp, double xm, double yp, double ym,
-                                           double zp, double zm) const
+    __device__ __forceinline__ Own load(unsigned o) const
+    {
+        Own w;
+        w.rho = rho[o];
+        w.f = COARSE ? f[o] : 0.0;
+        return w;
+    }
+    __device__ __forceinline__ double cell(unsigned o, const Own& w, double uc, double xp, double xm, double yp,
+                                           double ym, double zp, double zm) const
     {
         const double S =
             wpow<FRN>(xp) + wpow<FRN>(xm) + wpow<FRN>(yp) + wpow<FRN>(ym) + wpow<FRN>(zp) + wpow<FRN>(zm);
         double OP = S - wpow<FRN>(uc) * 6.0;
         OP = OP * c.fct3r;
-        OP = OP + c.fct1r * rho[o];
+        OP = OP + c.fct1r * w.rho;
         OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
-        if constexpr (COARSE) OP = OP - f[o];
+        if constexpr (COARSE) OP = OP - w.f;
         if (out) out[o] = OP;
         return OP;
     }
 };
 
-// FP64 outer residual r = N(u) of the mixed loop, emitting the inner
-// problem's data: g = -r (FP32) and xb = u / u_bg (FP32)
-template <int FRN>
-struct FrOuterOp {
-    using T = double;
-    const double* u;
-    const double* delta;
-    float* g;
-    float* xb;
-    FrCoef c;
-    __device__ const double* field() const { return u; }
-    __device__ __forceinline__ double cell(unsigned o, double uc, double xp, double xm, double yp, double ym,
-                                           double zp, double zm) const
+// Fine cells seen from coarse column (I, J) in one fine plane z: the 2x2
+// children (x0,y0), (x1,y0), (x0,y1), (x1,y1) and the 8 face neighbours
+// (xl,y0), (xl,y1), (xu,y0), (xu,y1), (x0,yl), (x1,yl), (x0,yu), (x1,yu),
+// kept as colour parity and in-plane offset so that a plane loads with one
+// add per value
+struct Pts12 {
+    unsigned par[12], row[12], plane, half;
+    __device__ __forceinline__ void init(const Geo& gf, int I, int J)
     {
-        const double S =
-            wpow<FRN>(xp) + wpow<FRN>(xm) + wpow<FRN>(yp) + wpow<FRN>(ym) + wpow<FRN>(zp) + wpow<FRN>(zm);
-        double OP = S - wpow<FRN>(uc) * 6.0;
-        OP = OP * c.fct3r;
-        OP = OP + c.fct1r * delta[o];
-        OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
-        g[o] = (float)(-OP);
-        xb[o] = (float)(uc / c.ubg);
-        return OP;
+        const int N = gf.n;
+        const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1;
+        const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
+        const int yl = (J == 0) ? N - 1 : 2 * J - 1, yu = (2 * J + 2 == N) ? 0 : 2 * J + 2;
+        const int X[12] = {x0, x1, x0, x1, xl, xl, xu, xu, x0, x1, x0, x1};
+        const int Y[12] = {y0, y0, y1, y1, y0, y1, y0, y1, yl, yl, yu, yu};
+#pragma unroll
+        for (int q = 0; q < 12; q++) {
+            par[q] = (unsigned)((X[q] + Y[q]) & 1);
+            row[q] = (unsigned)Y[q] * (unsigned)gf.h + (unsigned)(X[q] >> 1);
+        }
+        plane = (unsigned)gf.n * (unsigned)gf.h;
+        half = (unsigned)gf.half;
+    }
+    __device__ __forceinline__ unsigned at(int q, int z) const
+    {
+        return ((par[q] + (unsigned)z) & 1u) * half + (unsigned)z * plane + row[q];
     }
 };
 
 // restrict_residual_fR(1): the 8 children's residuals averaged, recomputed
-// from u and delta, PMP2MGsolver_fR.f90:607-749
+// from u and delta, PMP2MGsolver_fR.f90:607-749.  A thread owns coarse column
+// (I, J) and marches over K; fine plane 2K+2 (the upper z faces of K) is the
+// lower child plane of K+1 and plane 2K-1 is carried from K-1, so u and delta
+// are read once.  The operation order is the reference's.
 template <int FRN>
 __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* __restrict__ u0,
                                                                   const double* __restrict__ delta,
                                                                   double* __restrict__ rres1, Geo gf, Geo gc,
-                                                                  FrCoef c, int TX, int TY, int XB, unsigned NB)
+                                                                  FrCoef c, int TX, int TY, int XB, int JB, int KC,
+                                                                  unsigned NB)
 {
-    int I;
-    unsigned row;
-    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
-    const int N = gf.n;
-    const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1, z0 = 2 * K, z1 = 2 * K + 1;
-    const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
-    const int yl = (J == 0) ? N - 1 : 2 * J - 1, yu = (2 * J + 2 == N) ? 0 : 2 * J + 2;
-    const int zl = (K == 0) ? N - 1 : 2 * K - 1, zu = (2 * K + 2 == N) ? 0 : 2 * K + 2;
-    const unsigned ch[8] = {sidx(gf, x0, y0, z0), sidx(gf, x1, y0, z0), sidx(gf, x0, y1, z0), sidx(gf, x1, y1, z0),
-                            sidx(gf, x0, y0, z1), sidx(gf, x1, y0, z1), sidx(gf, x0, y1, z1), sidx(gf, x1, y1, z1)};
-    double uc[8];
+    ZCol cc;
+    if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
+    const int I = cc.m, J = cc.j, N = gf.n;
+    Pts12 F;
+    F.init(gf, I, J);
+    // per fine plane: u^{n+1}, fct1 delta and fct4/u of the 4 children, and
+    // u^{n+1} of the 8 face neighbours
+    struct Pl {
+        double w[4], t1[4], t2[4], f[8];
+    };
+    auto load = [&](int z, Pl& p) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) uc[q] = u0[ch[q]];
-    double Q = -8.0 * c.fct2r;
+        for (int q = 0; q < 4; q++) {
+            const unsigned o = F.at(q, z);
+            const double u = u0[o];
+            p.w[q] = wpow<FRN>(u);
+            p.t1[q] = c.fct1r * delta[o];
+            p.t2[q] = c.fct4_1 / u;
+        }
 #pragma unroll
-    for (int q = 0; q < 8; q++) Q = Q + c.fct1r * delta[ch[q]] + c.fct4_1 / uc[q];
-    double s8 = wpow<FRN>(uc[0]);
+        for (int q = 0; q < 8; q++) p.f[q] = wpow<FRN>(u0[F.at(4 + q, z)]);
+    };
+    Pl p0, p1, p2;
+    double wl[4];
+    const int zs = 2 * cc.k0;
+    load(zs, p0);
+    {
+        const int zl = zs == 0 ? N - 1 : zs - 1;
 #pragma unroll
-    for (int q = 1; q < 8; q++) s8 = s8 + wpow<FRN>(uc[q]);
-    double P = -3.0 * s8;
-#define W_(x, y, z) wpow<FRN>(u0[sidx(gf, x, y, z)])
-    P = P + W_(xl, y0, z0) + W_(xl, y1, z0) + W_(xl, y0, z1) + W_(xl, y1, z1);
-    P = P + W_(xu, y0, z0) + W_(xu, y1, z0) + W_(xu, y0, z1) + W_(xu, y1, z1);
-    P = P + W_(x0, yl, z0) + W_(x1, yl, z0) + W_(x0, yl, z1) + W_(x1, yl, z1);
-    P = P + W_(x0, yu, z0) + W_(x1, yu, z0) + W_(x0, yu, z1) + W_(x1, yu, z1);
-    P = P + W_(x0, y0, zl) + W_(x1, y0, zl) + W_(x0, y1, zl) + W_(x1, y1, zl);
-    P = P + W_(x0, y0, zu) + W_(x1, y0, zu) + W_(x0, y1, zu) + W_(x1, y1, zu);
-#undef W_
-    rres1[sidx(gc, I, J, K)] = (P * c.fct3_1 + Q) / 8.0;
+        for (int q = 0; q < 4; q++) wl[q] = wpow<FRN>(u0[F.at(q, zl)]);
+    }
+    for (int K = cc.k0; K < cc.k1; K++) {
+        load(2 * K + 1, p1);
+        load(2 * K + 2 == N ? 0 : 2 * K + 2, p2);
+        double Q = -8.0 * c.fct2r;
+#pragma unroll
+        for (int q = 0; q < 4; q++) Q = Q + p0.t1[q] + p0.t2[q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) Q = Q + p1.t1[q] + p1.t2[q];
+        double s8 = p0.w[0];
+        s8 = s8 + p0.w[1];
+        s8 = s8 + p0.w[2];
+        s8 = s8 + p0.w[3];
+#pragma unroll
+        for (int q = 0; q < 4; q++) s8 = s8 + p1.w[q];
+        double P = -3.0 * s8;
+        P = P + p0.f[0] + p0.f[1] + p1.f[0] + p1.f[1];  // x-1 faces
+        P = P + p0.f[2] + p0.f[3] + p1.f[2] + p1.f[3];  // x+2 faces
+        P = P + p0.f[4] + p0.f[5] + p1.f[4] + p1.f[5];  // y-1 faces
+        P = P + p0.f[6] + p0.f[7] + p1.f[6] + p1.f[7];  // y+2 faces
+        P = P + wl[0] + wl[1] + wl[2] + wl[3];          // z-1 faces
+        P = P + p2.w[0] + p2.w[1] + p2.w[2] + p2.w[3];  // z+2 faces
+        rres1[sidx(gc, I, J, K)] = (P * c.fct3_1 + Q) / 8.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) wl[q] = p1.w[q];
+        p0 = p2;
+    }
 }
 
 // 8-child average (restrict_density, restrict_scalar_field,
@@ -308,50 +373,112 @@ __global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T
     if (out2) out2[o] = v;
 }
 
-// correct_solution, PMP2extradof.f90:1361-1578: u_h += P(u_H - R u_h) with the
-// 27/9/3/1 (/64) weights; guard: keep the old value unless the new one is > 0.
-// Each coarse value of the 3x3x3 block is loaded once; the sum keeps the
-// reference's order P = P + w uH - w Ru over the 8 contributors of a son.
-__global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH, const double* __restrict__ ru,
-                                                   double* __restrict__ uf, Geo gf, Geo gc, int guard, int TX, int TY,
-                                                   int XB, unsigned NB)
-{
-    int I;
-    unsigned row;
-    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
-    const int nc = gc.n;
-    const double a = 1.0 / 64.0, b = 3.0 * a, c9 = 9.0 * a, d = 27.0 * a;
-    const double wt[8] = {a, b, b, c9, b, c9, c9, d};
-    const int Xs[3] = {wrap_dec(I, nc), I, wrap_inc(I, nc)};
-    const int Ys[3] = {wrap_dec(J, nc), J, wrap_inc(J, nc)};
-    const int Zs[3] = {wrap_dec(K, nc), K, wrap_inc(K, nc)};
-    double vh[27], vr[27];
-#pragma unroll
-    for (int z = 0; z < 3; z++)
+// Coarse 3x3 block around column (I, J) of a coarse level: the colour
+// parity and in-plane offset of each (x, y) of the block, so that a plane K
+// of the block loads with one add per value
+struct Block9 {
+    unsigned par[9], row[9], plane, half;
+    __device__ __forceinline__ void init(const Geo& gc, int I, int J)
+    {
+        const int nc = gc.n;
+        const int Xs[3] = {wrap_dec(I, nc), I, wrap_inc(I, nc)};
+        const int Ys[3] = {wrap_dec(J, nc), J, wrap_inc(J, nc)};
 #pragma unroll
         for (int y = 0; y < 3; y++)
 #pragma unroll
             for (int x = 0; x < 3; x++) {
-                const unsigned q = sidx(gc, Xs[x], Ys[y], Zs[z]);
-                vh[x + 3 * y + 9 * z] = uH[q];
-                vr[x + 3 * y + 9 * z] = ru[q];
+                par[x + 3 * y] = (unsigned)((Xs[x] + Ys[y]) & 1);
+                row[x + 3 * y] = (unsigned)Ys[y] * (unsigned)gc.h + (unsigned)(Xs[x] >> 1);
             }
+        plane = (unsigned)gc.n * (unsigned)gc.h;
+        half = (unsigned)gc.half;
+    }
+    __device__ __forceinline__ unsigned at(int q, int K) const
+    {
+        return ((par[q] + (unsigned)K) & 1u) * half + (unsigned)K * plane + row[q];
+    }
+};
+
+// offset of son (2I+s1, 2J+s2, 2K+s3) on the fine level: colour (s1+s2+s3)&1,
+// m = I
+__device__ __forceinline__ unsigned son_index(const Geo& gf, int I, int J, int K, int s1, int s2, int s3)
+{
+    return (unsigned)((s1 + s2 + s3) & 1) * (unsigned)gf.half +
+           ((unsigned)(2 * K + s3) * (unsigned)gf.n + (unsigned)(2 * J + s2)) * (unsigned)gf.h + (unsigned)I;
+}
+
+// correct_solution, PMP2extradof.f90:1361-1578: u_h += P(u_H - R u_h) with the
+// 27/9/3/1 (/64) weights; guard: keep the old value unless the new one is > 0.
+// A thread owns coarse column (I, J) and marches over K with two planes of
+// the 3x3 coarse block in registers: the sons in fine plane 2K use planes
+// K-1 and K, then plane K+1 replaces K-1 for the sons in plane 2K+1.  The sum
+// keeps the reference's order P = P + w u_H - w R u over the 8 contributors
+// of a son.
+template <int S3>
+__device__ __forceinline__ void correct64_sons(const double* hz, const double* rz, const double* hk,
+                                               const double* rk, double* __restrict__ uf, const Geo& gf, int I, int J,
+                                               int K, int guard)
+{
+    const double a = 1.0 / 64.0, b = 3.0 * a, c9 = 9.0 * a, d = 27.0 * a;
+    const double wt[8] = {a, b, b, c9, b, c9, c9, d};
+    double old[4];
 #pragma unroll
-    for (int sind = 0; sind < 8; sind++) {
-        const int s1 = sind & 1, s2 = (sind >> 1) & 1, s3 = (sind >> 2) & 1;
+    for (int t = 0; t < 4; t++) old[t] = uf[son_index(gf, I, J, K, t & 1, t >> 1, S3)];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int s1 = t & 1, s2 = t >> 1;
         double P = 0.0;
 #pragma unroll
         for (int ind = 0; ind < 8; ind++) {
             const int x = (ind & 1) ? 1 : (s1 ? 2 : 0);
             const int y = (ind & 2) ? 1 : (s2 ? 2 : 0);
-            const int z = (ind & 4) ? 1 : (s3 ? 2 : 0);
-            P = P + wt[ind] * vh[x + 3 * y + 9 * z];
-            P = P - wt[ind] * vr[x + 3 * y + 9 * z];
+            const int q = x + 3 * y;
+            // contributor in plane K (ind & 4) or in the plane beside it
+            const double vh = (ind & 4) ? hk[q] : hz[q];
+            const double vr = (ind & 4) ? rk[q] : rz[q];
+            P = P + wt[ind] * vh;
+            P = P - wt[ind] * vr;
         }
-        const unsigned fi = sidx(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3);
-        const double tmpt = uf[fi] + P;
-        if (!guard || tmpt > 0.0) uf[fi] = tmpt;
+        const double tmpt = old[t] + P;
+        if (!guard || tmpt > 0.0) uf[son_index(gf, I, J, K, s1, s2, S3)] = tmpt;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH, const double* __restrict__ ru,
+                                                   double* __restrict__ uf, Geo gf, Geo gc, int guard, int TX, int TY,
+                                                   int XB, int JB, int KC, unsigned NB)
+{
+    ZCol cc;
+    if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
+    const int I = cc.m, J = cc.j;
+    Block9 B;
+    B.init(gc, I, J);
+    double hs[9], rs[9], hk[9], rk[9];  // plane beside K (K-1, then K+1) and plane K
+    const int Km = wrap_dec(cc.k0, gc.n);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        hs[q] = uH[B.at(q, Km)];
+        rs[q] = ru[B.at(q, Km)];
+        hk[q] = uH[B.at(q, cc.k0)];
+        rk[q] = ru[B.at(q, cc.k0)];
+    }
+    for (int K = cc.k0; K < cc.k1; K++) {
+        correct64_sons<0>(hs, rs, hk, rk, uf, gf, I, J, K, guard);
+        const int Kp = wrap_inc(K, gc.n);
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            hs[q] = uH[B.at(q, Kp)];
+            rs[q] = ru[B.at(q, Kp)];
+        }
+        correct64_sons<1>(hs, rs, hk, rk, uf, gf, I, J, K, guard);
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const double th = hk[q], tr = rk[q];
+            hk[q] = hs[q];
+            rk[q] = rs[q];
+            hs[q] = th;
+            rs[q] = tr;
+        }
     }
 }
 
@@ -481,15 +608,106 @@ __device__ __forceinline__ float df_solve(float S, float xb, float r, const DfCo
 template <int FRN>
 struct DfSmoothOp {
     using T = float;
+    struct Own {
+        float xb, rhs;
+    };
     float* d;
     const float* xb;
     const float* rhs;
     DfCoef c;
     __device__ float* field() const { return d; }
-    __device__ __forceinline__ void update(unsigned o, float xp, float xm, float yp, float ym, float zp, float zm) const
+    __device__ __forceinline__ Own load(unsigned o, unsigned) const { return Own{xb[o], rhs[o]}; }
+    __device__ __forceinline__ void update(unsigned o, unsigned, const Own& w, float, float xp, float xm, float yp,
+                                           float ym, float zp, float zm) const
     {
         const float S = xp + xm + yp + ym + zp + zm;
-        d[o] = df_solve<FRN>(S, xb[o], rhs[o], c);
+        d[o] = df_solve<FRN>(S, w.xb, w.rhs, c);
+    }
+};
+
+// u + u_bg (x(d) - xb) at one cell, evaluated in FP64 as u_bg d / dden(x, xb)
+// (u unchanged where xb^{n+1} + d is not positive)
+template <int FRN>
+__device__ __forceinline__ double outer_updated(double u, float d, float xb, double ubg)
+{
+    const double dl = d, xbc = xb;
+    const double w = wpow<FRN>(xbc) + dl;
+    if (!(w > 0.0)) return u;
+    const double x = xroot<FRN>(w);
+    return u + ubg * (dl / dden<FRN>(x, xbc));
+}
+
+// the last colour-1 pass of the inner cycle fused with the outer update: the
+// new colour-1 increment goes straight into u (d itself is not needed
+// afterwards) and every colour-0 cell, visited through the window, is
+// updated from its final increment
+template <int FRN>
+struct DfSmoothUpdateOp {
+    using T = float;
+    struct Own {
+        float xb, rhs, xbo;
+        double u, uo;
+    };
+    const float* d;
+    const float* xb;
+    const float* rhs;
+    double* u;
+    DfCoef c;
+    double ubg;
+    __device__ const float* field() const { return d; }
+    __device__ __forceinline__ Own load(unsigned o, unsigned oo) const
+    {
+        Own w;
+        w.xb = xb[o];
+        w.rhs = rhs[o];
+        w.u = u[o];
+        w.xbo = xb[oo];
+        w.uo = u[oo];
+        return w;
+    }
+    __device__ __forceinline__ void update(unsigned o, unsigned oo, const Own& w, float dother, float xp, float xm,
+                                           float yp, float ym, float zp, float zm) const
+    {
+        const float S = xp + xm + yp + ym + zp + zm;
+        u[o] = outer_updated<FRN>(w.u, df_solve<FRN>(S, w.xb, w.rhs, c), w.xb, ubg);
+        u[oo] = outer_updated<FRN>(w.uo, dother, w.xbo, ubg);
+    }
+};
+
+// FP64 outer residual r = N(u) of the mixed loop (the plain path's residual),
+// emitting the inner problem's data: g = -r and xb = u / u_bg in FP32, and
+// the inner increment after its first colour-0 pass from d = 0 (where every
+// neighbour sum vanishes): d = solve(0) on colour 0, d = 0 on colour 1
+template <int FRN>
+struct FrOuterOp {
+    using T = double;
+    const double* u;
+    const double* delta;
+    float* g;
+    float* xb;
+    float* d;
+    FrCoef c;
+    DfCoef dc;
+    unsigned half;
+    struct Own {
+        double delta;
+    };
+    __device__ const double* field() const { return u; }
+    __device__ __forceinline__ Own load(unsigned o) const { return Own{delta[o]}; }
+    __device__ __forceinline__ double cell(unsigned o, const Own& w, double uc, double xp, double xm, double yp,
+                                           double ym, double zp, double zm) const
+    {
+        const double S =
+            wpow<FRN>(xp) + wpow<FRN>(xm) + wpow<FRN>(yp) + wpow<FRN>(ym) + wpow<FRN>(zp) + wpow<FRN>(zm);
+        double OP = S - wpow<FRN>(uc) * 6.0;
+        OP = OP * c.fct3r;
+        OP = OP + c.fct1r * w.delta;
+        OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
+        const float gv = (float)(-OP), xv = (float)(uc / c.ubg);
+        g[o] = gv;
+        xb[o] = xv;
+        d[o] = o < half ? df_solve<FRN>(0.0f, xv, gv, dc) : 0.0f;
+        return OP;
     }
 };
 
@@ -503,55 +721,90 @@ struct DfResOp {
     const float* rhs;
     float* out;
     DfCoef c;
+    struct Own {
+        float xb, rhs;
+    };
     __device__ const float* field() const { return d; }
-    __device__ __forceinline__ double cell(unsigned o, float dc, float xp, float xm, float yp, float ym, float zp,
-                                           float zm) const
+    __device__ __forceinline__ Own load(unsigned o) const { return Own{xb[o], rhs[o]}; }
+    __device__ __forceinline__ double cell(unsigned o, const Own& w, float dc, float xp, float xm, float yp, float ym,
+                                           float zp, float zm) const
     {
         const float S = xp + xm + yp + ym + zp + zm;
-        const float xbc = xb[o];
+        const float xbc = w.xb;
         const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
-        const float r = c.A * (S - 6.0f * dc) + dlocal<FRN>(dc, x, xbc, c.C) - rhs[o];
+        const float r = c.A * (S - 6.0f * dc) + dlocal<FRN>(dc, x, xbc, c.C) - w.rhs;
         out[o] = r;
         return 0.0;
     }
 };
 
-// level-0 inner residual D(d) - g averaged over the 8 children; the
-// children's Laplacians collapse to -3 sum(children) + sum(24 outer faces)
+// inner residual D(d) - rhs of a level averaged over the 8 children of each
+// coarse cell (rhs = g on level 0, the FAS right-hand side below); the
+// children's Laplacians collapse to -3 sum(children) + sum(24 outer faces).
+// z-marching like k_fr_restrict_residual1_64.
 template <int FRN>
-__global__ __launch_bounds__(256) void k_df_restrict_residual1(const float* __restrict__ d0,
-                                                               const float* __restrict__ xb0,
-                                                               const float* __restrict__ g0, float* __restrict__ rres1,
-                                                               Geo gf, Geo gc, DfCoef c, int TX, int TY, int XB,
-                                                               unsigned NB)
+__global__ __launch_bounds__(256) void k_df_restrict_residual(const float* __restrict__ d0,
+                                                              const float* __restrict__ xb0,
+                                                              const float* __restrict__ g0, float* __restrict__ rres1,
+                                                              Geo gf, Geo gc, DfCoef c, int TX, int TY, int XB, int JB,
+                                                              int KC, unsigned NB)
 {
-    int I;
-    unsigned row;
-    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
-    const int N = gf.n;
-    const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1, z0 = 2 * K, z1 = 2 * K + 1;
-    const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
-    const int yl = (J == 0) ? N - 1 : 2 * J - 1, yu = (2 * J + 2 == N) ? 0 : 2 * J + 2;
-    const int zl = (K == 0) ? N - 1 : 2 * K - 1, zu = (2 * K + 2 == N) ? 0 : 2 * K + 2;
-    float loc = 0.0f, s8 = 0.0f;
+    ZCol cc;
+    if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
+    const int I = cc.m, J = cc.j, N = gf.n;
+    Pts12 F;
+    F.init(gf, I, J);
+    // per fine plane: local term minus rhs and increment of the 4 children,
+    // increments of the 8 face neighbours
+    struct Pl {
+        float l[4], d[4], f[8];
+    };
+    auto load = [&](int z, Pl& p) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const unsigned o = sidx(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1));
-        const float dc = d0[o], xbc = xb0[o];
-        const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
-        loc += dlocal<FRN>(dc, x, xbc, c.C) - g0[o];
-        s8 += dc;
+        for (int q = 0; q < 4; q++) {
+            const unsigned o = F.at(q, z);
+            const float dc = d0[o], xbc = xb0[o];
+            const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
+            p.l[q] = dlocal<FRN>(dc, x, xbc, c.C) - g0[o];
+            p.d[q] = dc;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) p.f[q] = d0[F.at(4 + q, z)];
+    };
+    Pl p0, p1, p2;
+    float dl[4];
+    const int zs = 2 * cc.k0;
+    load(zs, p0);
+    {
+        const int zl = zs == 0 ? N - 1 : zs - 1;
+#pragma unroll
+        for (int q = 0; q < 4; q++) dl[q] = d0[F.at(q, zl)];
     }
-#define D_(x, y, z) d0[sidx(gf, x, y, z)]
-    float outer = D_(xl, y0, z0) + D_(xl, y1, z0) + D_(xl, y0, z1) + D_(xl, y1, z1);
-    outer += D_(xu, y0, z0) + D_(xu, y1, z0) + D_(xu, y0, z1) + D_(xu, y1, z1);
-    outer += D_(x0, yl, z0) + D_(x1, yl, z0) + D_(x0, yl, z1) + D_(x1, yl, z1);
-    outer += D_(x0, yu, z0) + D_(x1, yu, z0) + D_(x0, yu, z1) + D_(x1, yu, z1);
-    outer += D_(x0, y0, zl) + D_(x1, y0, zl) + D_(x0, y1, zl) + D_(x1, y1, zl);
-    outer += D_(x0, y0, zu) + D_(x1, y0, zu) + D_(x0, y1, zu) + D_(x1, y1, zu);
-#undef D_
-    rres1[sidx(gc, I, J, K)] = (c.A * (outer - 3.0f * s8) + loc) * 0.125f;
+    for (int K = cc.k0; K < cc.k1; K++) {
+        load(2 * K + 1, p1);
+        load(2 * K + 2 == N ? 0 : 2 * K + 2, p2);
+        float loc = 0.0f, s8 = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            loc += p0.l[q];
+            s8 += p0.d[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            loc += p1.l[q];
+            s8 += p1.d[q];
+        }
+        float outer = p0.f[0] + p0.f[1] + p1.f[0] + p1.f[1];
+        outer += p0.f[2] + p0.f[3] + p1.f[2] + p1.f[3];
+        outer += p0.f[4] + p0.f[5] + p1.f[4] + p1.f[5];
+        outer += p0.f[6] + p0.f[7] + p1.f[6] + p1.f[7];
+        outer += dl[0] + dl[1] + dl[2] + dl[3];
+        outer += p2.d[0] + p2.d[1] + p2.d[2] + p2.d[3];
+        rres1[sidx(gc, I, J, K)] = (c.A * (outer - 3.0f * s8) + loc) * 0.125f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) dl[q] = p1.d[q];
+        p0 = p2;
+    }
 }
 
 // restricted base: xb_c = (mean of the children's xb^{n+1})^{1/(n+1)}
@@ -571,46 +824,70 @@ __global__ __launch_bounds__(256) void k_restrict_base(const float* __restrict__
 }
 
 // inner coarse-grid correction d_h += P(d_H - R d_h), kept only where the
-// corrected x stays positive (the delta-form image of the reference's guard)
-template <int FRN>
-__global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH, const float* __restrict__ rd,
-                                                    float* __restrict__ df, const float* __restrict__ xbf, Geo gf,
-                                                    Geo gc, int TX, int TY, int XB, unsigned NB)
+// corrected x stays positive (the delta-form image of the reference's guard);
+// z-marching like k_correct64, on e = d_H - R d_h
+template <int FRN, int S3>
+__device__ __forceinline__ void df_correct_sons(const float* ez, const float* ek, float* __restrict__ df,
+                                                const float* __restrict__ xbf, const Geo& gf, int I, int J, int K)
 {
-    int I;
-    unsigned row;
-    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
-    const int nc = gc.n;
     const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
     const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
-    const int Xs[3] = {wrap_dec(I, nc), I, wrap_inc(I, nc)};
-    const int Ys[3] = {wrap_dec(J, nc), J, wrap_inc(J, nc)};
-    const int Zs[3] = {wrap_dec(K, nc), K, wrap_inc(K, nc)};
-    float e[27];
+    float old[4], wb[4];
 #pragma unroll
-    for (int z = 0; z < 3; z++)
+    for (int t = 0; t < 4; t++) {
+        const unsigned fi = son_index(gf, I, J, K, t & 1, t >> 1, S3);
+        old[t] = df[fi];
+        wb[t] = wpow<FRN>(xbf[fi]);
+    }
 #pragma unroll
-        for (int y = 0; y < 3; y++)
-#pragma unroll
-            for (int x = 0; x < 3; x++) {
-                const unsigned qq = sidx(gc, Xs[x], Ys[y], Zs[z]);
-                e[x + 3 * y + 9 * z] = dH[qq] - rd[qq];
-            }
-#pragma unroll
-    for (int sind = 0; sind < 8; sind++) {
-        const int s1 = sind & 1, s2 = (sind >> 1) & 1, s3 = (sind >> 2) & 1;
+    for (int t = 0; t < 4; t++) {
+        const int s1 = t & 1, s2 = t >> 1;
         float P = 0.0f;
 #pragma unroll
         for (int ind = 0; ind < 8; ind++) {
             const int x = (ind & 1) ? 1 : (s1 ? 2 : 0);
             const int y = (ind & 2) ? 1 : (s2 ? 2 : 0);
-            const int z = (ind & 4) ? 1 : (s3 ? 2 : 0);
-            P += wt[ind] * e[x + 3 * y + 9 * z];
+            const int q = x + 3 * y;
+            P += wt[ind] * ((ind & 4) ? ek[q] : ez[q]);
         }
-        const unsigned fi = sidx(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3);
-        const float dn = df[fi] + P;
-        if (wpow<FRN>(xbf[fi]) + dn > 0.0f) df[fi] = dn;
+        const float dn = old[t] + P;
+        if (wb[t] + dn > 0.0f) df[son_index(gf, I, J, K, s1, s2, S3)] = dn;
+    }
+}
+
+template <int FRN>
+__global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH, const float* __restrict__ rd,
+                                                    float* __restrict__ df, const float* __restrict__ xbf, Geo gf,
+                                                    Geo gc, int TX, int TY, int XB, int JB, int KC, unsigned NB)
+{
+    ZCol cc;
+    if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
+    const int I = cc.m, J = cc.j;
+    Block9 B;
+    B.init(gc, I, J);
+    float es[9], ek[9];  // plane beside K (K-1, then K+1) and plane K
+    const int Km = wrap_dec(cc.k0, gc.n);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        const unsigned om = B.at(q, Km), o0 = B.at(q, cc.k0);
+        es[q] = dH[om] - rd[om];
+        ek[q] = dH[o0] - rd[o0];
+    }
+    for (int K = cc.k0; K < cc.k1; K++) {
+        df_correct_sons<FRN, 0>(es, ek, df, xbf, gf, I, J, K);
+        const int Kp = wrap_inc(K, gc.n);
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const unsigned op = B.at(q, Kp);
+            es[q] = dH[op] - rd[op];
+        }
+        df_correct_sons<FRN, 1>(es, ek, df, xbf, gf, I, J, K);
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const float t = ek[q];
+            ek[q] = es[q];
+            es[q] = t;
+        }
     }
 }
 
@@ -647,21 +924,6 @@ __global__ __launch_bounds__(1024) void k_df_coarsest(float* __restrict__ d, con
             }
             __syncthreads();
         }
-    }
-}
-
-// u += u_bg (x(d) - xb), evaluated in FP64 as u_bg d / dden(x, xb)
-template <int FRN>
-__global__ __launch_bounds__(256) void k_outer_update(double* __restrict__ u, const float* __restrict__ d,
-                                                      const float* __restrict__ xb, size_t count, double ubg)
-{
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        const double dl = d[i], xbc = xb[i];
-        const double w = wpow<FRN>(xbc) + dl;
-        if (!(w > 0.0)) continue;
-        const double x = xroot<FRN>(w);
-        u[i] = u[i] + ubg * (dl / dden<FRN>(x, xbc));
     }
 }
 
@@ -792,8 +1054,8 @@ void launch_fr_restrict_residual1_64(int frn, const double* u0, const double* de
                                      const FrCoef& c0, hipStream_t s)
 {
     const Geo gf = make_geo(N), gc = make_geo(N / 2);
-    const Launch L = make_launch(gc.n, (long)gc.n * gc.n);
-    FRN_DISPATCH(frn, k_fr_restrict_residual1_64, u0, delta, rres1, gf, gc, c0, LAUNCH_ARGS);
+    const ZLaunch L = make_zlaunch(gc.n, gc.n);
+    FRN_DISPATCH(frn, k_fr_restrict_residual1_64, u0, delta, rres1, gf, gc, c0, ZARGS);
 }
 
 void launch_restrict8_64(const double* fine, double* out1, double* out2, int nc, hipStream_t s)
@@ -822,8 +1084,8 @@ void launch_fr_rhs64(int frn, const double* ru, const double* rho, const double*
 void launch_correct64(const double* uH, const double* ru, double* ufine, int nc, int guard, hipStream_t s)
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
-    const Launch L = make_launch(nc, (long)nc * nc);
-    hipLaunchKernelGGL(k_correct64, L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, LAUNCH_ARGS);
+    const ZLaunch L = make_zlaunch(nc, nc);
+    hipLaunchKernelGGL(k_correct64, L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, ZARGS);
 }
 
 // partials[0..count) -> out[0]; stage 1 writes its chunk sums after the
@@ -841,22 +1103,23 @@ void launch_sum_partials(const double* partials, int count, double* out, hipStre
     hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(1024), 0, s, (const double*)mid, nchunk, nchunk, out);
 }
 
-int launch_outer_residual(int frn, const double* u, const double* delta, float* g32, float* xb, double* partials,
-                          int n, const FrCoef& c, hipStream_t s)
+int launch_outer_residual(int frn, const double* u, const double* delta, float* g32, float* xb, float* d,
+                          double* partials, int n, const FrCoef& c, const DfCoef& dc, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    if (frn == 0) return both_colours(FrOuterOp<0>{u, delta, g32, xb, c}, partials, g, s);
-    if (frn == 1) return both_colours(FrOuterOp<1>{u, delta, g32, xb, c}, partials, g, s);
-    return both_colours(FrOuterOp<2>{u, delta, g32, xb, c}, partials, g, s);
+    const unsigned half = (unsigned)g.half;
+    if (frn == 0) return both_colours(FrOuterOp<0>{u, delta, g32, xb, d, c, dc, half}, partials, g, s);
+    if (frn == 1) return both_colours(FrOuterOp<1>{u, delta, g32, xb, d, c, dc, half}, partials, g, s);
+    return both_colours(FrOuterOp<2>{u, delta, g32, xb, d, c, dc, half}, partials, g, s);
 }
 
-void launch_outer_update(int frn, double* u, const float* d, const float* xb, int n, double ubg, hipStream_t s)
+void launch_df_smooth_update(int frn, const float* d, const float* xb, const float* rhs, double* u, int n,
+                             const DfCoef& c, double ubg, hipStream_t s)
 {
-    const size_t count = (size_t)n * n * n;
-    const dim3 grid = flat_grid(count);
-    if (frn == 0) hipLaunchKernelGGL((k_outer_update<0>), grid, dim3(256), 0, s, u, d, xb, count, ubg);
-    else if (frn == 1) hipLaunchKernelGGL((k_outer_update<1>), grid, dim3(256), 0, s, u, d, xb, count, ubg);
-    else hipLaunchKernelGGL((k_outer_update<2>), grid, dim3(256), 0, s, u, d, xb, count, ubg);
+    const Geo g = make_geo(n);
+    if (frn == 0) colour_pass(DfSmoothUpdateOp<0>{d, xb, rhs, u, c, ubg}, g, 1, s);
+    else if (frn == 1) colour_pass(DfSmoothUpdateOp<1>{d, xb, rhs, u, c, ubg}, g, 1, s);
+    else colour_pass(DfSmoothUpdateOp<2>{d, xb, rhs, u, c, ubg}, g, 1, s);
 }
 
 void launch_restrict_base(int frn, const float* xb_f, float* xb_c, int nc, hipStream_t s)
@@ -884,12 +1147,12 @@ void launch_df_residual(int frn, const float* d, const float* xb, const float* r
     else both_colours(DfResOp<2>{d, xb, rhs, res, c}, nullptr, g, s);
 }
 
-void launch_df_restrict_residual1(int frn, const float* d0, const float* xb0, const float* g0, float* rres1, int N,
-                                  const DfCoef& c0, hipStream_t s)
+void launch_df_restrict_residual(int frn, const float* d0, const float* xb0, const float* g0, float* rres1, int N,
+                                 const DfCoef& c0, hipStream_t s)
 {
     const Geo gf = make_geo(N), gc = make_geo(N / 2);
-    const Launch L = make_launch(gc.n, (long)gc.n * gc.n);
-    FRN_DISPATCH(frn, k_df_restrict_residual1, d0, xb0, g0, rres1, gf, gc, c0, LAUNCH_ARGS);
+    const ZLaunch L = make_zlaunch(gc.n, gc.n);
+    FRN_DISPATCH(frn, k_df_restrict_residual, d0, xb0, g0, rres1, gf, gc, c0, ZARGS);
 }
 
 void launch_df_rhs(int frn, const float* rd, const float* xb, const float* rres, float* f, int n, const DfCoef& c,
@@ -902,8 +1165,8 @@ void launch_df_correct(int frn, const float* dH, const float* rd, float* dfine, 
                        hipStream_t s)
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
-    const Launch L = make_launch(nc, (long)nc * nc);
-    FRN_DISPATCH(frn, k_df_correct, dH, rd, dfine, xbfine, gf, gc, LAUNCH_ARGS);
+    const ZLaunch L = make_zlaunch(nc, nc);
+    FRN_DISPATCH(frn, k_df_correct, dH, rd, dfine, xbfine, gf, gc, ZARGS);
 }
 
 void launch_df_coarsest(int frn, float* d, const float* xb, const float* rhs, int n, int sweeps, const DfCoef& c,

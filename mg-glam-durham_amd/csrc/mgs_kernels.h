@@ -34,16 +34,22 @@ void launch_correct64(const double* uH, const double* ru, double* ufine, int nc,
 void launch_sum_partials(const double* partials, int count, double* out, hipStream_t s);
 
 // ---- mixed precision f(R) path: FP64 outer loop, FP32 delta-form FAS ----
-int launch_outer_residual(int frn, const double* u, const double* delta, float* g, float* xb, double* partials,
-                          int n, const FrCoef& c, hipStream_t s);
-void launch_outer_update(int frn, double* u, const float* d, const float* xb, int n, double ubg, hipStream_t s);
+// FP64 residual of u (partials for its norm), g = -r and xb = u/u_bg in FP32,
+// and the inner increment d after its first colour-0 pass from zero
+int launch_outer_residual(int frn, const double* u, const double* delta, float* g, float* xb, float* d,
+                          double* partials, int n, const FrCoef& c, const DfCoef& dc, hipStream_t s);
+// the last colour-1 inner pass fused with u += u_bg (x(d) - xb) on all cells
+void launch_df_smooth_update(int frn, const float* d, const float* xb, const float* rhs, double* u, int n,
+                             const DfCoef& c, double ubg, hipStream_t s);
 void launch_restrict_base(int frn, const float* xb_f, float* xb_c, int nc, hipStream_t s);
 void launch_df_smooth(int frn, float* d, const float* xb, const float* rhs, int n, int colour, const DfCoef& c,
                       hipStream_t s);
 void launch_df_residual(int frn, const float* d, const float* xb, const float* rhs, float* res, int n,
                         const DfCoef& c, hipStream_t s);
-void launch_df_restrict_residual1(int frn, const float* d0, const float* xb0, const float* g0, float* rres1, int N,
-                                  const DfCoef& c0, hipStream_t s);
+// residual of the fine level (N cells a side, right-hand side g) restricted
+// to the next level in one pass
+void launch_df_restrict_residual(int frn, const float* d, const float* xb, const float* g, float* rres, int N,
+                                 const DfCoef& c, hipStream_t s);
 void launch_restrict8_32(const float* fine, float* out1, float* out2, int nc, hipStream_t s);
 void launch_df_rhs(int frn, const float* rd, const float* xb, const float* rres, float* f, int n, const DfCoef& c,
                    hipStream_t s);

@@ -13,6 +13,8 @@
 // k+-1 a block needs).
 #pragma once
 
+#include <cstdlib>
+
 #include "mgs_device.h"
 
 namespace mgs {
@@ -23,25 +25,39 @@ struct ZLaunch {
     unsigned NB;
 };
 
+// planes marched per thread; MGS_ZRUN overrides it (launch-shape tuning)
 inline int zrun_length(int n)
 {
-    if (n >= 256) return 16;
-    if (n >= 64) return 8;
-    if (n >= 16) return 4;
-    return n;
+    static const int forced = [] {
+        const char* e = std::getenv("MGS_ZRUN");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (forced > 0) return forced < n ? forced : n;
+    // measured at 512^3 (runs of 2, 4, 6, 8, 16, 32 planes): 4 is best; the
+    // window's two prologue loads per run cost less than the lost
+    // parallelism of longer runs
+    return n >= 4 ? 4 : n;
 }
 
-inline ZLaunch make_zlaunch(const Geo& g)
+// (TX x TY)-thread blocks over `width` columns x n rows, each thread marching
+// a run of KC planes; k-run-major block order
+inline ZLaunch make_zlaunch(int width, int n)
 {
+    // widest block row; MGS_ZTX overrides it (launch-shape tuning)
+    static const int max_tx = [] {
+        const char* e = std::getenv("MGS_ZTX");
+        const int v = e ? std::atoi(e) : 0;
+        return (v == 32 || v == 64 || v == 128) ? v : 256;
+    }();
     ZLaunch L;
-    L.TX = g.h < 256 ? g.h : 256;
+    L.TX = width < max_tx ? width : max_tx;
     int ty = 256 / L.TX;
-    if (ty > g.n) ty = g.n;
+    if (ty > n) ty = n;
     L.TY = ty < 1 ? 1 : ty;
-    L.XB = (g.h + L.TX - 1) / L.TX;
-    L.JB = (g.n + L.TY - 1) / L.TY;
-    L.KC = zrun_length(g.n);
-    const int kb = (g.n + L.KC - 1) / L.KC;
+    L.XB = (width + L.TX - 1) / L.TX;
+    L.JB = (n + L.TY - 1) / L.TY;
+    L.KC = zrun_length(n);
+    const int kb = (n + L.KC - 1) / L.KC;
     L.NB = (unsigned)(L.XB * L.JB * kb);
     const unsigned per = (L.NB + 7) / 8;
     L.grid = dim3(per * 8);
@@ -49,12 +65,15 @@ inline ZLaunch make_zlaunch(const Geo& g)
     return L;
 }
 
+// stencil passes over one colour half: columns are the n/2 values of m
+inline ZLaunch make_zlaunch(const Geo& g) { return make_zlaunch(g.h, g.n); }
+
 struct ZCol {
     int m, j, k0, k1;
 };
 
 // column and plane run of this thread; false for idle threads
-__device__ __forceinline__ bool zcol(const Geo& g, int TX, int TY, int XB, int JB, int KC, unsigned NB, ZCol& c)
+__device__ __forceinline__ bool zcol(int width, int n, int TX, int TY, int XB, int JB, int KC, unsigned NB, ZCol& c)
 {
     unsigned lb;
     if (!map_block(NB, lb)) return false;
@@ -65,13 +84,26 @@ __device__ __forceinline__ bool zcol(const Geo& g, int TX, int TY, int XB, int J
     c.m = (int)(xb * TX + threadIdx.x);
     c.j = (int)(jb * TY + threadIdx.y);
     c.k0 = (int)kb * KC;
-    c.k1 = c.k0 + KC < g.n ? c.k0 + KC : g.n;
-    return c.m < g.h && c.j < g.n;
+    c.k1 = c.k0 + KC < n ? c.k0 + KC : n;
+    return c.m < width && c.j < n;
+}
+
+__device__ __forceinline__ bool zcol(const Geo& g, int TX, int TY, int XB, int JB, int KC, unsigned NB, ZCol& c)
+{
+    return zcol(g.h, g.n, TX, TY, XB, JB, KC, NB, c);
 }
 
 // Column window over one colour half `a` (32-bit offsets; a level holds at
-// most 2^30 cells).  load() at step k gives the six neighbour values of the
-// opposite-colour cell at (m, j, k), whose x-parity is s.
+// most 2^30 cells).  fetch(k) issues the loads that give the six neighbour
+// values of the opposite-colour cell at (m, j, k), whose x-parity is s;
+// apply() turns them into the neighbours and advances the window to k+1.
+// Splitting the two lets a loop issue the loads of plane k+1 before it
+// computes and stores plane k.
+template <typename T>
+struct ZPre {
+    T next, xo, yp, ym;
+};
+
 template <typename T>
 struct ZWin {
     const T* __restrict__ a;
@@ -93,31 +125,40 @@ struct ZWin {
         cur = a[(unsigned)k0 * plane + jrow + m];
     }
 
-    // neighbours of the other colour's cell at plane k (x-parity s); advances
-    // the window to k+1
-    __device__ __forceinline__ void step(int k, int s, T& xp, T& xm, T& yp, T& ym, T& zp, T& zm)
+    __device__ __forceinline__ ZPre<T> fetch(int k, int s) const
     {
         const unsigned kp = (unsigned)(k + 1 == (int)n ? 0 : k + 1);
         const unsigned pk = (unsigned)k * plane;
-        const T next = a[kp * plane + jrow + m];
         unsigned mo;
         if (s) mo = (m + 1 == h) ? 0u : m + 1;
         else mo = (m == 0) ? h - 1 : m - 1;
-        const T xo = a[pk + jrow + mo];
-        yp = a[pk + jprow + m];
-        ym = a[pk + jmrow + m];
-        xp = s ? xo : cur;
-        xm = s ? cur : xo;
-        zp = next;
+        ZPre<T> p;
+        p.next = a[kp * plane + jrow + m];
+        p.xo = a[pk + jrow + mo];
+        p.yp = a[pk + jprow + m];
+        p.ym = a[pk + jmrow + m];
+        return p;
+    }
+
+    __device__ __forceinline__ void apply(const ZPre<T>& p, int s, T& xp, T& xm, T& yp, T& ym, T& zp, T& zm)
+    {
+        xp = s ? p.xo : cur;
+        xm = s ? cur : p.xo;
+        yp = p.yp;
+        ym = p.ym;
+        zp = p.next;
         zm = prev;
         prev = cur;
-        cur = next;
+        cur = p.next;
     }
 };
 
-// One colour pass: OP::update(offset of the cell in the full level array,
-// xp, xm, yp, ym, zp, zm) with neighbours in the reference's order
-// M1u, M1l, M2u, M2l, M3u, M3l.
+// One colour pass.  OP::load(own offset, other offset) loads the cell's own
+// data (everything but the marched field); OP::update(own offset, other
+// offset, own data, value of the other-colour cell at the same (m, j, k),
+// xp, xm, yp, ym, zp, zm) applies the update, with neighbours in the
+// reference's order M1u, M1l, M2u, M2l, M3u, M3l.  Offsets index the full
+// level array.  The loads of plane k+1 are issued before plane k is updated.
 template <class OP>
 __global__ __launch_bounds__(256) void k_colour_pass(OP op, Geo g, int colour, int TX, int TY, int XB, int JB, int KC,
                                                      unsigned NB)
@@ -125,20 +166,38 @@ __global__ __launch_bounds__(256) void k_colour_pass(OP op, Geo g, int colour, i
     ZCol c;
     if (!zcol(g, TX, TY, XB, JB, KC, NB, c)) return;
     using T = typename OP::T;
+    using Own = typename OP::Own;
     ZWin<T> w;
     w.init(op.field() + (size_t)(1 - colour) * g.half, g, c.m, c.j, c.k0);
     const unsigned own = (unsigned)colour * (unsigned)g.half;
-    for (int k = c.k0; k < c.k1; k++) {
+    const unsigned oth = (unsigned)(1 - colour) * (unsigned)g.half;
+    const unsigned col = w.jrow + w.m;
+    int k = c.k0;
+    ZPre<T> pc = w.fetch(k, (colour + c.j + k) & 1);
+    Own oc = op.load(own + (unsigned)k * w.plane + col, oth + (unsigned)k * w.plane + col);
+    for (; k < c.k1; k++) {
+        ZPre<T> pn = pc;
+        Own on = oc;
+        if (k + 1 < c.k1) {
+            const unsigned rel = (unsigned)(k + 1) * w.plane + col;
+            pn = w.fetch(k + 1, (colour + c.j + k + 1) & 1);
+            on = op.load(own + rel, oth + rel);
+        }
+        const unsigned rel = (unsigned)k * w.plane + col;
+        const T other = w.cur;
         T xp, xm, yp, ym, zp, zm;
-        const int s = (colour + c.j + k) & 1;
-        w.step(k, s, xp, xm, yp, ym, zp, zm);
-        op.update(own + (unsigned)k * w.plane + w.jrow + w.m, xp, xm, yp, ym, zp, zm);
+        w.apply(pc, (colour + c.j + k) & 1, xp, xm, yp, ym, zp, zm);
+        op.update(own + rel, oth + rel, oc, other, xp, xm, yp, ym, zp, zm);
+        pc = pn;
+        oc = on;
     }
 }
 
-// Both colours per column: OP::cell(offset, centre, xp, xm, yp, ym, zp, zm)
-// returns a value whose square is accumulated; the block sum goes to
-// partials[logical block] when partials is non-null.
+// Both colours per column: OP::load(offset) loads a cell's own data;
+// OP::cell(offset, own data, centre, xp, xm, yp, ym, zp, zm) returns a value
+// whose square is accumulated; the block sum goes to partials[logical block]
+// when partials is non-null.  Loads of plane k+1 are issued before plane k
+// is evaluated.
 template <class OP>
 __global__ __launch_bounds__(256) void k_both_colours(OP op, double* __restrict__ partials, Geo g, int TX, int TY,
                                                       int XB, int JB, int KC, unsigned NB)
@@ -147,21 +206,41 @@ __global__ __launch_bounds__(256) void k_both_colours(OP op, double* __restrict_
     double acc = 0.0;
     if (zcol(g, TX, TY, XB, JB, KC, NB, c)) {
         using T = typename OP::T;
+        using Own = typename OP::Own;
         ZWin<T> w0, w1;  // windows over the red (0) and black (1) halves
         w0.init(op.field(), g, c.m, c.j, c.k0);
         w1.init(op.field() + g.half, g, c.m, c.j, c.k0);
         const unsigned half = (unsigned)g.half;
-        for (int k = c.k0; k < c.k1; k++) {
+        const unsigned col = w0.jrow + w0.m;
+        int k = c.k0;
+        // red cell: neighbours in the black half, x-parity (j+k)&1;
+        // black cell: neighbours in the red half, x-parity (1+j+k)&1
+        ZPre<T> p1 = w1.fetch(k, (c.j + k) & 1);
+        ZPre<T> p0 = w0.fetch(k, (1 + c.j + k) & 1);
+        Own o0 = op.load((unsigned)k * w0.plane + col);
+        Own o1 = op.load(half + (unsigned)k * w0.plane + col);
+        for (; k < c.k1; k++) {
+            ZPre<T> q1 = p1, q0 = p0;
+            Own n0 = o0, n1 = o1;
+            if (k + 1 < c.k1) {
+                const unsigned rel = (unsigned)(k + 1) * w0.plane + col;
+                q1 = w1.fetch(k + 1, (c.j + k + 1) & 1);
+                q0 = w0.fetch(k + 1, (1 + c.j + k + 1) & 1);
+                n0 = op.load(rel);
+                n1 = op.load(half + rel);
+            }
             const T c0 = w0.cur, c1 = w1.cur;
-            const unsigned off = (unsigned)k * w0.plane + w0.jrow + w0.m;
+            const unsigned off = (unsigned)k * w0.plane + col;
             T xp, xm, yp, ym, zp, zm;
-            // red cell: neighbours in the black half, x-parity (j+k)&1
-            w1.step(k, (c.j + k) & 1, xp, xm, yp, ym, zp, zm);
-            const double r0 = op.cell(off, c0, xp, xm, yp, ym, zp, zm);
-            // black cell: neighbours in the red half, x-parity (1+j+k)&1
-            w0.step(k, (1 + c.j + k) & 1, xp, xm, yp, ym, zp, zm);
-            const double r1 = op.cell(half + off, c1, xp, xm, yp, ym, zp, zm);
+            w1.apply(p1, (c.j + k) & 1, xp, xm, yp, ym, zp, zm);
+            const double r0 = op.cell(off, o0, c0, xp, xm, yp, ym, zp, zm);
+            w0.apply(p0, (1 + c.j + k) & 1, xp, xm, yp, ym, zp, zm);
+            const double r1 = op.cell(half + off, o1, c1, xp, xm, yp, ym, zp, zm);
             acc += r0 * r0 + r1 * r1;
+            p1 = q1;
+            p0 = q0;
+            o0 = n0;
+            o1 = n1;
         }
     }
     if (partials) store_partial(partials, NB, block_sum(acc));
