@@ -99,6 +99,19 @@ def cpu_reference_baseline(delta32, N, box, fr0, gpu_cycles, ncycles):
     }
 
 
+def profiled_traffic():
+    """HBM bytes per launch of the roofline kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by a copy probe there)."""
+    path = os.path.join(ROOT, "profiles", "fr512_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(gib):
     """A device copy of known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
     import torch
@@ -225,7 +238,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic() if args.workload == "fr512" else None,
+            "traffic_source": "profiles/fr512_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
             "launches": sm[0],

@@ -102,6 +102,7 @@ FrCoef fr_coefs(const mgs_params& p, int level)
     c.fct4_1 = c.fct2r * std::pow(std::fabs(fR_bg), 1.0 / (1.0 + (double)n));
     c.ubg = std::pow(std::fabs(fR_bg), 1.0 / (1.0 + (double)n));
     c.inv_w = 1.0 / powi_d(c.ubg, n + 1);
+    c.inv_ubg = 1.0 / c.ubg;
     c.reference_roots = (p.flags & MGS_FLAG_REFERENCE_ROOTS) ? 1 : 0;
     return c;
 }

@@ -64,6 +64,7 @@ struct FrCoef {
     // normalisation of the local polynomial for the FP32 seed:
     // x = u / ubg solves x^{n+2} + (P inv_w) x + Q2 inv_w = 0, inv_w = ubg^-(n+1)
     double inv_w;
+    double inv_ubg;  // 1 / ubg
     int reference_roots;  // 1: evaluate the reference's closed form literally
 };
 

@@ -699,11 +699,15 @@ struct FrOuterOp {
     {
         const double S =
             wpow<FRN>(xp) + wpow<FRN>(xm) + wpow<FRN>(yp) + wpow<FRN>(ym) + wpow<FRN>(zp) + wpow<FRN>(zm);
+        // the plain residual's operator, with 1/u from the hardware
+        // reciprocal refined once (a few ulp: the inner problem only needs
+        // g and xb in FP32; the norm is unaffected at any tolerance used)
+        const double ru = rcp64(uc);
         double OP = S - wpow<FRN>(uc) * 6.0;
         OP = OP * c.fct3r;
         OP = OP + c.fct1r * w.delta;
-        OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
-        const float gv = (float)(-OP), xv = (float)(uc / c.ubg);
+        OP = OP + c.fct2r * (c.a_fR_rt * ru - 1.0);
+        const float gv = (float)(-OP), xv = (float)(uc * c.inv_ubg);
         g[o] = gv;
         xb[o] = xv;
         d[o] = o < half ? df_solve<FRN>(0.0f, xv, gv, dc) : 0.0f;

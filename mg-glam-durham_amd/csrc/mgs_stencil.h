@@ -57,11 +57,18 @@ inline ZLaunch make_zlaunch(int width, int n)
     L.XB = (width + L.TX - 1) / L.TX;
     L.JB = (n + L.TY - 1) / L.TY;
     L.KC = zrun_length(n);
+    // block order: MGS_ZORDER=k makes the plane runs of one column block
+    // consecutive (flagged to the kernels by a negative JB)
+    static const bool kfast = [] {
+        const char* e = std::getenv("MGS_ZORDER");
+        return e && e[0] == 'k';
+    }();
     const int kb = (n + L.KC - 1) / L.KC;
     L.NB = (unsigned)(L.XB * L.JB * kb);
     const unsigned per = (L.NB + 7) / 8;
     L.grid = dim3(per * 8);
     L.block = dim3(L.TX, L.TY);
+    if (kfast) L.JB = -L.JB;
     return L;
 }
 
@@ -77,10 +84,19 @@ __device__ __forceinline__ bool zcol(int width, int n, int TX, int TY, int XB, i
 {
     unsigned lb;
     if (!map_block(NB, lb)) return false;
-    const unsigned xb = XB == 1 ? 0u : lb % (unsigned)XB;
-    const unsigned t = XB == 1 ? lb : lb / (unsigned)XB;
-    const unsigned jb = t % (unsigned)JB;
-    const unsigned kb = t / (unsigned)JB;
+    unsigned xb, jb, kb;
+    if (JB > 0) {  // x blocks fastest, then rows, then plane runs
+        xb = XB == 1 ? 0u : lb % (unsigned)XB;
+        const unsigned t = XB == 1 ? lb : lb / (unsigned)XB;
+        jb = t % (unsigned)JB;
+        kb = t / (unsigned)JB;
+    } else {  // plane runs fastest
+        const unsigned KB = (unsigned)((n + KC - 1) / KC);
+        kb = lb % KB;
+        const unsigned t = lb / KB;
+        jb = t % (unsigned)(-JB);
+        xb = t / (unsigned)(-JB);
+    }
     c.m = (int)(xb * TX + threadIdx.x);
     c.j = (int)(jb * TY + threadIdx.y);
     c.k0 = (int)kb * KC;
