@@ -198,6 +198,66 @@ int mgs_use_graphs(mgs_handle* h, int32_t enable);
  * (for zero-copy consumers on the same device) */
 int mgs_device_field(mgs_handle* h, void** ptr, size_t* bytes);
 
+/* ---- serving: continuous-batching solve server ------------------------
+ * No counterpart in the reference (a build-only component).  A server owns a
+ * worker thread and one HIP stream on the current device.  Requests (f(R),
+ * MGS_PREC_MIXED, ngrid >= 16) wait in a FIFO queue and are admitted into
+ * one of max_active slots when a slot is free and the request's hierarchy
+ * fits in hbm_budget; admitted requests keep their hierarchies in HBM and
+ * advance together: every batched step (outer residual, each level's colour
+ * passes, transfers, coarsest solves, update) is one persistent launch over
+ * the tiles of all active requests, replayed from a captured hipGraph.
+ * Requests join at the next outer iteration after admission and leave when
+ * converged, so each request's result is bit-identical alone, in any mix,
+ * and to mgs_solve with the same parameters (mode 1, inner_cycles 1). */
+typedef struct mgs_server mgs_server;
+
+typedef struct mgs_server_opts {
+    uint64_t hbm_budget;     /* bytes for request hierarchies; 0: 60 % of the device's free memory */
+    int32_t max_active;      /* requests solved together (slots), 1..64 */
+    int32_t queue_capacity;  /* waiting requests; beyond it mgs_server_submit returns MGS_E_BUSY */
+    int32_t use_graphs;      /* 1: replay each batch step from a captured hipGraph */
+    int32_t reserved;
+} mgs_server_opts;
+
+typedef struct mgs_request {
+    mgs_params params;       /* model MGS_MODEL_FR, precision MGS_PREC_MIXED, fr0 < 0, 16 <= ngrid <= 1024 */
+    const void* delta;       /* host density contrast, ngrid^3 in the reference's order; copied by submit */
+    int32_t dtype;           /* MGS_F32 / MGS_F64 */
+    int32_t max_outer;       /* cap on outer iterations */
+    double rel_tol;          /* stop when RMS(r)/RMS(r0) < rel_tol */
+} mgs_request;
+
+typedef struct mgs_request_result {
+    int32_t status;          /* 0 or the MGS_E_* code the request failed with */
+    int32_t outer;           /* outer iterations */
+    int32_t converged;
+    int32_t reserved;
+    double res0, res_final;  /* FP64 RMS residuals */
+    double queued_ms;        /* submit -> admission */
+    double latency_ms;       /* submit -> result ready */
+} mgs_request_result;
+
+typedef struct mgs_server_stats {
+    int64_t submitted, completed, rejected;
+    int64_t steps;           /* batched outer-iteration steps executed */
+    double p50_ms, p99_ms, mean_ms;   /* latency submit -> result of completed requests */
+    double solves_per_s;     /* completed / (last completion - first submission) */
+    int32_t max_batch;       /* most requests advanced by one batched step */
+    int32_t reserved;
+} mgs_server_stats;
+
+int mgs_server_create(const mgs_server_opts* opts, mgs_server** out);
+int mgs_server_destroy(mgs_server* srv);
+const char* mgs_server_last_error(const mgs_server* srv);   /* also valid for srv == NULL */
+/* enqueue a request; *ticket identifies it for mgs_server_wait */
+int mgs_server_submit(mgs_server* srv, const mgs_request* req, int64_t* ticket);
+/* block until the request is done; copy its field u (raw, x scale) to the
+ * host array `u` (may be NULL) and its report to `res` (may be NULL); a
+ * ticket can be waited for once */
+int mgs_server_wait(mgs_server* srv, int64_t ticket, void* u, int32_t dtype, double scale, mgs_request_result* res);
+int mgs_server_get_stats(mgs_server* srv, mgs_server_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,14 +124,14 @@ DfCoef df_coefs(const mgs_params& p, int level)
 // ---------------------------------------------------------------------------
 // hierarchy
 // ---------------------------------------------------------------------------
-static int levelmax_of(int N)
+int levelmax_of(int N)
 {
     int l = 0;
     while ((1 << l) < N) l++;
     return l;
 }
 
-static int levelmin_of(int N)
+int levelmin_of(int N)
 {
     int odd = N;
     while (odd % 2 == 0) odd /= 2;
@@ -670,8 +670,16 @@ static int check_params(const mgs_params* p, std::string* why)
         *why = "grid hierarchy: every level must have an even number of cells";
         return MGS_E_ARG;
     }
+    if (p->precision == MGS_PREC_MIXED && lc < 1) {
+        *why = "the mixed-precision solve needs at least two grid levels";
+        return MGS_E_ARG;
+    }
     return 0;
 }
+
+namespace mgs {
+int validate_params(const mgs_params* p, std::string* why) { return check_params(p, why); }
+}  // namespace mgs
 
 extern "C" {
 

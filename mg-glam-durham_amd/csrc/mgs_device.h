@@ -138,4 +138,19 @@ __device__ inline double block_sum(double v)
     return t;
 }
 
+// sum of p[lo, hi) by one block: thread t adds p[lo + t], p[lo + t + B], ...
+// (B = blockDim.x, a power of two <= 1024), then a tree over the block in
+// s[0..B); the result is s[0].  The order depends only on (lo, hi, B).
+__device__ inline void block_chunk_sum(const double* __restrict__ p, int lo, int hi, double* s)
+{
+    double v = 0.0;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += blockDim.x) v += p[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+}
+
 }  // namespace mgs

@@ -96,6 +96,10 @@ void prof_begin(mgs_handle* h, int kc, double bytes);
 void prof_end(mgs_handle* h);
 int vcycle_plain(mgs_handle* h, int l);
 int op_residual(mgs_handle* h, int l, bool store, double* rms);
+int levelmax_of(int N);
+int levelmin_of(int N);
+// the parameter checks of mgs_create (0 or an MGS_E_* code with a reason)
+int validate_params(const mgs_params* p, std::string* why);
 
 // mgs_dgp.hip
 DgpCoef dgp_coefs(const mgs_params& p, int level);

@@ -79,11 +79,11 @@ struct ZCol {
     int m, j, k0, k1;
 };
 
-// column and plane run of this thread; false for idle threads
-__device__ __forceinline__ bool zcol(int width, int n, int TX, int TY, int XB, int JB, int KC, unsigned NB, ZCol& c)
+// column and plane run of thread (tx, ty) of logical block lb; false for
+// idle threads
+__device__ __forceinline__ bool zcol_at(int width, int n, int TX, int TY, int XB, int JB, int KC, unsigned lb, int tx,
+                                        int ty, ZCol& c)
 {
-    unsigned lb;
-    if (!map_block(NB, lb)) return false;
     unsigned xb, jb, kb;
     if (JB > 0) {  // x blocks fastest, then rows, then plane runs
         xb = XB == 1 ? 0u : lb % (unsigned)XB;
@@ -97,11 +97,19 @@ __device__ __forceinline__ bool zcol(int width, int n, int TX, int TY, int XB, i
         jb = t % (unsigned)(-JB);
         xb = t / (unsigned)(-JB);
     }
-    c.m = (int)(xb * TX + threadIdx.x);
-    c.j = (int)(jb * TY + threadIdx.y);
+    c.m = (int)(xb * TX + tx);
+    c.j = (int)(jb * TY + ty);
     c.k0 = (int)kb * KC;
     c.k1 = c.k0 + KC < n ? c.k0 + KC : n;
     return c.m < width && c.j < n;
+}
+
+// the same for this thread of a standalone launch (XCD-remapped block id)
+__device__ __forceinline__ bool zcol(int width, int n, int TX, int TY, int XB, int JB, int KC, unsigned NB, ZCol& c)
+{
+    unsigned lb;
+    if (!map_block(NB, lb)) return false;
+    return zcol_at(width, n, TX, TY, XB, JB, KC, lb, threadIdx.x, threadIdx.y, c);
 }
 
 __device__ __forceinline__ bool zcol(const Geo& g, int TX, int TY, int XB, int JB, int KC, unsigned NB, ZCol& c)
@@ -176,11 +184,8 @@ struct ZWin {
 // reference's order M1u, M1l, M2u, M2l, M3u, M3l.  Offsets index the full
 // level array.  The loads of plane k+1 are issued before plane k is updated.
 template <class OP>
-__global__ __launch_bounds__(256) void k_colour_pass(OP op, Geo g, int colour, int TX, int TY, int XB, int JB, int KC,
-                                                     unsigned NB)
+__device__ __forceinline__ void colour_pass_col(const OP& op, const Geo& g, int colour, const ZCol& c)
 {
-    ZCol c;
-    if (!zcol(g, TX, TY, XB, JB, KC, NB, c)) return;
     using T = typename OP::T;
     using Own = typename OP::Own;
     ZWin<T> w;
@@ -209,56 +214,71 @@ __global__ __launch_bounds__(256) void k_colour_pass(OP op, Geo g, int colour, i
     }
 }
 
+template <class OP>
+__global__ __launch_bounds__(256) void k_colour_pass(OP op, Geo g, int colour, int TX, int TY, int XB, int JB, int KC,
+                                                     unsigned NB)
+{
+    ZCol c;
+    if (!zcol(g, TX, TY, XB, JB, KC, NB, c)) return;
+    colour_pass_col(op, g, colour, c);
+}
+
 // Both colours per column: OP::load(offset) loads a cell's own data;
 // OP::cell(offset, own data, centre, xp, xm, yp, ym, zp, zm) returns a value
 // whose square is accumulated; the block sum goes to partials[logical block]
 // when partials is non-null.  Loads of plane k+1 are issued before plane k
 // is evaluated.
 template <class OP>
+__device__ __forceinline__ double both_colours_col(const OP& op, const Geo& g, const ZCol& c)
+{
+    using T = typename OP::T;
+    using Own = typename OP::Own;
+    double acc = 0.0;
+    ZWin<T> w0, w1;  // windows over the red (0) and black (1) halves
+    w0.init(op.field(), g, c.m, c.j, c.k0);
+    w1.init(op.field() + g.half, g, c.m, c.j, c.k0);
+    const unsigned half = (unsigned)g.half;
+    const unsigned col = w0.jrow + w0.m;
+    int k = c.k0;
+    // red cell: neighbours in the black half, x-parity (j+k)&1;
+    // black cell: neighbours in the red half, x-parity (1+j+k)&1
+    ZPre<T> p1 = w1.fetch(k, (c.j + k) & 1);
+    ZPre<T> p0 = w0.fetch(k, (1 + c.j + k) & 1);
+    Own o0 = op.load((unsigned)k * w0.plane + col);
+    Own o1 = op.load(half + (unsigned)k * w0.plane + col);
+    for (; k < c.k1; k++) {
+        ZPre<T> q1 = p1, q0 = p0;
+        Own n0 = o0, n1 = o1;
+        if (k + 1 < c.k1) {
+            const unsigned rel = (unsigned)(k + 1) * w0.plane + col;
+            q1 = w1.fetch(k + 1, (c.j + k + 1) & 1);
+            q0 = w0.fetch(k + 1, (1 + c.j + k + 1) & 1);
+            n0 = op.load(rel);
+            n1 = op.load(half + rel);
+        }
+        const T c0 = w0.cur, c1 = w1.cur;
+        const unsigned off = (unsigned)k * w0.plane + col;
+        T xp, xm, yp, ym, zp, zm;
+        w1.apply(p1, (c.j + k) & 1, xp, xm, yp, ym, zp, zm);
+        const double r0 = op.cell(off, o0, c0, xp, xm, yp, ym, zp, zm);
+        w0.apply(p0, (1 + c.j + k) & 1, xp, xm, yp, ym, zp, zm);
+        const double r1 = op.cell(half + off, o1, c1, xp, xm, yp, ym, zp, zm);
+        acc += r0 * r0 + r1 * r1;
+        p1 = q1;
+        p0 = q0;
+        o0 = n0;
+        o1 = n1;
+    }
+    return acc;
+}
+
+template <class OP>
 __global__ __launch_bounds__(256) void k_both_colours(OP op, double* __restrict__ partials, Geo g, int TX, int TY,
                                                       int XB, int JB, int KC, unsigned NB)
 {
     ZCol c;
     double acc = 0.0;
-    if (zcol(g, TX, TY, XB, JB, KC, NB, c)) {
-        using T = typename OP::T;
-        using Own = typename OP::Own;
-        ZWin<T> w0, w1;  // windows over the red (0) and black (1) halves
-        w0.init(op.field(), g, c.m, c.j, c.k0);
-        w1.init(op.field() + g.half, g, c.m, c.j, c.k0);
-        const unsigned half = (unsigned)g.half;
-        const unsigned col = w0.jrow + w0.m;
-        int k = c.k0;
-        // red cell: neighbours in the black half, x-parity (j+k)&1;
-        // black cell: neighbours in the red half, x-parity (1+j+k)&1
-        ZPre<T> p1 = w1.fetch(k, (c.j + k) & 1);
-        ZPre<T> p0 = w0.fetch(k, (1 + c.j + k) & 1);
-        Own o0 = op.load((unsigned)k * w0.plane + col);
-        Own o1 = op.load(half + (unsigned)k * w0.plane + col);
-        for (; k < c.k1; k++) {
-            ZPre<T> q1 = p1, q0 = p0;
-            Own n0 = o0, n1 = o1;
-            if (k + 1 < c.k1) {
-                const unsigned rel = (unsigned)(k + 1) * w0.plane + col;
-                q1 = w1.fetch(k + 1, (c.j + k + 1) & 1);
-                q0 = w0.fetch(k + 1, (1 + c.j + k + 1) & 1);
-                n0 = op.load(rel);
-                n1 = op.load(half + rel);
-            }
-            const T c0 = w0.cur, c1 = w1.cur;
-            const unsigned off = (unsigned)k * w0.plane + col;
-            T xp, xm, yp, ym, zp, zm;
-            w1.apply(p1, (c.j + k) & 1, xp, xm, yp, ym, zp, zm);
-            const double r0 = op.cell(off, o0, c0, xp, xm, yp, ym, zp, zm);
-            w0.apply(p0, (1 + c.j + k) & 1, xp, xm, yp, ym, zp, zm);
-            const double r1 = op.cell(half + off, o1, c1, xp, xm, yp, ym, zp, zm);
-            acc += r0 * r0 + r1 * r1;
-            p1 = q1;
-            p0 = q0;
-            o0 = n0;
-            o1 = n1;
-        }
-    }
+    if (zcol(g, TX, TY, XB, JB, KC, NB, c)) acc = both_colours_col(op, g, c);
     if (partials) store_partial(partials, NB, block_sum(acc));
 }
 

@@ -65,6 +65,35 @@ class SolveReport(C.Structure):
     ]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("queue_capacity", C.c_int32),
+        ("use_graphs", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("params", Params), ("delta", C.c_void_p), ("dtype", C.c_int32), ("max_outer", C.c_int32),
+        ("rel_tol", C.c_double),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32), ("reserved", C.c_int32),
+        ("res0", C.c_double), ("res_final", C.c_double), ("queued_ms", C.c_double), ("latency_ms", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64), ("steps", C.c_int64),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("solves_per_s", C.c_double),
+        ("max_batch", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 _lib = None
 
 
@@ -110,6 +139,12 @@ def load():
         "mgs_synchronize": ([H], C.c_int),
         "mgs_use_graphs": ([H, i32], C.c_int),
         "mgs_device_field": ([H, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)], C.c_int),
+        "mgs_server_create": ([C.POINTER(ServerOpts), C.POINTER(H)], C.c_int),
+        "mgs_server_destroy": ([H], C.c_int),
+        "mgs_server_last_error": ([H], C.c_char_p),
+        "mgs_server_submit": ([H, C.POINTER(Request), C.POINTER(C.c_int64)], C.c_int),
+        "mgs_server_wait": ([H, C.c_int64, vp, i32, C.c_double, C.POINTER(RequestResult)], C.c_int),
+        "mgs_server_get_stats": ([H, C.POINTER(ServerStats)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -314,3 +349,65 @@ def exported_symbols_declared_in_header(header_path):
     import re
     txt = open(header_path).read()
     return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mgs_\w+)\s*\(", txt, re.M)))
+
+
+class Server:
+    """Continuous-batching solve server (mgs_server_* in include/mgsolve.h):
+    f(R) requests in mixed precision, admitted by HBM footprint into
+    `max_active` slots and advanced together by batched launches."""
+
+    def __init__(self, max_active=16, hbm_budget=0, queue_capacity=0, use_graphs=True):
+        self.L = load()
+        self.h = C.c_void_p()
+        o = ServerOpts(int(hbm_budget), int(max_active), int(queue_capacity), 1 if use_graphs else 0, 0)
+        rc = self.L.mgs_server_create(C.byref(o), C.byref(self.h))
+        if rc:
+            raise MgsError(f"mgs_server_create failed ({rc}): {self.L.mgs_server_last_error(None).decode()}")
+
+    def close(self):
+        if self.h:
+            self.L.mgs_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, cfg: Config, delta, rel_tol=1e-8, max_outer=60):
+        """Queue a solve of cfg (f(R), PREC_MIXED) on density delta; returns a ticket."""
+        a = np.ascontiguousarray(delta)
+        dt = F32 if a.dtype == np.float32 else F64
+        if dt == F64:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+        r = Request(cfg.c(), _np_ptr(a), dt, int(max_outer), float(rel_tol))
+        t = C.c_int64()
+        rc = self.L.mgs_server_submit(self.h, C.byref(r), C.byref(t))
+        if rc:
+            raise MgsError(f"mgs_server_submit failed ({rc}): {self.L.mgs_server_last_error(self.h).decode()}")
+        return t.value
+
+    def wait(self, ticket, N, dtype=np.float64, scale=1.0):
+        """Block until the ticket is done; returns (field u, result dict)."""
+        u = np.empty((N, N, N), dtype=dtype)
+        res = RequestResult()
+        rc = self.L.mgs_server_wait(self.h, int(ticket), _np_ptr(u), F32 if dtype == np.float32 else F64,
+                                    float(scale), C.byref(res))
+        out = {k: getattr(res, k) for k, _ in RequestResult._fields_}
+        if rc:
+            raise MgsError(f"mgs_server_wait failed ({rc}): {self.L.mgs_server_last_error(self.h).decode()}")
+        return u, out
+
+    def stats(self):
+        st = ServerStats()
+        rc = self.L.mgs_server_get_stats(self.h, C.byref(st))
+        if rc:
+            raise MgsError(f"mgs_server_get_stats failed ({rc})")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

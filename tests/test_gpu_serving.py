@@ -1,0 +1,115 @@
+"""The continuous-batching solve server (mgs_server_* in include/mgsolve.h).
+
+Each request's result must be the same alone, in a mix with other requests
+of other sizes and parameters, and from the single-request solve
+(mgs_solve, mixed precision, mode 1): the server runs the same per-request
+arithmetic, tiling and reduction order, so the bar is bitwise equality.
+Admission by HBM footprint is exercised with a budget that holds only one
+request at a time.  Error paths are checked without a GPU."""
+import numpy as np
+import pytest
+
+import oracle as O
+import mgsolve as M
+from conftest import gpu_available
+
+
+def cfg(N, fr0=-1e-5, frn=1, box=64.0, a=1.0):
+    return M.Config(N=N, model=M.MODEL_FR, fr_n=frn, box=box, omega_m=0.3, omega_l=0.7, aexpn=a, fr0=fr0,
+                    precision=M.PREC_MIXED)
+
+
+# a mix of sizes (incl. a non-power-of-two grid), models n and parameters
+MIX = [
+    (32, -1e-5, 1, 64.0, 1.0, 1.0, 11),
+    (16, -1e-4, 1, 32.0, 1.0, 0.8, 12),
+    (48, -1e-6, 1, 96.0, 1.0, 1.3, 13),
+    (32, -1e-5, 0, 64.0, 0.8, 1.0, 14),
+    (64, -1e-5, 2, 128.0, 1.0, 1.1, 15),
+    (24, -1e-5, 1, 48.0, 1.0, 1.2, 16),
+    (32, -1e-6, 1, 64.0, 0.5, 1.5, 17),
+]
+
+
+def density(N, box, sigma, seed, f32):
+    d = O.lognormal_density(N, box, sigma, seed=seed)
+    return d.astype(np.float32) if f32 else d
+
+
+def standalone(c, delta, tol):
+    with M.Solver(c) as S:
+        S.set_density(delta)
+        rep = S.solve(mode=1, rel_tol=tol, max_cycles=60, init_guess=True, inner_cycles=1)
+        return S.get_field(), rep
+
+
+@pytest.mark.gpu
+def test_results_equal_alone_in_mix_and_standalone():
+    tol = 1e-8
+    reqs = []
+    for k, (N, fr0, frn, box, a, sigma, seed) in enumerate(MIX):
+        reqs.append((cfg(N, fr0, frn, box, a), density(N, box, sigma, seed, f32=(k % 2 == 0))))
+    print(f"[serving] {len(reqs)} requests, mixed then alone", flush=True)
+    with M.Server(max_active=4) as srv:
+        tickets = [srv.submit(c, d, rel_tol=tol) for c, d in reqs]
+        mixed = [srv.wait(t, c.N) for t, (c, _) in zip(tickets, reqs)]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["max_batch"] >= 2, st
+    for (c, d), (u_mix, r_mix) in zip(reqs, mixed):
+        print(f"[serving] N={c.N} n={c.fr_n} fr0={c.fr0}", flush=True)
+        assert r_mix["status"] == 0 and r_mix["converged"], r_mix
+        assert r_mix["res_final"] < tol * r_mix["res0"]
+        with M.Server(max_active=1) as srv:
+            u_alone, r_alone = srv.wait(srv.submit(c, d, rel_tol=tol), c.N)
+        assert np.array_equal(u_alone, u_mix)
+        assert r_alone["outer"] == r_mix["outer"] and r_alone["res_final"] == r_mix["res_final"]
+        u_ref, rep = standalone(c, d, tol)
+        assert np.array_equal(u_ref, u_mix)
+        assert rep["outer"] == r_mix["outer"] and rep["res_final"] == r_mix["res_final"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_graphs", [True, False])
+def test_admission_by_footprint_and_graph_replay(use_graphs):
+    """A budget that holds one 32^3 hierarchy: requests queue and are
+    admitted one after another; results do not depend on graph replay."""
+    N = 32
+    c = cfg(N)
+    ds = [density(N, 64.0, 1.0, 40 + i, f32=True) for i in range(3)]
+    print(f"[serving] admission, graphs={use_graphs}", flush=True)
+    with M.Server(max_active=8, use_graphs=True) as big:
+        ref = [big.wait(big.submit(c, d), N)[0] for d in ds]
+    budget = 40 * N ** 3  # one hierarchy (about 35 bytes per cell), not two
+    with M.Server(max_active=8, hbm_budget=budget, use_graphs=use_graphs) as srv:
+        ts = [srv.submit(c, d) for d in ds]
+        out = [srv.wait(t, N) for t in ts]
+        st = srv.stats()
+    assert st["max_batch"] == 1, st
+    for (u, r), u_ref in zip(out, ref):
+        assert r["status"] == 0 and r["converged"]
+        assert np.array_equal(u, u_ref)
+    # requests queued behind the first waited for admission
+    assert out[2][1]["queued_ms"] >= out[0][1]["queued_ms"]
+
+
+@pytest.mark.gpu
+def test_server_rejects_bad_requests():
+    with M.Server(max_active=2) as srv:
+        d = np.zeros((16,) * 3, np.float32)
+        with pytest.raises(M.MgsError, match="MGS_PREC_MIXED"):
+            srv.submit(M.Config(N=16, precision=M.PREC_FP64), d)
+        with pytest.raises(M.MgsError, match="ngrid"):
+            srv.submit(cfg(8), np.zeros((8,) * 3, np.float32))
+        bad = d.copy()
+        bad[1, 2, 3] = np.inf
+        with pytest.raises(M.MgsError, match="finite"):
+            srv.submit(cfg(16), bad)
+        with pytest.raises(M.MgsError, match="budget"):
+            with M.Server(max_active=1, hbm_budget=1024) as tiny:
+                tiny.submit(cfg(16), d)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
+def test_server_without_gpu_is_an_error():
+    with pytest.raises(M.MgsError, match="no CPU fallback"):
+        M.Server()
