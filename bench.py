@@ -99,6 +99,145 @@ def cpu_reference_baseline(delta32, N, box, fr0, gpu_cycles, ncycles):
     }
 
 
+def dgp_compare(N, box, sigma, seed, steps):
+    """nDGP normal branch, H0rc = 1, a = 1: the plain FP64 solve and the mixed
+    solve of the same seeded lognormal density, each run to the reference's
+    stagnation exit (the residual of a periodic DGP problem levels off at the
+    mean of its nonlinear source, so a relative 1e-8 target does not apply);
+    times, cycles, final residuals and the field / fifth-force P(k) agreement."""
+    out = {}
+    fields = {}
+    delta = None
+    for prec in (M.PREC_FP64, M.PREC_MIXED):
+        c = M.Config(N=N, model=M.MODEL_DGP, branch=1, box=box, omega_m=0.3, omega_l=0.7, aexpn=1.0, h0rc=1.0,
+                     precision=prec)
+        with M.Solver(c) as S:
+            if delta is None:
+                S.make_lognormal_density(seed, sigma)
+                delta = S.get_density(np.float64)
+            else:
+                S.set_density(delta)
+            S.solve(mode=0, res_conv=1e-30, max_cycles=60)
+            S.synchronize()
+            reps = [S.solve(mode=0, res_conv=1e-30, max_cycles=60) for _ in range(steps)]
+            fields[prec] = S.get_field()
+            _, pk, _ = S.power_spectrum(2)
+        r = reps[-1]
+        key = "fp64" if prec == M.PREC_FP64 else "mixed"
+        out[key] = {"ms_median": float(np.median([x["seconds"] * 1e3 for x in reps])), "cycles": r["cycles"],
+                    "outer": r["outer"], "res0": r["res0"], "res_final": r["res_final"]}
+        out[key + "_pk"] = pk
+    u64, um = fields[M.PREC_FP64], fields[M.PREC_MIXED]
+    d = np.abs(um - u64)
+    pk64, pkm = out.pop("fp64_pk"), out.pop("mixed_pk")
+    out["mixed_vs_fp64"] = {
+        "u_rel_linf": float(d.max() / np.abs(u64).max()),
+        "u_rel_l2": float(np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(u64 ** 2))),
+        "fifth_force_pk_max_abs_ratio_minus_1": float(np.max(np.abs(pkm[pk64 > 0] / pk64[pk64 > 0] - 1.0))),
+    }
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["ms_median"] / out["mixed"]["ms_median"]
+    out["workload"] = (f"nDGP normal branch, H0rc=1, Om=0.3, a=1, {N}^3, box {box:g} Mpc/h, lognormal sigma_g={sigma} "
+                       f"seed={seed}, both solved to the reference's stagnation exit")
+    log(f"[dgp] fp64 {out['fp64']['ms_median']:.1f} ms ({out['fp64']['cycles']} cycles), mixed "
+        f"{out['mixed']['ms_median']:.1f} ms ({out['mixed']['outer']} outer), u rel L_inf "
+        f"{out['mixed_vs_fp64']['u_rel_linf']:.2e}")
+    return out
+
+
+def serving_stream(n_req, seed, max_active):
+    """The serving stream: n_req f(R) n=1 requests, ngrid uniform over
+    {64, 128, 192, 256} (cell 2 Mpc/h), |fR0| from {1e-4, 1e-5, 1e-6},
+    seeded lognormal densities with sigma_g ~ U[0.8, 1.6] (generated on the
+    GPU, kept on the host as float32), each solved to relative residual 1e-8.
+    First all requests at once (saturated throughput), then Poisson arrivals
+    at 0.8x that rate (latency).  Three requests are re-solved alone and
+    compared with their in-mix results."""
+    import threading
+
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([64, 128, 192, 256], n_req)
+    fr0s = -rng.choice([1e-4, 1e-5, 1e-6], n_req)
+    sigmas = rng.uniform(0.8, 1.6, n_req)
+    cfgs = [M.Config(N=int(N), model=M.MODEL_FR, fr_n=1, box=2.0 * int(N), omega_m=0.3, omega_l=0.7, aexpn=1.0,
+                     fr0=float(f), precision=M.PREC_MIXED) for N, f in zip(sizes, fr0s)]
+    dens = []
+    gens = {}
+    t0 = time.perf_counter()
+    for i, c in enumerate(cfgs):
+        g = gens.get(c.N)
+        if g is None:
+            g = gens[c.N] = M.Solver(M.Config(N=c.N, box=c.box, precision=M.PREC_FP64))
+        g.make_lognormal_density(2000 + i, float(sigmas[i]))
+        dens.append(g.get_density(np.float32))
+    for g in gens.values():
+        g.close()
+    log(f"[serve] {n_req} densities generated in {time.perf_counter() - t0:.1f} s")
+    # saturated: everything queued at once
+    with M.Server(max_active=max_active) as srv:
+        ts = time.perf_counter()
+        tickets = [srv.submit(c, d) for c, d in zip(cfgs, dens)]
+        outs = [srv.wait(t, c.N, np.float32)[1] for t, c in zip(tickets, cfgs)]
+        sat_s = time.perf_counter() - ts
+        sat = srv.stats()
+    rate = 0.8 * n_req / sat_s
+    log(f"[serve] saturated: {n_req / sat_s:.1f} solves/s, p50 {sat['p50_ms']:.1f} ms, batch <= {sat['max_batch']}")
+    bad = [o for o in outs if o["status"] != 0 or not o["converged"]]
+    # Poisson arrivals at 0.8x the saturated rate
+    gaps = rng.exponential(1.0 / rate, n_req)
+    fields = {}
+    keep = {0, n_req // 2, n_req - 1}
+    with M.Server(max_active=max_active) as srv:
+        tickets = [None] * n_req
+
+        def producer():
+            start = time.perf_counter()
+            at = 0.0
+            for i in range(n_req):
+                at += gaps[i]
+                dt = start + at - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[i] = srv.submit(cfgs[i], dens[i])
+
+        th = threading.Thread(target=producer)
+        tp = time.perf_counter()
+        th.start()
+        for i in range(n_req):
+            while tickets[i] is None:
+                time.sleep(0.0005)
+            u, o = srv.wait(tickets[i], cfgs[i].N, np.float32)
+            if i in keep:
+                fields[i] = u
+            if o["status"] != 0 or not o["converged"]:
+                bad.append(o)
+        th.join()
+        pois_s = time.perf_counter() - tp
+        st = srv.stats()
+    log(f"[serve] Poisson {rate:.1f}/s: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms, "
+        f"{st['solves_per_s']:.1f} solves/s, batch <= {st['max_batch']}")
+    # the same requests alone
+    diff = 0.0
+    with M.Server(max_active=1) as srv:
+        for i in sorted(keep):
+            u, _ = srv.wait(srv.submit(cfgs[i], dens[i]), cfgs[i].N, np.float32)
+            diff = max(diff, float(np.max(np.abs(u - fields[i]))))
+    return {
+        "requests": n_req,
+        "sizes": [64, 128, 192, 256],
+        "max_active": max_active,
+        "saturated_solves_per_s": n_req / sat_s,
+        "arrival_rate_per_s": rate,
+        "p50_ms": st["p50_ms"],
+        "p99_ms": st["p99_ms"],
+        "mean_ms": st["mean_ms"],
+        "solves_per_s": st["solves_per_s"],
+        "wall_s": pois_s,
+        "max_batch": st["max_batch"],
+        "failed_or_unconverged": len(bad),
+        "alone_vs_mix_max_abs_diff": diff,
+    }
+
+
 def profiled_traffic():
     """HBM bytes per launch of the roofline kernel, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by a copy probe there)."""
@@ -135,6 +274,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--copy-probe", type=float, default=0.0, help="only run a device copy of this many GiB")
+    ap.add_argument("--serve-requests", type=int, default=512, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--dgp-n", type=int, default=512, help="grid of the nDGP comparison (0: skip)")
     args = ap.parse_args()
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
@@ -192,6 +334,14 @@ def main():
     else:
         delta32 = Smix.get_density(np.float32)
 
+    dgp = None
+    if args.dgp_n > 0:
+        dgp = dgp_compare(args.dgp_n, float(args.dgp_n), 1.2, 1005, 1)
+
+    serving = None
+    if args.serve_requests > 0:
+        serving = serving_stream(args.serve_requests, 4000, args.serve_slots)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_reference_baseline(delta32, N, box, fr0, fp64["cycles"] if fp64 else reps[-1]["outer"],
@@ -222,7 +372,11 @@ def main():
             "N": N,
             "inner_cycles_per_outer": args.inner,
         },
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "latency_ms": ({"p50": serving["p50_ms"], "p99": serving["p99_ms"],
+                        "source": "serving stream (Poisson arrivals), submit -> result"} if serving else None),
+        "solve_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "serving": serving,
+        "dgp": dgp,
         "mixed": {
             "outer_iterations": reps[-1]["outer"],
             "inner_vcycles": reps[-1]["cycles"],

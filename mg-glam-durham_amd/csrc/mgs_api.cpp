@@ -179,8 +179,10 @@ static int alloc_hierarchy(mgs_handle* h)
             if ((rc = dalloc(h, &v.d, n3))) return rc;
             if ((rc = dalloc(h, &v.xb, n3))) return rc;
             if ((rc = dalloc(h, &v.f, n3))) return rc;
-            if (l > 0) {
+            if (l > 0 || h->p.model == MGS_MODEL_DGP) {
                 if ((rc = dalloc(h, &v.res, n3))) return rc;
+            }
+            if (l > 0) {
                 if ((rc = dalloc(h, &v.rres, n3))) return rc;
                 if ((rc = dalloc(h, &v.rd, n3))) return rc;
             }
@@ -572,17 +574,84 @@ static int solve_plain(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
     return 0;
 }
 
+// ---- nDGP: inner V-cycle on the FP32 increment e (see mgs_dgp.hip) ----------
+static int vcycle_inner_dgp(mgs_handle* h, int l)
+{
+    const int Nsmth = 2;
+    LevelF& v = h->M[l];
+    LevelD& V = h->L[l];
+    const DgpCoef dc = dgp_coefs(h->p, l);
+    float* tmp = (float*)h->dgp_tmp;
+    const double s = 4.0;
+    const int cls = l == 0 ? MGS_KC_SMOOTH : MGS_KC_COARSE;
+    if (l >= h->lc) {
+        prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(v.n) * s);
+        launch_dgpm_coarsest(v.d, v.xb, V.rho, v.f, tmp, v.n, 8, dc, h->s);
+        prof_end(h);
+        return 0;
+    }
+    for (int it = 0; it < 2 * Nsmth; it++) {
+        prof_begin(h, cls, 3.0 * n3d(v.n) * s);
+        launch_dgpm_smooth(v.d, v.xb, V.rho, v.f, tmp, v.n, (it & 1) ? 1 : 0, dc, h->s);
+        prof_end(h);
+    }
+    LevelF& c = h->M[l + 1];
+    prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, 6.0 * n3d(v.n) * s);
+    launch_dgpm_residual(v.d, v.xb, V.rho, v.f, v.res, v.n, dc, h->s);
+    launch_restrict8_32(v.res, c.rres, nullptr, c.n, h->s);
+    launch_restrict8_32(v.d, c.rd, c.d, c.n, h->s);
+    prof_end(h);
+    auto coarse = [h, l, &c]() {
+        prof_begin(h, MGS_KC_COARSE, 5.0 * n3d(c.n) * 4.0);
+        launch_dgpm_residual(c.rd, c.xb, h->L[l + 1].rho, c.rres, c.f, c.n, dgp_coefs(h->p, l + 1), h->s);
+        prof_end(h);
+        return vcycle_inner_dgp(h, l + 1);
+    };
+    if (l == 0) {
+        int rc = run_captured(h, &h->g_inner, coarse);
+        if (rc) return rc;
+    } else {
+        coarse();
+    }
+    prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * s);
+    launch_dgpm_correct(c.d, c.rd, v.d, c.n, h->s);
+    prof_end(h);
+    for (int it = 0; it < 2 * Nsmth; it++) {
+        prof_begin(h, cls, 3.0 * n3d(v.n) * s);
+        launch_dgpm_smooth(v.d, v.xb, V.rho, v.f, tmp, v.n, (it & 1) ? 1 : 0, dc, h->s);
+        prof_end(h);
+    }
+    return 0;
+}
+
+static int outer_residual_dgp(mgs_handle* h, double* rms)
+{
+    LevelD& v = h->L[0];
+    LevelF& m = h->M[0];
+    prof_begin(h, MGS_KC_RESIDUAL, 28.0 * n3d(v.n));
+    const int nb = launch_dgpm_outer(v.u, v.rho, m.f, m.xb, m.d, h->partials, v.n, dgp_coefs(h->p, 0), h->s);
+    prof_end(h);
+    launch_sum_partials(h->partials, nb, h->dsum, h->s);
+    HIPCHK(h, hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(h, hipStreamSynchronize(h->s));
+    *rms = std::sqrt(h->hsum[0] / n3d(v.n));
+    return 0;
+}
+
+// FP64 defect correction around the FP32 inner FAS (f(R) and nDGP)
 static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep)
 {
     int rc;
+    const bool fr = h->p.model == MGS_MODEL_FR;
     const int frn = h->p.fr_n;
     const int inner = o.inner_cycles > 0 ? o.inner_cycles : 1;
+    if (!fr && (rc = cache_densities(h))) return rc;
     double res_old_1 = 1.0e5, res_old_2 = 1.0e10;
     bool conv = false;
     int cycles = 0;
     for (int it = 1;; it++) {
         double res = 0.0;
-        if ((rc = outer_residual(h, &res))) return rc;
+        if ((rc = fr ? outer_residual(h, &res) : outer_residual_dgp(h, &res))) return rc;
         if (!std::isfinite(res)) return set_err(h, MGS_E_DOMAIN, "residual is not finite");
         if (it == 1) rep->res0 = res;
         rep->res_final = res;
@@ -606,14 +675,27 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
         // inner problem: base restricted to every level, increment from zero
         for (int l = 1; l <= h->lc; l++) {
             prof_begin(h, l == 1 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(h->M[l - 1].n) + n3d(h->M[l].n)) * 4.0);
-            launch_restrict_base(frn, h->M[l - 1].xb, h->M[l].xb, h->M[l].n, h->s);
+            if (fr)
+                launch_restrict_base(frn, h->M[l - 1].xb, h->M[l].xb, h->M[l].n, h->s);
+            else
+                launch_restrict8_32(h->M[l - 1].xb, h->M[l].xb, nullptr, h->M[l].n, h->s);
             prof_end(h);
         }
-        // d = 0 with its first colour-0 pass came from the outer residual;
-        // the last inner pass also applies u += u_bg (x(d) - xb)
-        for (int ic = 0; ic < inner; ic++) {
-            vcycle_inner(h, 0, ic == 0, ic == inner - 1);
-            cycles++;
+        if (fr) {
+            // d = 0 with its first colour-0 pass came from the outer residual;
+            // the last inner pass also applies u += u_bg (x(d) - xb)
+            for (int ic = 0; ic < inner; ic++) {
+                vcycle_inner(h, 0, ic == 0, ic == inner - 1);
+                cycles++;
+            }
+        } else {
+            for (int ic = 0; ic < inner; ic++) {
+                vcycle_inner_dgp(h, 0);
+                cycles++;
+            }
+            prof_begin(h, MGS_KC_UPDATE, 20.0 * n3d(h->L[0].n));
+            launch_dgpm_update(h->L[0].u, h->M[0].d, h->L[0].n, h->s);
+            prof_end(h);
         }
     }
     rep->cycles = cycles;
@@ -646,10 +728,6 @@ static int check_params(const mgs_params* p, std::string* why)
     }
     if (p->precision != MGS_PREC_FP64 && p->precision != MGS_PREC_MIXED) {
         *why = "precision must be MGS_PREC_FP64 or MGS_PREC_MIXED";
-        return MGS_E_ARG;
-    }
-    if (p->precision == MGS_PREC_MIXED && p->model == MGS_MODEL_DGP) {
-        *why = "mixed-precision DGP solve is not available; use MGS_PREC_FP64";
         return MGS_E_ARG;
     }
     if (p->precision == MGS_PREC_MIXED && p->model == MGS_MODEL_FR && !(p->fr0 < 0.0)) {
@@ -1033,7 +1111,7 @@ int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* repor
     if (h->p.precision == MGS_PREC_MIXED && h->p.model == MGS_MODEL_FR)
         rc = solve_mixed(h, *opts, report);
     else if (h->p.precision == MGS_PREC_MIXED)
-        rc = solve_mixed_dgp(h, *opts, report);
+        rc = solve_mixed(h, *opts, report);
     else
         rc = solve_plain(h, *opts, report);
     HIPCHK(h, hipEventRecord(e1, h->s));

@@ -270,11 +270,272 @@ int dgp_alloc(mgs_handle* h)
     return 0;
 }
 
-int solve_mixed_dgp(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep)
+// ===========================================================================
+// Mixed precision: FP32 delta-form FAS inside the FP64 defect correction
+// ===========================================================================
+// The inner unknown is the increment e of the field over an FP32 base
+// phi_b (the current FP64 iterate rounded): D(e) = L(phi_b + e) - L(phi_b),
+// written so that every term is proportional to e,
+//   D(e) = lap(e)/dx^2 - 2 sgn(alpha) dSigma / (S(Sigma_b + dSigma) + S(Sigma_b)),
+//   S(x) = sqrt(alpha^2 + 8/3 x),
+//   dSigma = [2/3 (B.eps + Q(eps)) + 1/8 (2 C_b.eta + |eta|^2)] / dx^4,
+// where eps = (e_xx, e_yy, e_zz) and eta = (e_xy, e_xz, e_yz) are e's second
+// differences, Q(a) = a_x^2 + a_y^2 + a_z^2 - a_x a_y - a_x a_z - a_y a_z,
+// B = grad Q at the base's (a_xx, a_yy, a_zz) and C_b the base's cross
+// differences: Sigma(phi_b + e) - Sigma(phi_b) exactly, Sigma being quadratic.
+// FP32 rounding then scales with the increment and the FP64 outer residual
+// keeps converging.  The smoother is the reference's update
+// e <- e - (D(e) - rhs) / (-6/dx^2), colour by colour, out of place.
+
+// second differences, face sum and centre of a field at (i, j, k)
+struct D2f {
+    float xx, yy, zz, xy, xz, yz, s6, c0;
+};
+
+__device__ __forceinline__ D2f d2_at(const float* __restrict__ u, const Geo& g, int i, int j, int k)
 {
-    (void)o;
-    (void)rep;
-    return set_err(h, MGS_E_ARG, "mixed-precision DGP solve is not available; use MGS_PREC_FP64");
+    const int n = g.n;
+    const int im = wrap_dec(i, n), ip = wrap_inc(i, n);
+    const int jm = wrap_dec(j, n), jp = wrap_inc(j, n);
+    const int km = wrap_dec(k, n), kp = wrap_inc(k, n);
+    D2f d;
+    d.c0 = u[split_index(g, i, j, k)];
+    const float xp = u[split_index(g, ip, j, k)], xm = u[split_index(g, im, j, k)];
+    const float yp = u[split_index(g, i, jp, k)], ym = u[split_index(g, i, jm, k)];
+    const float zp = u[split_index(g, i, j, kp)], zm = u[split_index(g, i, j, km)];
+    d.xx = xp + xm - 2.0f * d.c0;
+    d.yy = yp + ym - 2.0f * d.c0;
+    d.zz = zp + zm - 2.0f * d.c0;
+    d.xy = u[split_index(g, ip, jp, k)] + u[split_index(g, im, jm, k)] - u[split_index(g, ip, jm, k)] -
+           u[split_index(g, im, jp, k)];
+    d.xz = u[split_index(g, ip, j, kp)] + u[split_index(g, im, j, km)] - u[split_index(g, ip, j, km)] -
+           u[split_index(g, im, j, kp)];
+    d.yz = u[split_index(g, i, jp, kp)] + u[split_index(g, i, jm, km)] - u[split_index(g, i, jp, km)] -
+           u[split_index(g, i, jm, kp)];
+    d.s6 = xp + xm + yp + ym + zp + zm;
+    return d;
+}
+
+struct DgpF {
+    float alpha, fct1, inv_dx2, inv_dx4, sgn;
+};
+
+static DgpF dgpf(const DgpCoef& c)
+{
+    return DgpF{(float)c.alpha, (float)c.fct1, (float)(1.0 / c.dx2), (float)(1.0 / c.dx4),
+                (float)std::copysign(1.0, c.alpha)};
+}
+
+// D(e) at a cell from the base's and the increment's second differences
+__device__ __forceinline__ float dgp_dop(const D2f& b, const D2f& e, float rho, const DgpF& c)
+{
+    const float Qb = b.xx * b.xx + b.yy * b.yy + b.zz * b.zz - b.xx * b.yy - b.xx * b.zz - b.yy * b.zz;
+    const float Cb = b.xy * b.xy + b.xz * b.xz + b.yz * b.yz;
+    const float Sig = c.fct1 * rho + ((2.0f / 3.0f) * Qb + 0.125f * Cb) * c.inv_dx4;
+    const float Bx = 2.0f * b.xx - b.yy - b.zz, By = 2.0f * b.yy - b.xx - b.zz, Bz = 2.0f * b.zz - b.xx - b.yy;
+    const float Qe = e.xx * e.xx + e.yy * e.yy + e.zz * e.zz - e.xx * e.yy - e.xx * e.zz - e.yy * e.zz;
+    const float lin = Bx * e.xx + By * e.yy + Bz * e.zz;
+    const float cross = 2.0f * (b.xy * e.xy + b.xz * e.xz + b.yz * e.yz) + e.xy * e.xy + e.xz * e.xz + e.yz * e.yz;
+    const float dSig = ((2.0f / 3.0f) * (lin + Qe) + 0.125f * cross) * c.inv_dx4;
+    const float a2 = c.alpha * c.alpha;
+    const float So = sqrtf(fmaxf(a2 + (8.0f / 3.0f) * Sig, 0.0f));
+    const float Sn = sqrtf(fmaxf(a2 + (8.0f / 3.0f) * (Sig + dSig), 0.0f));
+    const float den = So + Sn;
+    const float nl = den > 0.0f ? 2.0f * c.sgn * dSig / den : 0.0f;
+    return (e.s6 - 6.0f * e.c0) * c.inv_dx2 - nl;
+}
+
+// one colour of the inner smoother, out of place into tmp (one colour half)
+__global__ __launch_bounds__(256) void k_dgpm_smooth(const float* __restrict__ e, const float* __restrict__ pb,
+                                                     const double* __restrict__ rho, const float* __restrict__ rhs,
+                                                     float* __restrict__ tmp, Geo g, int colour, DgpF c, int TX,
+                                                     int TY, int XB, unsigned NB)
+{
+    int m;
+    unsigned row;
+    if (!map_xy(g.h, (long)g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const int j = (int)(row % (unsigned)g.n), k = (int)(row / (unsigned)g.n);
+    const int i = 2 * m + ((colour + j + k) & 1);
+    const size_t me = (size_t)colour * g.half + row_base(g, j, k) + m;
+    const D2f de = d2_at(e, g, i, j, k), db = d2_at(pb, g, i, j, k);
+    const float D = dgp_dop(db, de, (float)rho[me], c) - rhs[me];
+    tmp[row_base(g, j, k) + m] = de.c0 + D / (6.0f * c.inv_dx2);
+}
+
+// out = D(e) - rhs on both colours (inner residual; with (e, rhs, out) =
+// (R e, R r, f) the coarse FAS right-hand side)
+__global__ __launch_bounds__(256) void k_dgpm_residual(const float* __restrict__ e, const float* __restrict__ pb,
+                                                       const double* __restrict__ rho,
+                                                       const float* __restrict__ rhs, float* __restrict__ out, Geo g,
+                                                       DgpF c, int TX, int TY, int XB, unsigned NB)
+{
+    int m;
+    unsigned row;
+    if (!map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) return;
+    const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+    const int col = row >= nn ? 1 : 0;
+    const unsigned r2 = row - (unsigned)col * nn;
+    const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
+    const int i = 2 * m + ((col + j + k) & 1);
+    const size_t me = (size_t)col * g.half + row_base(g, j, k) + m;
+    out[me] = dgp_dop(d2_at(pb, g, i, j, k), d2_at(e, g, i, j, k), (float)rho[me], c) - rhs[me];
+}
+
+// FP64 outer residual r = L(phi) (partials of r^2), and the inner problem's
+// data: g = -r and phi_b = phi in FP32, e = 0
+__global__ __launch_bounds__(256) void k_dgpm_outer(const double* __restrict__ u, const double* __restrict__ rho,
+                                                    float* __restrict__ g32, float* __restrict__ pb,
+                                                    float* __restrict__ e, double* __restrict__ partials, Geo g,
+                                                    DgpCoef c, int TX, int TY, int XB, unsigned NB)
+{
+    int m;
+    unsigned row;
+    double acc = 0.0;
+    if (map_xy(g.h, 2L * g.n * g.n, TX, TY, XB, NB, m, row)) {
+        const unsigned nn = (unsigned)g.n * (unsigned)g.n;
+        const int col = row >= nn ? 1 : 0;
+        const unsigned r2 = row - (unsigned)col * nn;
+        const int j = (int)(r2 % (unsigned)g.n), k = (int)(r2 / (unsigned)g.n);
+        const int i = 2 * m + ((col + j + k) & 1);
+        const size_t me = (size_t)col * g.half + row_base(g, j, k) + m;
+        double c0;
+        const double OP = dgp_L(u, g, i, j, k, rho[me], c, c0);
+        g32[me] = (float)(-OP);
+        pb[me] = (float)c0;
+        e[me] = 0.0f;
+        acc = OP * OP;
+    }
+    store_partial(partials, NB, block_sum(acc));
+}
+
+// coarsest level: sweeps of the inner smoother by one workgroup, the update
+// evaluated in FP64 (the level holds at most a few thousand cells)
+__global__ __launch_bounds__(1024) void k_dgpm_coarsest(float* __restrict__ e, const float* __restrict__ pb,
+                                                        const double* __restrict__ rho,
+                                                        const float* __restrict__ rhs, float* __restrict__ tmp, Geo g,
+                                                        int sweeps, DgpF c)
+{
+    for (int sw = 0; sw < sweeps; sw++) {
+        for (int colour = 0; colour < 2; colour++) {
+            for (int t = threadIdx.x; t < (int)g.half; t += blockDim.x) {
+                const int m = t % g.h, row = t / g.h;
+                const int j = row % g.n, k = row / g.n;
+                const int i = 2 * m + ((colour + j + k) & 1);
+                const size_t me = (size_t)colour * g.half + t;
+                const D2f de = d2_at(e, g, i, j, k);
+                const double D = (double)dgp_dop(d2_at(pb, g, i, j, k), de, (float)rho[me], c) - (double)rhs[me];
+                tmp[t] = (float)((double)de.c0 + D / (6.0 * (double)c.inv_dx2));
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < (int)g.half; t += blockDim.x) e[(size_t)colour * g.half + t] = tmp[t];
+            __syncthreads();
+        }
+    }
+}
+
+// e_h += P(e_H - R e_h) (no guard for DGP)
+__global__ __launch_bounds__(256) void k_dgpm_correct(const float* __restrict__ eH, const float* __restrict__ rd,
+                                                      float* __restrict__ ef, Geo gf, Geo gc, int TX, int TY, int XB,
+                                                      unsigned NB)
+{
+    int I;
+    unsigned row;
+    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
+    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
+    const int nc = gc.n;
+    const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
+    const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
+    const int Xs[3] = {wrap_dec(I, nc), I, wrap_inc(I, nc)};
+    const int Ys[3] = {wrap_dec(J, nc), J, wrap_inc(J, nc)};
+    const int Zs[3] = {wrap_dec(K, nc), K, wrap_inc(K, nc)};
+    float ev[27];
+#pragma unroll
+    for (int z = 0; z < 3; z++)
+#pragma unroll
+        for (int y = 0; y < 3; y++)
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                const size_t q = split_index(gc, Xs[x], Ys[y], Zs[z]);
+                ev[x + 3 * y + 9 * z] = eH[q] - rd[q];
+            }
+    float old[8];
+#pragma unroll
+    for (int sind = 0; sind < 8; sind++)
+        old[sind] = ef[split_index(gf, 2 * I + (sind & 1), 2 * J + ((sind >> 1) & 1), 2 * K + (sind >> 2))];
+#pragma unroll
+    for (int sind = 0; sind < 8; sind++) {
+        const int s1 = sind & 1, s2 = (sind >> 1) & 1, s3 = sind >> 2;
+        float P = 0.0f;
+#pragma unroll
+        for (int ind = 0; ind < 8; ind++) {
+            const int x = (ind & 1) ? 1 : (s1 ? 2 : 0);
+            const int y = (ind & 2) ? 1 : (s2 ? 2 : 0);
+            const int z = (ind & 4) ? 1 : (s3 ? 2 : 0);
+            P += wt[ind] * ev[x + 3 * y + 9 * z];
+        }
+        ef[split_index(gf, 2 * I + s1, 2 * J + s2, 2 * K + s3)] = old[sind] + P;
+    }
+}
+
+// phi += e
+__global__ __launch_bounds__(256) void k_dgpm_update(double* __restrict__ u, const float* __restrict__ e, size_t count)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) u[i] = u[i] + (double)e[i];
+}
+
+int launch_dgpm_outer(const double* u, const double* rho, float* g32, float* pb, float* e, double* partials, int n,
+                      const DgpCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+    hipLaunchKernelGGL(k_dgpm_outer, L.grid, L.block, 0, s, u, rho, g32, pb, e, partials, g, c, L.TX, L.TY, L.XB,
+                       L.NB);
+    return (int)L.NB;
+}
+
+void launch_dgpm_smooth(float* e, const float* pb, const double* rho, const float* rhs, float* tmp, int n, int colour,
+                        const DgpCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, (long)n * n);
+    hipLaunchKernelGGL(k_dgpm_smooth, L.grid, L.block, 0, s, e, pb, rho, rhs, tmp, g, colour, dgpf(c), L.TX, L.TY,
+                       L.XB, L.NB);
+    launch_copy_f32(tmp, e + (size_t)colour * g.half, g.half, s);
+}
+
+void launch_dgpm_residual(const float* e, const float* pb, const double* rho, const float* rhs, float* out, int n,
+                          const DgpCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const Launch L = make_launch(g.h, 2L * n * n);
+    hipLaunchKernelGGL(k_dgpm_residual, L.grid, L.block, 0, s, e, pb, rho, rhs, out, g, dgpf(c), L.TX, L.TY, L.XB,
+                       L.NB);
+}
+
+void launch_dgpm_coarsest(float* e, const float* pb, const double* rho, const float* rhs, float* tmp, int n,
+                          int sweeps, const DgpCoef& c, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    int threads = (int)g.half;
+    if (threads > 1024) threads = 1024;
+    threads = ((threads + 63) / 64) * 64;
+    hipLaunchKernelGGL(k_dgpm_coarsest, dim3(1), dim3(threads), 0, s, e, pb, rho, rhs, tmp, g, sweeps, dgpf(c));
+}
+
+void launch_dgpm_correct(const float* eH, const float* rd, float* ef, int nc, hipStream_t s)
+{
+    const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
+    const Launch L = make_launch(nc, (long)nc * nc);
+    hipLaunchKernelGGL(k_dgpm_correct, L.grid, L.block, 0, s, eH, rd, ef, gf, gc, L.TX, L.TY, L.XB, L.NB);
+}
+
+void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s)
+{
+    const size_t count = (size_t)n * n * n;
+    size_t b = (count + 255) / 256;
+    if (b > 65536) b = 65536;
+    hipLaunchKernelGGL(k_dgpm_update, dim3((unsigned)b), dim3(256), 0, s, u, e, count);
 }
 
 }  // namespace mgs

@@ -328,6 +328,11 @@ void launch_copy_f64(const double* src, double* dst, size_t count, hipStream_t s
     hipLaunchKernelGGL((k_copy<double>), flat_grid(count), dim3(256), 0, s, src, dst, count);
 }
 
+void launch_copy_f32(const float* src, float* dst, size_t count, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_copy<float>), flat_grid(count), dim3(256), 0, s, src, dst, count);
+}
+
 void launch_fr_smooth64(int frn, int level, double* u, const double* rho, const double* f, int n, int colour,
                         const FrCoef& c, hipStream_t s)
 {

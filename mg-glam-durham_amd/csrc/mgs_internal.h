@@ -112,7 +112,17 @@ void launch_dgp_restrict_residual1_64(const double* u0, const double* delta, dou
                                       hipStream_t s);
 void launch_dgp_rhs64(const double* ru, const double* rho, const double* rres, double* f, int n, const DgpCoef& c,
                       hipStream_t s);
-int solve_mixed_dgp(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep);
+// mixed-precision DGP (FP32 delta-form inner problem)
+int launch_dgpm_outer(const double* u, const double* rho, float* g32, float* pb, float* e, double* partials, int n,
+                      const DgpCoef& c, hipStream_t s);
+void launch_dgpm_smooth(float* e, const float* pb, const double* rho, const float* rhs, float* tmp, int n, int colour,
+                        const DgpCoef& c, hipStream_t s);
+void launch_dgpm_residual(const float* e, const float* pb, const double* rho, const float* rhs, float* out, int n,
+                          const DgpCoef& c, hipStream_t s);
+void launch_dgpm_coarsest(float* e, const float* pb, const double* rho, const float* rhs, float* tmp, int n,
+                          int sweeps, const DgpCoef& c, hipStream_t s);
+void launch_dgpm_correct(const float* eH, const float* rd, float* ef, int nc, hipStream_t s);
+void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s);
 
 // mgs_fft.hip
 void fft_release(mgs_handle* h);

@@ -15,6 +15,7 @@ void launch_unpack_f64_to_f32(const double* split, float* lex, int n, double sca
 void launch_fill_f64(double* p, size_t count, double v, hipStream_t s);
 void launch_fill_f32(float* p, size_t count, float v, hipStream_t s);
 void launch_copy_f64(const double* src, double* dst, size_t count, hipStream_t s);
+void launch_copy_f32(const float* src, float* dst, size_t count, hipStream_t s);
 
 // ---- plain FP64 f(R) path (the reference's operators in FP64) ----------
 void launch_fr_smooth64(int frn, int level, double* u, const double* rho, const double* f, int n, int colour,
