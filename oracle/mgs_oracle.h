@@ -69,6 +69,7 @@ typedef struct orc_solve_opts {
     int32_t init_guess;   /* 1: start from u = u_bg (f(R)) / phi = 0 (DGP); 0: use given field */
     int32_t normalize;    /* f(R): 1 = divide the result by u_bg on exit (PMP2extradof.f90:131-146) */
     int32_t dgp_lexi;     /* DGP smoother order: 1 = reference sequential, 0 = colour-parallel */
+    int32_t cycle;        /* 0 V_cycle, 1 F_cycle, 2 W_cycle (the reference's Vcycle/Fcycle/Wcycle) */
 } orc_solve_opts;
 
 typedef struct orc_solve_report {
@@ -109,7 +110,9 @@ double orc_fr_fbg(const orc_params* p);
     int orc_solve_##SFX(const orc_params* p, STOR* u, const STOR* delta,                     \
                         const orc_solve_opts* o, orc_solve_report* rep);                    \
     int orc_vcycles_##SFX(const orc_params* p, STOR* u, const STOR* delta, int ncycles,      \
-                          int dgp_lexi, double* res_after);
+                          int dgp_lexi, double* res_after);                                 \
+    int orc_cycles_##SFX(const orc_params* p, STOR* u, const STOR* delta, int ncycles,       \
+                         int cycle, int dgp_lexi, double* res_after);
 
 ORC_DECLARE(f32, float)
 ORC_DECLARE(f64, double)

@@ -591,20 +591,23 @@ static void FN(smooth_any)(const orc_params* p, int l, int red, int dgp_lexi, FN
         FN(orc_dgp_smooth)(p, l, red, dgp_lexi, h->u[l], h->rho[l], h->f[l]);
 }
 
-static void FN(vcycle)(const orc_params* p, int l, int dgp_lexi, FN(orc_hier) * h)
+static void FN(cycle)(const orc_params* p, int l, int kind, int dgp_lexi, FN(orc_hier) * h);
+
+/* 2 x (red, black) sweeps, the Nsmth = 2 loop of every cycle type */
+static void FN(smooth2)(const orc_params* p, int l, int dgp_lexi, FN(orc_hier) * h)
 {
     const int Nsmth = 2;
-    if (l >= h->lc) {
-        for (int s = 0; s < Nsmth; s++) {
-            FN(smooth_any)(p, l, 1, dgp_lexi, h);
-            FN(smooth_any)(p, l, 0, dgp_lexi, h);
-        }
-        return;
-    }
     for (int s = 0; s < Nsmth; s++) {
         FN(smooth_any)(p, l, 1, dgp_lexi, h);
         FN(smooth_any)(p, l, 0, dgp_lexi, h);
     }
+}
+
+/* coarse-grid correction from level l: residual, restrictions, FAS RHS, a
+ * `kind` cycle on level l+1 and the prolongation (PMP2extradof.f90:418-545
+ * for V_cycle; the same sequence in F_cycle :650-840 and W_cycle :935-1140) */
+static void FN(cgc)(const orc_params* p, int l, int kind, int dgp_lexi, FN(orc_hier) * h)
+{
     const int nc = p->N >> (l + 1);
     if (l == 0) {
         /* the level-0 residual the reference computes here is never read:
@@ -628,12 +631,28 @@ static void FN(vcycle)(const orc_params* p, int l, int dgp_lexi, FN(orc_hier) * 
         FN(orc_fr_rhs)(p, l + 1, h->ru[l + 1], h->rho[l + 1], h->rres[l + 1], h->f[l + 1]);
     else
         FN(orc_dgp_rhs)(p, l + 1, h->ru[l + 1], h->rho[l + 1], h->rres[l + 1], h->f[l + 1]);
-    FN(vcycle)(p, l + 1, dgp_lexi, h);
+    FN(cycle)(p, l + 1, kind, dgp_lexi, h);
     FN(orc_correct)(nc, h->u[l + 1], h->ru[l + 1], h->u[l], p->model == 1);
-    for (int s = 0; s < Nsmth; s++) {
-        FN(smooth_any)(p, l, 1, dgp_lexi, h);
-        FN(smooth_any)(p, l, 0, dgp_lexi, h);
-    }
+}
+
+/* V_cycle (PMP2extradof.f90:352-555): S2, CGC(V), S2.
+ * F_cycle (:563-858): S2, CGC(F), S2, CGC(V), S2.
+ * W_cycle (:867-1157): S2, CGC(W), S2, CGC(W), S2.
+ * On the coarsest level each is S2 only. */
+static void FN(cycle)(const orc_params* p, int l, int kind, int dgp_lexi, FN(orc_hier) * h)
+{
+    FN(smooth2)(p, l, dgp_lexi, h);
+    if (l >= h->lc) return;
+    FN(cgc)(p, l, kind, dgp_lexi, h);
+    FN(smooth2)(p, l, dgp_lexi, h);
+    if (kind == 0) return;
+    FN(cgc)(p, l, kind == 1 ? 0 : 2, dgp_lexi, h);
+    FN(smooth2)(p, l, dgp_lexi, h);
+}
+
+static void FN(vcycle)(const orc_params* p, int l, int dgp_lexi, FN(orc_hier) * h)
+{
+    FN(cycle)(p, l, 0, dgp_lexi, h);
 }
 
 static double FN(res0)(const orc_params* p, FN(orc_hier) * h)
@@ -654,6 +673,19 @@ int FN(orc_vcycles)(const orc_params* p, STOR* u, const STOR* delta, int ncycles
     return 0;
 }
 
+int FN(orc_cycles)(const orc_params* p, STOR* u, const STOR* delta, int ncycles, int cycle, int dgp_lexi,
+                   double* res_after)
+{
+    FN(orc_hier) h;
+    FN(hier_alloc)(p, &h, u, delta);
+    for (int c = 0; c < ncycles; c++) {
+        FN(cycle)(p, 0, cycle, dgp_lexi, &h);
+        if (res_after) res_after[c] = FN(res0)(p, &h);
+    }
+    FN(hier_free)(&h);
+    return 0;
+}
+
 int FN(orc_solve)(const orc_params* p, STOR* u, const STOR* delta, const orc_solve_opts* o, orc_solve_report* rep)
 {
     const size_t N3 = (size_t)p->N * p->N * p->N;
@@ -668,7 +700,7 @@ int FN(orc_solve)(const orc_params* p, STOR* u, const STOR* delta, const orc_sol
     double res_old_1 = 1.0e5, res_old_2 = 1.0e10;
     int iter = 1;
     for (;;) {
-        FN(vcycle)(p, 0, o->dgp_lexi, &h);
+        FN(cycle)(p, 0, o->cycle, o->dgp_lexi, &h);
         const double res = FN(res0)(p, &h);
         if (iter <= 256) rep->history[iter - 1] = res;
         rep->cycles = iter;

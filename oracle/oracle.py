@@ -46,7 +46,7 @@ class OrcOpts(C.Structure):
     _fields_ = [
         ("mode", C.c_int32), ("max_cycles", C.c_int32), ("res_conv", C.c_double),
         ("rel_tol", C.c_double), ("init_guess", C.c_int32), ("normalize", C.c_int32),
-        ("dgp_lexi", C.c_int32),
+        ("dgp_lexi", C.c_int32), ("cycle", C.c_int32),
     ]
 
 
@@ -114,6 +114,8 @@ def lib():
             getattr(L, f"orc_dgp_rhs_{prec}").argtypes = [P, C.c_int, ptr, ptr, ptr, ptr]
             getattr(L, f"orc_solve_{prec}").argtypes = [P, ptr, ptr, C.POINTER(OrcOpts), C.POINTER(OrcReport)]
             getattr(L, f"orc_vcycles_{prec}").argtypes = [P, ptr, ptr, C.c_int, C.c_int, C.POINTER(C.c_double)]
+            getattr(L, f"orc_cycles_{prec}").argtypes = [P, ptr, ptr, C.c_int, C.c_int, C.c_int,
+                                                         C.POINTER(C.c_double)]
         L.orc_levelmax.argtypes = [C.c_int]
         L.orc_levelmin.argtypes = [C.c_int]
         L.orc_fr_ubg.argtypes = [P]
@@ -220,15 +222,25 @@ def vcycles(p, u, delta, ncycles, dgp_lexi=False, prec="f64"):
     return np.array(hist[:ncycles])
 
 
+def cycles(p, u, delta, ncycles, cycle=0, dgp_lexi=False, prec="f64"):
+    """ncycles V (0), F (1) or W (2) cycles in place on u; returns the level-0
+    RMS residual after each."""
+    hist = (C.c_double * max(ncycles, 1))()
+    getattr(lib(), f"orc_cycles_{prec}")(C.byref(p.c()), _ptr(u, prec), _ptr(delta, prec), ncycles, int(cycle),
+                                         int(dgp_lexi), hist)
+    return np.array(hist[:ncycles])
+
+
 def solve(p, delta, u=None, mode=1, rel_tol=1e-8, res_conv=1e-8, max_cycles=60, normalize=False,
-          dgp_lexi=False, prec="f64"):
+          dgp_lexi=False, prec="f64", cycle=0):
     """scalar_field-style solve.  mode 0: the reference's exits; mode 1: relative residual.
     Returns (u, report dict)."""
     dt = _dt(prec)
     delta = np.ascontiguousarray(delta, dtype=dt)
     init = u is None
     u = np.zeros_like(delta) if init else np.ascontiguousarray(u, dtype=dt).copy()
-    o = OrcOpts(mode, max_cycles, res_conv, rel_tol, 1 if init else 0, 1 if normalize else 0, int(dgp_lexi))
+    o = OrcOpts(mode, max_cycles, res_conv, rel_tol, 1 if init else 0, 1 if normalize else 0, int(dgp_lexi),
+                int(cycle))
     r = OrcReport()
     getattr(lib(), f"orc_solve_{prec}")(C.byref(p.c()), _ptr(u, prec), _ptr(delta, prec), C.byref(o), C.byref(r))
     rep = dict(cycles=r.cycles, converged=bool(r.converged), res0=r.res0, res_final=r.res_final,
@@ -274,6 +286,7 @@ class RefLib:
         L.ref_residual.argtypes = [C.c_int]
         L.ref_residual.restype = C.c_double
         L.ref_vcycles.argtypes = [C.c_int, C.POINTER(C.c_double)]
+        L.ref_cycles.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double)]
         self.L = L
         self.N = None
 
@@ -328,6 +341,12 @@ class RefLib:
     def vcycles(self, n):
         h = (C.c_double * n)()
         self.L.ref_vcycles(n, h)
+        return np.array(h[:])
+
+    def cycles(self, n, cycle):
+        """n F_cycle(0) (cycle 1) or W_cycle(0) (cycle 2) calls, residual after each."""
+        h = (C.c_double * n)()
+        self.L.ref_cycles(n, cycle, h)
         return np.array(h[:])
 
     def scalar_field(self, istep=1):

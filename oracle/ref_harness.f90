@@ -185,6 +185,29 @@ contains
       end do
    end subroutine
 
+   subroutine ref_cycles(ncycles, cycle, hist) bind(C, name='ref_cycles')
+      integer(c_int), value :: ncycles, cycle
+      real(c_double), intent(out) :: hist(*)
+      integer :: c, l0
+      real*8 :: res
+      l0 = 0
+      do c = 1, ncycles
+         if (cycle == 1) then
+            call F_cycle(l0)
+         else if (cycle == 2) then
+            call W_cycle(l0)
+         else
+            call V_cycle(l0)
+         end if
+         if (MG_model == 1) then
+            call calculate_residual_fR(l0, res)
+         else
+            call calculate_residual_DGP(l0, res)
+         end if
+         hist(c) = res
+      end do
+   end subroutine
+
    subroutine ref_scalar_field(istep) bind(C, name='ref_scalar_field')
       integer(c_int), value :: istep
       integer :: s

@@ -19,6 +19,9 @@ Fixtures (all little-endian numpy .npz, float32 fields):
                                 field, 32^3: residual history + final field
   scalar_field_fr.npz           scalar_field(1) with the reference's own
                                 exits, 32^3: final normalised field
+  cycles_{F,W}_{fr,dgp}.npz     3 F_cycle / W_cycle calls, 16^3: residual
+                                history + final field
+(`python make_golden.py cycles` writes only the last group.)
 """
 import os
 import sys
@@ -174,5 +177,30 @@ def main():
     print("scalar_field_fr done")
 
 
+def cycle_fixtures():
+    """F_cycle and W_cycle (PMP2extradof.f90:563-1157), 3 calls each from
+    the reference's initial field, 16^3, both models."""
+    R = O.RefLib()
+    rng = np.random.default_rng(20261202)
+    N = 16
+    for model, name in ((1, "fr"), (2, "dgp")):
+        for cyc, cname in ((1, "F"), (2, "W")):
+            p = O.Params(N=N, model=model, fr_n=1, fr0=-1.0e-5, h0rc=1.0, **PSETS["a05"])
+            R.init(p, iter_max=3)
+            delta = (rng.lognormal(0.0, 1.0, (N, N, N)) - np.exp(0.5)).astype(np.float32)
+            R.put(1, delta)
+            u0 = R.get(2)
+            hist = R.cycles(3, cyc)
+            np.savez_compressed(os.path.join(HERE, f"cycles_{cname}_{name}.npz"), delta=delta, u_in=u0, hist=hist,
+                                u_out=R.get(2), cycle=np.array([cyc]),
+                                params=np.array([p.N, p.model, p.fr_n, p.branch, p.box, p.om, p.oml, p.a, p.fr0,
+                                                 p.h0rc]))
+            print(f"cycles_{cname}_{name}", hist)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["cycles"]:
+        cycle_fixtures()
+    else:
+        main()
+        cycle_fixtures()

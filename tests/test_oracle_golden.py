@@ -95,6 +95,18 @@ def test_vcycles_match_reference(name, model):
     same(u, fx["u_out"], "field after 6 V-cycles")
 
 
+@pytest.mark.parametrize("name", ["cycles_F_fr", "cycles_W_fr", "cycles_F_dgp", "cycles_W_dgp"])
+def test_f_and_w_cycles_match_reference(name):
+    """F_cycle and W_cycle (PMP2extradof.f90:563-1157): three calls from the
+    reference's initial field, residual after each and the final field."""
+    fx = np.load(os.path.join(GOLDEN, name + ".npz"))
+    p = params_of(fx)
+    u = fx["u_in"].copy()
+    hist = O.cycles(p, u, fx["delta"].copy(), len(fx["hist"]), cycle=int(fx["cycle"][0]), dgp_lexi=True, prec="f32")
+    assert np.allclose(hist, fx["hist"], rtol=1e-12, atol=0)
+    same(u, fx["u_out"], f"field after 3 {name[7]}-cycles")
+
+
 def test_scalar_field_matches_reference():
     """scalar_field(1) with the reference's exits and iter_count_max = 8,
     including the final division by u_bg (PMP2extradof.f90:131-146)."""
