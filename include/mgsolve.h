@@ -16,6 +16,7 @@
  *   scalar_field(i_step)                        PMP2extradof.f90:12
  *   initialise_extradof                         PMP2extradof.f90:181
  *   V_cycle(ilevel)                             PMP2extradof.f90:352
+ *   F_cycle(ilevel), W_cycle(ilevel)            PMP2extradof.f90:563, 867
  *   restrict_density(ilevel)                    PMP2extradof.f90:1168
  *   restrict_scalar_field(ilevel)               PMP2extradof.f90:1262
  *   correct_solution(ilevel)                    PMP2extradof.f90:1361
@@ -106,6 +107,9 @@ typedef struct mgs_solve_opts {
     double rel_tol;          /* mode 1: stop when RMS(r)/RMS(r0) < rel_tol */
     int32_t init_guess;      /* 1: start from u = u_bg (f(R)) / phi = 0 (DGP), as initialise_extradof */
     int32_t inner_cycles;    /* mixed: inner FP32 V-cycles per outer FP64 iteration (default 1) */
+    int32_t cycle;           /* plain FP64: 0 V_cycle, 1 F_cycle, 2 W_cycle (the reference's Vcycle/Fcycle/
+                                Wcycle switches, PMP2extradof.f90:86-88); the mixed form uses V-cycles */
+    int32_t reserved;
 } mgs_solve_opts;
 
 typedef struct mgs_solve_report {
@@ -156,6 +160,8 @@ int mgs_restrict_density(mgs_handle* h, int32_t level);             /* restrict_
 int mgs_rhs(mgs_handle* h, int32_t level);                          /* calculate_physical_right_hand_side_* */
 int mgs_correct(mgs_handle* h, int32_t level);                      /* correct_solution */
 int mgs_vcycle(mgs_handle* h, int32_t level);                       /* V_cycle */
+int mgs_fcycle(mgs_handle* h, int32_t level);                       /* F_cycle, PMP2extradof.f90:563 */
+int mgs_wcycle(mgs_handle* h, int32_t level);                       /* W_cycle, PMP2extradof.f90:867 */
 
 /* ---- whole solve (scalar_field) -------------------------------------- */
 int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* report);

@@ -443,6 +443,47 @@ int vcycle_plain(mgs_handle* h, int l)
     return 0;
 }
 
+// coarse-grid correction from level l with a `kind` cycle (0 V, 1 F, 2 W) on
+// level l+1: the sequence V_cycle, F_cycle and W_cycle share
+// (PMP2extradof.f90:418-545, 650-840, 935-1140)
+static int cycle_plain(mgs_handle* h, int l, int kind);
+
+static int cgc_plain(mgs_handle* h, int l, int kind)
+{
+    if (l > 0) op_residual(h, l, true, nullptr);
+    op_restrict_residual(h, l + 1);
+    op_restrict_field(h, l + 1);
+    op_rhs(h, l + 1);
+    int rc = cycle_plain(h, l + 1, kind);
+    if (rc) return rc;
+    op_correct(h, l + 1);
+    return 0;
+}
+
+static void smooth2_plain(mgs_handle* h, int l)
+{
+    const int Nsmth = 2;
+    for (int s = 0; s < Nsmth; s++) {
+        op_smooth(h, l, 1);
+        op_smooth(h, l, 0);
+    }
+}
+
+// F_cycle (PMP2extradof.f90:563-858): S2, CGC(F), S2, CGC(V), S2;
+// W_cycle (:867-1157): S2, CGC(W), S2, CGC(W), S2; S2 only on the coarsest
+static int cycle_plain(mgs_handle* h, int l, int kind)
+{
+    if (kind == 0) return vcycle_plain(h, l);
+    int rc;
+    smooth2_plain(h, l);
+    if (l >= h->lc) return 0;
+    if ((rc = cgc_plain(h, l, kind))) return rc;
+    smooth2_plain(h, l);
+    if ((rc = cgc_plain(h, l, kind == 1 ? 0 : 2))) return rc;
+    smooth2_plain(h, l);
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // mixed precision: FP32 delta-form FAS V-cycle (inner) and FP64 outer loop
 // ---------------------------------------------------------------------------
@@ -550,7 +591,7 @@ static int solve_plain(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
     int iter = 1;
     bool conv = false;
     for (;;) {
-        vcycle_plain(h, 0);
+        if ((rc = cycle_plain(h, 0, o.cycle))) return rc;
         double res = 0.0;
         if ((rc = op_residual(h, 0, false, &res))) return rc;
         if (iter <= 128) rep->history[iter - 1] = res;
@@ -1092,12 +1133,29 @@ int mgs_vcycle(mgs_handle* h, int32_t level)
     SYNC_RET(h);
 }
 
+static int cycle_entry(mgs_handle* h, int32_t level, int kind)
+{
+    if (!h) return MGS_E_ARG;
+    if (check_level(h, level, 0)) return MGS_E_ARG;
+    if (!h->has_density) return set_err(h, MGS_E_STATE, "no density set");
+    cache_densities(h);
+    cycle_plain(h, level, kind);
+    SYNC_RET(h);
+}
+
+int mgs_fcycle(mgs_handle* h, int32_t level) { return cycle_entry(h, level, 1); }
+
+int mgs_wcycle(mgs_handle* h, int32_t level) { return cycle_entry(h, level, 2); }
+
 int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* report)
 {
     if (!h || !opts || !report) return set_err(h, MGS_E_ARG, "NULL argument");
     if (!h->has_density) return set_err(h, MGS_E_STATE, "no density set (mgs_set_density)");
     std::memset(report, 0, sizeof(*report));
     if (opts->max_cycles < 1) return set_err(h, MGS_E_ARG, "max_cycles must be >= 1");
+    if (opts->cycle < 0 || opts->cycle > 2) return set_err(h, MGS_E_ARG, "cycle must be 0 (V), 1 (F) or 2 (W)");
+    if (opts->cycle != 0 && h->p.precision == MGS_PREC_MIXED)
+        return set_err(h, MGS_E_ARG, "the mixed-precision solve uses V-cycles (cycle = 0)");
     hipEvent_t e0, e1;
     HIPCHK(h, hipEventCreate(&e0));
     HIPCHK(h, hipEventCreate(&e1));
@@ -1108,9 +1166,7 @@ int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* repor
         launch_fill_f64(h->L[0].u, (size_t)n3d(h->p.ngrid), h->p.model == MGS_MODEL_FR ? ubg : 0.0, h->s);
     }
     int rc;
-    if (h->p.precision == MGS_PREC_MIXED && h->p.model == MGS_MODEL_FR)
-        rc = solve_mixed(h, *opts, report);
-    else if (h->p.precision == MGS_PREC_MIXED)
+    if (h->p.precision == MGS_PREC_MIXED)
         rc = solve_mixed(h, *opts, report);
     else
         rc = solve_plain(h, *opts, report);

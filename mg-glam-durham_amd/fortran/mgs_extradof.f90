@@ -5,6 +5,7 @@
 !   scalar_field(i_step)                          PMP2extradof.f90:12-178
 !   initialise_extradof                           PMP2extradof.f90:181-344
 !   V_cycle(ilevel)                               PMP2extradof.f90:352-555
+!   F_cycle(ilevel), W_cycle(ilevel)              PMP2extradof.f90:563-1157
 !   restrict_density(ilevel)                      PMP2extradof.f90:1168
 !   restrict_scalar_field(ilevel)                 PMP2extradof.f90:1262
 !   correct_solution(ilevel)                      PMP2extradof.f90:1361
@@ -33,7 +34,7 @@ module mgs_extradof
 
   public :: mgs_configure, mgs_attach_fields, mgs_release, mgs_fields_to_device, mgs_fields_to_host
   public :: mgs_handle_of
-  public :: scalar_field, initialise_extradof, V_cycle
+  public :: scalar_field, initialise_extradof, V_cycle, F_cycle, W_cycle
   public :: restrict_density, restrict_scalar_field, correct_solution
   public :: relaxation_iterations_fR, calculate_residual_fR, restrict_residual_fR
   public :: calculate_physical_right_hand_side_fR
@@ -43,6 +44,8 @@ module mgs_extradof
   ! run control of the reference's scalar_field (PMP2mod_tools.f90 scalars)
   integer, public, save :: iter_count_max = 100
   real(8), public, save :: res_conv = 1.0d-8
+  ! cycle type of scalar_field, as the reference's switches (PMP2extradof.f90:86-88)
+  integer, public, save :: Vcycle = 1, Fcycle = 0, Wcycle = 0
   logical, public, save :: print_residuals = .true.
 
   type(c_ptr), save :: h = c_null_ptr
@@ -170,6 +173,10 @@ contains
     o%rel_tol = 0.0d0
     o%init_guess = 0
     o%inner_cycles = 1
+    o%cycle = 0
+    if (Fcycle == 1) o%cycle = 1
+    if (Wcycle == 1) o%cycle = 2
+    o%reserved = 0
     call check(mgs_solve(h, o, r), 'mgs_solve')
     if (print_residuals) then
       do it = 1, min(int(r%outer), 128)
@@ -201,6 +208,18 @@ contains
     call ensure_handle()
     call check(mgs_vcycle(h, ilevel), 'V_cycle')
   end subroutine V_cycle
+
+  subroutine F_cycle(ilevel)
+    integer, intent(in) :: ilevel
+    call ensure_handle()
+    call check(mgs_fcycle(h, ilevel), 'F_cycle')
+  end subroutine F_cycle
+
+  subroutine W_cycle(ilevel)
+    integer, intent(in) :: ilevel
+    call ensure_handle()
+    call check(mgs_wcycle(h, ilevel), 'W_cycle')
+  end subroutine W_cycle
 
   subroutine restrict_density(ilevel)
     integer, intent(in) :: ilevel

@@ -26,6 +26,7 @@ module mgsolve
     integer(c_int32_t) :: mode, max_cycles
     real(c_double) :: res_conv, rel_tol
     integer(c_int32_t) :: init_guess, inner_cycles
+    integer(c_int32_t) :: cycle, reserved
   end type mgs_solve_opts
 
   type, bind(C) :: mgs_solve_report
@@ -137,6 +138,16 @@ module mgsolve
       integer(c_int32_t), value :: level
     end function
     integer(c_int) function mgs_vcycle(h, level) bind(C, name='mgs_vcycle')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: level
+    end function
+    integer(c_int) function mgs_fcycle(h, level) bind(C, name='mgs_fcycle')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: level
+    end function
+    integer(c_int) function mgs_wcycle(h, level) bind(C, name='mgs_wcycle')
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: h
       integer(c_int32_t), value :: level

@@ -13,6 +13,8 @@ the reference's Fortran routines:
     Solver.rhs(level)                calculate_physical_right_hand_side_fR/_DGP
     Solver.correct(level)            correct_solution
     Solver.vcycle(level)             V_cycle
+    Solver.fcycle(level)             F_cycle               PMP2extradof.f90:563
+    Solver.wcycle(level)             W_cycle               PMP2extradof.f90:867
     Solver.solve(...)                scalar_field
 
 There is no CPU fallback: if the library is missing, or no GPU is visible,
@@ -54,7 +56,7 @@ class Params(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [
         ("mode", C.c_int32), ("max_cycles", C.c_int32), ("res_conv", C.c_double), ("rel_tol", C.c_double),
-        ("init_guess", C.c_int32), ("inner_cycles", C.c_int32),
+        ("init_guess", C.c_int32), ("inner_cycles", C.c_int32), ("cycle", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -130,6 +132,8 @@ def load():
         "mgs_rhs": ([H, i32], C.c_int),
         "mgs_correct": ([H, i32], C.c_int),
         "mgs_vcycle": ([H, i32], C.c_int),
+        "mgs_fcycle": ([H, i32], C.c_int),
+        "mgs_wcycle": ([H, i32], C.c_int),
         "mgs_solve": ([H, C.POINTER(SolveOpts), C.POINTER(SolveReport)], C.c_int),
         "mgs_fifth_force": ([H, vp, vp, vp, vp, i32, i32], C.c_int),
         "mgs_power_spectrum": ([H, i32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
@@ -299,8 +303,16 @@ class Solver:
     def vcycle(self, level=0):
         self._chk(self.L.mgs_vcycle(self.h, level), "mgs_vcycle")
 
-    def solve(self, mode=1, max_cycles=60, res_conv=1.0e-8, rel_tol=1.0e-8, init_guess=True, inner_cycles=1):
-        o = SolveOpts(mode, max_cycles, res_conv, rel_tol, 1 if init_guess else 0, inner_cycles)
+    def fcycle(self, level=0):
+        self._chk(self.L.mgs_fcycle(self.h, level), "mgs_fcycle")
+
+    def wcycle(self, level=0):
+        self._chk(self.L.mgs_wcycle(self.h, level), "mgs_wcycle")
+
+    def solve(self, mode=1, max_cycles=60, res_conv=1.0e-8, rel_tol=1.0e-8, init_guess=True, inner_cycles=1,
+              cycle=0):
+        """scalar_field: cycle 0 V_cycle, 1 F_cycle, 2 W_cycle (plain FP64)."""
+        o = SolveOpts(mode, max_cycles, res_conv, rel_tol, 1 if init_guess else 0, inner_cycles, int(cycle), 0)
         r = SolveReport()
         self._chk(self.L.mgs_solve(self.h, C.byref(o), C.byref(r)), "mgs_solve")
         n = r.outer if self.cfg.precision == PREC_MIXED else r.cycles

@@ -191,6 +191,45 @@ def test_vcycles_fp64_vs_oracle(model):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("model,cycle", [(1, 1), (1, 2), (2, 1), (2, 2)])
+def test_f_and_w_cycles_fp64_vs_oracle(model, cycle):
+    """Three F_cycle / W_cycle calls (PMP2extradof.f90:563-1157) from the
+    same start: residual history and field against the FP64 oracle, and a
+    scalar_field solve with that cycle type."""
+    name = f"cycles_{'F' if cycle == 1 else 'W'}_{'fr' if model == 1 else 'dgp'}"
+    fx = np.load(os.path.join(GOLDEN, name + ".npz"))
+    op, gc = cfgs(fx["params"])
+    print(f"[cycles] {name} N={op.N}", flush=True)
+    delta = fx["delta"].astype(np.float64)
+    u0 = fx["u_in"].astype(np.float64)
+    u = u0.copy()
+    hist_o = O.cycles(op, u, delta, 3, cycle=cycle, dgp_lexi=False, prec="f64")
+    with M.Solver(gc) as S:
+        S.set_density(delta)
+        S.set_field(u0)
+        hist_g = []
+        for _ in range(3):
+            (S.fcycle if cycle == 1 else S.wcycle)(0)
+            hist_g.append(S.residual(0))
+        ug = S.get_field()
+        S.set_field(u0)
+        rep = S.solve(mode=0, res_conv=1e-30, max_cycles=4, init_guess=False, cycle=cycle)
+        us = S.get_field()
+    hist_g = np.array(hist_g)
+    u_s = u0.copy()
+    hist_s = O.cycles(op, u_s, delta, rep["cycles"], cycle=cycle, dgp_lexi=False, prec="f64")
+    if model == 1:
+        assert np.allclose(hist_g, hist_o, rtol=1e-9, atol=1e-12 * hist_o[0]), (hist_g, hist_o)
+        assert max_rel(ug, u) <= 1e-9
+        assert max_rel(us, u_s) <= 1e-9
+    else:
+        assert np.allclose(hist_g, hist_o, rtol=1e-13, atol=0), (hist_g, hist_o)
+        exact(ug, u, f"DGP field after 3 {name} cycles")
+        exact(us, u_s, "DGP field after the solve")
+    assert np.allclose(rep["history"], hist_s, rtol=1e-9, atol=1e-12 * hist_s[0])
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("frn", [0, 1, 2])
 def test_fp64_solve_matches_oracle(frn):
     """scalar_field to relative residual 1e-8 on a seeded lognormal field."""
