@@ -504,6 +504,19 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
         prof_end(h);
         return 0;
     }
+    if (l > 0 && v.n <= TAIL_LEVEL_MAX_N && h->use_tail) {
+        // the small levels in one launch
+        TailLevelIn lv[16];
+        const int nlev = h->lc - l + 1;
+        for (int i = 0; i < nlev && i < 16; i++) {
+            const LevelF& m = h->M[l + i];
+            lv[i] = TailLevelIn{m.n, m.d, m.xb, m.f, m.rres, m.rd, df_coefs(h->p, l + i)};
+        }
+        prof_begin(h, MGS_KC_COARSE, 12.0 * n3d(v.n) * 4.0);
+        const bool ok = launch_df_tail(frn, nlev, lv, h->s);
+        prof_end(h);
+        if (ok) return 0;
+    }
     const double s = 4.0;
     for (int it = (l == 0 && first) ? 1 : 0; it < 2 * Nsmth; it++) {
         prof_begin(h, cls, 2.0 * n3d(v.n) * s);
@@ -814,6 +827,10 @@ int mgs_create(const mgs_params* params, mgs_handle** out)
         return set_err(nullptr, MGS_E_HIP, "no HIP device available: libmgsolve has no CPU fallback");
     mgs_handle* h = new mgs_handle();
     h->p = *params;
+    {
+        const char* e = std::getenv("MGS_NO_TAIL");
+        h->use_tail = !(e && e[0] == '1');
+    }
     h->lmax = levelmax_of(params->ngrid);
     h->lmin = levelmin_of(params->ngrid);
     h->lc = h->lmax - h->lmin < 0 ? 0 : h->lmax - h->lmin;

@@ -246,6 +246,12 @@ __global__ __launch_bounds__(1024) void k_df_coarsest(float* __restrict__ d, con
     df_coarsest_block<FRN>(d, xb, rhs, g, sweeps, c);
 }
 
+template <int FRN>
+__global__ __launch_bounds__(1024) void k_df_tail(TailDesc T)
+{
+    df_tail_vcycle<FRN>(T);
+}
+
 // ===========================================================================
 // launchers
 // ===========================================================================
@@ -491,6 +497,26 @@ void launch_df_correct(int frn, const float* dH, const float* rd, float* dfine, 
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
     const ZLaunch L = make_zlaunch(nc, nc);
     FRN_DISPATCH(frn, k_df_correct, dH, rd, dfine, xbfine, gf, gc, ZARGS);
+}
+
+bool launch_df_tail(int frn, int nlev, const TailLevelIn* lv, hipStream_t s)
+{
+    if (nlev < 2 || nlev > TAIL_MAXL) return false;
+    TailDesc T;
+    T.nl = nlev - 1;
+    for (int i = 0; i < nlev; i++) {
+        T.L[i].g = make_geo(lv[i].n);
+        T.L[i].d = lv[i].d;
+        T.L[i].xb = lv[i].xb;
+        T.L[i].f = lv[i].f;
+        T.L[i].rres = lv[i].rres;
+        T.L[i].rd = lv[i].rd;
+        T.L[i].dc = lv[i].dc;
+    }
+    if (frn == 0) hipLaunchKernelGGL((k_df_tail<0>), dim3(1), dim3(1024), 0, s, T);
+    else if (frn == 1) hipLaunchKernelGGL((k_df_tail<1>), dim3(1), dim3(1024), 0, s, T);
+    else hipLaunchKernelGGL((k_df_tail<2>), dim3(1), dim3(1024), 0, s, T);
+    return true;
 }
 
 void launch_df_coarsest(int frn, float* d, const float* xb, const float* rhs, int n, int sweeps, const DfCoef& c,

@@ -702,7 +702,8 @@ __device__ __forceinline__ void df_coarsest_block(float* __restrict__ d, const f
             }
             __syncthreads();
         }
-    }}
+    }
+}
 
 // 8-child average (restrict_density, restrict_scalar_field,
 // restrict_residual_fR at levels > 1, PMP2extradof.f90:1198-1247,
@@ -725,5 +726,104 @@ __device__ __forceinline__ void restrict8_cell(const T* __restrict__ fine, T* __
     const T v = P / (T)8.0;
     out1[o] = v;
     if (out2) out2[o] = v;}
+
+
+// ---------------------------------------------------------------------------
+// Tail of the inner V-cycle: every level from l0 (at most 16^3 cells) down to
+// the coarsest, run by one workgroup in one launch.  The coarse levels are
+// too small to fill the GPU and a launch per operation would cost more than
+// the work; here each operation is one pass of the workgroup over the
+// level's cells, separated by barriers.  Per cell the arithmetic is that of
+// the level kernels (a red-black pass, the restrictions and the
+// prolongation do not depend on the order cells are visited in), so the
+// results are identical to the launch-per-operation cycle.
+// ---------------------------------------------------------------------------
+struct TailLevel {
+    Geo g;
+    float *d, *xb, *f, *rres, *rd;
+    DfCoef dc;
+};
+
+constexpr int TAIL_MAXL = 8;
+constexpr int TAIL_MAX_N = 16;  // levels with n <= 16 run in the tail
+
+struct TailDesc {
+    int nl;  // levels below the first (L[nl] is the coarsest)
+    TailLevel L[TAIL_MAXL];
+};
+
+// cell t of a colour half -> the six neighbours' offsets and its own
+__device__ __forceinline__ Nb tail_cell(const Geo& g, int colour, int t)
+{
+    const int m = t % g.h, row = t / g.h;
+    return neighbours(g, colour, m, row % g.n, row / g.n);
+}
+
+template <class OP>
+__device__ __forceinline__ void tail_update(const OP& op, const Geo& g, int colour, int t)
+{
+    const Nb q = tail_cell(g, colour, t);
+    const auto* a = op.field();
+    const unsigned oo = q.me + (colour ? 0u - (unsigned)g.half : (unsigned)g.half);
+    op.update(q.me, oo, op.load(q.me, oo), a[oo], a[q.xp], a[q.xm], a[q.yp], a[q.ym], a[q.zp], a[q.zm]);
+}
+
+template <class OP>
+__device__ __forceinline__ void tail_cell_op(const OP& op, const Geo& g, int t)
+{
+    const int colour = t >= (int)g.half ? 1 : 0;
+    const Nb q = tail_cell(g, colour, t - colour * (int)g.half);
+    const auto* a = op.field();
+    (void)op.cell(q.me, op.load(q.me), a[q.me], a[q.xp], a[q.xm], a[q.yp], a[q.ym], a[q.zp], a[q.zm]);
+}
+
+__device__ __forceinline__ void coarse_ijk(const Geo& gc, int t, int& I, int& J, int& K)
+{
+    const unsigned n = (unsigned)gc.n;
+    I = (int)((unsigned)t % n);
+    J = (int)(((unsigned)t / n) % n);
+    K = (int)((unsigned)t / (n * n));
+}
+
+template <int FRN>
+__device__ void df_tail_vcycle(const TailDesc& T)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int i = 0; i < T.nl; i++) {
+        const TailLevel &L = T.L[i], &C = T.L[i + 1];
+        for (int it = 0; it < 4; it++) {
+            for (int t = tid; t < (int)L.g.half; t += nthr)
+                tail_update(DfSmoothOp<FRN>{L.d, L.xb, L.f, L.dc}, L.g, it & 1, t);
+            __syncthreads();
+        }
+        const int ncc = C.g.n * C.g.n * C.g.n;
+        for (int t = tid; t < ncc; t += nthr) {
+            int I, J, K;
+            coarse_ijk(C.g, t, I, J, K);
+            df_restrict_residual_col<FRN>(L.d, L.xb, L.f, C.rres, L.g, C.g, L.dc, ZCol{I, J, K, K + 1});
+            restrict8_cell<float>(L.d, C.rd, C.d, L.g, C.g, I, J, K);
+        }
+        __syncthreads();
+        for (int t = tid; t < ncc; t += nthr) tail_cell_op(DfResOp<FRN>{C.rd, C.xb, C.rres, C.f, C.dc}, C.g, t);
+        __syncthreads();
+    }
+    const TailLevel& B = T.L[T.nl];
+    df_coarsest_block<FRN>(B.d, B.xb, B.f, B.g, 8, B.dc);
+    for (int i = T.nl - 1; i >= 0; i--) {
+        const TailLevel &L = T.L[i], &C = T.L[i + 1];
+        const int ncc = C.g.n * C.g.n * C.g.n;
+        for (int t = tid; t < ncc; t += nthr) {
+            int I, J, K;
+            coarse_ijk(C.g, t, I, J, K);
+            df_correct_col<FRN>(C.d, C.rd, L.d, L.xb, L.g, C.g, ZCol{I, J, K, K + 1});
+        }
+        __syncthreads();
+        for (int it = 0; it < 4; it++) {
+            for (int t = tid; t < (int)L.g.half; t += nthr)
+                tail_update(DfSmoothOp<FRN>{L.d, L.xb, L.f, L.dc}, L.g, it & 1, t);
+            __syncthreads();
+        }
+    }
+}
 
 }  // namespace mgs

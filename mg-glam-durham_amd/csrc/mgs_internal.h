@@ -79,6 +79,7 @@ struct mgs_handle {
     hipGraphExec_t g_plain = nullptr;
     hipGraphExec_t g_inner = nullptr;
     bool use_graphs = true;
+    bool use_tail = true;  // mixed f(R): small inner levels in one launch (MGS_NO_TAIL=1 turns it off)
     // profiling
     bool prof = false;
     std::vector<mgs::ProfRec> recs;

@@ -58,5 +58,14 @@ void launch_df_correct(int frn, const float* dH, const float* rd, float* dfine, 
                        hipStream_t s);
 void launch_df_coarsest(int frn, float* d, const float* xb, const float* rhs, int n, int sweeps, const DfCoef& c,
                         hipStream_t s);
+// the inner V-cycle from level lv[0] (<= 16^3 cells) to the coarsest lv[nlev-1]
+// in one single-workgroup launch; false if nlev is out of range
+struct TailLevelIn {
+    int n;
+    float *d, *xb, *f, *rres, *rd;
+    DfCoef dc;
+};
+constexpr int TAIL_LEVEL_MAX_N = 16;
+bool launch_df_tail(int frn, int nlev, const TailLevelIn* lv, hipStream_t s);
 
 }  // namespace mgs
