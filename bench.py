@@ -127,12 +127,14 @@ def dgp_compare(N, box, sigma, seed, steps):
         out[key] = {"ms_median": float(np.median([x["seconds"] * 1e3 for x in reps])), "cycles": r["cycles"],
                     "outer": r["outer"], "res0": r["res0"], "res_final": r["res_final"]}
         out[key + "_pk"] = pk
-    u64, um = fields[M.PREC_FP64], fields[M.PREC_MIXED]
+    # phi is defined up to a constant (L involves only its differences)
+    u64 = fields[M.PREC_FP64] - fields[M.PREC_FP64].mean()
+    um = fields[M.PREC_MIXED] - fields[M.PREC_MIXED].mean()
     d = np.abs(um - u64)
     pk64, pkm = out.pop("fp64_pk"), out.pop("mixed_pk")
     out["mixed_vs_fp64"] = {
-        "u_rel_linf": float(d.max() / np.abs(u64).max()),
-        "u_rel_l2": float(np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(u64 ** 2))),
+        "u_rel_linf_mean_removed": float(d.max() / np.abs(u64).max()),
+        "u_rel_l2_mean_removed": float(np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(u64 ** 2))),
         "fifth_force_pk_max_abs_ratio_minus_1": float(np.max(np.abs(pkm[pk64 > 0] / pk64[pk64 > 0] - 1.0))),
     }
     out["speedup_mixed_vs_fp64"] = out["fp64"]["ms_median"] / out["mixed"]["ms_median"]
@@ -140,7 +142,7 @@ def dgp_compare(N, box, sigma, seed, steps):
                        f"seed={seed}, both solved to the reference's stagnation exit")
     log(f"[dgp] fp64 {out['fp64']['ms_median']:.1f} ms ({out['fp64']['cycles']} cycles), mixed "
         f"{out['mixed']['ms_median']:.1f} ms ({out['mixed']['outer']} outer), u rel L_inf "
-        f"{out['mixed_vs_fp64']['u_rel_linf']:.2e}")
+        f"{out['mixed_vs_fp64']['u_rel_linf_mean_removed']:.2e} (mean removed)")
     return out
 
 

@@ -302,6 +302,10 @@ def test_mixed_dgp_matches_fp64(branch, sigma):
           f"mixed: outer={rm['outer']} res -> {rm['res_final']:.3e}", flush=True)
     # the outer residual is the FP64 operator's: same floor
     assert rm["res_final"] <= 1.01 * r64["res_final"] + 1e-12 * r64["res0"]
+    # L(phi) involves only differences of phi: a constant shift of phi is
+    # free, and each solver drifts in it differently; compare without it
+    u64 = u64 - u64.mean()
+    um = um - um.mean()
     scale = np.max(np.abs(u64))
     err = np.max(np.abs(um - u64)) / scale
     print(f"  field rel L_inf {err:.2e}", flush=True)
