@@ -271,7 +271,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fr512", choices=sorted(WORKLOADS))
-    ap.add_argument("--inner", type=int, default=1, help="inner FP32 V-cycles per outer FP64 iteration")
+    ap.add_argument("--inner", type=int, default=0,
+                    help="inner FP32 V-cycles per outer FP64 iteration (0: chosen per outer iteration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=2)
@@ -372,7 +373,7 @@ def main():
             "workload": f"f(R) Hu-Sawicki n=1 |fR0|={abs(fr0):g} Om=0.3 a=1, {N}^3, box {box:g} Mpc/h, "
                         f"lognormal sigma_g={sigma} seed={seed}, rel. residual 1e-8 from u=u_bg",
             "N": N,
-            "inner_cycles_per_outer": args.inner,
+            "inner_cycles_per_outer": args.inner if args.inner > 0 else "auto",
         },
         "latency_ms": ({"p50": serving["p50_ms"], "p99": serving["p99_ms"],
                         "source": "serving stream (Poisson arrivals), submit -> result"} if serving else None),

@@ -106,7 +106,9 @@ typedef struct mgs_solve_opts {
     double res_conv;         /* mode 0 threshold on the level-0 RMS residual */
     double rel_tol;          /* mode 1: stop when RMS(r)/RMS(r0) < rel_tol */
     int32_t init_guess;      /* 1: start from u = u_bg (f(R)) / phi = 0 (DGP), as initialise_extradof */
-    int32_t inner_cycles;    /* mixed: inner FP32 V-cycles per outer FP64 iteration (default 1) */
+    int32_t inner_cycles;    /* mixed: inner FP32 V-cycles per outer FP64 iteration; 0: mode 1 picks each
+                                outer iteration's count (1 to 4) from the contraction measured so far,
+                                mode 0 runs 1 */
     int32_t cycle;           /* plain FP64: 0 V_cycle, 1 F_cycle, 2 W_cycle (the reference's Vcycle/Fcycle/
                                 Wcycle switches, PMP2extradof.f90:86-88); the mixed form uses V-cycles */
     int32_t reserved;

@@ -518,7 +518,8 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
         if (ok) return 0;
     }
     const double s = 4.0;
-    for (int it = (l == 0 && first) ? 1 : 0; it < 2 * Nsmth; it++) {
+    const int npre = l == 0 ? h->sweeps0[0] : Nsmth;
+    for (int it = (l == 0 && first) ? 1 : 0; it < 2 * npre; it++) {
         prof_begin(h, cls, 2.0 * n3d(v.n) * s);
         launch_df_smooth(frn, v.d, v.xb, v.f, v.n, (it & 1) ? 1 : 0, dc, h->s);
         prof_end(h);
@@ -545,7 +546,8 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
     prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * s);
     launch_df_correct(frn, c.d, c.rd, v.d, v.xb, c.n, h->s);
     prof_end(h);
-    const int npost = (l == 0 && last) ? 2 * Nsmth - 1 : 2 * Nsmth;
+    const int nps = l == 0 ? h->sweeps0[1] : Nsmth;
+    const int npost = (l == 0 && last) ? 2 * nps - 1 : 2 * nps;
     for (int it = 0; it < npost; it++) {
         prof_begin(h, cls, 2.0 * n3d(v.n) * s);
         launch_df_smooth(frn, v.d, v.xb, v.f, v.n, (it & 1) ? 1 : 0, dc, h->s);
@@ -692,15 +694,36 @@ static int outer_residual_dgp(mgs_handle* h, double* rms)
     return 0;
 }
 
+// Mode 1 with inner_cycles 0: the inner cycles of the next outer iteration.
+// An inner cycle contracts the FP64 residual by a nearly constant factor rho
+// until the FP32 increment's accuracy is reached (a gain of 1e-5 to 1e-6 per
+// outer iteration, hence the cap), so the count is the one that reaches 0.7
+// of the target at the rho measured over the previous outer iteration (the
+// next one contracts up to 1.5 times less per cycle), between 1 and the cap;
+// the first outer iteration runs the cap.  Fewer outer iterations save their
+// FP64 residual, update and base restriction, about half an inner cycle each.
+static int auto_inner(const mgs_handle* h, double res, double target, double res_prev, int m_prev)
+{
+    if (m_prev <= 0 || !(res_prev > 0.0) || !(res > 0.0) || !(target > 0.0)) return h->inner_max;
+    const double rho = std::pow(res / res_prev, 1.0 / m_prev);
+    if (!(rho > 0.0) || !(rho < 0.9)) return h->inner_max;
+    const double m = std::ceil(std::log(0.7 * target / res) / std::log(rho));
+    if (!(m >= 1.0)) return 1;
+    return m >= (double)h->inner_max ? h->inner_max : (int)m;
+}
+
 // FP64 defect correction around the FP32 inner FAS (f(R) and nDGP)
 static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep)
 {
     int rc;
     const bool fr = h->p.model == MGS_MODEL_FR;
     const int frn = h->p.fr_n;
-    const int inner = o.inner_cycles > 0 ? o.inner_cycles : 1;
+    const bool automatic = o.inner_cycles == 0 && o.mode == 1;
+    int inner = o.inner_cycles > 0 ? o.inner_cycles : 1;
     if (!fr && (rc = cache_densities(h))) return rc;
     double res_old_1 = 1.0e5, res_old_2 = 1.0e10;
+    double res_prev = 0.0;
+    int m_prev = 0;
     bool conv = false;
     int cycles = 0;
     for (int it = 1;; it++) {
@@ -726,6 +749,11 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
             can_exit = true;
         }
         if (can_exit) break;
+        if (automatic) {
+            inner = auto_inner(h, res, o.rel_tol * rep->res0, res_prev, m_prev);
+            res_prev = res;
+            m_prev = inner;
+        }
         // inner problem: base restricted to every level, increment from zero
         for (int l = 1; l <= h->lc; l++) {
             prof_begin(h, l == 1 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(h->M[l - 1].n) + n3d(h->M[l].n)) * 4.0);
@@ -830,6 +858,15 @@ int mgs_create(const mgs_params* params, mgs_handle** out)
     {
         const char* e = std::getenv("MGS_NO_TAIL");
         h->use_tail = !(e && e[0] == '1');
+        // tuning overrides of the mixed inner cycle's shape
+        const char* w = std::getenv("MGS_INNER_SWEEPS");
+        int a = 0, b = 0;
+        if (w && std::sscanf(w, "%d,%d", &a, &b) == 2 && a >= 1 && b >= 1 && a <= 8 && b <= 8) {
+            h->sweeps0[0] = a;
+            h->sweeps0[1] = b;
+        }
+        const char* im = std::getenv("MGS_INNER_MAX");
+        if (im && std::atoi(im) >= 1 && std::atoi(im) <= 64) h->inner_max = std::atoi(im);
     }
     h->lmax = levelmax_of(params->ngrid);
     h->lmin = levelmin_of(params->ngrid);

@@ -278,7 +278,7 @@ static void colour_pass(const OP& op, const Geo& g, int colour, hipStream_t s)
 template <class OP>
 static int both_colours(const OP& op, double* partials, const Geo& g, hipStream_t s)
 {
-    const ZLaunch L = make_zlaunch(g);
+    const ZLaunch L = make_zlaunch_both(g);
     hipLaunchKernelGGL((k_both_colours<OP>), L.grid, L.block, 0, s, op, partials, g, ZARGS);
     return (int)L.NB;
 }

@@ -46,6 +46,18 @@ struct ProfClass {
     double bytes = 0.0;
 };
 
+// Level-0 pre/post sweeps of the mixed f(R) inner V-cycle (coarser inner
+// levels keep the reference's Nsmth = 2).  One pre-sweep is enough on the
+// finest level of the increment's problem: its smooth error comes straight
+// from the FP64 residual, and the coarse correction removes it.  The serving
+// engine runs the same sweeps, so its results stay bit-identical to
+// mgs_solve's.
+constexpr int MIXED_PRE0 = 1;
+constexpr int MIXED_POST0 = 2;
+// mode 1 with inner_cycles 0: cap on the automatic inner cycles per outer
+// iteration (the FP32 increment's accuracy bounds what one outer step gains)
+constexpr int MIXED_INNER_MAX = 4;
+
 // DGP constants of one level, PMP2MGsolver_DGP.f90:36-48
 struct DgpCoef {
     double dx2, dx4, alpha, beta, fct1;
@@ -80,6 +92,11 @@ struct mgs_handle {
     hipGraphExec_t g_inner = nullptr;
     bool use_graphs = true;
     bool use_tail = true;  // mixed f(R): small inner levels in one launch (MGS_NO_TAIL=1 turns it off)
+    // mixed f(R): level-0 pre/post sweeps of the inner cycle (MGS_INNER_SWEEPS
+    // "a,b" overrides them for mgs_solve only) and the automatic inner-cycle
+    // cap (MGS_INNER_MAX)
+    int sweeps0[2] = {mgs::MIXED_PRE0, mgs::MIXED_POST0};
+    int inner_max = mgs::MIXED_INNER_MAX;
     // profiling
     bool prof = false;
     std::vector<mgs::ProfRec> recs;

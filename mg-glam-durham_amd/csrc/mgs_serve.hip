@@ -54,6 +54,7 @@ struct LevelDesc {
     float *d, *xb, *f, *rres, *rd;
     DfCoef dc;
     Shape z;   // stencil passes on this level
+    Shape zb;  // both-colour passes on this level (residual-type)
     Shape zc;  // coarse-column transfers between this level and the next finer one (levels >= 1)
 };
 
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void kb_outer_residual(const SlotDesc* __restr
         const unsigned lb = t - P->res_pre[i];
         ZCol c;
         double acc = 0.0;
-        if (tile_col(L.g, L.g.h, L.z, lb, c)) {
+        if (tile_col(L.g, L.g.h, L.zb, lb, c)) {
             FRN_SWITCH(sd.frn, acc = both_colours_col(FrOuterOp<FRN>{sd.u, sd.rho, L.f, L.xb, L.d, sd.fc, L.dc,
                                                                      (unsigned)L.g.half},
                                                       L.g, c));
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(1024) void kb_sums(const SlotDesc* __restrict__ S, 
     for (int i = blockIdx.x; i < P->nres; i += gridDim.x) {
         const int slot = P->res_slot[i];
         const SlotDesc& sd = S[slot];
-        const int count = (int)sd.L[0].z.NB;
+        const int count = (int)sd.L[0].zb.NB;
         double r;
         if (count <= 8192) {
             block_chunk_sum(sd.partials, 0, count, s);
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(256) void kb_rhs(const SlotDesc* __restrict__ S, co
         const SlotDesc& sd = S[P->inner_slot[l][i]];
         const LevelDesc& C = sd.L[l + 1];
         ZCol c;
-        if (tile_col(C.g, C.g.h, C.z, t - P->zn_pre[l][i], c)) {
+        if (tile_col(C.g, C.g.h, C.zb, t - P->zn_pre[l][i], c)) {
             FRN_SWITCH(sd.frn, both_colours_col(DfResOp<FRN>{C.rd, C.xb, C.rres, C.f, C.dc}, C.g, c));
         }
     }
@@ -344,7 +345,7 @@ static size_t layout(Hier& h, int N, char* base)
     const size_t n3 = (size_t)N * N * N;
     h.u = (double*)take(n3 * 8);
     h.rho = (double*)take(n3 * 8);
-    h.partials = (double*)take(((size_t)make_zlaunch(make_geo(N)).NB + 64) * 8);
+    h.partials = (double*)take(((size_t)make_zlaunch_both(make_geo(N)).NB + 64) * 8);
     for (int l = 0; l <= h.lc; l++) {
         const size_t m = (size_t)(N >> l) * (N >> l) * (N >> l);
         h.d[l] = (float*)take(m * 4);
@@ -489,6 +490,7 @@ void fill_desc(SlotDesc& sd, const Hier& h, const mgs_params& p)
         L.rd = h.rd[l];
         L.dc = df_coefs(p, l);
         L.z = shape_of(make_zlaunch(L.g));
+        L.zb = shape_of(make_zlaunch_both(L.g));
         if (l > 0) L.zc = shape_of(make_zlaunch(L.g.n, L.g.n));
     }
 }
@@ -618,7 +620,7 @@ void build_plan(mgs_server* S, int* depth, bool* any_inner)
         if (!sl.r) continue;
         const SlotDesc& sd = S->hS[i];
         P.res_slot[P.nres] = i;
-        P.res_pre[P.nres + 1] = P.res_pre[P.nres] + sd.L[0].z.NB;
+        P.res_pre[P.nres + 1] = P.res_pre[P.nres] + sd.L[0].zb.NB;
         P.nres++;
         if (sl.fresh) continue;
         *any_inner = true;
@@ -631,7 +633,7 @@ void build_plan(mgs_server* S, int* depth, bool* any_inner)
             P.z_pre[l][k + 1] = P.z_pre[l][k] + sd.L[l].z.NB;
             P.zc_pre[l][k + 1] = P.zc_pre[l][k] + C.zc.NB;
             P.cell_pre[l][k + 1] = P.cell_pre[l][k] + (nc * nc * nc + 255u) / 256u;
-            P.zn_pre[l][k + 1] = P.zn_pre[l][k] + C.z.NB;
+            P.zn_pre[l][k + 1] = P.zn_pre[l][k] + C.zb.NB;
             P.ninner[l]++;
         }
         P.coarse_slot[sd.lc][P.ncoarse[sd.lc]++] = i;
@@ -646,13 +648,15 @@ void vcycle_batched(mgs_server* S, int l, int depth)
     hipLaunchKernelGGL(kb_coarsest, dim3(MAXS), dim3(1024), 0, s, S->dS, S->dP, l);
     if (l >= depth) return;
     // pre-smoothing (level 0: the first colour-0 pass came with the outer residual)
-    for (int it = l == 0 ? 1 : 0; it < 4; it++) hipLaunchKernelGGL(kb_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
+    const int npre = 2 * (l == 0 ? MIXED_PRE0 : 2);
+    for (int it = l == 0 ? 1 : 0; it < npre; it++) hipLaunchKernelGGL(kb_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
     hipLaunchKernelGGL(kb_restrict_residual, G, B, 0, s, S->dS, S->dP, l);
     hipLaunchKernelGGL(kb_restrict_d, G, B, 0, s, S->dS, S->dP, l);
     hipLaunchKernelGGL(kb_rhs, G, B, 0, s, S->dS, S->dP, l);
     vcycle_batched(S, l + 1, depth);
     hipLaunchKernelGGL(kb_correct, G, B, 0, s, S->dS, S->dP, l);
-    const int npost = l == 0 ? 3 : 4;
+    // post-smoothing (level 0: the last colour-1 pass is fused with the update)
+    const int npost = l == 0 ? 2 * MIXED_POST0 - 1 : 4;
     for (int it = 0; it < npost; it++) hipLaunchKernelGGL(kb_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
     if (l == 0) hipLaunchKernelGGL(kb_smooth_update, G, B, 0, s, S->dS, S->dP);
 }

@@ -41,7 +41,7 @@ inline int zrun_length(int n)
 
 // (TX x TY)-thread blocks over `width` columns x n rows, each thread marching
 // a run of KC planes; k-run-major block order
-inline ZLaunch make_zlaunch(int width, int n)
+inline ZLaunch make_zlaunch(int width, int n, int kc)
 {
     // widest block row; MGS_ZTX overrides it (launch-shape tuning)
     static const int max_tx = [] {
@@ -56,7 +56,7 @@ inline ZLaunch make_zlaunch(int width, int n)
     L.TY = ty < 1 ? 1 : ty;
     L.XB = (width + L.TX - 1) / L.TX;
     L.JB = (n + L.TY - 1) / L.TY;
-    L.KC = zrun_length(n);
+    L.KC = kc < 1 ? 1 : (kc > n ? n : kc);
     // block order: MGS_ZORDER=k makes the plane runs of one column block
     // consecutive (flagged to the kernels by a negative JB)
     static const bool kfast = [] {
@@ -72,8 +72,21 @@ inline ZLaunch make_zlaunch(int width, int n)
     return L;
 }
 
+inline ZLaunch make_zlaunch(int width, int n) { return make_zlaunch(width, n, zrun_length(n)); }
+
 // stencil passes over one colour half: columns are the n/2 values of m
 inline ZLaunch make_zlaunch(const Geo& g) { return make_zlaunch(g.h, g.n); }
+
+// passes over both colours (residual-type kernels, two windows per column);
+// MGS_ZRUN_BOTH overrides their run length (launch-shape tuning)
+inline ZLaunch make_zlaunch_both(const Geo& g)
+{
+    static const int forced = [] {
+        const char* e = std::getenv("MGS_ZRUN_BOTH");
+        return e ? std::atoi(e) : 0;
+    }();
+    return make_zlaunch(g.h, g.n, forced > 0 ? forced : zrun_length(g.n));
+}
 
 struct ZCol {
     int m, j, k0, k1;

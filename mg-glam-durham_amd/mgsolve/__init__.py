@@ -309,9 +309,10 @@ class Solver:
     def wcycle(self, level=0):
         self._chk(self.L.mgs_wcycle(self.h, level), "mgs_wcycle")
 
-    def solve(self, mode=1, max_cycles=60, res_conv=1.0e-8, rel_tol=1.0e-8, init_guess=True, inner_cycles=1,
+    def solve(self, mode=1, max_cycles=60, res_conv=1.0e-8, rel_tol=1.0e-8, init_guess=True, inner_cycles=0,
               cycle=0):
-        """scalar_field: cycle 0 V_cycle, 1 F_cycle, 2 W_cycle (plain FP64)."""
+        """scalar_field: cycle 0 V_cycle, 1 F_cycle, 2 W_cycle (plain FP64).  Mixed precision: inner_cycles
+        FP32 V-cycles per outer FP64 iteration, 0 = chosen per outer iteration (mode 1; 1 in mode 0)."""
         o = SolveOpts(mode, max_cycles, res_conv, rel_tol, 1 if init_guess else 0, inner_cycles, int(cycle), 0)
         r = SolveReport()
         self._chk(self.L.mgs_solve(self.h, C.byref(o), C.byref(r)), "mgs_solve")
