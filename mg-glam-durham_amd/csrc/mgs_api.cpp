@@ -354,12 +354,18 @@ int op_rhs(mgs_handle* h, int l)
     return 0;
 }
 
-int op_correct(mgs_handle* h, int l)
+// in_cycle: the correction of a V/F/W cycle, which a colour-0 relaxation
+// pass of level l-1 follows; for f(R) only the colour-1 cells are corrected
+// then (the pass overwrites colour 0 without reading it).  mgs_correct, the
+// reference's standalone correct_solution, corrects every cell.
+int op_correct(mgs_handle* h, int l, bool in_cycle)
 {
     LevelD& c = h->L[l];
     LevelD& f = h->L[l - 1];
-    prof_begin(h, l == 1 ? MGS_KC_CORRECT : MGS_KC_COARSE, (2.0 * n3d(f.n) + 2.0 * n3d(c.n)) * 8.0);
-    launch_correct64(c.u, c.ru, f.u, c.n, h->p.model == MGS_MODEL_FR ? 1 : 0, h->s);
+    const bool fr = h->p.model == MGS_MODEL_FR;
+    const bool only1 = fr && in_cycle;
+    prof_begin(h, l == 1 ? MGS_KC_CORRECT : MGS_KC_COARSE, ((only1 ? 1.0 : 2.0) * n3d(f.n) + 2.0 * n3d(c.n)) * 8.0);
+    launch_correct64(c.u, c.ru, f.u, c.n, fr ? 1 : 0, only1, h->s);
     prof_end(h);
     return 0;
 }
@@ -435,7 +441,7 @@ int vcycle_plain(mgs_handle* h, int l)
         op_rhs(h, l + 1);
         vcycle_plain(h, l + 1);
     }
-    op_correct(h, l + 1);
+    op_correct(h, l + 1, true);
     for (int s = 0; s < Nsmth; s++) {
         op_smooth(h, l, 1);
         op_smooth(h, l, 0);
@@ -456,7 +462,7 @@ static int cgc_plain(mgs_handle* h, int l, int kind)
     op_rhs(h, l + 1);
     int rc = cycle_plain(h, l + 1, kind);
     if (rc) return rc;
-    op_correct(h, l + 1);
+    op_correct(h, l + 1, true);
     return 0;
 }
 
@@ -1173,7 +1179,7 @@ int mgs_correct(mgs_handle* h, int32_t level)
 {
     if (!h) return MGS_E_ARG;
     if (check_level(h, level, 1)) return MGS_E_ARG;
-    op_correct(h, level);
+    op_correct(h, level, false);
     SYNC_RET(h);
 }
 

@@ -73,6 +73,43 @@ __device__ inline void store_partial(double* partials, unsigned NB, double t)
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
+
+// Element i of a device array addressed by a 32-bit byte offset: the access
+// then takes the (uniform) array base from scalar registers and one 32-bit
+// vector offset, instead of a 64-bit vector address per access.  Requires
+// i * sizeof(T) < 2^32: a level holds at most 2^30 cells, so any float index
+// of a level, or any index within one colour half of a double level.
+template <typename T>
+__device__ __forceinline__ const T& elem(const T* a, unsigned i)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a) + i * (unsigned)sizeof(T));
+}
+
+template <typename T>
+__device__ __forceinline__ T& elem(T* a, unsigned i)
+{
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(a) + i * (unsigned)sizeof(T));
+}
+
+// element i of a whole level's array (up to 2^30 cells): elem() where the
+// byte offset fits 32 bits for any index (4-byte elements), a[i] otherwise
+template <typename T>
+__device__ __forceinline__ const T& lvl(const T* a, unsigned i)
+{
+    if constexpr (sizeof(T) <= 4)
+        return elem(a, i);
+    else
+        return a[i];
+}
+
+template <typename T>
+__device__ __forceinline__ T& lvl(T* a, unsigned i)
+{
+    if constexpr (sizeof(T) <= 4)
+        return elem(a, i);
+    else
+        return a[i];
+}
 template <int FRN, typename T>
 __device__ __forceinline__ T wpow(T x)
 {

@@ -112,7 +112,9 @@ __global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T
     restrict8_cell<T>(fine, out1, out2, gf, gc, I, (int)(row % (unsigned)gc.n), (int)(row / (unsigned)gc.n));
 }
 
-// correct_solution (PMP2extradof.f90:1361-1578), one coarse column per thread
+// correct_solution (PMP2extradof.f90:1361-1578), one coarse column per thread;
+// ONLY1: colour-1 sons only (correct64_sons)
+template <bool ONLY1>
 __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH, const double* __restrict__ ru,
                                                    double* __restrict__ uf, Geo gf, Geo gc, int guard, int TX, int TY,
                                                    int XB, int JB, int KC, unsigned NB)
@@ -132,14 +134,14 @@ __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH
         rk[q] = ru[B.at(q, cc.k0)];
     }
     for (int K = cc.k0; K < cc.k1; K++) {
-        correct64_sons<0>(hs, rs, hk, rk, uf, gf, I, J, K, guard);
+        correct64_sons<0, ONLY1>(hs, rs, hk, rk, uf, gf, I, J, K, guard);
         const int Kp = wrap_inc(K, gc.n);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             hs[q] = uH[B.at(q, Kp)];
             rs[q] = ru[B.at(q, Kp)];
         }
-        correct64_sons<1>(hs, rs, hk, rk, uf, gf, I, J, K, guard);
+        correct64_sons<1, ONLY1>(hs, rs, hk, rk, uf, gf, I, J, K, guard);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const double th = hk[q], tr = rk[q];
@@ -411,11 +413,15 @@ void launch_fr_rhs64(int frn, const double* ru, const double* rho, const double*
     else both_colours(FrResOp<2, true>{ru, rho, rres, f, c}, nullptr, g, s);
 }
 
-void launch_correct64(const double* uH, const double* ru, double* ufine, int nc, int guard, hipStream_t s)
+void launch_correct64(const double* uH, const double* ru, double* ufine, int nc, int guard, bool only1,
+                      hipStream_t s)
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
     const ZLaunch L = make_zlaunch(nc, nc);
-    hipLaunchKernelGGL(k_correct64, L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, ZARGS);
+    if (only1)
+        hipLaunchKernelGGL(k_correct64<true>, L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, ZARGS);
+    else
+        hipLaunchKernelGGL(k_correct64<false>, L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, ZARGS);
 }
 
 // partials[0..count) -> out[0]; stage 1 writes its chunk sums after the

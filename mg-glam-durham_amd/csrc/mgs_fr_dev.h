@@ -296,7 +296,20 @@ __device__ __forceinline__ unsigned son_index(const Geo& gf, int I, int J, int K
 // K-1 and K, then plane K+1 replaces K-1 for the sons in plane 2K+1.  The sum
 // keeps the reference's order P = P + w u_H - w R u over the 8 contributors
 // of a son.
-template <int S3>
+//
+// ONLY1: correct the colour-1 sons only.  Inside the f(R) cycles a
+// correction is always followed by a colour-0 relaxation pass, whose new
+// values depend on the colour-1 neighbours alone (the local root does not
+// read the cell's old value), so corrected colour-0 values would be
+// overwritten unread: skipping them halves the fine-level traffic and leaves
+// every result bit-identical.
+template <int S3, bool ONLY1 = false>
+__device__ __forceinline__ bool son_corrected(int t)
+{
+    return !ONLY1 || (((t & 1) + (t >> 1) + S3) & 1) == 1;
+}
+
+template <int S3, bool ONLY1>
 __device__ __forceinline__ void correct64_sons(const double* hz, const double* rz, const double* hk,
                                                const double* rk, double* __restrict__ uf, const Geo& gf, int I, int J,
                                                int K, int guard)
@@ -305,9 +318,11 @@ __device__ __forceinline__ void correct64_sons(const double* hz, const double* r
     const double wt[8] = {a, b, b, c9, b, c9, c9, d};
     double old[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++) old[t] = uf[son_index(gf, I, J, K, t & 1, t >> 1, S3)];
+    for (int t = 0; t < 4; t++)
+        if (son_corrected<S3, ONLY1>(t)) old[t] = uf[son_index(gf, I, J, K, t & 1, t >> 1, S3)];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
+        if (!son_corrected<S3, ONLY1>(t)) continue;
         const int s1 = t & 1, s2 = t >> 1;
         double P = 0.0;
 #pragma unroll
@@ -378,11 +393,15 @@ __device__ __forceinline__ float df_solve(float S, float xb, float r, const DfCo
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
         if (it > 0) x = xroot<FRN>(fmaxf(wb + d, 1.0e-30f));
-        const float rx = __builtin_amdgcn_rcpf(x);
-        // C (1/xb - 1/x) = C d / (dden(x, xb) x xb)
-        const float loc = c.C * d * __builtin_amdgcn_rcpf(dden<FRN>(x, xb)) * rx * rxb;
+        // one reciprocal q = 1 / (x dden(x, xb)) gives 1/x = dden q and
+        // C (1/xb - 1/x) = C d / (dden(x, xb) x xb) = C d q / xb
+        const float dd = dden<FRN>(x, xb);
+        const float q = __builtin_amdgcn_rcpf(x * dd);
+        const float rx = dd * q;
+        const float loc = c.C * d * q * rxb;
         const float F = c.A * (S - 6.0f * d) + loc - r;
-        const float Fp = -6.0f * c.A + c.C * (1.0f / (float)(FRN + 1)) * rx * __builtin_amdgcn_rcpf(wpow<FRN>(x));
+        // dF/dd = -6A + C / ((n+1) x^{n+2})
+        const float Fp = -6.0f * c.A + c.C * (1.0f / (float)(FRN + 1)) * rx * wpow<FRN>(rx);
         d = d - F * __builtin_amdgcn_rcpf(Fp);
     }
     // stay inside the domain x > 0
@@ -401,12 +420,12 @@ struct DfSmoothOp {
     const float* rhs;
     DfCoef c;
     __device__ float* field() const { return d; }
-    __device__ __forceinline__ Own load(unsigned o, unsigned) const { return Own{xb[o], rhs[o]}; }
+    __device__ __forceinline__ Own load(unsigned o, unsigned) const { return Own{elem(xb, o), elem(rhs, o)}; }
     __device__ __forceinline__ void update(unsigned o, unsigned, const Own& w, float, float xp, float xm, float yp,
                                            float ym, float zp, float zm) const
     {
         const float S = xp + xm + yp + ym + zp + zm;
-        d[o] = df_solve<FRN>(S, w.xb, w.rhs, c);
+        elem(d, o) = df_solve<FRN>(S, w.xb, w.rhs, c);
     }
 };
 
@@ -443,10 +462,10 @@ struct DfSmoothUpdateOp {
     __device__ __forceinline__ Own load(unsigned o, unsigned oo) const
     {
         Own w;
-        w.xb = xb[o];
-        w.rhs = rhs[o];
+        w.xb = elem(xb, o);
+        w.rhs = elem(rhs, o);
         w.u = u[o];
-        w.xbo = xb[oo];
+        w.xbo = elem(xb, oo);
         w.uo = u[oo];
         return w;
     }
@@ -493,9 +512,9 @@ struct FrOuterOp {
         OP = OP + c.fct1r * w.delta;
         OP = OP + c.fct2r * (c.a_fR_rt * ru - 1.0);
         const float gv = (float)(-OP), xv = (float)(uc * c.inv_ubg);
-        g[o] = gv;
-        xb[o] = xv;
-        d[o] = o < half ? df_solve<FRN>(0.0f, xv, gv, dc) : 0.0f;
+        elem(g, o) = gv;
+        elem(xb, o) = xv;
+        elem(d, o) = o < half ? df_solve<FRN>(0.0f, xv, gv, dc) : 0.0f;
         return OP;
     }
 };
@@ -514,7 +533,7 @@ struct DfResOp {
         float xb, rhs;
     };
     __device__ const float* field() const { return d; }
-    __device__ __forceinline__ Own load(unsigned o) const { return Own{xb[o], rhs[o]}; }
+    __device__ __forceinline__ Own load(unsigned o) const { return Own{elem(xb, o), elem(rhs, o)}; }
     __device__ __forceinline__ double cell(unsigned o, const Own& w, float dc, float xp, float xm, float yp, float ym,
                                            float zp, float zm) const
     {
@@ -522,7 +541,7 @@ struct DfResOp {
         const float xbc = w.xb;
         const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
         const float r = c.A * (S - 6.0f * dc) + dlocal<FRN>(dc, x, xbc, c.C) - w.rhs;
-        out[o] = r;
+        elem(out, o) = r;
         return 0.0;
     }
 };
@@ -549,13 +568,13 @@ __device__ __forceinline__ void df_restrict_residual_col(const float* __restrict
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const unsigned o = F.at(q, z);
-            const float dc = d0[o], xbc = xb0[o];
+            const float dc = elem(d0, o), xbc = elem(xb0, o);
             const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
-            p.l[q] = dlocal<FRN>(dc, x, xbc, c.C) - g0[o];
+            p.l[q] = dlocal<FRN>(dc, x, xbc, c.C) - elem(g0, o);
             p.d[q] = dc;
         }
 #pragma unroll
-        for (int q = 0; q < 8; q++) p.f[q] = d0[F.at(4 + q, z)];
+        for (int q = 0; q < 8; q++) p.f[q] = elem(d0, F.at(4 + q, z));
     };
     Pl p0, p1, p2;
     float dl[4];
@@ -564,7 +583,7 @@ __device__ __forceinline__ void df_restrict_residual_col(const float* __restrict
     {
         const int zl = zs == 0 ? N - 1 : zs - 1;
 #pragma unroll
-        for (int q = 0; q < 4; q++) dl[q] = d0[F.at(q, zl)];
+        for (int q = 0; q < 4; q++) dl[q] = elem(d0, F.at(q, zl));
     }
     for (int K = cc.k0; K < cc.k1; K++) {
         load(2 * K + 1, p1);
@@ -601,8 +620,8 @@ __device__ __forceinline__ void restrict_base_cell(const float* __restrict__ xf,
     float w = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; q++)
-        w += wpow<FRN>(xf[sidx(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1))]);
-    xc[sidx(gc, I, J, K)] = xroot<FRN>(w * 0.125f);}
+        w += wpow<FRN>(elem(xf, sidx(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1))));
+    elem(xc, sidx(gc, I, J, K)) = xroot<FRN>(w * 0.125f);}
 
 // inner coarse-grid correction d_h += P(d_H - R d_h), kept only where the
 // corrected x stays positive (the delta-form image of the reference's guard);
@@ -611,17 +630,22 @@ template <int FRN, int S3>
 __device__ __forceinline__ void df_correct_sons(const float* ez, const float* ek, float* __restrict__ df,
                                                 const float* __restrict__ xbf, const Geo& gf, int I, int J, int K)
 {
+    // colour-1 sons only: every inner correction is followed by a colour-0
+    // pass (see son_corrected)
+    constexpr bool ONLY1 = true;
     const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
     const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
     float old[4], wb[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
+        if (!son_corrected<S3, ONLY1>(t)) continue;
         const unsigned fi = son_index(gf, I, J, K, t & 1, t >> 1, S3);
-        old[t] = df[fi];
-        wb[t] = wpow<FRN>(xbf[fi]);
+        old[t] = elem(df, fi);
+        wb[t] = wpow<FRN>(elem(xbf, fi));
     }
 #pragma unroll
     for (int t = 0; t < 4; t++) {
+        if (!son_corrected<S3, ONLY1>(t)) continue;
         const int s1 = t & 1, s2 = t >> 1;
         float P = 0.0f;
 #pragma unroll
@@ -632,7 +656,7 @@ __device__ __forceinline__ void df_correct_sons(const float* ez, const float* ek
             P += wt[ind] * ((ind & 4) ? ek[q] : ez[q]);
         }
         const float dn = old[t] + P;
-        if (wb[t] + dn > 0.0f) df[son_index(gf, I, J, K, s1, s2, S3)] = dn;
+        if (wb[t] + dn > 0.0f) elem(df, son_index(gf, I, J, K, s1, s2, S3)) = dn;
     }
 }
 
@@ -649,8 +673,8 @@ __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, con
 #pragma unroll
     for (int q = 0; q < 9; q++) {
         const unsigned om = B.at(q, Km), o0 = B.at(q, cc.k0);
-        es[q] = dH[om] - rd[om];
-        ek[q] = dH[o0] - rd[o0];
+        es[q] = elem(dH, om) - elem(rd, om);
+        ek[q] = elem(dH, o0) - elem(rd, o0);
     }
     for (int K = cc.k0; K < cc.k1; K++) {
         df_correct_sons<FRN, 0>(es, ek, df, xbf, gf, I, J, K);
@@ -658,7 +682,7 @@ __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, con
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const unsigned op = B.at(q, Kp);
-            es[q] = dH[op] - rd[op];
+            es[q] = elem(dH, op) - elem(rd, op);
         }
         df_correct_sons<FRN, 1>(es, ek, df, xbf, gf, I, J, K);
 #pragma unroll
@@ -714,18 +738,18 @@ __device__ __forceinline__ void restrict8_cell(const T* __restrict__ fine, T* __
                                                const Geo& gf, const Geo& gc, int I, int J, int K)
 {
     // children in the reference's order (x fastest), accumulated in T
-    T P = fine[sidx(gf, 2 * I, 2 * J, 2 * K)];
-    P = P + fine[sidx(gf, 2 * I + 1, 2 * J, 2 * K)];
-    P = P + fine[sidx(gf, 2 * I, 2 * J + 1, 2 * K)];
-    P = P + fine[sidx(gf, 2 * I + 1, 2 * J + 1, 2 * K)];
-    P = P + fine[sidx(gf, 2 * I, 2 * J, 2 * K + 1)];
-    P = P + fine[sidx(gf, 2 * I + 1, 2 * J, 2 * K + 1)];
-    P = P + fine[sidx(gf, 2 * I, 2 * J + 1, 2 * K + 1)];
-    P = P + fine[sidx(gf, 2 * I + 1, 2 * J + 1, 2 * K + 1)];
+    T P = lvl(fine, sidx(gf, 2 * I, 2 * J, 2 * K));
+    P = P + lvl(fine, sidx(gf, 2 * I + 1, 2 * J, 2 * K));
+    P = P + lvl(fine, sidx(gf, 2 * I, 2 * J + 1, 2 * K));
+    P = P + lvl(fine, sidx(gf, 2 * I + 1, 2 * J + 1, 2 * K));
+    P = P + lvl(fine, sidx(gf, 2 * I, 2 * J, 2 * K + 1));
+    P = P + lvl(fine, sidx(gf, 2 * I + 1, 2 * J, 2 * K + 1));
+    P = P + lvl(fine, sidx(gf, 2 * I, 2 * J + 1, 2 * K + 1));
+    P = P + lvl(fine, sidx(gf, 2 * I + 1, 2 * J + 1, 2 * K + 1));
     const unsigned o = sidx(gc, I, J, K);
     const T v = P / (T)8.0;
-    out1[o] = v;
-    if (out2) out2[o] = v;}
+    lvl(out1, o) = v;
+    if (out2) lvl(out2, o) = v;}
 
 
 // ---------------------------------------------------------------------------

@@ -29,7 +29,10 @@ void launch_fr_restrict_residual1_64(int frn, const double* u0, const double* de
 void launch_restrict8_64(const double* fine, double* out1, double* out2, int nc, hipStream_t s);
 void launch_fr_rhs64(int frn, const double* ru, const double* rho, const double* rres, double* f, int n,
                      const FrCoef& c, hipStream_t s);
-void launch_correct64(const double* uH, const double* ru, double* ufine, int nc, int guard, hipStream_t s);
+// only1: colour-1 fine cells only, for a correction that a colour-0 f(R)
+// relaxation pass follows (the colour-0 values would be overwritten unread)
+void launch_correct64(const double* uH, const double* ru, double* ufine, int nc, int guard, bool only1,
+                      hipStream_t s);
 
 // deterministic sum of `count` partials into out[0] (single block)
 void launch_sum_partials(const double* partials, int count, double* out, hipStream_t s);

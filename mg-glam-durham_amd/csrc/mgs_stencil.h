@@ -158,8 +158,8 @@ struct ZWin {
         jrow = (unsigned)j * h;
         jprow = (unsigned)wrap_inc(j, g.n) * h;
         jmrow = (unsigned)wrap_dec(j, g.n) * h;
-        prev = a[(unsigned)wrap_dec(k0, g.n) * plane + jrow + m];
-        cur = a[(unsigned)k0 * plane + jrow + m];
+        prev = elem(a, (unsigned)wrap_dec(k0, g.n) * plane + jrow + m);
+        cur = elem(a, (unsigned)k0 * plane + jrow + m);
     }
 
     __device__ __forceinline__ ZPre<T> fetch(int k, int s) const
@@ -170,10 +170,10 @@ struct ZWin {
         if (s) mo = (m + 1 == h) ? 0u : m + 1;
         else mo = (m == 0) ? h - 1 : m - 1;
         ZPre<T> p;
-        p.next = a[kp * plane + jrow + m];
-        p.xo = a[pk + jrow + mo];
-        p.yp = a[pk + jprow + m];
-        p.ym = a[pk + jmrow + m];
+        p.next = elem(a, kp * plane + jrow + m);
+        p.xo = elem(a, pk + jrow + mo);
+        p.yp = elem(a, pk + jprow + m);
+        p.ym = elem(a, pk + jmrow + m);
         return p;
     }
 
@@ -196,6 +196,36 @@ struct ZWin {
 // xp, xm, yp, ym, zp, zm) applies the update, with neighbours in the
 // reference's order M1u, M1l, M2u, M2l, M3u, M3l.  Offsets index the full
 // level array.  The loads of plane k+1 are issued before plane k is updated.
+// A run of exactly R planes, unrolled: every load of the run is issued
+// before the first update (R + 2 window values, R own records), and the
+// window needs no register rotation.
+template <int R, class OP>
+__device__ __forceinline__ void colour_pass_run(const OP& op, ZWin<typename OP::T>& w, unsigned own, unsigned oth,
+                                                unsigned col, int s0, int k0)
+{
+    using T = typename OP::T;
+    using Own = typename OP::Own;
+    ZPre<T> p[R];
+    Own o[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const unsigned rel = (unsigned)(k0 + r) * w.plane + col;
+        p[r] = w.fetch(k0 + r, s0 ^ (r & 1));
+        o[r] = op.load(own + rel, oth + rel);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const unsigned rel = (unsigned)(k0 + r) * w.plane + col;
+        const T other = w.cur;
+        T xp, xm, yp, ym, zp, zm;
+        w.apply(p[r], s0 ^ (r & 1), xp, xm, yp, ym, zp, zm);
+        op.update(own + rel, oth + rel, o[r], other, xp, xm, yp, ym, zp, zm);
+    }
+}
+
+// plane-run length of the unrolled path (the default run, zrun_length)
+constexpr int ZRUN_UNROLLED = 4;
+
 template <class OP>
 __device__ __forceinline__ void colour_pass_col(const OP& op, const Geo& g, int colour, const ZCol& c)
 {
@@ -206,6 +236,10 @@ __device__ __forceinline__ void colour_pass_col(const OP& op, const Geo& g, int 
     const unsigned own = (unsigned)colour * (unsigned)g.half;
     const unsigned oth = (unsigned)(1 - colour) * (unsigned)g.half;
     const unsigned col = w.jrow + w.m;
+    if (c.k1 - c.k0 == ZRUN_UNROLLED) {
+        colour_pass_run<ZRUN_UNROLLED>(op, w, own, oth, col, (colour + c.j + c.k0) & 1, c.k0);
+        return;
+    }
     int k = c.k0;
     ZPre<T> pc = w.fetch(k, (colour + c.j + k) & 1);
     Own oc = op.load(own + (unsigned)k * w.plane + col, oth + (unsigned)k * w.plane + col);
