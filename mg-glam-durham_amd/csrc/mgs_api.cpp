@@ -531,11 +531,8 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
         prof_end(h);
     }
     LevelF& c = h->M[l + 1];
-    prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (3.0 * n3d(v.n) + n3d(c.n)) * s);
-    launch_df_restrict_residual(frn, v.d, v.xb, v.f, c.rres, v.n, dc, h->s);
-    prof_end(h);
-    prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(v.n) + 2.0 * n3d(c.n)) * s);
-    launch_restrict8_32(v.d, c.rd, c.d, c.n, h->s);
+    prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 3.0 * n3d(c.n)) * s);
+    launch_df_restrict_residual(frn, v.d, v.xb, v.f, c.rres, c.rd, c.d, v.n, dc, h->s);
     prof_end(h);
     auto coarse = [h, l, frn, &c]() {
         prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(c.n) * 4.0);

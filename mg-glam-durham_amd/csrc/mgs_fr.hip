@@ -213,12 +213,13 @@ template <int FRN>
 __global__ __launch_bounds__(256) void k_df_restrict_residual(const float* __restrict__ d0,
                                                               const float* __restrict__ xb0,
                                                               const float* __restrict__ g0, float* __restrict__ rres1,
-                                                              Geo gf, Geo gc, DfCoef c, int TX, int TY, int XB, int JB,
-                                                              int KC, unsigned NB)
+                                                              float* __restrict__ rd1, float* __restrict__ d1, Geo gf,
+                                                              Geo gc, DfCoef c, int TX, int TY, int XB, int JB, int KC,
+                                                              unsigned NB)
 {
     ZCol cc;
     if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
-    df_restrict_residual_col<FRN>(d0, xb0, g0, rres1, gf, gc, c, cc);
+    df_restrict_residual_col<FRN>(d0, xb0, g0, rres1, rd1, d1, gf, gc, c, cc);
 }
 
 template <int FRN>
@@ -483,12 +484,12 @@ void launch_df_residual(int frn, const float* d, const float* xb, const float* r
     else both_colours(DfResOp<2>{d, xb, rhs, res, c}, nullptr, g, s);
 }
 
-void launch_df_restrict_residual(int frn, const float* d0, const float* xb0, const float* g0, float* rres1, int N,
-                                 const DfCoef& c0, hipStream_t s)
+void launch_df_restrict_residual(int frn, const float* d0, const float* xb0, const float* g0, float* rres1,
+                                 float* rd1, float* d1, int N, const DfCoef& c0, hipStream_t s)
 {
     const Geo gf = make_geo(N), gc = make_geo(N / 2);
     const ZLaunch L = make_zlaunch(gc.n, gc.n);
-    FRN_DISPATCH(frn, k_df_restrict_residual, d0, xb0, g0, rres1, gf, gc, c0, ZARGS);
+    FRN_DISPATCH(frn, k_df_restrict_residual, d0, xb0, g0, rres1, rd1, d1, gf, gc, c0, ZARGS);
 }
 
 void launch_df_rhs(int frn, const float* rd, const float* xb, const float* rres, float* f, int n, const DfCoef& c,

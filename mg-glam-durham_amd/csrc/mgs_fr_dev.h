@@ -550,10 +550,12 @@ struct DfResOp {
 // coarse cell (rhs = g on level 0, the FAS right-hand side below); the
 // children's Laplacians collapse to -3 sum(children) + sum(24 outer faces).
 // One coarse column (I, J) = (cc.m, cc.j) marching over its run of K, like
-// k_fr_restrict_residual1_64.
+// k_fr_restrict_residual1_64.  The children's increments it reads also give
+// R d, stored as the frozen copy rd1 and the coarse start value d1.
 template <int FRN>
 __device__ __forceinline__ void df_restrict_residual_col(const float* __restrict__ d0, const float* __restrict__ xb0,
                                                          const float* __restrict__ g0, float* __restrict__ rres1,
+                                                         float* __restrict__ rd1, float* __restrict__ d1,
                                                          const Geo& gf, const Geo& gc, const DfCoef& c, const ZCol& cc)
 {
     const int I = cc.m, J = cc.j, N = gf.n;
@@ -588,11 +590,13 @@ __device__ __forceinline__ void df_restrict_residual_col(const float* __restrict
     for (int K = cc.k0; K < cc.k1; K++) {
         load(2 * K + 1, p1);
         load(2 * K + 2 == N ? 0 : 2 * K + 2, p2);
-        float loc = 0.0f, s8 = 0.0f;
+        // s8 adds the children in restrict8_cell's order (x fastest, from
+        // the first child), so R d below is bit-identical to it
+        float loc = 0.0f, s8 = p0.d[0];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             loc += p0.l[q];
-            s8 += p0.d[q];
+            if (q > 0) s8 += p0.d[q];
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -605,7 +609,11 @@ __device__ __forceinline__ void df_restrict_residual_col(const float* __restrict
         outer += p0.f[6] + p0.f[7] + p1.f[6] + p1.f[7];
         outer += dl[0] + dl[1] + dl[2] + dl[3];
         outer += p2.d[0] + p2.d[1] + p2.d[2] + p2.d[3];
-        rres1[sidx(gc, I, J, K)] = (c.A * (outer - 3.0f * s8) + loc) * 0.125f;
+        const unsigned oc = sidx(gc, I, J, K);
+        elem(rres1, oc) = (c.A * (outer - 3.0f * s8) + loc) * 0.125f;
+        const float rdv = s8 / 8.0f;
+        elem(rd1, oc) = rdv;
+        elem(d1, oc) = rdv;
 #pragma unroll
         for (int q = 0; q < 4; q++) dl[q] = p1.d[q];
         p0 = p2;
@@ -824,8 +832,7 @@ __device__ void df_tail_vcycle(const TailDesc& T)
         for (int t = tid; t < ncc; t += nthr) {
             int I, J, K;
             coarse_ijk(C.g, t, I, J, K);
-            df_restrict_residual_col<FRN>(L.d, L.xb, L.f, C.rres, L.g, C.g, L.dc, ZCol{I, J, K, K + 1});
-            restrict8_cell<float>(L.d, C.rd, C.d, L.g, C.g, I, J, K);
+            df_restrict_residual_col<FRN>(L.d, L.xb, L.f, C.rres, C.rd, C.d, L.g, C.g, L.dc, ZCol{I, J, K, K + 1});
         }
         __syncthreads();
         for (int t = tid; t < ncc; t += nthr) tail_cell_op(DfResOp<FRN>{C.rd, C.xb, C.rres, C.f, C.dc}, C.g, t);

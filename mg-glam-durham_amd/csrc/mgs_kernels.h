@@ -51,9 +51,10 @@ void launch_df_smooth(int frn, float* d, const float* xb, const float* rhs, int 
 void launch_df_residual(int frn, const float* d, const float* xb, const float* rhs, float* res, int n,
                         const DfCoef& c, hipStream_t s);
 // residual of the fine level (N cells a side, right-hand side g) restricted
-// to the next level in one pass
-void launch_df_restrict_residual(int frn, const float* d, const float* xb, const float* g, float* rres, int N,
-                                 const DfCoef& c, hipStream_t s);
+// to the next level in one pass, with R d into rd (frozen copy) and dc
+// (start value of the coarse increment)
+void launch_df_restrict_residual(int frn, const float* d, const float* xb, const float* g, float* rres, float* rd,
+                                 float* dc, int N, const DfCoef& c, hipStream_t s);
 void launch_restrict8_32(const float* fine, float* out1, float* out2, int nc, hipStream_t s);
 void launch_df_rhs(int frn, const float* rd, const float* xb, const float* rres, float* f, int n, const DfCoef& c,
                    hipStream_t s);

@@ -227,7 +227,8 @@ __global__ __launch_bounds__(256) void kb_smooth_update(const SlotDesc* __restri
     }
 }
 
-// residual of level l restricted to level l+1
+// residual of level l restricted to level l+1, with R d (frozen copy rd and
+// the start value d of level l+1)
 __global__ __launch_bounds__(256) void kb_restrict_residual(const SlotDesc* __restrict__ S,
                                                             const Plan* __restrict__ P, int l)
 {
@@ -238,24 +239,8 @@ __global__ __launch_bounds__(256) void kb_restrict_residual(const SlotDesc* __re
         const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
         ZCol c;
         if (tile_col(C.g, C.g.n, C.zc, t - P->zc_pre[l][i], c)) {
-            FRN_SWITCH(sd.frn, df_restrict_residual_col<FRN>(F.d, F.xb, F.f, C.rres, F.g, C.g, F.dc, c));
-        }
-    }
-}
-
-// R d of level l onto level l+1 (frozen copy rd and the start value d)
-__global__ __launch_bounds__(256) void kb_restrict_d(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l)
-{
-    const unsigned total = P->cell_pre[l][P->ninner[l]];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_slot(P->cell_pre[l], P->ninner[l], t);
-        const SlotDesc& sd = S[P->inner_slot[l][i]];
-        const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
-        const unsigned nc = (unsigned)C.g.n;
-        const unsigned cell = (t - P->cell_pre[l][i]) * 256u + threadIdx.x;
-        if (cell < nc * nc * nc) {
-            const int I = (int)(cell % nc), J = (int)((cell / nc) % nc), K = (int)(cell / (nc * nc));
-            restrict8_cell<float>(F.d, C.rd, C.d, F.g, C.g, I, J, K);
+            FRN_SWITCH(sd.frn,
+                       df_restrict_residual_col<FRN>(F.d, F.xb, F.f, C.rres, C.rd, C.d, F.g, C.g, F.dc, c));
         }
     }
 }
@@ -651,7 +636,6 @@ void vcycle_batched(mgs_server* S, int l, int depth)
     const int npre = 2 * (l == 0 ? MIXED_PRE0 : 2);
     for (int it = l == 0 ? 1 : 0; it < npre; it++) hipLaunchKernelGGL(kb_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
     hipLaunchKernelGGL(kb_restrict_residual, G, B, 0, s, S->dS, S->dP, l);
-    hipLaunchKernelGGL(kb_restrict_d, G, B, 0, s, S->dS, S->dP, l);
     hipLaunchKernelGGL(kb_rhs, G, B, 0, s, S->dS, S->dP, l);
     vcycle_batched(S, l + 1, depth);
     hipLaunchKernelGGL(kb_correct, G, B, 0, s, S->dS, S->dP, l);
