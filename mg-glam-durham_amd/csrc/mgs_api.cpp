@@ -311,7 +311,8 @@ int op_restrict_residual(mgs_handle* h, int l)
     if (l == 1) {
         prof_begin(h, MGS_KC_RESTRICT, (2.0 * n3d(f.n) + n3d(c.n)) * 8.0);
         if (h->p.model == MGS_MODEL_FR)
-            launch_fr_restrict_residual1_64(h->p.fr_n, f.u, f.rho, c.rres, f.n, fr_coefs(h->p, 0), h->s);
+            launch_fr_restrict_residual1_64(h->p.fr_n, f.u, f.rho, c.rres, nullptr, nullptr, f.n, fr_coefs(h->p, 0),
+                                            h->s);
         else
             launch_dgp_restrict_residual1_64(f.u, f.rho, c.rres, f.n, dgp_coefs(h->p, 0), h->s);
     } else {
@@ -330,6 +331,22 @@ int op_restrict_field(mgs_handle* h, int l)
     launch_restrict8_64(f.u, c.ru, c.u, c.n, h->s);
     prof_end(h);
     return 0;
+}
+
+// restrict_residual then restrict_scalar_field onto level l, as a cycle runs
+// them; for f(R) onto level 1 one pass reads u once for both
+static int op_restrict_both(mgs_handle* h, int l)
+{
+    if (l == 1 && h->p.model == MGS_MODEL_FR) {
+        LevelD& c = h->L[l];
+        const LevelD& f = h->L[l - 1];
+        prof_begin(h, MGS_KC_RESTRICT, (2.0 * n3d(f.n) + 3.0 * n3d(c.n)) * 8.0);
+        launch_fr_restrict_residual1_64(h->p.fr_n, f.u, f.rho, c.rres, c.ru, c.u, f.n, fr_coefs(h->p, 0), h->s);
+        prof_end(h);
+        return 0;
+    }
+    op_restrict_residual(h, l);
+    return op_restrict_field(h, l);
 }
 
 int op_restrict_density(mgs_handle* h, int l)
@@ -429,8 +446,7 @@ int vcycle_plain(mgs_handle* h, int l)
         op_smooth(h, l, 0);
     }
     if (l > 0) op_residual(h, l, true, nullptr);
-    op_restrict_residual(h, l + 1);
-    op_restrict_field(h, l + 1);
+    op_restrict_both(h, l + 1);
     if (l == 0) {
         int rc = run_captured(h, &h->g_plain, [h]() {
             op_rhs(h, 1);
@@ -457,8 +473,7 @@ static int cycle_plain(mgs_handle* h, int l, int kind);
 static int cgc_plain(mgs_handle* h, int l, int kind)
 {
     if (l > 0) op_residual(h, l, true, nullptr);
-    op_restrict_residual(h, l + 1);
-    op_restrict_field(h, l + 1);
+    op_restrict_both(h, l + 1);
     op_rhs(h, l + 1);
     int rc = cycle_plain(h, l + 1, kind);
     if (rc) return rc;

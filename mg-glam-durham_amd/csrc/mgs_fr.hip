@@ -35,12 +35,17 @@ namespace mgs {
 // (I, J) and marches over K; fine plane 2K+2 (the upper z faces of K) is the
 // lower child plane of K+1 and plane 2K-1 is carried from K-1, so u and delta
 // are read once.  The operation order is the reference's.
+//
+// With ru1 non-null the same pass also restricts u (restrict_scalar_field,
+// the 8-child average in restrict8_cell's order, so bit-identical to it) into
+// ru1 and the coarse start value u1.
 template <int FRN>
 __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* __restrict__ u0,
                                                                   const double* __restrict__ delta,
-                                                                  double* __restrict__ rres1, Geo gf, Geo gc,
-                                                                  FrCoef c, int TX, int TY, int XB, int JB, int KC,
-                                                                  unsigned NB)
+                                                                  double* __restrict__ rres1,
+                                                                  double* __restrict__ ru1, double* __restrict__ u1,
+                                                                  Geo gf, Geo gc, FrCoef c, int TX, int TY, int XB,
+                                                                  int JB, int KC, unsigned NB)
 {
     ZCol cc;
     if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
@@ -48,11 +53,12 @@ __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* 
     Pts12 F;
     F.init(gf, I, J);
     // per fine plane: u^{n+1}, fct1 delta and fct4/u of the 4 children, and
-    // u^{n+1} of the 8 face neighbours
+    // u^{n+1} of the 8 face neighbours; us: running sum of the children's u
+    // (x fastest), continued from `base` for the upper plane of a coarse cell
     struct Pl {
-        double w[4], t1[4], t2[4], f[8];
+        double w[4], t1[4], t2[4], f[8], us;
     };
-    auto load = [&](int z, Pl& p) {
+    auto load = [&](int z, Pl& p, bool cont, double base) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const unsigned o = F.at(q, z);
@@ -60,6 +66,10 @@ __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* 
             p.w[q] = wpow<FRN>(u);
             p.t1[q] = c.fct1r * delta[o];
             p.t2[q] = c.fct4_1 / u;
+            if (q == 0)
+                p.us = cont ? base + u : u;
+            else
+                p.us = p.us + u;
         }
 #pragma unroll
         for (int q = 0; q < 8; q++) p.f[q] = wpow<FRN>(u0[F.at(4 + q, z)]);
@@ -67,15 +77,15 @@ __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* 
     Pl p0, p1, p2;
     double wl[4];
     const int zs = 2 * cc.k0;
-    load(zs, p0);
+    load(zs, p0, false, 0.0);
     {
         const int zl = zs == 0 ? N - 1 : zs - 1;
 #pragma unroll
         for (int q = 0; q < 4; q++) wl[q] = wpow<FRN>(u0[F.at(q, zl)]);
     }
     for (int K = cc.k0; K < cc.k1; K++) {
-        load(2 * K + 1, p1);
-        load(2 * K + 2 == N ? 0 : 2 * K + 2, p2);
+        load(2 * K + 1, p1, true, p0.us);
+        load(2 * K + 2 == N ? 0 : 2 * K + 2, p2, false, 0.0);
         double Q = -8.0 * c.fct2r;
 #pragma unroll
         for (int q = 0; q < 4; q++) Q = Q + p0.t1[q] + p0.t2[q];
@@ -94,7 +104,13 @@ __global__ __launch_bounds__(256) void k_fr_restrict_residual1_64(const double* 
         P = P + p0.f[6] + p0.f[7] + p1.f[6] + p1.f[7];  // y+2 faces
         P = P + wl[0] + wl[1] + wl[2] + wl[3];          // z-1 faces
         P = P + p2.w[0] + p2.w[1] + p2.w[2] + p2.w[3];  // z+2 faces
-        rres1[sidx(gc, I, J, K)] = (P * c.fct3_1 + Q) / 8.0;
+        const unsigned oc = sidx(gc, I, J, K);
+        rres1[oc] = (P * c.fct3_1 + Q) / 8.0;
+        if (ru1) {
+            const double v = p1.us / 8.0;
+            ru1[oc] = v;
+            u1[oc] = v;
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) wl[q] = p1.w[q];
         p0 = p2;
@@ -383,12 +399,12 @@ int launch_fr_residual64(int frn, int level, const double* u, const double* rho,
 #undef RS
 }
 
-void launch_fr_restrict_residual1_64(int frn, const double* u0, const double* delta, double* rres1, int N,
-                                     const FrCoef& c0, hipStream_t s)
+void launch_fr_restrict_residual1_64(int frn, const double* u0, const double* delta, double* rres1, double* ru1,
+                                     double* u1, int N, const FrCoef& c0, hipStream_t s)
 {
     const Geo gf = make_geo(N), gc = make_geo(N / 2);
     const ZLaunch L = make_zlaunch(gc.n, gc.n);
-    FRN_DISPATCH(frn, k_fr_restrict_residual1_64, u0, delta, rres1, gf, gc, c0, ZARGS);
+    FRN_DISPATCH(frn, k_fr_restrict_residual1_64, u0, delta, rres1, ru1, u1, gf, gc, c0, ZARGS);
 }
 
 void launch_restrict8_64(const double* fine, double* out1, double* out2, int nc, hipStream_t s)

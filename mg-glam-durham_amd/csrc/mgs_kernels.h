@@ -24,8 +24,9 @@ void launch_fr_smooth64(int frn, int level, double* u, const double* rho, const 
 // partials receives one double per block; returns the number of blocks
 int launch_fr_residual64(int frn, int level, const double* u, const double* rho, const double* f, double* res,
                          double* partials, int n, const FrCoef& c, hipStream_t s);
-void launch_fr_restrict_residual1_64(int frn, const double* u0, const double* delta, double* rres1, int N,
-                                     const FrCoef& c0, hipStream_t s);
+// ru1/u1 non-null: the same pass also stores R u (restrict_scalar_field) there
+void launch_fr_restrict_residual1_64(int frn, const double* u0, const double* delta, double* rres1, double* ru1,
+                                     double* u1, int N, const FrCoef& c0, hipStream_t s);
 void launch_restrict8_64(const double* fine, double* out1, double* out2, int nc, hipStream_t s);
 void launch_fr_rhs64(int frn, const double* ru, const double* rho, const double* rres, double* f, int n,
                      const FrCoef& c, hipStream_t s);

@@ -275,6 +275,39 @@ __global__ __launch_bounds__(256) void k_colour_pass(OP op, Geo g, int colour, i
 // whose square is accumulated; the block sum goes to partials[logical block]
 // when partials is non-null.  Loads of plane k+1 are issued before plane k
 // is evaluated.
+// A run of exactly R planes of both colours, unrolled like colour_pass_run;
+// the squares are accumulated in the same order as the loop below.
+template <int R, class OP>
+__device__ __forceinline__ double both_colours_run(const OP& op, ZWin<typename OP::T>& w0, ZWin<typename OP::T>& w1,
+                                                   unsigned half, unsigned col, int j, int k0)
+{
+    using T = typename OP::T;
+    using Own = typename OP::Own;
+    ZPre<T> p0[R], p1[R];
+    Own o0[R], o1[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const unsigned rel = (unsigned)(k0 + r) * w0.plane + col;
+        p1[r] = w1.fetch(k0 + r, (j + k0 + r) & 1);
+        p0[r] = w0.fetch(k0 + r, (1 + j + k0 + r) & 1);
+        o0[r] = op.load(rel);
+        o1[r] = op.load(half + rel);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const T c0 = w0.cur, c1 = w1.cur;
+        const unsigned off = (unsigned)(k0 + r) * w0.plane + col;
+        T xp, xm, yp, ym, zp, zm;
+        w1.apply(p1[r], (j + k0 + r) & 1, xp, xm, yp, ym, zp, zm);
+        const double r0 = op.cell(off, o0[r], c0, xp, xm, yp, ym, zp, zm);
+        w0.apply(p0[r], (1 + j + k0 + r) & 1, xp, xm, yp, ym, zp, zm);
+        const double r1 = op.cell(half + off, o1[r], c1, xp, xm, yp, ym, zp, zm);
+        acc += r0 * r0 + r1 * r1;
+    }
+    return acc;
+}
+
 template <class OP>
 __device__ __forceinline__ double both_colours_col(const OP& op, const Geo& g, const ZCol& c)
 {
@@ -286,6 +319,7 @@ __device__ __forceinline__ double both_colours_col(const OP& op, const Geo& g, c
     w1.init(op.field() + g.half, g, c.m, c.j, c.k0);
     const unsigned half = (unsigned)g.half;
     const unsigned col = w0.jrow + w0.m;
+    if (c.k1 - c.k0 == ZRUN_UNROLLED) return both_colours_run<ZRUN_UNROLLED>(op, w0, w1, half, col, c.j, c.k0);
     int k = c.k0;
     // red cell: neighbours in the black half, x-parity (j+k)&1;
     // black cell: neighbours in the red half, x-parity (1+j+k)&1
