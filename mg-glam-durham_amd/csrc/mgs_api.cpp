@@ -508,10 +508,11 @@ static int cycle_plain(mgs_handle* h, int l, int kind)
 // ---------------------------------------------------------------------------
 // mixed precision: FP32 delta-form FAS V-cycle (inner) and FP64 outer loop
 // ---------------------------------------------------------------------------
-// One inner V(2,2) cycle at level l.  On level 0, `first` means d was set by
-// the fused outer residual (colour 0 already relaxed once from zero: that
-// pass is skipped) and `last` fuses the final colour-1 pass with the outer
-// update of u.
+// One inner V-cycle at level l: V(sweeps0) on level 0, V(2,2) below.  On
+// level 0, `first` means colour 0 of d was set by the fused outer residual
+// (relaxed once from zero: that pass is skipped, and colour 1, not written
+// there, is set by the colour-1 pass that follows) and `last` fuses the final
+// colour-1 pass with the outer update of u.
 static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = false)
 {
     const int Nsmth = 2;
@@ -580,13 +581,13 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
     return 0;
 }
 
-// FP64 outer residual; also writes the inner problem's g, xb and d after its
-// first colour-0 pass (reads u, delta; writes g, xb, d: 28 B/cell)
+// FP64 outer residual; also writes the inner problem's g, xb and colour 0 of
+// d after its first colour-0 pass (reads u, delta; writes g, xb, d: 26 B/cell)
 static int outer_residual(mgs_handle* h, double* rms)
 {
     LevelD& v = h->L[0];
     LevelF& m = h->M[0];
-    prof_begin(h, MGS_KC_RESIDUAL, 28.0 * n3d(v.n));
+    prof_begin(h, MGS_KC_RESIDUAL, 26.0 * n3d(v.n));
     const int nb = launch_outer_residual(h->p.fr_n, v.u, v.rho, m.f, m.xb, m.d, h->partials, v.n, fr_coefs(h->p, 0),
                                          df_coefs(h->p, 0), h->s);
     prof_end(h);

@@ -481,7 +481,9 @@ struct DfSmoothUpdateOp {
 // FP64 outer residual r = N(u) of the mixed loop (the plain path's residual),
 // emitting the inner problem's data: g = -r and xb = u / u_bg in FP32, and
 // the inner increment after its first colour-0 pass from d = 0 (where every
-// neighbour sum vanishes): d = solve(0) on colour 0, d = 0 on colour 1
+// neighbour sum vanishes): d = solve(0) on colour 0.  Colour 1 of d is not
+// written: the inner cycle's next pass is the colour-1 pass, which sets it
+// from colour 0 without reading it.
 template <int FRN>
 struct FrOuterOp {
     using T = double;
@@ -514,7 +516,7 @@ struct FrOuterOp {
         const float gv = (float)(-OP), xv = (float)(uc * c.inv_ubg);
         elem(g, o) = gv;
         elem(xb, o) = xv;
-        elem(d, o) = o < half ? df_solve<FRN>(0.0f, xv, gv, dc) : 0.0f;
+        if (o < half) elem(d, o) = df_solve<FRN>(0.0f, xv, gv, dc);
         return OP;
     }
 };
@@ -668,30 +670,56 @@ __device__ __forceinline__ void df_correct_sons(const float* ez, const float* ek
     }
 }
 
-template <int FRN>
+// e = dH - rd over the 3x3 coarse block of plane K.  SHFL: each row's centre
+// is loaded and its x-neighbours come from the adjacent lanes of the wave,
+// which hold columns I-1 and I+1 of the same row and plane run where lok /
+// rok say so (else they are loaded too): a third of the loads, same values.
+template <bool SHFL>
+__device__ __forceinline__ void e_block9(const float* __restrict__ dH, const float* __restrict__ rd, const Block9& B,
+                                         int K, bool lok, bool rok, float* e)
+{
+#pragma unroll
+    for (int y = 0; y < 3; y++) {
+        if constexpr (SHFL) {
+            const unsigned oc = B.at(1 + 3 * y, K);
+            const float c = elem(dH, oc) - elem(rd, oc);
+            float l = __shfl_up(c, 1u, 64);
+            float r = __shfl_down(c, 1u, 64);
+            if (!lok) {
+                const unsigned o = B.at(3 * y, K);
+                l = elem(dH, o) - elem(rd, o);
+            }
+            if (!rok) {
+                const unsigned o = B.at(2 + 3 * y, K);
+                r = elem(dH, o) - elem(rd, o);
+            }
+            e[3 * y] = l;
+            e[3 * y + 1] = c;
+            e[3 * y + 2] = r;
+        } else {
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                const unsigned o = B.at(x + 3 * y, K);
+                e[x + 3 * y] = elem(dH, o) - elem(rd, o);
+            }
+        }
+    }
+}
+
+template <int FRN, bool SHFL = false>
 __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, const float* __restrict__ rd,
                                                float* __restrict__ df, const float* __restrict__ xbf, const Geo& gf,
-                                               const Geo& gc, const ZCol& cc)
+                                               const Geo& gc, const ZCol& cc, bool lok = false, bool rok = false)
 {
     const int I = cc.m, J = cc.j;
     Block9 B;
     B.init(gc, I, J);
     float es[9], ek[9];  // plane beside K (K-1, then K+1) and plane K
-    const int Km = wrap_dec(cc.k0, gc.n);
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-        const unsigned om = B.at(q, Km), o0 = B.at(q, cc.k0);
-        es[q] = elem(dH, om) - elem(rd, om);
-        ek[q] = elem(dH, o0) - elem(rd, o0);
-    }
+    e_block9<SHFL>(dH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, es);
+    e_block9<SHFL>(dH, rd, B, cc.k0, lok, rok, ek);
     for (int K = cc.k0; K < cc.k1; K++) {
         df_correct_sons<FRN, 0>(es, ek, df, xbf, gf, I, J, K);
-        const int Kp = wrap_inc(K, gc.n);
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-            const unsigned op = B.at(q, Kp);
-            es[q] = elem(dH, op) - elem(rd, op);
-        }
+        e_block9<SHFL>(dH, rd, B, wrap_inc(K, gc.n), lok, rok, es);
         df_correct_sons<FRN, 1>(es, ek, df, xbf, gf, I, J, K);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
