@@ -178,6 +178,40 @@ int mgs_fifth_force(mgs_handle* h, void* psi, void* gx, void* gy, void* gz, int3
  * k [h/Mpc], P [(Mpc/h)^3], count of modes */
 int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, double* count);
 
+/* ---- PM step around the solve: particles, CIC deposit, Poisson FFT, push -- */
+/* Particles are the reference's REAL*4 XPAR, YPAR, ZPAR, VX, VY, VZ
+ * (PMP2mod_tools.f90:46): positions in grid units in [1, N+1), cell
+ * I = INT(X).  mgs_particles_set copies them into device arrays owned by the
+ * handle (where = MGS_HOST or MGS_DEVICE for the source; NULL velocities are
+ * zero); they stay resident across calls.  mgs_particles_get copies them out
+ * (NULL outputs are skipped). */
+int mgs_particles_set(mgs_handle* h, const float* x, const float* y, const float* z, const float* vx, const float* vy,
+                      const float* vz, int64_t npart, int32_t where);
+int mgs_particles_get(mgs_handle* h, float* x, float* y, float* z, float* vx, float* vy, float* vz, int32_t where);
+/* DENSIT's CIC deposit, PMP2mod_density.f90:235-281 (FI = -1, then each
+ * particle adds WPAR = N^3/npart over its 8 nearest cells): writes the
+ * handle's density contrast, which mgs_solve then uses.  FP64 accumulation.
+ * MGS_E_DOMAIN if any particle lies outside [1, N+1) or is not finite (it is
+ * skipped). */
+int mgs_cic_deposit(mgs_handle* h);
+/* POTENTfft5, PMP2main.f90:705-867: the potential phi with
+ * L_d phi = trfi delta (L_d the 7-point discrete Laplacian, grid units; the
+ * mean mode is multiplied by trfi), trfi = 1.5 Omega_m / a as the reference
+ * sets it (:727-731).  phi is kept on the device (N^3 FP64, x fastest). */
+int mgs_potential(mgs_handle* h, double trfi);
+/* POTENT_FOR_MG, PMP2main.f90:869-935 (f(R) and nDGP branches): phi += the
+ * scalar field's potential (psi of mgs_fifth_force) from the current u. */
+int mgs_potential_add_fifth_force(mgs_handle* h);
+/* phi to dst (N^3 in the reference's FI order) as MGS_F32 or MGS_F64 */
+int mgs_get_potential(mgs_handle* h, void* dst, int32_t dtype, int32_t where);
+/* MOVE, PMP2main.f90:304-685, f(R)/nDGP branch: each particle's velocity
+ * gets pconst times the CIC interpolation of the doubled central-difference
+ * gradients of phi at its 8 nodes, its position xconst times the new
+ * velocity, with the periodic wrap and the REAL*4 edge rule of :677-679;
+ * *svel = sum of v^2 WPAR (may be NULL).  pconst and xconst are the
+ * reference's PCONST and XCONST (:390-393). */
+int mgs_move(mgs_handle* h, double pconst, double xconst, double* svel);
+
 /* ---- synthetic input ------------------------------------------------- */
 /* seeded lognormal density delta = exp(g - sigma_g^2/2) - 1, g Gaussian with
  * P_g(k) ~ k^slope exp(-k^2/(2 k_c^2)), k_c = kcut_frac k_Nyq (Philox-4x32-10
@@ -192,7 +226,8 @@ int mgs_make_lognormal_density(mgs_handle* h, uint64_t seed, double sigma_g, dou
 #define MGS_KC_CORRECT 3     /* level-1 -> 0 prolongation */
 #define MGS_KC_COARSE 4      /* everything on levels >= 1 */
 #define MGS_KC_UPDATE 5      /* mixed: outer update */
-#define MGS_KC_COUNT 6
+#define MGS_KC_PM 6          /* PM step: deposit, Poisson FFT, fifth-force potential, move */
+#define MGS_KC_COUNT 7
 int mgs_profile(mgs_handle* h, int32_t enable);
 /* totals since enable: launches, milliseconds, algorithmic bytes */
 int mgs_profile_get(mgs_handle* h, int32_t kclass, int64_t* launches, double* ms, double* bytes);

@@ -922,6 +922,7 @@ int mgs_destroy(mgs_handle* h)
     if (h->staging) (void)hipFree(h->staging);
     if (h->hsum) (void)hipHostFree(h->hsum);
     fft_release(h);
+    pm_release(h);
     if (h->s) (void)hipStreamDestroy(h->s);
     delete h;
     return 0;

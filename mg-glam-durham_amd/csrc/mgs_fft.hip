@@ -236,6 +236,56 @@ __global__ __launch_bounds__(256) void k_bin_reduce(const double* __restrict__ p
     bins[t] = s;
 }
 
+// POTENTfft5's Green's function, PMP2main.f90:736-742, 805-816: the discrete
+// Laplacian's eigenvalue A = (G(ky) - 6 + G(kx)) + G(kz), G(k) = 2cos(P16 k/N)
+// with the reference's P16 = 6.28318530718, and A = 1 for the mean mode
+// (|A| < 1e-7).  The half spectrum (x fastest, N/2+1 of it) is multiplied by
+// trfi / A and by 1/N^3, hipFFT's transforms being unnormalised where
+// FFTPACK5's forward one divides by N per dimension.
+__global__ __launch_bounds__(256) void k_green(hipfftDoubleComplex* __restrict__ c, int N, double trfi,
+                                               double inv_n3)
+{
+    const size_t nh = (size_t)(N / 2 + 1);
+    const size_t total = nh * N * N;
+    const double P16 = 6.28318530718;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+        const int kx = (int)(q % nh);
+        const size_t r = q / nh;
+        const int ky = (int)(r % N), kz = (int)(r / N);
+        const double gx = 2.0 * cos(P16 * kx / N), gy = 2.0 * cos(P16 * ky / N), gz = 2.0 * cos(P16 * kz / N);
+        double A = gy - 6.0;
+        A = gx + A;
+        A = A + gz;
+        if (fabs(A) < 1.0e-7) A = 1.0;
+        hipfftDoubleComplex v = c[q];
+        v.x = v.x * trfi / A * inv_n3;
+        v.y = v.y * trfi / A * inv_n3;
+        c[q] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ a, float* __restrict__ b, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) b[q] = (float)a[q];
+}
+
+// POTENT_FOR_MG, PMP2main.f90:869-935: the potential (x fastest) plus the
+// scalar field's contribution at each cell
+__global__ __launch_bounds__(256) void k_add_psi(double* __restrict__ phi, const double* __restrict__ u, Geo g,
+                                                 int model, int frn, double fct, double inv_ubg)
+{
+    const size_t n3 = 2 * g.half;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n3; q += stride) {
+        const int i = (int)(q % g.n);
+        const size_t r = q / g.n;
+        const int j = (int)(r % g.n), k = (int)(r / g.n);
+        phi[q] = phi[q] + psi_at(u, g, i, j, k, model, frn, fct, inv_ubg);
+    }
+}
+
 static dim3 grid_for(size_t count)
 {
     size_t b = (count + 255) / 256;
@@ -402,6 +452,72 @@ int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, doub
         pk[b - 1] = c > 0 ? box * box * box * hb[b] / c : 0.0;
         if (count) count[b - 1] = c;
     }
+    return 0;
+}
+
+int mgs_potential(mgs_handle* h, double trfi)
+{
+    if (!h) return MGS_E_ARG;
+    if (!std::isfinite(trfi)) return set_err(h, MGS_E_ARG, "trfi not finite");
+    if (!h->has_density) return set_err(h, MGS_E_STATE, "no density set");
+    int rc = fft_ensure(h);
+    if (rc) return rc;
+    double* phi = pm_potential(h);
+    if (!phi) return set_err(h, MGS_E_NOMEM, "potential buffer");
+    FftState* f = fft_of(h);
+    const int N = h->p.ngrid;
+    const double n3 = (double)N * N * N;
+    prof_begin(h, MGS_KC_PM, 8.0 * n3 * 2.0);
+    launch_unpack_f64_to_f64(h->L[0].rho, f->real, N, 1.0, h->s);
+    if (hipfftExecD2Z(f->fwd, f->real, f->cplx) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecD2Z");
+    hipLaunchKernelGGL(k_green, grid_for((size_t)N * N * (N / 2 + 1)), dim3(256), 0, h->s, f->cplx, N, trfi, 1.0 / n3);
+    if (hipfftExecZ2D(f->inv, f->cplx, phi) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecZ2D");
+    prof_end(h);
+    if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "potential");
+    return 0;
+}
+
+int mgs_potential_add_fifth_force(mgs_handle* h)
+{
+    if (!h) return MGS_E_ARG;
+    double* phi = pm_potential(h);
+    if (!phi) return set_err(h, MGS_E_NOMEM, "potential buffer");
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    double ubg = 1.0;
+    const double fbg = fr_background(h->p, &ubg);
+    const double ctilde = 2.99792458e3 * (double)N / h->p.box;
+    prof_begin(h, MGS_KC_PM, 24.0 * (double)n3);
+    hipLaunchKernelGGL(k_add_psi, grid_for(n3), dim3(256), 0, h->s, phi, h->L[0].u, make_geo(N), h->p.model,
+                       h->p.fr_n, ctilde * ctilde * 0.5 * fbg, 1.0 / ubg);
+    prof_end(h);
+    if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "fifth-force potential");
+    return 0;
+}
+
+int mgs_get_potential(mgs_handle* h, void* dst, int32_t dtype, int32_t where)
+{
+    if (!h || !dst) return set_err(h, MGS_E_ARG, "NULL argument");
+    if (dtype != MGS_F32 && dtype != MGS_F64) return set_err(h, MGS_E_ARG, "dtype");
+    if (where != MGS_HOST && where != MGS_DEVICE) return set_err(h, MGS_E_ARG, "where");
+    if (!h->pm) return set_err(h, MGS_E_STATE, "no potential: call mgs_potential first");
+    double* phi = pm_potential(h);
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    const size_t bytes = n3 * (dtype == MGS_F32 ? 4 : 8);
+    void* out = dst;
+    if (where == MGS_HOST) {
+        int rc = ensure_staging(h, bytes);
+        if (rc) return rc;
+        out = h->staging;
+    }
+    if (dtype == MGS_F32)
+        hipLaunchKernelGGL(k_to_f32, grid_for(n3), dim3(256), 0, h->s, phi, (float*)out, n3);
+    else if (hipMemcpyAsync(out, phi, bytes, hipMemcpyDeviceToDevice, h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, "potential copy");
+    if (where == MGS_HOST && hipMemcpyAsync(dst, out, bytes, hipMemcpyDeviceToHost, h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, "potential copy");
+    if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "potential copy");
     return 0;
 }
 

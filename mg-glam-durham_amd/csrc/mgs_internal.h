@@ -86,6 +86,8 @@ struct mgs_handle {
     double* dgp_rms = nullptr;
     // FFT plans (density generator, power spectrum)
     void* fft = nullptr;
+    // particles and the potential of the PM step (mgs_pm.hip)
+    void* pm = nullptr;
     // captured coarse-level parts of the V-cycles (levels >= 1), replayed
     // every cycle; rebuilt when the model scalars change
     hipGraphExec_t g_plain = nullptr;
@@ -144,6 +146,11 @@ void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s);
 
 // mgs_fft.hip
 void fft_release(mgs_handle* h);
+
+// mgs_pm.hip
+void pm_release(mgs_handle* h);
+// the handle's N^3 FP64 potential (x fastest), allocated on first use; null on failure
+double* pm_potential(mgs_handle* h);
 }  // namespace mgs
 
 int set_err(mgs_handle* h, int code, const std::string& msg);

@@ -1,0 +1,343 @@
+// mgs_pm.hip -- the particle side of a PM step around the scalar-field solve:
+// CIC density deposit (DENSIT) and the particle push with the CIC-gathered
+// potential gradient (MOVE), on particles resident in HBM.
+//
+// Particles are the reference's REAL*4 arrays XPAR..VZ (PMP2mod_tools.f90:46)
+// in its convention: positions in grid units in [1, N+1), cell I = INT(X).
+// They live on the device in six float arrays owned by the handle.
+//
+// Deposit: one thread per particle adds its 8 CIC weights with hardware FP64
+// atomics straight into the solver's level-0 density (colour-split FP64, the
+// array mgs_set_density fills), so a solve can follow without a host trip.
+// FP64 accumulation makes the sum independent of the atomic order to ~1e-16
+// (the reference adds into REAL*4 FI).
+//
+// Move: one thread per particle gathers the 32 potential values of its 2x2x2
+// node block and their outer neighbours (PMP2main.f90:496-558), forms the
+// doubled central-difference gradients at the 8 nodes, interpolates them
+// with the CIC weights, kicks, drifts, wraps, and adds v^2 WPAR to a block
+// partial of SVEL.  The potential (mgs_potential, + mgs_potential_add_fifth_force)
+// is FP64 in x-fastest order.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/mgsolve.h"
+#include "mgs_device.h"
+#include "mgs_internal.h"
+#include "mgs_kernels.h"
+
+namespace mgs {
+
+struct PmState {
+    int64_t np = 0;
+    float* buf = nullptr;  // x, y, z, vx, vy, vz: 6 * np floats
+    double* phi = nullptr; // N^3 potential, x fastest
+    double* part = nullptr;
+    int* bad = nullptr;    // particles outside [1, N+1) seen by the last kernel
+    int npart_blocks = 0;
+};
+
+static PmState* pm_of(mgs_handle* h) { return (PmState*)h->pm; }
+
+constexpr int PM_BLOCK = 256;
+constexpr int PM_GRID_MAX = 8192;  // persistent blocks of the particle kernels
+
+static int pm_grid(int64_t np)
+{
+    const int64_t b = (np + PM_BLOCK - 1) / PM_BLOCK;
+    return (int)(b < PM_GRID_MAX ? (b < 1 ? 1 : b) : PM_GRID_MAX);
+}
+
+void pm_release(mgs_handle* h)
+{
+    PmState* p = pm_of(h);
+    if (!p) return;
+    if (p->buf) (void)hipFree(p->buf);
+    if (p->phi) (void)hipFree(p->phi);
+    if (p->part) (void)hipFree(p->part);
+    if (p->bad) (void)hipFree(p->bad);
+    delete p;
+    h->pm = nullptr;
+}
+
+double* pm_potential(mgs_handle* h)
+{
+    PmState* p = pm_of(h);
+    if (!p) {
+        p = new PmState();
+        h->pm = p;
+    }
+    if (!p->phi) {
+        const size_t n3 = (size_t)h->p.ngrid * h->p.ngrid * h->p.ngrid;
+        if (hipMalloc((void**)&p->phi, n3 * sizeof(double)) != hipSuccess) {
+            p->phi = nullptr;
+            return nullptr;
+        }
+    }
+    return p->phi;
+}
+
+// INT(X) of the reference for X in [1, N+1): the 1-based cell; -1 outside
+__device__ __forceinline__ int cell_of(double X, int N)
+{
+    if (!(X >= 1.0 && X < (double)N + 1.0)) return -1;
+    return (int)X;
+}
+
+// DENSIT's CIC loop, PMP2mod_density.f90:235-281 (0-based cells here)
+__global__ __launch_bounds__(PM_BLOCK) void k_cic_deposit(const float* __restrict__ xs, const float* __restrict__ ys,
+                                                          const float* __restrict__ zs, int64_t np,
+                                                          double* __restrict__ rho, Geo g, double wpar,
+                                                          int* __restrict__ bad)
+{
+    const int N = g.n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += stride) {
+        const double X = xs[p], Y = ys[p], Z = zs[p];
+        const int I = cell_of(X, N), J = cell_of(Y, N), K = cell_of(Z, N);
+        if (I < 0 || J < 0 || K < 0) {
+            atomicAdd(bad, 1);
+            continue;
+        }
+        const double D1 = X - (double)I, D2 = Y - (double)J, D3 = Z - (double)K;
+        const double T1 = 1.0 - D1, T2 = 1.0 - D2, T3 = 1.0 - D3;
+        const double T2W = T2 * wpar, D2W = D2 * wpar;
+        const int i0 = I - 1, j0 = J - 1, k0 = K - 1;
+        const int i1 = I == N ? 0 : I, j1 = J == N ? 0 : J, k1 = K == N ? 0 : K;
+        unsafeAtomicAdd(&rho[split_index(g, i0, j0, k0)], T3 * T1 * T2W);
+        unsafeAtomicAdd(&rho[split_index(g, i1, j0, k0)], T3 * D1 * T2W);
+        unsafeAtomicAdd(&rho[split_index(g, i0, j1, k0)], T3 * T1 * D2W);
+        unsafeAtomicAdd(&rho[split_index(g, i1, j1, k0)], T3 * D1 * D2W);
+        unsafeAtomicAdd(&rho[split_index(g, i0, j0, k1)], D3 * T1 * T2W);
+        unsafeAtomicAdd(&rho[split_index(g, i1, j0, k1)], D3 * D1 * T2W);
+        unsafeAtomicAdd(&rho[split_index(g, i0, j1, k1)], D3 * T1 * D2W);
+        unsafeAtomicAdd(&rho[split_index(g, i1, j1, k1)], D3 * D1 * D2W);
+    }
+}
+
+// MOVE's particle loop for the f(R) and nDGP models (no friction or mass
+// terms, PMP2main.f90:488-684): kick with the CIC-interpolated gradient of
+// the potential, drift, periodic wrap, SVEL partial per block
+__global__ __launch_bounds__(PM_BLOCK) void k_move(float* __restrict__ xs, float* __restrict__ ys,
+                                                   float* __restrict__ zs, float* __restrict__ vxs,
+                                                   float* __restrict__ vys, float* __restrict__ vzs, int64_t np,
+                                                   const double* __restrict__ phi, int N, double pconst,
+                                                   double xconst, double wpar, double* __restrict__ part,
+                                                   int* __restrict__ bad)
+{
+    // XN = FLOAT(NGRID)+1.-1.E-8 and YN = FLOAT(NGRID), evaluated as the
+    // reference does (default REAL: the 1e-8 is below float resolution)
+    const double XN = (double)((float)N + 1.0f - 1.0e-8f);
+    const double YN = (double)N;
+    const size_t n = (size_t)N;
+    double svel = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += stride) {
+        double X = xs[p], Y = ys[p], Z = zs[p];
+        double VVX = vxs[p], VVY = vys[p], VVZ = vzs[p];
+        const int I = cell_of(X, N), J = cell_of(Y, N), K = cell_of(Z, N);
+        if (I < 0 || J < 0 || K < 0) {
+            atomicAdd(bad, 1);
+            continue;
+        }
+        const double D1 = X - (double)I, D2 = Y - (double)J, D3 = Z - (double)K;
+        const double T1 = 1.0 - D1, T2 = 1.0 - D2, T3 = 1.0 - D3;
+        const double T2T1 = T2 * T1, T2D1 = T2 * D1, D2T1 = D2 * T1, D2D1 = D2 * D1;
+        // 0-based indices of I-1 .. I+2 (x), J-1 .. J+2 (y), K-1 .. K+2 (z)
+        const int x1 = I - 1, x2 = I == N ? 0 : I, x3 = I + 1 >= N ? I + 1 - N : I + 1, x0 = I == 1 ? N - 1 : I - 2;
+        const int y1 = J - 1, y2 = J == N ? 0 : J, y3 = J + 1 >= N ? J + 1 - N : J + 1, y0 = J == 1 ? N - 1 : J - 2;
+        const int z1 = K - 1, z2 = K == N ? 0 : K, z3 = K + 1 >= N ? K + 1 - N : K + 1, z0 = K == 1 ? N - 1 : K - 2;
+#define PHI(a, b, c) phi[(size_t)(a) + n * ((size_t)(b) + n * (size_t)(c))]
+        const double F111 = PHI(x1, y1, z1), F211 = PHI(x2, y1, z1), F121 = PHI(x1, y2, z1), F221 = PHI(x2, y2, z1);
+        const double F112 = PHI(x1, y1, z2), F212 = PHI(x2, y1, z2), F122 = PHI(x1, y2, z2), F222 = PHI(x2, y2, z2);
+        const double F113 = PHI(x1, y1, z3), F213 = PHI(x2, y1, z3), F123 = PHI(x1, y2, z3), F223 = PHI(x2, y2, z3);
+        const double F110 = PHI(x1, y1, z0), F210 = PHI(x2, y1, z0), F120 = PHI(x1, y2, z0), F220 = PHI(x2, y2, z0);
+        const double F311 = PHI(x3, y1, z1), F321 = PHI(x3, y2, z1), F131 = PHI(x1, y3, z1), F231 = PHI(x2, y3, z1);
+        const double F312 = PHI(x3, y1, z2), F322 = PHI(x3, y2, z2), F132 = PHI(x1, y3, z2), F232 = PHI(x2, y3, z2);
+        const double F011 = PHI(x0, y1, z1), F021 = PHI(x0, y2, z1), F101 = PHI(x1, y0, z1), F201 = PHI(x2, y0, z1);
+        const double F012 = PHI(x0, y1, z2), F022 = PHI(x0, y2, z2), F102 = PHI(x1, y0, z2), F202 = PHI(x2, y0, z2);
+#undef PHI
+        // doubled gradients at the 8 nodes (the 1/2 is in pconst)
+        const double GX111 = F211 - F011, GX211 = F311 - F111, GX121 = F221 - F021, GX221 = F321 - F121;
+        const double GX112 = F212 - F012, GX212 = F312 - F112, GX122 = F222 - F022, GX222 = F322 - F122;
+        const double GY111 = F121 - F101, GY211 = F221 - F201, GY121 = F131 - F111, GY221 = F231 - F211;
+        const double GY112 = F122 - F102, GY212 = F222 - F202, GY122 = F132 - F112, GY222 = F232 - F212;
+        const double GZ111 = F112 - F110, GZ211 = F212 - F210, GZ121 = F122 - F120, GZ221 = F222 - F220;
+        const double GZ112 = F113 - F111, GZ212 = F213 - F211, GZ122 = F123 - F121, GZ222 = F223 - F221;
+        const double GX = pconst * (T3 * (T2T1 * GX111 + T2D1 * GX211 + D2T1 * GX121 + D2D1 * GX221) +
+                                    D3 * (T2T1 * GX112 + T2D1 * GX212 + D2T1 * GX122 + D2D1 * GX222));
+        const double GY = pconst * (T3 * (T2T1 * GY111 + T2D1 * GY211 + D2T1 * GY121 + D2D1 * GY221) +
+                                    D3 * (T2T1 * GY112 + T2D1 * GY212 + D2T1 * GY122 + D2D1 * GY222));
+        const double GZ = pconst * (T3 * (T2T1 * GZ111 + T2D1 * GZ211 + D2T1 * GZ121 + D2D1 * GZ221) +
+                                    D3 * (T2T1 * GZ112 + T2D1 * GZ212 + D2T1 * GZ122 + D2D1 * GZ222));
+        VVX = VVX + GX;
+        VVY = VVY + GY;
+        VVZ = VVZ + GZ;
+        X = X + VVX * xconst;
+        Y = Y + VVY * xconst;
+        Z = Z + VVZ * xconst;
+        if (X < 1.0) X = X + YN;
+        if (X >= XN) X = X - YN;
+        if (Y < 1.0) Y = Y + YN;
+        if (Y >= XN) Y = Y - YN;
+        if (Z < 1.0) Z = Z + YN;
+        if (Z >= XN) Z = Z - YN;
+        svel = svel + (VVX * VVX + VVY * VVY + VVZ * VVZ) * wpar;
+        float xf = (float)X, yf = (float)Y, zf = (float)Z;
+        // a coordinate that rounds to N+1 in REAL*4 is pulled back inside
+        if ((int)xf == N + 1) xf = xf - 1.0e-3f;
+        if ((int)yf == N + 1) yf = yf - 1.0e-3f;
+        if ((int)zf == N + 1) zf = zf - 1.0e-3f;
+        xs[p] = xf;
+        ys[p] = yf;
+        zs[p] = zf;
+        vxs[p] = (float)VVX;
+        vys[p] = (float)VVY;
+        vzs[p] = (float)VVZ;
+    }
+    const double t = block_sum(svel);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// WPAR = YN**3/FLOAT(Nparticles) (PMP2mod_density.f90:40, PMP2main.f90:470):
+// the particle count is converted to default REAL first, as there
+static double wpar_of(int N, int64_t np)
+{
+    const double yn = (double)N;
+    return yn * yn * yn / (double)(float)np;
+}
+
+static int pm_ensure_particles(mgs_handle* h, int64_t np)
+{
+    PmState* p = pm_of(h);
+    if (!p) {
+        p = new PmState();
+        h->pm = p;
+    }
+    if (!p->bad && hipMalloc((void**)&p->bad, sizeof(int)) != hipSuccess) {
+        p->bad = nullptr;
+        return set_err(h, MGS_E_NOMEM, "particle counters");
+    }
+    if (!p->part) {
+        if (hipMalloc((void**)&p->part, (PM_GRID_MAX + 1024) * sizeof(double)) != hipSuccess) {
+            p->part = nullptr;
+            return set_err(h, MGS_E_NOMEM, "particle partials");
+        }
+    }
+    if (p->np == np && p->buf) return 0;
+    if (p->buf) (void)hipFree(p->buf);
+    p->buf = nullptr;
+    p->np = 0;
+    if (hipMalloc((void**)&p->buf, (size_t)np * 6 * sizeof(float)) != hipSuccess) {
+        p->buf = nullptr;
+        return set_err(h, MGS_E_NOMEM, "particle arrays (" + std::to_string(np) + " particles)");
+    }
+    p->np = np;
+    return 0;
+}
+
+// out-of-range particles counted by the last kernel (synchronises)
+static int pm_check_bad(mgs_handle* h, const char* what)
+{
+    int bad = 0;
+    if (hipMemcpyAsync(&bad, pm_of(h)->bad, sizeof(int), hipMemcpyDeviceToHost, h->s) != hipSuccess ||
+        hipStreamSynchronize(h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, std::string(what) + ": HIP error");
+    if (bad)
+        return set_err(h, MGS_E_DOMAIN, std::string(what) + ": " + std::to_string(bad) +
+                                            " particle(s) outside [1, N+1) or not finite were skipped");
+    return 0;
+}
+
+}  // namespace mgs
+
+using namespace mgs;
+
+extern "C" {
+
+int mgs_particles_set(mgs_handle* h, const float* x, const float* y, const float* z, const float* vx, const float* vy,
+                      const float* vz, int64_t npart, int32_t where)
+{
+    if (!h) return MGS_E_ARG;
+    if (!x || !y || !z) return set_err(h, MGS_E_ARG, "positions are NULL");
+    if (npart < 1) return set_err(h, MGS_E_ARG, "npart < 1");
+    if (where != MGS_HOST && where != MGS_DEVICE) return set_err(h, MGS_E_ARG, "where");
+    int rc = pm_ensure_particles(h, npart);
+    if (rc) return rc;
+    PmState* p = pm_of(h);
+    const size_t bytes = (size_t)npart * sizeof(float);
+    const hipMemcpyKind kind = where == MGS_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    const float* src[6] = {x, y, z, vx, vy, vz};
+    for (int a = 0; a < 6; a++) {
+        float* dst = p->buf + (size_t)a * npart;
+        const hipError_t e = src[a] ? hipMemcpyAsync(dst, src[a], bytes, kind, h->s) : hipMemsetAsync(dst, 0, bytes, h->s);
+        if (e != hipSuccess) return set_err(h, MGS_E_HIP, "particle upload");
+    }
+    if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "particle upload");
+    return 0;
+}
+
+int mgs_particles_get(mgs_handle* h, float* x, float* y, float* z, float* vx, float* vy, float* vz, int32_t where)
+{
+    if (!h) return MGS_E_ARG;
+    PmState* p = pm_of(h);
+    if (!p || !p->buf) return set_err(h, MGS_E_STATE, "no particles set");
+    if (where != MGS_HOST && where != MGS_DEVICE) return set_err(h, MGS_E_ARG, "where");
+    const size_t bytes = (size_t)p->np * sizeof(float);
+    const hipMemcpyKind kind = where == MGS_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    float* dst[6] = {x, y, z, vx, vy, vz};
+    for (int a = 0; a < 6; a++)
+        if (dst[a] && hipMemcpyAsync(dst[a], p->buf + (size_t)a * p->np, bytes, kind, h->s) != hipSuccess)
+            return set_err(h, MGS_E_HIP, "particle download");
+    if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "particle download");
+    return 0;
+}
+
+int mgs_cic_deposit(mgs_handle* h)
+{
+    if (!h) return MGS_E_ARG;
+    PmState* p = pm_of(h);
+    if (!p || !p->buf) return set_err(h, MGS_E_STATE, "no particles set");
+    const int N = h->p.ngrid;
+    const Geo g = make_geo(N);
+    prof_begin(h, MGS_KC_PM, (double)p->np * 12.0 + 8.0 * 2.0 * (double)N * N * N);
+    launch_fill_f64(h->L[0].rho, (size_t)N * N * N, -1.0, h->s);
+    if (hipMemsetAsync(p->bad, 0, sizeof(int), h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "deposit");
+    hipLaunchKernelGGL(k_cic_deposit, dim3(pm_grid(p->np)), dim3(PM_BLOCK), 0, h->s, p->buf, p->buf + p->np,
+                       p->buf + 2 * p->np, p->np, h->L[0].rho, g, wpar_of(N, p->np), p->bad);
+    prof_end(h);
+    int rc = pm_check_bad(h, "CIC deposit");
+    h->has_density = true;
+    h->rho_cached = false;
+    return rc;
+}
+
+int mgs_move(mgs_handle* h, double pconst, double xconst, double* svel)
+{
+    if (!h) return MGS_E_ARG;
+    PmState* p = pm_of(h);
+    if (!p || !p->buf) return set_err(h, MGS_E_STATE, "no particles set");
+    if (!p->phi) return set_err(h, MGS_E_STATE, "no potential: call mgs_potential first");
+    if (!std::isfinite(pconst) || !std::isfinite(xconst)) return set_err(h, MGS_E_ARG, "pconst/xconst not finite");
+    const int N = h->p.ngrid;
+    const int grid = pm_grid(p->np);
+    prof_begin(h, MGS_KC_PM, (double)p->np * 48.0);
+    if (hipMemsetAsync(p->bad, 0, sizeof(int), h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "move");
+    const int64_t np = p->np;
+    float* b = p->buf;
+    hipLaunchKernelGGL(k_move, dim3(grid), dim3(PM_BLOCK), 0, h->s, b, b + np, b + 2 * np, b + 3 * np, b + 4 * np,
+                       b + 5 * np, np, p->phi, N, pconst, xconst, wpar_of(N, np), p->part, p->bad);
+    launch_sum_partials(p->part, grid, h->dsum, h->s);
+    prof_end(h);
+    double s = 0.0;
+    if (hipMemcpyAsync(&s, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, "move");
+    int rc = pm_check_bad(h, "move");
+    if (svel) *svel = s;
+    return rc;
+}
+
+}  // extern "C"

@@ -38,7 +38,7 @@ F32, F64 = 0, 1
 HOST, DEVICE = 0, 1
 ARR_U, ARR_RHO, ARR_F, ARR_RES, ARR_RRES, ARR_RU = range(6)
 KC_SMOOTH, KC_RESIDUAL, KC_RESTRICT, KC_CORRECT, KC_COARSE, KC_UPDATE = range(6)
-KC_NAMES = ["smooth", "residual", "restrict", "correct", "coarse", "update"]
+KC_NAMES = ["smooth", "residual", "restrict", "correct", "coarse", "update", "pm"]
 
 
 class MgsError(RuntimeError):
@@ -138,6 +138,13 @@ def load():
         "mgs_fifth_force": ([H, vp, vp, vp, vp, i32, i32], C.c_int),
         "mgs_power_spectrum": ([H, i32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "mgs_make_lognormal_density": ([H, C.c_uint64, C.c_double, C.c_double, C.c_double], C.c_int),
+        "mgs_particles_set": ([H, vp, vp, vp, vp, vp, vp, C.c_int64, i32], C.c_int),
+        "mgs_particles_get": ([H, vp, vp, vp, vp, vp, vp, i32], C.c_int),
+        "mgs_cic_deposit": ([H], C.c_int),
+        "mgs_potential": ([H, C.c_double], C.c_int),
+        "mgs_potential_add_fifth_force": ([H], C.c_int),
+        "mgs_get_potential": ([H, vp, i32, i32], C.c_int),
+        "mgs_move": ([H, C.c_double, C.c_double, C.POINTER(C.c_double)], C.c_int),
         "mgs_profile": ([H, i32], C.c_int),
         "mgs_profile_get": ([H, i32, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "mgs_synchronize": ([H], C.c_int),
@@ -340,6 +347,41 @@ class Solver:
     def make_lognormal_density(self, seed, sigma_g, slope=-1.5, kcut_frac=0.5):
         self._chk(self.L.mgs_make_lognormal_density(self.h, seed, sigma_g, slope, kcut_frac),
                   "mgs_make_lognormal_density")
+
+    # ---- PM step: particles, deposit, potential, push ----------------------
+    def set_particles(self, x, y, z, vx=None, vy=None, vz=None):
+        """Positions in grid units in [1, N+1) (the reference's XPAR convention)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, z, vx, vy, vz)]
+        n = arrs[0].size
+        if any(a is not None and a.size != n for a in arrs):
+            raise MgsError("particle arrays differ in length")
+        ptrs = [None if a is None else _np_ptr(a) for a in arrs]
+        self._chk(self.L.mgs_particles_set(self.h, *ptrs, n, HOST), "mgs_particles_set")
+        self._np = n
+
+    def get_particles(self):
+        outs = [np.empty(self._np, np.float32) for _ in range(6)]
+        self._chk(self.L.mgs_particles_get(self.h, *[_np_ptr(a) for a in outs], HOST), "mgs_particles_get")
+        return tuple(outs)
+
+    def cic_deposit(self):
+        self._chk(self.L.mgs_cic_deposit(self.h), "mgs_cic_deposit")
+
+    def potential(self, trfi):
+        self._chk(self.L.mgs_potential(self.h, trfi), "mgs_potential")
+
+    def potential_add_fifth_force(self):
+        self._chk(self.L.mgs_potential_add_fifth_force(self.h), "mgs_potential_add_fifth_force")
+
+    def get_potential(self, dtype=np.float64):
+        out = np.empty((self.N,) * 3, dtype)
+        self._chk(self.L.mgs_get_potential(self.h, _np_ptr(out), self._dt(out), HOST), "mgs_get_potential")
+        return out
+
+    def move(self, pconst, xconst):
+        s = C.c_double()
+        self._chk(self.L.mgs_move(self.h, pconst, xconst, C.byref(s)), "mgs_move")
+        return s.value
 
     # ---- profiling ---------------------------------------------------------
     def profile(self, enable=True):

@@ -1,0 +1,125 @@
+"""PM step around the solve on the GPU (mgs_particles_*, mgs_cic_deposit,
+mgs_potential, mgs_potential_add_fifth_force, mgs_move) against the numpy
+restatement in oracle/pm.py.
+
+Bars: the deposit and the potential to FP64 rounding (the deposit's atomic
+order and hipFFT vs numpy's FFT differ only there); the push bit-exact
+(FP64 arithmetic in the reference's operation order on REAL*4 particles, no
+contraction on either side); SVEL to FP64 rounding (block order)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import pm as P
+import mgsolve as M
+from conftest import gpu_available
+
+
+def particles(N, n, seed):
+    rng = np.random.default_rng(seed)
+    x, y, z = (rng.uniform(1.0, N + 1.0, n).astype(np.float32) for _ in range(3))
+    # edge cases: on the first node, just below N+1, on the wrap face
+    x[:3] = np.float32([1.0, np.nextafter(np.float32(N + 1), np.float32(0)), N + 0.5])
+    v = [rng.normal(0.0, 0.3, n).astype(np.float32) for _ in range(3)]
+    return x, y, z, v
+
+
+def fr_cfg(N, prec=M.PREC_FP64):
+    return M.Config(N=N, model=M.MODEL_FR, fr_n=1, box=2.0 * N, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=-1e-5,
+                    precision=prec)
+
+
+@pytest.mark.gpu
+def test_deposit_and_potential_match_oracle():
+    N = 32
+    x, y, z, _ = particles(N, 40000, 7)
+    print(f"[pm] deposit N={N} np={x.size}", flush=True)
+    with M.Solver(fr_cfg(N)) as S:
+        S.set_particles(x, y, z)
+        S.cic_deposit()
+        d = S.get_density(np.float64)
+        d_o = P.cic_deposit(x, y, z, N)
+        w = P.wpar(N, x.size)
+        assert np.allclose(d, d_o, rtol=0, atol=1e-12 * w), np.max(np.abs(d - d_o))
+        assert abs(np.sum(d + 1.0) - N ** 3) <= 1e-9 * N ** 3
+        trfi = 1.5 * 0.3 / 1.0
+        S.potential(trfi)
+        phi = S.get_potential()
+        phi_o = P.poisson_fft(d_o, trfi)
+        assert np.allclose(phi, phi_o, rtol=0, atol=1e-11 * np.max(np.abs(phi_o))), np.max(np.abs(phi - phi_o))
+        phi32 = S.get_potential(np.float32)
+        assert np.array_equal(phi32, phi.astype(np.float32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [M.MODEL_FR, M.MODEL_DGP])
+def test_potential_plus_fifth_force_after_a_solve(model):
+    N = 32
+    x, y, z, _ = particles(N, 60000, 11)
+    if model == M.MODEL_FR:
+        c = fr_cfg(N, M.PREC_MIXED)
+    else:
+        c = M.Config(N=N, model=M.MODEL_DGP, branch=1, box=2.0 * N, omega_m=0.3, omega_l=0.7, aexpn=1.0, h0rc=1.0,
+                     precision=M.PREC_FP64)
+    print(f"[pm] deposit -> solve -> potential + fifth force, model={model}", flush=True)
+    with M.Solver(c) as S:
+        S.set_particles(x, y, z)
+        S.cic_deposit()
+        if model == M.MODEL_FR:
+            rep = S.solve(mode=1, rel_tol=1e-8, max_cycles=60)
+        else:
+            rep = S.solve(mode=0, res_conv=1e-30, max_cycles=30)
+        assert rep["res_final"] < rep["res0"]
+        d = S.get_density(np.float64)
+        u = S.get_field(np.float64)
+        S.potential(0.45)
+        S.potential_add_fifth_force()
+        phi = S.get_potential()
+    p = O.Params(N=N, model=model, fr_n=1, branch=1, box=2.0 * N, om=0.3, oml=0.7, a=1.0, fr0=-1e-5, h0rc=1.0)
+    un = u / O.fr_ubg(p) if model == M.MODEL_FR else u
+    expect = P.poisson_fft(d, 0.45) + O.fifth_force_potential(p, un)
+    assert np.allclose(phi, expect, rtol=0, atol=1e-11 * np.max(np.abs(expect)))
+
+
+@pytest.mark.gpu
+def test_move_matches_oracle_bitwise():
+    N = 32
+    x, y, z, (vx, vy, vz) = particles(N, 50000, 13)
+    rng = np.random.default_rng(2)
+    phi = rng.standard_normal((N, N, N))
+    pc, xc = P.pconst_xconst(0.5, 0.02, 0.3, 0.7)
+    print(f"[pm] move N={N} np={x.size}", flush=True)
+    with M.Solver(fr_cfg(N)) as S:
+        # a rough potential from a random density; the oracle pushes with the
+        # potential the GPU holds
+        S.set_density(P.laplacian7(phi))
+        S.potential(0.7)
+        phi_g = S.get_potential()
+        S.set_particles(x, y, z, vx, vy, vz)
+        svel = S.move(pc, xc)
+        got = S.get_particles()
+    expect = P.move(phi_g, x, y, z, vx, vy, vz, pc, xc)
+    for a, b, name in zip(got, expect[:6], ("x", "y", "z", "vx", "vy", "vz")):
+        assert np.array_equal(a, b), (name, int(np.sum(a != b)))
+    assert svel == pytest.approx(expect[6], rel=1e-12)
+    assert np.all(got[0] >= 1.0) and np.all(got[0] < N + 1.0)
+
+
+@pytest.mark.gpu
+def test_pm_errors():
+    N = 16
+    with M.Solver(fr_cfg(N)) as S:
+        with pytest.raises(M.MgsError, match="no particles"):
+            S.cic_deposit()
+        S.set_particles(np.float32([1.5, 17.5]), np.float32([2.0, 2.0]), np.float32([3.0, 3.0]))
+        with pytest.raises(M.MgsError, match="outside"):
+            S.cic_deposit()
+        with pytest.raises(M.MgsError, match="no potential"):
+            S.move(-0.1, 0.1)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
+def test_pm_without_gpu_is_an_error():
+    with pytest.raises(M.MgsError):
+        with M.Solver(fr_cfg(16)) as S:
+            S.cic_deposit()
