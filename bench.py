@@ -146,6 +146,75 @@ def dgp_compare(N, box, sigma, seed, steps):
     return out
 
 
+def pm_step(N, seed, steps):
+    """One PM step around the solve, as the reference's time step runs it
+    (PMP2main.f90:39-58): CIC deposit of N^3 particles, the mixed f(R) solve,
+    the Poisson FFT, the fifth-force potential and the push.  Particles start
+    on a lattice displaced by up to half a cell (seeded); device times per
+    operation from the library's hipEvent profiler, algorithmic bytes: deposit
+    12 B per particle + 8 B per cell, potential 16 B per cell through the
+    FFTs' traffic not counted, fifth force 16 B per cell, push 48 B per
+    particle (the potential gathers hit the caches)."""
+    rng = np.random.default_rng(seed)
+    g = np.arange(1, N + 1, dtype=np.float32)
+    pos = []
+    for ax in range(3):  # x varies fastest over the particle index, like the grid
+        shape = [1, 1, 1]
+        shape[2 - ax] = N
+        base = np.broadcast_to(g.reshape(shape), (N, N, N)).reshape(-1)
+        p = base + rng.uniform(-0.5, 0.5, N ** 3).astype(np.float32)
+        p = np.where(p < 1.0, p + N, p)
+        p = np.where(p >= N + 1.0, p - N, p).astype(np.float32)
+        pos.append(p)
+        del base
+    vel = [rng.normal(0.0, 0.05, N ** 3).astype(np.float32) for _ in range(3)]
+    c = M.Config(N=N, model=M.MODEL_FR, fr_n=1, box=float(N), omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=-1e-6,
+                 precision=M.PREC_MIXED)
+    pc, xc = -0.5 * 0.02, 0.02
+    out = {}
+    with M.Solver(c) as S:
+        S.set_particles(*pos, *vel)
+        del pos, vel
+        times = {"deposit": [], "solve": [], "potential": [], "fifth_force": [], "move": []}
+        for it in range(steps + 1):
+            S.profile(True)
+            last = 0.0
+            per = {}
+            for name, fn in (("deposit", S.cic_deposit),
+                             ("solve", lambda: S.solve(mode=1, rel_tol=1e-8, max_cycles=60)),
+                             ("potential", lambda: S.potential(1.5 * 0.3)),
+                             ("fifth_force", S.potential_add_fifth_force),
+                             ("move", lambda: S.move(pc, xc))):
+                t0 = time.perf_counter()
+                r = fn()
+                S.synchronize()
+                wall = (time.perf_counter() - t0) * 1e3
+                if name == "solve":
+                    per[name] = r["seconds"] * 1e3
+                else:
+                    tot = S.profile_get(6)[1]
+                    per[name] = tot - last
+                    last = tot
+                per[name + "_wall"] = wall
+            S.profile(False)
+            if it > 0:
+                for k in times:
+                    times[k].append(per[k])
+        n3 = float(N) ** 3
+        med = {k: float(np.median(v)) for k, v in times.items()}
+        alg = {"deposit": 12.0 * n3 + 8.0 * n3, "fifth_force": 16.0 * n3, "move": 48.0 * n3}
+        out = {
+            "workload": f"{N}^3 grid, {N}^3 particles (lattice + U(-0.5, 0.5) cell displacements), f(R) n=1 "
+                        f"|fR0|=1e-6, mixed solve to 1e-8",
+            "ms": med,
+            "alg_GBs": {k: alg[k] / (med[k] * 1e-3) / 1e9 for k in alg},
+            "particles_per_s": {"deposit": n3 / (med["deposit"] * 1e-3), "move": n3 / (med["move"] * 1e-3)},
+            "step_ms": sum(med.values()),
+        }
+    log(f"[pm] {N}^3: " + ", ".join(f"{k} {v:.2f} ms" for k, v in med.items()))
+    return out
+
+
 def serving_stream(n_req, seed, max_active):
     """The serving stream: n_req f(R) n=1 requests, ngrid uniform over
     {64, 128, 192, 256} (cell 2 Mpc/h), |fR0| from {1e-4, 1e-5, 1e-6},
@@ -280,6 +349,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=512, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--dgp-n", type=int, default=512, help="grid of the nDGP comparison (0: skip)")
+    ap.add_argument("--pm-n", type=int, default=512, help="grid (and particles per side) of the PM step (0: skip)")
     args = ap.parse_args()
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
@@ -341,6 +411,10 @@ def main():
     if args.dgp_n > 0:
         dgp = dgp_compare(args.dgp_n, float(args.dgp_n), 1.2, 1005, 1)
 
+    pm = None
+    if args.pm_n > 0:
+        pm = pm_step(args.pm_n, 77, 2)
+
     serving = None
     if args.serve_requests > 0:
         serving = serving_stream(args.serve_requests, 4000, args.serve_slots)
@@ -380,6 +454,7 @@ def main():
         "solve_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "serving": serving,
         "dgp": dgp,
+        "pm_step": pm,
         "mixed": {
             "outer_iterations": reps[-1]["outer"],
             "inner_vcycles": reps[-1]["cycles"],

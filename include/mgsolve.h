@@ -31,6 +31,9 @@
  *   POTENT_FOR_MG (MG potential)                PMP2main.f90:869
  *   MOVE gradient stencil (2x grad by central difference) PMP2main.f90:565-594
  *   POWERfft5 (NGP-binned power spectrum)       PMP2mod_power.f90:209
+ *   DENSIT, CIC deposit branch                  PMP2mod_density.f90:235-281
+ *   POTENTfft5 (FFT Poisson solve)              PMP2main.f90:705
+ *   MOVE (f(R)/nDGP particle push)              PMP2main.f90:304
  */
 #ifndef MGSOLVE_H
 #define MGSOLVE_H
@@ -197,10 +200,12 @@ int mgs_cic_deposit(mgs_handle* h);
 /* POTENTfft5, PMP2main.f90:705-867: the potential phi with
  * L_d phi = trfi delta (L_d the 7-point discrete Laplacian, grid units; the
  * mean mode is multiplied by trfi), trfi = 1.5 Omega_m / a as the reference
- * sets it (:727-731).  phi is kept on the device (N^3 FP64, x fastest). */
+ * sets it (:727-731).  phi is kept on the device (N^3, x fastest) in REAL*4
+ * like the reference's FI, rounded once from the FP64 transforms. */
 int mgs_potential(mgs_handle* h, double trfi);
 /* POTENT_FOR_MG, PMP2main.f90:869-935 (f(R) and nDGP branches): phi += the
- * scalar field's potential (psi of mgs_fifth_force) from the current u. */
+ * scalar field's potential (psi of mgs_fifth_force) from the current u,
+ * summed in FP64 and rounded to REAL*4 as FI = FI + sf there. */
 int mgs_potential_add_fifth_force(mgs_handle* h);
 /* phi to dst (N^3 in the reference's FI order) as MGS_F32 or MGS_F64 */
 int mgs_get_potential(mgs_handle* h, void* dst, int32_t dtype, int32_t where);

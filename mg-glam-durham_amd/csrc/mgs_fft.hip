@@ -5,7 +5,10 @@
 //   * the fifth-force potential psi and its central-difference gradient
 //     (POTENT_FOR_MG, PMP2main.f90:884-950, and MOVE's stencil :565-594);
 //   * the NGP-binned power spectrum with POWERfft5's bins
-//     (PMP2mod_power.f90:323-418).
+//     (PMP2mod_power.f90:323-418);
+//   * the PM step's potential: POTENTfft5's discrete-Green's-function
+//     Poisson solve (PMP2main.f90:705-867) and POTENT_FOR_MG's addition of
+//     the scalar field (:869-935), into the handle's REAL*4 potential.
 
 #include <hipfft/hipfft.h>
 
@@ -271,9 +274,16 @@ __global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ a, fl
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) b[q] = (float)a[q];
 }
 
-// POTENT_FOR_MG, PMP2main.f90:869-935: the potential (x fastest) plus the
-// scalar field's contribution at each cell
-__global__ __launch_bounds__(256) void k_add_psi(double* __restrict__ phi, const double* __restrict__ u, Geo g,
+__global__ __launch_bounds__(256) void k_to_f64(const float* __restrict__ a, double* __restrict__ b, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) b[q] = (double)a[q];
+}
+
+// POTENT_FOR_MG, PMP2main.f90:869-935: the REAL*4 potential (x fastest) plus
+// the scalar field's contribution at each cell, summed in FP64 and rounded
+// once, as FI = FI + sf does there
+__global__ __launch_bounds__(256) void k_add_psi(float* __restrict__ phi, const double* __restrict__ u, Geo g,
                                                  int model, int frn, double fct, double inv_ubg)
 {
     const size_t n3 = 2 * g.half;
@@ -282,7 +292,7 @@ __global__ __launch_bounds__(256) void k_add_psi(double* __restrict__ phi, const
         const int i = (int)(q % g.n);
         const size_t r = q / g.n;
         const int j = (int)(r % g.n), k = (int)(r / g.n);
-        phi[q] = phi[q] + psi_at(u, g, i, j, k, model, frn, fct, inv_ubg);
+        phi[q] = (float)((double)phi[q] + psi_at(u, g, i, j, k, model, frn, fct, inv_ubg));
     }
 }
 
@@ -462,16 +472,18 @@ int mgs_potential(mgs_handle* h, double trfi)
     if (!h->has_density) return set_err(h, MGS_E_STATE, "no density set");
     int rc = fft_ensure(h);
     if (rc) return rc;
-    double* phi = pm_potential(h);
+    float* phi = pm_potential(h);
     if (!phi) return set_err(h, MGS_E_NOMEM, "potential buffer");
     FftState* f = fft_of(h);
     const int N = h->p.ngrid;
     const double n3 = (double)N * N * N;
-    prof_begin(h, MGS_KC_PM, 8.0 * n3 * 2.0);
+    prof_begin(h, MGS_KC_PM, 8.0 * n3 * 2.0 + 12.0 * n3);
     launch_unpack_f64_to_f64(h->L[0].rho, f->real, N, 1.0, h->s);
     if (hipfftExecD2Z(f->fwd, f->real, f->cplx) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecD2Z");
     hipLaunchKernelGGL(k_green, grid_for((size_t)N * N * (N / 2 + 1)), dim3(256), 0, h->s, f->cplx, N, trfi, 1.0 / n3);
-    if (hipfftExecZ2D(f->inv, f->cplx, phi) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecZ2D");
+    if (hipfftExecZ2D(f->inv, f->cplx, f->real) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecZ2D");
+    // FI is REAL*4 between the reference's transform passes and after them
+    hipLaunchKernelGGL(k_to_f32, grid_for((size_t)n3), dim3(256), 0, h->s, f->real, phi, (size_t)n3);
     prof_end(h);
     if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "potential");
     return 0;
@@ -480,14 +492,14 @@ int mgs_potential(mgs_handle* h, double trfi)
 int mgs_potential_add_fifth_force(mgs_handle* h)
 {
     if (!h) return MGS_E_ARG;
-    double* phi = pm_potential(h);
+    float* phi = pm_potential(h);
     if (!phi) return set_err(h, MGS_E_NOMEM, "potential buffer");
     const int N = h->p.ngrid;
     const size_t n3 = (size_t)N * N * N;
     double ubg = 1.0;
     const double fbg = fr_background(h->p, &ubg);
     const double ctilde = 2.99792458e3 * (double)N / h->p.box;
-    prof_begin(h, MGS_KC_PM, 24.0 * (double)n3);
+    prof_begin(h, MGS_KC_PM, 16.0 * (double)n3);
     hipLaunchKernelGGL(k_add_psi, grid_for(n3), dim3(256), 0, h->s, phi, h->L[0].u, make_geo(N), h->p.model,
                        h->p.fr_n, ctilde * ctilde * 0.5 * fbg, 1.0 / ubg);
     prof_end(h);
@@ -501,7 +513,7 @@ int mgs_get_potential(mgs_handle* h, void* dst, int32_t dtype, int32_t where)
     if (dtype != MGS_F32 && dtype != MGS_F64) return set_err(h, MGS_E_ARG, "dtype");
     if (where != MGS_HOST && where != MGS_DEVICE) return set_err(h, MGS_E_ARG, "where");
     if (!h->pm) return set_err(h, MGS_E_STATE, "no potential: call mgs_potential first");
-    double* phi = pm_potential(h);
+    float* phi = pm_potential(h);
     const int N = h->p.ngrid;
     const size_t n3 = (size_t)N * N * N;
     const size_t bytes = n3 * (dtype == MGS_F32 ? 4 : 8);
@@ -511,8 +523,8 @@ int mgs_get_potential(mgs_handle* h, void* dst, int32_t dtype, int32_t where)
         if (rc) return rc;
         out = h->staging;
     }
-    if (dtype == MGS_F32)
-        hipLaunchKernelGGL(k_to_f32, grid_for(n3), dim3(256), 0, h->s, phi, (float*)out, n3);
+    if (dtype == MGS_F64)
+        hipLaunchKernelGGL(k_to_f64, grid_for(n3), dim3(256), 0, h->s, phi, (double*)out, n3);
     else if (hipMemcpyAsync(out, phi, bytes, hipMemcpyDeviceToDevice, h->s) != hipSuccess)
         return set_err(h, MGS_E_HIP, "potential copy");
     if (where == MGS_HOST && hipMemcpyAsync(dst, out, bytes, hipMemcpyDeviceToHost, h->s) != hipSuccess)

@@ -149,8 +149,8 @@ void fft_release(mgs_handle* h);
 
 // mgs_pm.hip
 void pm_release(mgs_handle* h);
-// the handle's N^3 FP64 potential (x fastest), allocated on first use; null on failure
-double* pm_potential(mgs_handle* h);
+// the handle's N^3 REAL*4 potential (x fastest), allocated on first use; null on failure
+float* pm_potential(mgs_handle* h);
 }  // namespace mgs
 
 int set_err(mgs_handle* h, int code, const std::string& msg);

@@ -17,7 +17,8 @@
 // doubled central-difference gradients at the 8 nodes, interpolates them
 // with the CIC weights, kicks, drifts, wraps, and adds v^2 WPAR to a block
 // partial of SVEL.  The potential (mgs_potential, + mgs_potential_add_fifth_force)
-// is FP64 in x-fastest order.
+// is REAL*4 in x-fastest order, as the reference's FI; the gradients and
+// weights are formed in FP64 from it, as there.
 
 #include <hip/hip_runtime.h>
 
@@ -34,7 +35,7 @@ namespace mgs {
 struct PmState {
     int64_t np = 0;
     float* buf = nullptr;  // x, y, z, vx, vy, vz: 6 * np floats
-    double* phi = nullptr; // N^3 potential, x fastest
+    float* phi = nullptr;  // N^3 potential, x fastest, REAL*4 like the reference's FI
     double* part = nullptr;
     int* bad = nullptr;    // particles outside [1, N+1) seen by the last kernel
     int npart_blocks = 0;
@@ -63,7 +64,7 @@ void pm_release(mgs_handle* h)
     h->pm = nullptr;
 }
 
-double* pm_potential(mgs_handle* h)
+float* pm_potential(mgs_handle* h)
 {
     PmState* p = pm_of(h);
     if (!p) {
@@ -72,7 +73,7 @@ double* pm_potential(mgs_handle* h)
     }
     if (!p->phi) {
         const size_t n3 = (size_t)h->p.ngrid * h->p.ngrid * h->p.ngrid;
-        if (hipMalloc((void**)&p->phi, n3 * sizeof(double)) != hipSuccess) {
+        if (hipMalloc((void**)&p->phi, n3 * sizeof(float)) != hipSuccess) {
             p->phi = nullptr;
             return nullptr;
         }
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(PM_BLOCK) void k_cic_deposit(const float* __restric
 __global__ __launch_bounds__(PM_BLOCK) void k_move(float* __restrict__ xs, float* __restrict__ ys,
                                                    float* __restrict__ zs, float* __restrict__ vxs,
                                                    float* __restrict__ vys, float* __restrict__ vzs, int64_t np,
-                                                   const double* __restrict__ phi, int N, double pconst,
+                                                   const float* __restrict__ phi, int N, double pconst,
                                                    double xconst, double wpar, double* __restrict__ part,
                                                    int* __restrict__ bad)
 {
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(PM_BLOCK) void k_move(float* __restrict__ xs, float
         const int x1 = I - 1, x2 = I == N ? 0 : I, x3 = I + 1 >= N ? I + 1 - N : I + 1, x0 = I == 1 ? N - 1 : I - 2;
         const int y1 = J - 1, y2 = J == N ? 0 : J, y3 = J + 1 >= N ? J + 1 - N : J + 1, y0 = J == 1 ? N - 1 : J - 2;
         const int z1 = K - 1, z2 = K == N ? 0 : K, z3 = K + 1 >= N ? K + 1 - N : K + 1, z0 = K == 1 ? N - 1 : K - 2;
-#define PHI(a, b, c) phi[(size_t)(a) + n * ((size_t)(b) + n * (size_t)(c))]
+#define PHI(a, b, c) (double)phi[(size_t)(a) + n * ((size_t)(b) + n * (size_t)(c))]
         const double F111 = PHI(x1, y1, z1), F211 = PHI(x2, y1, z1), F121 = PHI(x1, y2, z1), F221 = PHI(x2, y2, z1);
         const double F112 = PHI(x1, y1, z2), F212 = PHI(x2, y1, z2), F122 = PHI(x1, y2, z2), F222 = PHI(x2, y2, z2);
         const double F113 = PHI(x1, y1, z3), F213 = PHI(x2, y1, z3), F123 = PHI(x1, y2, z3), F223 = PHI(x2, y2, z3);

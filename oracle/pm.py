@@ -68,11 +68,22 @@ def green_eigen(N):
 def poisson_fft(delta, trfi):
     """POTENTfft5, PMP2main.f90:705-867: phi with hat(phi) = trfi hat(delta)
     / A(k) (A = 1 for the mean mode); FFTPACK5's normalised forward and
-    unnormalised backward transforms compose to numpy's rfftn/irfftn."""
+    unnormalised backward transforms compose to numpy's rfftn/irfftn.  FP64
+    result (the REAL*4 FI it is stored in: potential_fi)."""
     d = np.asarray(delta, np.float64)
     N = d.shape[0]
     F = np.fft.rfftn(d)
     return np.fft.irfftn(F * trfi / green_eigen(N), s=d.shape, axes=(0, 1, 2))
+
+
+def potential_fi(delta, trfi, psi=None):
+    """The REAL*4 potential FI after POTENTfft5 and, with psi, after
+    POTENT_FOR_MG's FI = FI + sf (PMP2main.f90:884-935: the sum in FP64,
+    rounded once)."""
+    fi = poisson_fft(delta, trfi).astype(np.float32)
+    if psi is not None:
+        fi = (fi.astype(np.float64) + np.asarray(psi, np.float64)).astype(np.float32)
+    return fi
 
 
 def laplacian7(phi):

@@ -2,8 +2,9 @@
 mgs_potential, mgs_potential_add_fifth_force, mgs_move) against the numpy
 restatement in oracle/pm.py.
 
-Bars: the deposit and the potential to FP64 rounding (the deposit's atomic
-order and hipFFT vs numpy's FFT differ only there); the push bit-exact
+Bars: the deposit to FP64 rounding (atomic order), the REAL*4 potential to
+one REAL*4 ulp (its FP64 value differs from numpy's FFT only at rounding
+level, so the stored values are equal but for ties); the push bit-exact
 (FP64 arithmetic in the reference's operation order on REAL*4 particles, no
 contraction on either side); SVEL to FP64 rounding (block order)."""
 import numpy as np
@@ -46,7 +47,10 @@ def test_deposit_and_potential_match_oracle():
         S.potential(trfi)
         phi = S.get_potential()
         phi_o = P.poisson_fft(d_o, trfi)
-        assert np.allclose(phi, phi_o, rtol=0, atol=1e-11 * np.max(np.abs(phi_o))), np.max(np.abs(phi - phi_o))
+        # REAL*4 storage of an FP64 result that agrees to ~1e-15: equal but
+        # for ties at a rounding boundary (one REAL*4 ulp)
+        assert np.allclose(phi, phi_o, rtol=0, atol=1.2e-7 * np.max(np.abs(phi_o))), np.max(np.abs(phi - phi_o))
+        assert np.mean(phi.astype(np.float32) == P.potential_fi(d_o, trfi)) > 0.999
         phi32 = S.get_potential(np.float32)
         assert np.array_equal(phi32, phi.astype(np.float32))
 
@@ -77,8 +81,10 @@ def test_potential_plus_fifth_force_after_a_solve(model):
         phi = S.get_potential()
     p = O.Params(N=N, model=model, fr_n=1, branch=1, box=2.0 * N, om=0.3, oml=0.7, a=1.0, fr0=-1e-5, h0rc=1.0)
     un = u / O.fr_ubg(p) if model == M.MODEL_FR else u
-    expect = P.poisson_fft(d, 0.45) + O.fifth_force_potential(p, un)
-    assert np.allclose(phi, expect, rtol=0, atol=1e-11 * np.max(np.abs(expect)))
+    expect = P.potential_fi(d, 0.45, O.fifth_force_potential(p, un))
+    scale = np.max(np.abs(expect))
+    assert np.allclose(phi, expect, rtol=0, atol=2.4e-7 * scale)
+    assert np.mean(phi.astype(np.float32) == expect) > 0.999
 
 
 @pytest.mark.gpu
