@@ -209,6 +209,8 @@ int mgs_potential(mgs_handle* h, double trfi);
 int mgs_potential_add_fifth_force(mgs_handle* h);
 /* phi to dst (N^3 in the reference's FI order) as MGS_F32 or MGS_F64 */
 int mgs_get_potential(mgs_handle* h, void* dst, int32_t dtype, int32_t where);
+/* phi from src (stored as REAL*4): a potential the host holds in FI */
+int mgs_set_potential(mgs_handle* h, const void* src, int32_t dtype, int32_t where);
 /* MOVE, PMP2main.f90:304-685, f(R)/nDGP branch: each particle's velocity
  * gets pconst times the CIC interpolation of the doubled central-difference
  * gradients of phi at its 8 nodes, its position xconst times the new

@@ -507,6 +507,35 @@ int mgs_potential_add_fifth_force(mgs_handle* h)
     return 0;
 }
 
+int mgs_set_potential(mgs_handle* h, const void* src, int32_t dtype, int32_t where)
+{
+    if (!h || !src) return set_err(h, MGS_E_ARG, "NULL argument");
+    if (dtype != MGS_F32 && dtype != MGS_F64) return set_err(h, MGS_E_ARG, "dtype");
+    if (where != MGS_HOST && where != MGS_DEVICE) return set_err(h, MGS_E_ARG, "where");
+    float* phi = pm_potential(h);
+    if (!phi) return set_err(h, MGS_E_NOMEM, "potential buffer");
+    const int N = h->p.ngrid;
+    const size_t n3 = (size_t)N * N * N;
+    const size_t bytes = n3 * (dtype == MGS_F32 ? 4 : 8);
+    if (dtype == MGS_F32) {
+        if (hipMemcpyAsync(phi, src, bytes, where == MGS_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                           h->s) != hipSuccess)
+            return set_err(h, MGS_E_HIP, "potential upload");
+    } else {
+        const void* d = src;
+        if (where == MGS_HOST) {
+            int rc = ensure_staging(h, bytes);
+            if (rc) return rc;
+            if (hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->s) != hipSuccess)
+                return set_err(h, MGS_E_HIP, "potential upload");
+            d = h->staging;
+        }
+        hipLaunchKernelGGL(k_to_f32, grid_for(n3), dim3(256), 0, h->s, (const double*)d, phi, n3);
+    }
+    if (hipStreamSynchronize(h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "potential upload");
+    return 0;
+}
+
 int mgs_get_potential(mgs_handle* h, void* dst, int32_t dtype, int32_t where)
 {
     if (!h || !dst) return set_err(h, MGS_E_ARG, "NULL argument");

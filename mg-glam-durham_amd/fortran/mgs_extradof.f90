@@ -17,13 +17,21 @@
 !   calculate_residual_DGP(ilevel, res)           PMP2MGsolver_DGP.f90:240
 !   restrict_residual_DGP(ilevel)                 PMP2MGsolver_DGP.f90:418
 !   calculate_physical_right_hand_side_DGP(ilevel) PMP2MGsolver_DGP.f90:793
+!   DENSIT (CIC branch)                           PMP2mod_density.f90:17-281
+!   POTENTfft5                                    PMP2main.f90:705-867
+!   POTENT_FOR_MG                                 PMP2main.f90:869-1015
+!   MOVE(FactV) (f(R)/nDGP)                       PMP2main.f90:304-685
 !
 ! The reference keeps its state in module Tools (FI = delta, FI2 = u or phi,
 ! FI3 = the packed hierarchy).  Here the hierarchy lives on the GPU, owned
 ! by one libmgsolve handle; the host arrays FI and FI2 are borrowed through
 ! mgs_attach_fields and synchronised by scalar_field (density in, field in
 ! and out), or explicitly by mgs_fields_to_device / mgs_fields_to_host for
-! callers that drive the per-level routines themselves.  Errors print
+! callers that drive the per-level routines themselves.  The PM routines
+! take the particle arrays XPAR..VZ borrowed through mgs_attach_particles
+! and, like the reference, read and write FI and those arrays on the host at
+! each call (the particles, density and potential also stay on the GPU for
+! callers of the C-ABI).  Errors print
 ! libmgsolve's message and STOP, as the reference does for an unsupported
 ! model (PMP2extradof.f90:408-411); non-convergence is a printed warning.
 module mgs_extradof
@@ -40,6 +48,7 @@ module mgs_extradof
   public :: calculate_physical_right_hand_side_fR
   public :: relaxation_iterations_DGP, calculate_residual_DGP, restrict_residual_DGP
   public :: calculate_physical_right_hand_side_DGP
+  public :: mgs_attach_particles, DENSIT, POTENTfft5, POTENT_FOR_MG, MOVE
 
   ! run control of the reference's scalar_field (PMP2mod_tools.f90 scalars)
   integer, public, save :: iter_count_max = 100
@@ -47,11 +56,16 @@ module mgs_extradof
   ! cycle type of scalar_field, as the reference's switches (PMP2extradof.f90:86-88)
   integer, public, save :: Vcycle = 1, Fcycle = 0, Wcycle = 0
   logical, public, save :: print_residuals = .true.
+  ! PM step: the time step ASTEP that MOVE reads from module Tools, and the
+  ! kinetic energy MOVE leaves there (ENKIN, PMP2main.f90:681)
+  real(8), public, save :: ASTEP = 0.0d0, ENKIN = 0.0d0
 
   type(c_ptr), save :: h = c_null_ptr
   type(mgs_params), save :: prm
   logical, save :: configured = .false.
   real(4), pointer, save :: FI(:,:,:) => null(), FI2(:,:,:) => null()
+  real(4), pointer, save :: XPAR(:) => null(), YPAR(:) => null(), ZPAR(:) => null()
+  real(4), pointer, save :: VX(:) => null(), VY(:) => null(), VZ(:) => null()
 
 contains
 
@@ -290,5 +304,84 @@ contains
     call ensure_handle()
     call check(mgs_rhs(h, ilevel), 'calculate_physical_right_hand_side_DGP')
   end subroutine calculate_physical_right_hand_side_DGP
+
+  ! ---- PM step around the solve -------------------------------------------
+
+  ! the particle arrays of module Tools (PMP2mod_tools.f90:46), REAL*4,
+  ! positions in grid units in [1, NGRID+1)
+  subroutine mgs_attach_particles(x, y, z, vx_, vy_, vz_)
+    real(4), target, intent(inout) :: x(:), y(:), z(:), vx_(:), vy_(:), vz_(:)
+    XPAR => x
+    YPAR => y
+    ZPAR => z
+    VX => vx_
+    VY => vy_
+    VZ => vz_
+  end subroutine mgs_attach_particles
+
+  subroutine need_particles()
+    if (.not. associated(XPAR)) then
+      write(*, '(A)') 'mgs_extradof: mgs_attach_particles(XPAR, YPAR, ZPAR, VX, VY, VZ) must be called first'
+      stop 1
+    end if
+  end subroutine need_particles
+
+  subroutine particles_to_device()
+    call need_particles()
+    call check(mgs_particles_set(h, c_loc(XPAR), c_loc(YPAR), c_loc(ZPAR), c_loc(VX), c_loc(VY), c_loc(VZ), &
+                                 int(size(XPAR), c_int64_t), MGS_HOST), 'mgs_particles_set')
+  end subroutine particles_to_device
+
+  ! DENSIT's CIC branch, PMP2mod_density.f90:37-55, 235-281: FI = -1 plus
+  ! WPAR times each particle's CIC weights
+  subroutine DENSIT()
+    call ensure_handle()
+    call need_fields()
+    call particles_to_device()
+    call check(mgs_cic_deposit(h), 'DENSIT')
+    call check(mgs_get_density(h, c_loc(FI), MGS_F32, MGS_HOST), 'DENSIT')
+  end subroutine DENSIT
+
+  ! POTENTfft5, PMP2main.f90:705-867: FI (delta) -> FI (potential), with
+  ! trfi = 1.5 Om / AEXPN (the reference's branch without massive neutrinos)
+  subroutine POTENTfft5()
+    call ensure_handle()
+    call need_fields()
+    call check(mgs_set_density(h, c_loc(FI), MGS_F32, MGS_HOST), 'POTENTfft5')
+    call check(mgs_potential(h, 1.5d0 * prm%omega_m / prm%aexpn), 'POTENTfft5')
+    call check(mgs_get_potential(h, c_loc(FI), MGS_F32, MGS_HOST), 'POTENTfft5')
+  end subroutine POTENTfft5
+
+  ! POTENT_FOR_MG, PMP2main.f90:869-935 (f(R), nDGP): FI += the scalar
+  ! field's potential, from the field of the last solve on the GPU
+  subroutine POTENT_FOR_MG()
+    call ensure_handle()
+    call need_fields()
+    call check(mgs_set_potential(h, c_loc(FI), MGS_F32, MGS_HOST), 'POTENT_FOR_MG')
+    call check(mgs_potential_add_fifth_force(h), 'POTENT_FOR_MG')
+    call check(mgs_get_potential(h, c_loc(FI), MGS_F32, MGS_HOST), 'POTENT_FOR_MG')
+  end subroutine POTENT_FOR_MG
+
+  ! MOVE(FactV), PMP2main.f90:304-685 for f(R) and nDGP: PCONST and XCONST
+  ! from AEXPN, ASTEP, Om, OmL (:390-393), the push with the gradient of FI,
+  ! ENKIN = SVEL / 2 / (AEXPN + ASTEP/2)^2 (:681)
+  subroutine MOVE(FactV)
+    real(8), intent(in) :: FactV
+    real(8) :: pconst, xconst, ahalf, svel, a, om, oml
+    call ensure_handle()
+    call need_fields()
+    a = prm%aexpn
+    om = prm%omega_m
+    oml = prm%omega_l
+    pconst = -sqrt(a / (om + oml * a**3)) * ASTEP * 0.5d0 / FactV
+    ahalf = a + ASTEP / 2.0d0
+    xconst = ASTEP / sqrt(ahalf * (om + oml * ahalf**3)) / ahalf
+    call particles_to_device()
+    call check(mgs_set_potential(h, c_loc(FI), MGS_F32, MGS_HOST), 'MOVE')
+    call check(mgs_move(h, pconst, xconst, svel), 'MOVE')
+    call check(mgs_particles_get(h, c_loc(XPAR), c_loc(YPAR), c_loc(ZPAR), c_loc(VX), c_loc(VY), c_loc(VZ), &
+                                 MGS_HOST), 'MOVE')
+    ENKIN = svel / 2.0d0 / (a + ASTEP / 2.0d0)**2
+  end subroutine MOVE
 
 end module mgs_extradof

@@ -167,6 +167,47 @@ module mgsolve
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function
+    ! PM step around the solve (particles, CIC deposit, Poisson FFT, push)
+    integer(c_int) function mgs_particles_set(h, x, y, z, vx, vy, vz, npart, where) bind(C, name='mgs_particles_set')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: h, x, y, z, vx, vy, vz
+      integer(c_int64_t), value :: npart
+      integer(c_int32_t), value :: where
+    end function
+    integer(c_int) function mgs_particles_get(h, x, y, z, vx, vy, vz, where) bind(C, name='mgs_particles_get')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h, x, y, z, vx, vy, vz
+      integer(c_int32_t), value :: where
+    end function
+    integer(c_int) function mgs_cic_deposit(h) bind(C, name='mgs_cic_deposit')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function mgs_potential(h, trfi) bind(C, name='mgs_potential')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: trfi
+    end function
+    integer(c_int) function mgs_potential_add_fifth_force(h) bind(C, name='mgs_potential_add_fifth_force')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function mgs_get_potential(h, dst, dtype, where) bind(C, name='mgs_get_potential')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h, dst
+      integer(c_int32_t), value :: dtype, where
+    end function
+    integer(c_int) function mgs_set_potential(h, src, dtype, where) bind(C, name='mgs_set_potential')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h, src
+      integer(c_int32_t), value :: dtype, where
+    end function
+    integer(c_int) function mgs_move(h, pconst, xconst, svel) bind(C, name='mgs_move')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: pconst, xconst
+      real(c_double), intent(out) :: svel
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s

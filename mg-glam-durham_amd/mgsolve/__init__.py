@@ -144,6 +144,7 @@ def load():
         "mgs_potential": ([H, C.c_double], C.c_int),
         "mgs_potential_add_fifth_force": ([H], C.c_int),
         "mgs_get_potential": ([H, vp, i32, i32], C.c_int),
+        "mgs_set_potential": ([H, vp, i32, i32], C.c_int),
         "mgs_move": ([H, C.c_double, C.c_double, C.POINTER(C.c_double)], C.c_int),
         "mgs_profile": ([H, i32], C.c_int),
         "mgs_profile_get": ([H, i32, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
@@ -377,6 +378,11 @@ class Solver:
         out = np.empty((self.N,) * 3, dtype)
         self._chk(self.L.mgs_get_potential(self.h, _np_ptr(out), self._dt(out), HOST), "mgs_get_potential")
         return out
+
+    def set_potential(self, phi):
+        a = np.ascontiguousarray(phi)
+        assert a.size == self.N ** 3
+        self._chk(self.L.mgs_set_potential(self.h, _np_ptr(a), self._dt(a), HOST), "mgs_set_potential")
 
     def move(self, pconst, xconst):
         s = C.c_double()

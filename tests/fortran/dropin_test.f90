@@ -13,11 +13,13 @@
 program dropin_test
   use mgs_extradof
   implicit none
-  integer :: ngrid = 16, mg_model = 1, fr_n = 1, n_branch = 1, precision = 0, iter_max = 30
-  real(8) :: box = 64d0, om = 0.3d0, oml = 0.7d0, aexpn = 1d0, fr0 = -1d-5, h0rc = 1d0, conv = 1d-10
-  namelist /run/ ngrid, mg_model, fr_n, n_branch, precision, iter_max, box, om, oml, aexpn, fr0, h0rc, conv
+  integer :: ngrid = 16, mg_model = 1, fr_n = 1, n_branch = 1, precision = 0, iter_max = 30, npart = 0
+  real(8) :: box = 64d0, om = 0.3d0, oml = 0.7d0, aexpn = 1d0, fr0 = -1d-5, h0rc = 1d0, conv = 1d-10, astep_in = 0.02d0
+  namelist /run/ ngrid, mg_model, fr_n, n_branch, precision, iter_max, box, om, oml, aexpn, fr0, h0rc, conv, &
+                 npart, astep_in
   character(len=512) :: dir
   real(4), allocatable, target :: FI(:,:,:), FI2(:,:,:)
+  real(4), allocatable, target :: XP(:), YP(:), ZP(:), VXP(:), VYP(:), VZP(:)
   real(8) :: r0, r1, r2
   integer :: u
 
@@ -77,6 +79,31 @@ program dropin_test
   open(newunit=u, file=trim(dir)//'/ops.txt', status='replace')
   write(u, '(3ES25.17)') r0, r1, r2
   close(u)
+
+  ! the PM step's routines (DENSIT, POTENTfft5, POTENT_FOR_MG, MOVE) with the
+  ! field the cycles above left on the GPU; particles.bin holds XPAR, YPAR,
+  ! ZPAR, VX, VY, VZ (npart REAL*4 each)
+  if (npart > 0) then
+    allocate(XP(npart), YP(npart), ZP(npart), VXP(npart), VYP(npart), VZP(npart))
+    open(newunit=u, file=trim(dir)//'/particles.bin', access='stream', form='unformatted', status='old')
+    read(u) XP, YP, ZP, VXP, VYP, VZP
+    close(u)
+    call mgs_attach_particles(XP, YP, ZP, VXP, VYP, VZP)
+    ASTEP = astep_in
+    call DENSIT()
+    call write_grid('pm_density.bin', FI)
+    call POTENTfft5()
+    call write_grid('pm_phi.bin', FI)
+    call POTENT_FOR_MG()
+    call write_grid('pm_phimg.bin', FI)
+    call MOVE(1.0d0)
+    open(newunit=u, file=trim(dir)//'/pm_part.bin', access='stream', form='unformatted', status='replace')
+    write(u) XP, YP, ZP, VXP, VYP, VZP
+    close(u)
+    open(newunit=u, file=trim(dir)//'/pm.txt', status='replace')
+    write(u, '(ES25.17)') ENKIN
+    close(u)
+  end if
   call mgs_release()
 
 contains

@@ -23,7 +23,8 @@ DROPIN = os.path.join(FDIR, "_build", "libmgs_dropin.so")
 ROUTINES = ["scalar_field", "initialise_extradof", "v_cycle", "restrict_density", "restrict_scalar_field",
             "correct_solution", "relaxation_iterations_fr", "calculate_residual_fr", "restrict_residual_fr",
             "calculate_physical_right_hand_side_fr", "relaxation_iterations_dgp", "calculate_residual_dgp",
-            "restrict_residual_dgp", "calculate_physical_right_hand_side_dgp"]
+            "restrict_residual_dgp", "calculate_physical_right_hand_side_dgp", "densit", "potentfft5",
+            "potent_for_mg", "move"]
 
 
 def test_dropin_library_exports_reference_routines():
@@ -51,6 +52,7 @@ CASES = [(M.MODEL_FR, M.PREC_FP64), (M.MODEL_FR, M.PREC_MIXED), (M.MODEL_DGP, M.
 @pytest.mark.parametrize("model,precision", CASES)
 def test_dropin_matches_python_binding(tmp_path, model, precision):
     N, box, iters, conv = 32, 64.0, 25, 1e-9
+    npart, astep = 20000, 0.02
     print(f"[dropin] model={model} precision={precision} N={N}", flush=True)
     fr0, h0rc = -1e-5, 1.0
     def d(x):  # a Fortran double literal
@@ -59,7 +61,7 @@ def test_dropin_matches_python_binding(tmp_path, model, precision):
     (tmp_path / "params.nml").write_text(
         f"&run ngrid={N}, mg_model={model}, fr_n=1, n_branch=1, precision={precision}, iter_max={iters}, "
         f"box={d(box)}, om={d(0.3)}, oml={d(0.7)}, aexpn={d(1.0)}, fr0={d(fr0)}, h0rc={d(h0rc)}, "
-        f"conv={d(conv)} /\n")
+        f"conv={d(conv)}, npart={npart}, astep_in={d(astep)} /\n")
     d1 = O.lognormal_density(N, box, 1.0, seed=31).astype(np.float32)
     d2 = O.lognormal_density(N, box, 1.1, seed=32).astype(np.float32)
     cfg = M.Config(N=N, model=model, fr_n=1, box=box, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=fr0, h0rc=h0rc,
@@ -73,6 +75,9 @@ def test_dropin_matches_python_binding(tmp_path, model, precision):
             f0 = (1e-3 * rng.standard_normal((N,) * 3)).astype(np.float32)
         for name, a in (("delta1", d1), ("delta2", d2), ("field0", f0)):
             a.tofile(tmp_path / f"{name}.bin")
+        part = [rng.uniform(1.0, N + 1.0, npart).astype(np.float32) for _ in range(3)] + \
+               [rng.normal(0.0, 0.2, npart).astype(np.float32) for _ in range(3)]
+        np.concatenate(part).tofile(tmp_path / "particles.bin")
         p = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
         assert p.returncode == 0, p.stdout + p.stderr
         assert "The residual is equal to" in p.stdout
@@ -109,7 +114,29 @@ def test_dropin_matches_python_binding(tmp_path, model, precision):
         S.vcycle(0)
         r2 = S.residual(0)
         ops = S.get_field(np.float32)
+        # the PM routines: host FI and particle arrays in and out at each call
+        S.set_particles(*part)
+        S.cic_deposit()
+        dens = S.get_density(np.float32)
+        S.set_density(dens)
+        S.potential(1.5 * 0.3 / 1.0)
+        phi = S.get_potential(np.float32)
+        S.set_potential(phi)
+        S.potential_add_fifth_force()
+        phimg = S.get_potential(np.float32)
+        S.set_particles(*part)
+        S.set_potential(phimg)
+        pc = -np.sqrt(1.0 / (0.3 + 0.7 * 1.0)) * astep * 0.5 / 1.0
+        ah = 1.0 + astep / 2.0
+        xc = astep / np.sqrt(ah * (0.3 + 0.7 * (ah * ah * ah))) / ah  # ahalf**3 as two products
+        svel = S.move(pc, xc)
+        moved = np.concatenate(S.get_particles())
     got = np.fromfile(tmp_path / "ops.bin", np.float32).reshape(N, N, N)
     assert np.array_equal(got, ops)
     rf = [float(x) for x in (tmp_path / "ops.txt").read_text().split()]
     assert rf == [r0, r1, r2]
+    for name, a in (("pm_density", dens), ("pm_phi", phi), ("pm_phimg", phimg)):
+        assert np.array_equal(np.fromfile(tmp_path / f"{name}.bin", np.float32).reshape(N, N, N), a), name
+    assert np.array_equal(np.fromfile(tmp_path / "pm_part.bin", np.float32), moved)
+    enkin = float((tmp_path / "pm.txt").read_text())
+    assert enkin == pytest.approx(svel / 2.0 / (1.0 + astep / 2.0) ** 2, rel=1e-15)
