@@ -250,8 +250,9 @@ int mgs_device_field(mgs_handle* h, void** ptr, size_t* bytes);
 
 /* ---- serving: continuous-batching solve server ------------------------
  * No counterpart in the reference (a build-only component).  A server owns a
- * worker thread and one HIP stream on the current device.  Requests (f(R),
- * MGS_PREC_MIXED, ngrid >= 16) wait in a FIFO queue and are admitted into
+ * worker thread and one HIP stream on the current device.  Requests (f(R) or
+ * nDGP, MGS_PREC_MIXED, ngrid >= 16; nDGP: ngrid a power of two) wait in a
+ * FIFO queue and are admitted into
  * one of max_active slots when a slot is free and the request's hierarchy
  * fits in hbm_budget; admitted requests keep their hierarchies in HBM and
  * advance together: every batched step (outer residual, each level's colour
@@ -259,7 +260,8 @@ int mgs_device_field(mgs_handle* h, void** ptr, size_t* bytes);
  * the tiles of all active requests, replayed from a captured hipGraph.
  * Requests join at the next outer iteration after admission and leave when
  * converged, so each request's result is bit-identical alone, in any mix,
- * and to mgs_solve with the same parameters (mode 1, inner_cycles 1). */
+ * and to mgs_solve with the same parameters and inner_cycles 1 (mode 1 for
+ * exit_rule MGS_EXIT_REL_TOL, mode 0 for MGS_EXIT_REFERENCE). */
 typedef struct mgs_server mgs_server;
 
 typedef struct mgs_server_opts {
@@ -270,12 +272,19 @@ typedef struct mgs_server_opts {
     int32_t reserved;
 } mgs_server_opts;
 
+enum { MGS_EXIT_REL_TOL = 0, MGS_EXIT_REFERENCE = 1 };
+
 typedef struct mgs_request {
-    mgs_params params;       /* model MGS_MODEL_FR, precision MGS_PREC_MIXED, fr0 < 0, 16 <= ngrid <= 1024 */
+    mgs_params params;       /* precision MGS_PREC_MIXED, 16 <= ngrid <= 1024; f(R) (fr0 < 0) or nDGP
+                                (ngrid a power of two) */
     const void* delta;       /* host density contrast, ngrid^3 in the reference's order; copied by submit */
     int32_t dtype;           /* MGS_F32 / MGS_F64 */
     int32_t max_outer;       /* cap on outer iterations */
-    double rel_tol;          /* stop when RMS(r)/RMS(r0) < rel_tol */
+    double rel_tol;          /* MGS_EXIT_REL_TOL: stop when RMS(r)/RMS(r0) < rel_tol */
+    int32_t exit_rule;       /* MGS_EXIT_REL_TOL (mgs_solve's mode 1) or MGS_EXIT_REFERENCE (mode 0: res_conv
+                                and the stagnation exits of PMP2extradof.f90:107-122) */
+    int32_t reserved;
+    double res_conv;         /* MGS_EXIT_REFERENCE: converged when RMS(r) <= res_conv */
 } mgs_request;
 
 typedef struct mgs_request_result {

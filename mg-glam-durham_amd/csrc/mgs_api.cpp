@@ -599,7 +599,7 @@ static int outer_residual(mgs_handle* h, double* rms)
 }
 
 // the reference's exits, PMP2extradof.f90:107-122
-static bool reference_exit(const mgs_solve_opts& o, int iter, double res, double res1, double res2, bool* conv)
+bool reference_exit(const mgs_solve_opts& o, int iter, double res, double res1, double res2, bool* conv)
 {
     bool can_exit = false;
     if (res <= o.res_conv) {

@@ -785,8 +785,8 @@ __device__ __forceinline__ void restrict8_cell(const T* __restrict__ fine, T* __
     const unsigned o = sidx(gc, I, J, K);
     const T v = P / (T)8.0;
     lvl(out1, o) = v;
-    if (out2) lvl(out2, o) = v;}
-
+    if (out2) lvl(out2, o) = v;
+}
 
 // ---------------------------------------------------------------------------
 // Tail of the inner V-cycle: every level from l0 (at most 16^3 cells) down to

@@ -120,6 +120,9 @@ int levelmax_of(int N);
 int levelmin_of(int N);
 // the parameter checks of mgs_create (0 or an MGS_E_* code with a reason)
 int validate_params(const mgs_params* p, std::string* why);
+// mode-0 stopping rule after outer/cycle `iter` with residual res and the two
+// before it (res1, res2); sets *conv when res <= res_conv
+bool reference_exit(const mgs_solve_opts& o, int iter, double res, double res1, double res2, bool* conv);
 
 // mgs_dgp.hip
 DgpCoef dgp_coefs(const mgs_params& p, int level);

@@ -1,4 +1,4 @@
-// mgs_serve.hip -- continuous-batching f(R) solve server (mgs_server_* C-ABI).
+// mgs_serve.hip -- continuous-batching f(R) / nDGP solve server (mgs_server_* C-ABI).
 //
 // Requests wait in a FIFO queue; a worker thread admits them into slots as
 // slots and HBM budget free up, and advances every active request by one
@@ -7,13 +7,18 @@
 // all participating requests (a prefix sum of their tile counts in a device
 // plan), so one launch smooths, restricts or corrects the same level of every
 // request at once, whatever their grid sizes and parameters.  The sequence
-// depends only on the deepest hierarchy in the batch, so it is captured once
-// per depth as a hipGraph and replayed; requests joining or leaving only
+// depends only on the deepest f(R) and nDGP hierarchies in the batch, so it
+// is captured once per pair of depths as a hipGraph and replayed; requests joining or leaving only
 // change the plan in device memory.
 //
 // Per request the arithmetic, the tiling and the reduction order are those of
 // the single-request solve (mgs_api.cpp, solve_mixed with one inner cycle),
-// so a result is bit-identical alone, in any mix, and to mgs_solve.
+// so a result is bit-identical alone, in any mix, and to mgs_solve.  f(R)
+// and nDGP requests share a step: each model has its own participation lists
+// and kernels (the nDGP inner cycle is vcycle_inner_dgp's: out-of-place
+// colour passes through a per-request scratch array, 19-point operators on
+// FP64 densities cached on every level); the outer residual and its norm are
+// one launch over both.
 
 #include <hip/hip_runtime.h>
 
@@ -30,6 +35,7 @@
 #include <vector>
 
 #include "../../include/mgsolve.h"
+#include "mgs_dgp_dev.h"
 #include "mgs_fr_dev.h"
 #include "mgs_internal.h"
 #include "mgs_kernels.h"
@@ -56,14 +62,23 @@ struct LevelDesc {
     Shape z;   // stencil passes on this level
     Shape zb;  // both-colour passes on this level (residual-type)
     Shape zc;  // coarse-column transfers between this level and the next finer one (levels >= 1)
+    // nDGP
+    const double* rhol;  // FP64 density of this level
+    DgpF gf;
 };
 
 struct SlotDesc {
-    int frn, lc, pad0, pad1;
+    int frn, lc, model, pad1;
     double* u;
     const double* rho;
     double* partials;
     FrCoef fc;
+    // nDGP: FP64 operator coefficients of level 0, the tiles of its outer
+    // residual (as make_launch(h, 2 n^2)), and scratch of ngrid^3 floats (the
+    // colour passes' out-of-place half, then the residual before restriction)
+    DgpCoef gc0;
+    Shape zo;
+    float* scr;
     LevelDesc L[MAXL];
 };
 
@@ -82,6 +97,17 @@ struct Plan {
     unsigned zn_pre[MAXL][MAXS + 1];    // level-(l+1) stencil tiles
     int ncoarse[MAXL];
     int coarse_slot[MAXL][MAXS];
+    // nDGP requests, in blocks of 256 cells
+    int nd[MAXL];
+    int d_slot[MAXL][MAXS];
+    unsigned dh_pre[MAXL][MAXS + 1];  // level-l cells of one colour
+    unsigned df_pre[MAXL][MAXS + 1];  // level-l cells
+    unsigned dc_pre[MAXL][MAXS + 1];  // level-(l+1) cells
+    int ndcoarse[MAXL];
+    int dcoarse_slot[MAXL][MAXS];
+    int nupd;
+    int upd_slot[MAXS];
+    unsigned upd_pre[MAXS + 1];  // level-0 cells (phi += e)
 };
 
 // ---------------------------------------------------------------------------
@@ -123,8 +149,21 @@ __device__ __forceinline__ bool tile_col(const Geo& g, int width, const Shape& z
     } break;                       \
     }
 
-// FP64 outer residual (and the inner problem's g, xb, d after its first
-// colour-0 pass) on level 0; one block partial per tile, as k_both_colours
+// thread (x, row) of tile lb of a make_launch(width, rows) grid, as map_xy
+// (the tile is block lb of that grid, thread threadIdx.x = tx + TX ty)
+__device__ __forceinline__ bool tile_xy(int width, unsigned rows, const Shape& z, unsigned lb, int& x, unsigned& row)
+{
+    const int tx = (int)threadIdx.x % z.TX, ty = (int)threadIdx.x / z.TX;
+    if (ty >= z.TY) return false;
+    const unsigned xb = z.XB == 1 ? 0u : lb % (unsigned)z.XB, rb = z.XB == 1 ? lb : lb / (unsigned)z.XB;
+    x = (int)(xb * (unsigned)z.TX) + tx;
+    row = rb * (unsigned)z.TY + (unsigned)ty;
+    return x < width && row < rows;
+}
+
+// FP64 outer residual (and the inner problem's g, xb, d: f(R) after its
+// first colour-0 pass, nDGP d = 0) on level 0; one block partial per tile, as
+// k_both_colours / k_dgpm_outer
 __global__ __launch_bounds__(256) void kb_outer_residual(const SlotDesc* __restrict__ S, const Plan* __restrict__ P)
 {
     const unsigned total = P->res_pre[P->nres];
@@ -133,12 +172,19 @@ __global__ __launch_bounds__(256) void kb_outer_residual(const SlotDesc* __restr
         const SlotDesc& sd = S[P->res_slot[i]];
         const LevelDesc& L = sd.L[0];
         const unsigned lb = t - P->res_pre[i];
-        ZCol c;
         double acc = 0.0;
-        if (tile_col(L.g, L.g.h, L.zb, lb, c)) {
-            FRN_SWITCH(sd.frn, acc = both_colours_col(FrOuterOp<FRN>{sd.u, sd.rho, L.f, L.xb, L.d, sd.fc, L.dc,
-                                                                     (unsigned)L.g.half},
-                                                      L.g, c));
+        if (sd.model == MGS_MODEL_DGP) {
+            int m;
+            unsigned row;
+            if (tile_xy(L.g.h, 2u * (unsigned)L.g.n * (unsigned)L.g.n, sd.zo, lb, m, row))
+                acc = dgpm_outer_cell(sd.u, sd.rho, L.f, L.xb, L.d, L.g, sd.gc0, m, row);
+        } else {
+            ZCol c;
+            if (tile_col(L.g, L.g.h, L.zb, lb, c)) {
+                FRN_SWITCH(sd.frn, acc = both_colours_col(FrOuterOp<FRN>{sd.u, sd.rho, L.f, L.xb, L.d, sd.fc, L.dc,
+                                                                         (unsigned)L.g.half},
+                                                          L.g, c));
+            }
         }
         const double bs = block_sum(acc);
         if (threadIdx.x == 0) sd.partials[lb] = bs;
@@ -155,7 +201,7 @@ __global__ __launch_bounds__(1024) void kb_sums(const SlotDesc* __restrict__ S, 
     for (int i = blockIdx.x; i < P->nres; i += gridDim.x) {
         const int slot = P->res_slot[i];
         const SlotDesc& sd = S[slot];
-        const int count = (int)sd.L[0].zb.NB;
+        const int count = (int)(P->res_pre[i + 1] - P->res_pre[i]);
         double r;
         if (count <= 8192) {
             block_chunk_sum(sd.partials, 0, count, s);
@@ -287,6 +333,136 @@ __global__ __launch_bounds__(1024) void kb_coarsest(const SlotDesc* __restrict__
     }
 }
 
+// ---- nDGP (vcycle_inner_dgp's steps) ----------------------------------------
+// Each launch walks blocks of 256 cells of every participating request; the
+// cell bodies are those of the single-request kernels, so the order cells are
+// visited in does not change a value.
+
+// body(request, cell) for every cell of the blocks of 256 cells listed by
+// the prefix sums `pre` of the requests `slots`
+template <typename Body>
+__device__ __forceinline__ void dgp_cells(const SlotDesc* __restrict__ S, const unsigned* pre, int count,
+                                          const int* slots, Body body)
+{
+    const unsigned total = pre[count];
+    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = find_slot(pre, count, t);
+        body(S[slots[i]], (t - pre[i]) * 256u + threadIdx.x);
+    }
+}
+
+// (I, J, K) of cell `cell` of an nc^3 level
+struct Ijk {
+    int I, J, K;
+};
+__device__ __forceinline__ Ijk ijk_of(int nc, unsigned cell)
+{
+    const unsigned n = (unsigned)nc;
+    return Ijk{(int)(cell % n), (int)((cell / n) % n), (int)(cell / (n * n))};
+}
+
+// xb of level l+1 from level l
+__global__ __launch_bounds__(256) void kbd_restrict_base(const SlotDesc* __restrict__ S, const Plan* __restrict__ P,
+                                                         int l)
+{
+    dgp_cells(S, P->dc_pre[l], P->nd[l], P->d_slot[l], [&](const SlotDesc& sd, unsigned cell) {
+        const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
+        if (cell < (unsigned)(2 * C.g.half)) {
+            const Ijk c = ijk_of(C.g.n, cell);
+            restrict8_cell<float>(F.xb, C.xb, nullptr, F.g, C.g, c.I, c.J, c.K);
+        }
+    });
+}
+
+// one colour pass of the inner smoother on level l, out of place into scr
+__global__ __launch_bounds__(256) void kbd_smooth(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l,
+                                                  int colour)
+{
+    dgp_cells(S, P->dh_pre[l], P->nd[l], P->d_slot[l], [&](const SlotDesc& sd, unsigned cell) {
+        const LevelDesc& L = sd.L[l];
+        if (cell < (unsigned)L.g.half)
+            dgpm_smooth_cell(L.d, L.xb, L.rhol, L.f, sd.scr, L.g, colour, L.gf, (int)(cell % (unsigned)L.g.h),
+                             cell / (unsigned)L.g.h);
+    });
+}
+
+// the pass's colour half back from scr
+__global__ __launch_bounds__(256) void kbd_copy(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l,
+                                                int colour)
+{
+    dgp_cells(S, P->dh_pre[l], P->nd[l], P->d_slot[l], [&](const SlotDesc& sd, unsigned cell) {
+        const LevelDesc& L = sd.L[l];
+        if (cell < (unsigned)L.g.half) L.d[(size_t)colour * L.g.half + cell] = sd.scr[cell];
+    });
+}
+
+// inner residual D(e) - f of level l into scr
+__global__ __launch_bounds__(256) void kbd_residual(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l)
+{
+    dgp_cells(S, P->df_pre[l], P->nd[l], P->d_slot[l], [&](const SlotDesc& sd, unsigned cell) {
+        const LevelDesc& L = sd.L[l];
+        if (cell < (unsigned)(2 * L.g.half))
+            dgpm_residual_cell(L.d, L.xb, L.rhol, L.f, sd.scr, L.g, L.gf, (int)(cell % (unsigned)L.g.h),
+                               cell / (unsigned)L.g.h);
+    });
+}
+
+// R r and R e of level l: rres, and rd and the start value d of level l+1
+__global__ __launch_bounds__(256) void kbd_restrict(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l)
+{
+    dgp_cells(S, P->dc_pre[l], P->nd[l], P->d_slot[l], [&](const SlotDesc& sd, unsigned cell) {
+        const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
+        if (cell < (unsigned)(2 * C.g.half)) {
+            const Ijk c = ijk_of(C.g.n, cell);
+            restrict8_cell<float>(sd.scr, C.rres, nullptr, F.g, C.g, c.I, c.J, c.K);
+            restrict8_cell<float>(F.d, C.rd, C.d, F.g, C.g, c.I, c.J, c.K);
+        }
+    });
+}
+
+// FAS right-hand side of level l+1: f = D(R e) - R r
+__global__ __launch_bounds__(256) void kbd_rhs(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l)
+{
+    dgp_cells(S, P->dc_pre[l], P->nd[l], P->d_slot[l], [&](const SlotDesc& sd, unsigned cell) {
+        const LevelDesc& C = sd.L[l + 1];
+        if (cell < (unsigned)(2 * C.g.half))
+            dgpm_residual_cell(C.rd, C.xb, C.rhol, C.rres, C.f, C.g, C.gf, (int)(cell % (unsigned)C.g.h),
+                               cell / (unsigned)C.g.h);
+    });
+}
+
+// coarse-grid correction of level l from level l+1
+__global__ __launch_bounds__(256) void kbd_correct(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l)
+{
+    dgp_cells(S, P->dc_pre[l], P->nd[l], P->d_slot[l], [&](const SlotDesc& sd, unsigned cell) {
+        const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
+        if (cell < (unsigned)(2 * C.g.half)) {
+            const Ijk c = ijk_of(C.g.n, cell);
+            dgpm_correct_cell(C.d, C.rd, F.d, F.g, C.g, c.I, c.J, c.K);
+        }
+    });
+}
+
+// phi += e on level 0
+__global__ __launch_bounds__(256) void kbd_update(const SlotDesc* __restrict__ S, const Plan* __restrict__ P)
+{
+    dgp_cells(S, P->upd_pre, P->nupd, P->upd_slot, [&](const SlotDesc& sd, unsigned cell) {
+        const LevelDesc& L = sd.L[0];
+        if (cell < (unsigned)(2 * L.g.half)) sd.u[cell] = sd.u[cell] + (double)L.d[cell];
+    });
+}
+
+// coarsest-level sweeps of the nDGP requests whose coarsest level is l
+__global__ __launch_bounds__(1024) void kbd_coarsest(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l)
+{
+    for (int i = blockIdx.x; i < P->ndcoarse[l]; i += gridDim.x) {
+        const SlotDesc& sd = S[P->dcoarse_slot[l][i]];
+        const LevelDesc& L = sd.L[l];
+        dgpm_coarsest_block(L.d, L.xb, L.rhol, L.f, sd.scr, L.g, SWEEPS, L.gf);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -298,15 +474,21 @@ static double ms_between(Clock::time_point a, Clock::time_point b)
 }
 
 // one request's device hierarchy: level-0 u and delta in FP64, the FP32
-// inner hierarchy, and the level-0 block partials, carved out of one
+// inner hierarchy, and the level-0 block partials (nDGP: also the FP64
+// density of every coarser level and the scratch array), carved out of one
 // allocation
 struct Hier {
-    int N = 0, lc = 0;
+    int N = 0, lc = 0, model = MGS_MODEL_FR;
     size_t bytes = 0;
     void* base = nullptr;
     double *u = nullptr, *rho = nullptr, *partials = nullptr;
     float *d[MAXL] = {}, *xb[MAXL] = {}, *f[MAXL] = {}, *rres[MAXL] = {}, *rd[MAXL] = {};
+    double* rhol[MAXL] = {};  // nDGP: rhol[0] = rho
+    float* scr = nullptr;     // nDGP
 };
+
+// idle hierarchies are pooled by (ngrid, model)
+static int pool_key(int N, int model) { return N * 4 + model; }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -317,9 +499,10 @@ static int coarsest_of(int N)
 }
 
 // carve (or just measure, when base is null) a hierarchy for ngrid N
-static size_t layout(Hier& h, int N, char* base)
+static size_t layout(Hier& h, int N, int model, char* base)
 {
     h.N = N;
+    h.model = model;
     h.lc = coarsest_of(N);
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -330,7 +513,15 @@ static size_t layout(Hier& h, int N, char* base)
     const size_t n3 = (size_t)N * N * N;
     h.u = (double*)take(n3 * 8);
     h.rho = (double*)take(n3 * 8);
-    h.partials = (double*)take(((size_t)make_zlaunch_both(make_geo(N)).NB + 64) * 8);
+    const bool dgp = model == MGS_MODEL_DGP;
+    const Geo g0 = make_geo(N);
+    const size_t nb = dgp ? (size_t)make_launch(g0.h, 2L * N * N).NB : (size_t)make_zlaunch_both(g0).NB;
+    h.partials = (double*)take((nb + 64) * 8);
+    if (dgp) {
+        h.rhol[0] = h.rho;
+        for (int l = 1; l <= h.lc; l++) h.rhol[l] = (double*)take((size_t)(N >> l) * (N >> l) * (N >> l) * 8);
+        h.scr = (float*)take(n3 * 4);
+    }
     for (int l = 0; l <= h.lc; l++) {
         const size_t m = (size_t)(N >> l) * (N >> l) * (N >> l);
         h.d[l] = (float*)take(m * 4);
@@ -352,6 +543,8 @@ struct Request {
     int32_t dtype = MGS_F32;
     int32_t max_outer = 60;
     double rel_tol = 1e-8;
+    int32_t exit_rule = MGS_EXIT_REL_TOL;
+    double res_conv = 0.0;
     Clock::time_point t_submit, t_admit, t_done;
     bool done = false;
     mgs_request_result res{};
@@ -364,6 +557,7 @@ struct Slot {
     bool fresh = true;  // outer residual of the start field not yet taken
     int it = 0;         // outer residuals taken so far
     double res0 = 0.0;
+    double res1 = 1.0e5, res2 = 1.0e10;  // the two residuals before the last (mode-0 rule)
 };
 
 }  // namespace serve
@@ -384,12 +578,12 @@ struct mgs_server {
     std::map<int64_t, Request*> reqs;
     int64_t next_ticket = 1;
     Slot slots[MAXS];
-    std::map<int, std::vector<Hier*>> pool;  // idle hierarchies by ngrid
+    std::map<int, std::vector<Hier*>> pool;  // idle hierarchies by pool_key
     size_t used = 0, budget = 0;
     SlotDesc *dS = nullptr, *hS = nullptr;
     Plan *dP = nullptr, *hP = nullptr;
     double *dsum = nullptr, *hsum = nullptr;
-    std::map<int, hipGraphExec_t> graphs;  // by batch depth
+    std::map<int, hipGraphExec_t> graphs;  // by the batch's f(R) and nDGP depths
     bool desc_dirty = true;
     // statistics
     std::vector<double> lat;
@@ -429,42 +623,52 @@ void trim_pool(mgs_server* S, size_t need)
     }
 }
 
-Hier* take_hier(mgs_server* S, int N)
+Hier* take_hier(mgs_server* S, int N, int model)
 {
-    auto it = S->pool.find(N);
+    auto it = S->pool.find(pool_key(N, model));
     if (it != S->pool.end() && !it->second.empty()) {
         Hier* h = it->second.back();
         it->second.pop_back();
         return h;
     }
     Hier probe;
-    const size_t bytes = layout(probe, N, nullptr);
+    const size_t bytes = layout(probe, N, model, nullptr);
     if (S->used + bytes > S->budget) trim_pool(S, bytes);
     if (S->used + bytes > S->budget) return nullptr;
     void* base = nullptr;
     if (hipMalloc(&base, bytes) != hipSuccess) return nullptr;
     Hier* h = new Hier();
-    layout(*h, N, (char*)base);
+    layout(*h, N, model, (char*)base);
     h->base = base;
     S->used += bytes;
     return h;
 }
 
-size_t footprint(int N)
+size_t footprint(int N, int model)
 {
     Hier probe;
-    return layout(probe, N, nullptr);
+    return layout(probe, N, model, nullptr);
 }
 
 void fill_desc(SlotDesc& sd, const Hier& h, const mgs_params& p)
 {
     std::memset(&sd, 0, sizeof(sd));
+    const bool dgp = p.model == MGS_MODEL_DGP;
     sd.frn = p.fr_n;
     sd.lc = h.lc;
+    sd.model = dgp ? MGS_MODEL_DGP : MGS_MODEL_FR;
     sd.u = h.u;
     sd.rho = h.rho;
     sd.partials = h.partials;
-    sd.fc = fr_coefs(p, 0);
+    if (dgp) {
+        const Geo g0 = make_geo(h.N);
+        const Launch lo = make_launch(g0.h, 2L * h.N * h.N);
+        sd.gc0 = dgp_coefs(p, 0);
+        sd.zo = Shape{lo.TX, lo.TY, lo.XB, 0, 0, lo.NB};
+        sd.scr = h.scr;
+    } else {
+        sd.fc = fr_coefs(p, 0);
+    }
     for (int l = 0; l <= h.lc; l++) {
         LevelDesc& L = sd.L[l];
         L.g = make_geo(h.N >> l);
@@ -473,6 +677,11 @@ void fill_desc(SlotDesc& sd, const Hier& h, const mgs_params& p)
         L.f = h.f[l];
         L.rres = h.rres[l];
         L.rd = h.rd[l];
+        if (dgp) {
+            L.rhol = h.rhol[l];
+            L.gf = dgpf(dgp_coefs(p, l));
+            continue;
+        }
         L.dc = df_coefs(p, l);
         L.z = shape_of(make_zlaunch(L.g));
         L.zb = shape_of(make_zlaunch_both(L.g));
@@ -480,7 +689,8 @@ void fill_desc(SlotDesc& sd, const Hier& h, const mgs_params& p)
     }
 }
 
-// upload the density of a freshly admitted request and start from u = u_bg
+// upload the density of a freshly admitted request (nDGP: and restrict it to
+// every level, as cache_densities) and start from u = u_bg (nDGP: phi = 0)
 int start_request(mgs_server* S, Slot& sl)
 {
     Request* r = sl.r;
@@ -496,8 +706,12 @@ int start_request(mgs_server* S, Slot& sl)
             return MGS_E_HIP;
         launch_pack_f64_from_f64(h->u, h->rho, N, S->s);
     }
-    double ubg = 1.0;
-    fr_background(r->p, &ubg);
+    double ubg = 0.0;
+    if (h->model == MGS_MODEL_DGP) {
+        for (int l = 1; l <= h->lc; l++) launch_restrict8_64(h->rhol[l - 1], h->rhol[l], nullptr, N >> l, S->s);
+    } else {
+        fr_background(r->p, &ubg);
+    }
     launch_fill_f64(h->u, n3, ubg, S->s);
     if (hipStreamSynchronize(S->s) != hipSuccess) return MGS_E_HIP;
     std::vector<char>().swap(r->delta);
@@ -526,7 +740,7 @@ int finish_request(mgs_server* S, Slot& sl, int status)
         r->res.queued_ms = ms_between(r->t_submit, r->t_admit);
         r->res.latency_ms = ms_between(r->t_submit, r->t_done);
         r->done = true;
-        S->pool[h->N].push_back(h);
+        S->pool[pool_key(h->N, h->model)].push_back(h);
         if (!rc) {
             S->lat.push_back(r->res.latency_ms);
             S->last_done = r->t_done;
@@ -558,7 +772,7 @@ int admit(mgs_server* S)
                 free_slot = i;
         }
         if (free_slot < 0) break;
-        Hier* h = take_hier(S, r->p.ngrid);
+        Hier* h = take_hier(S, r->p.ngrid, r->p.model);
         if (!h) {
             if (active == 0) {
                 // it cannot fit even alone: fail it rather than wait forever
@@ -583,6 +797,8 @@ int admit(mgs_server* S)
         sl.fresh = true;
         sl.it = 0;
         sl.res0 = 0.0;
+        sl.res1 = 1.0e5;
+        sl.res2 = 1.0e10;
         const int rc = start_request(S, sl);
         if (rc) {
             finish_request(S, sl, rc);
@@ -594,22 +810,41 @@ int admit(mgs_server* S)
     return admitted;
 }
 
-void build_plan(mgs_server* S, int* depth, bool* any_inner)
+static unsigned blocks256(size_t cells) { return (unsigned)((cells + 255) / 256); }
+
+// participation lists of the next step; *dfr / *ddgp: the deepest f(R) /
+// nDGP hierarchy taking an inner cycle, -1 when there is none
+void build_plan(mgs_server* S, int* dfr, int* ddgp)
 {
     Plan& P = *S->hP;
     std::memset(&P, 0, sizeof(P));
-    *depth = 0;
-    *any_inner = false;
+    *dfr = *ddgp = -1;
     for (int i = 0; i < S->o.max_active; i++) {
         Slot& sl = S->slots[i];
         if (!sl.r) continue;
         const SlotDesc& sd = S->hS[i];
+        const bool dgp = sd.model == MGS_MODEL_DGP;
         P.res_slot[P.nres] = i;
-        P.res_pre[P.nres + 1] = P.res_pre[P.nres] + sd.L[0].zb.NB;
+        P.res_pre[P.nres + 1] = P.res_pre[P.nres] + (dgp ? sd.zo.NB : sd.L[0].zb.NB);
         P.nres++;
         if (sl.fresh) continue;
-        *any_inner = true;
-        if (sd.lc > *depth) *depth = sd.lc;
+        if (dgp) {
+            *ddgp = std::max(*ddgp, sd.lc);
+            for (int l = 0; l < sd.lc; l++) {
+                const int k = P.nd[l];
+                P.d_slot[l][k] = i;
+                P.dh_pre[l][k + 1] = P.dh_pre[l][k] + blocks256(sd.L[l].g.half);
+                P.df_pre[l][k + 1] = P.df_pre[l][k] + blocks256(2 * sd.L[l].g.half);
+                P.dc_pre[l][k + 1] = P.dc_pre[l][k] + blocks256(2 * sd.L[l + 1].g.half);
+                P.nd[l]++;
+            }
+            P.dcoarse_slot[sd.lc][P.ndcoarse[sd.lc]++] = i;
+            P.upd_slot[P.nupd] = i;
+            P.upd_pre[P.nupd + 1] = P.upd_pre[P.nupd] + blocks256(2 * sd.L[0].g.half);
+            P.nupd++;
+            continue;
+        }
+        *dfr = std::max(*dfr, sd.lc);
         for (int l = 0; l < sd.lc; l++) {
             const int k = P.ninner[l];
             const LevelDesc& C = sd.L[l + 1];
@@ -625,7 +860,7 @@ void build_plan(mgs_server* S, int* depth, bool* any_inner)
     }
 }
 
-// the inner V-cycle of every non-fresh request, recursing to `depth`
+// the inner V-cycle of every non-fresh f(R) request, recursing to `depth`
 void vcycle_batched(mgs_server* S, int l, int depth)
 {
     hipStream_t s = S->s;
@@ -645,37 +880,67 @@ void vcycle_batched(mgs_server* S, int l, int depth)
     if (l == 0) hipLaunchKernelGGL(kb_smooth_update, G, B, 0, s, S->dS, S->dP);
 }
 
-// one step: inner cycle + update of the non-fresh requests, then the outer
-// residual and its norm for all; depth < 0: no inner cycle
-void enqueue_step(mgs_server* S, int depth)
+// the nDGP inner V-cycle of every non-fresh nDGP request, recursing to
+// `depth` (vcycle_inner_dgp: 2 x 2 colour passes before and after)
+void vcycle_dgp(mgs_server* S, int l, int depth)
 {
     hipStream_t s = S->s;
-    if (depth >= 0) {
-        for (int l = 0; l < depth; l++)
+    const dim3 G(GRID), B(256);
+    hipLaunchKernelGGL(kbd_coarsest, dim3(MAXS), dim3(1024), 0, s, S->dS, S->dP, l);
+    if (l >= depth) return;
+    for (int it = 0; it < 4; it++) {
+        hipLaunchKernelGGL(kbd_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
+        hipLaunchKernelGGL(kbd_copy, G, B, 0, s, S->dS, S->dP, l, it & 1);
+    }
+    hipLaunchKernelGGL(kbd_residual, G, B, 0, s, S->dS, S->dP, l);
+    hipLaunchKernelGGL(kbd_restrict, G, B, 0, s, S->dS, S->dP, l);
+    hipLaunchKernelGGL(kbd_rhs, G, B, 0, s, S->dS, S->dP, l);
+    vcycle_dgp(S, l + 1, depth);
+    hipLaunchKernelGGL(kbd_correct, G, B, 0, s, S->dS, S->dP, l);
+    for (int it = 0; it < 4; it++) {
+        hipLaunchKernelGGL(kbd_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
+        hipLaunchKernelGGL(kbd_copy, G, B, 0, s, S->dS, S->dP, l, it & 1);
+    }
+}
+
+// one step: inner cycle + update of the non-fresh requests, then the outer
+// residual and its norm for all; dfr / ddgp < 0: no f(R) / nDGP inner cycle
+void enqueue_step(mgs_server* S, int dfr, int ddgp)
+{
+    hipStream_t s = S->s;
+    if (dfr >= 0) {
+        for (int l = 0; l < dfr; l++)
             hipLaunchKernelGGL(kb_restrict_base, dim3(GRID), dim3(256), 0, s, S->dS, S->dP, l);
-        vcycle_batched(S, 0, depth);
+        vcycle_batched(S, 0, dfr);
+    }
+    if (ddgp >= 0) {
+        for (int l = 0; l < ddgp; l++)
+            hipLaunchKernelGGL(kbd_restrict_base, dim3(GRID), dim3(256), 0, s, S->dS, S->dP, l);
+        vcycle_dgp(S, 0, ddgp);
+        hipLaunchKernelGGL(kbd_update, dim3(GRID), dim3(256), 0, s, S->dS, S->dP);
     }
     hipLaunchKernelGGL(kb_outer_residual, dim3(GRID), dim3(256), 0, s, S->dS, S->dP);
     hipLaunchKernelGGL(kb_sums, dim3(MAXS), dim3(1024), 0, s, S->dS, S->dP, S->dsum);
 }
 
-int run_step(mgs_server* S, int depth)
+int run_step(mgs_server* S, int dfr, int ddgp)
 {
     if (!S->o.use_graphs) {
-        enqueue_step(S, depth);
+        enqueue_step(S, dfr, ddgp);
         return hipGetLastError() == hipSuccess ? 0 : MGS_E_HIP;
     }
-    auto it = S->graphs.find(depth);
+    const int key = (dfr + 1) * 64 + (ddgp + 1);
+    auto it = S->graphs.find(key);
     if (it == S->graphs.end()) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ex = nullptr;
         if (hipStreamBeginCapture(S->s, hipStreamCaptureModeThreadLocal) != hipSuccess) return MGS_E_HIP;
-        enqueue_step(S, depth);
+        enqueue_step(S, dfr, ddgp);
         if (hipStreamEndCapture(S->s, &g) != hipSuccess) return MGS_E_HIP;
         const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return MGS_E_HIP;
-        it = S->graphs.emplace(depth, ex).first;
+        it = S->graphs.emplace(key, ex).first;
     }
     return hipGraphLaunch(it->second, S->s) == hipSuccess ? 0 : MGS_E_HIP;
 }
@@ -709,21 +974,20 @@ void worker_main(mgs_server* S)
         // descriptors and plan for this batch
         for (int i = 0; i < S->o.max_active; i++)
             if (S->slots[i].r && S->desc_dirty) fill_desc(S->hS[i], *S->slots[i].h, S->slots[i].r->p);
-        int depth = 0;
-        bool any_inner = false;
-        build_plan(S, &depth, &any_inner);
+        int dfr = -1, ddgp = -1;
+        build_plan(S, &dfr, &ddgp);
         int rc = 0;
         if (hipMemcpyAsync(S->dS, S->hS, sizeof(SlotDesc) * MAXS, hipMemcpyHostToDevice, S->s) != hipSuccess ||
             hipMemcpyAsync(S->dP, S->hP, sizeof(Plan), hipMemcpyHostToDevice, S->s) != hipSuccess)
             rc = MGS_E_HIP;
         S->desc_dirty = false;
-        if (!rc) rc = run_step(S, any_inner ? depth : -1);
+        if (!rc) rc = run_step(S, dfr, ddgp);
         if (!rc && (hipMemcpyAsync(S->hsum, S->dsum, sizeof(double) * MAXS, hipMemcpyDeviceToHost, S->s) != hipSuccess ||
                     hipStreamSynchronize(S->s) != hipSuccess))
             rc = MGS_E_HIP;
         S->steps++;
         S->max_batch = std::max(S->max_batch, nact);
-        // per request: the stopping rule of solve_mixed (mode 1)
+        // per request: the stopping rule of solve_mixed (mode 1 or 0)
         for (int i = 0; i < S->o.max_active; i++) {
             Slot& sl = S->slots[i];
             if (!sl.r) continue;
@@ -743,7 +1007,14 @@ void worker_main(mgs_server* S)
             if (sl.it == 1) {
                 sl.res0 = res;
                 r->res.res0 = res;
-                if (res == 0.0) stop = conv = true;
+                if (res == 0.0 && r->exit_rule == MGS_EXIT_REL_TOL) stop = conv = true;
+            } else if (r->exit_rule == MGS_EXIT_REFERENCE) {
+                mgs_solve_opts o{};
+                o.max_cycles = r->max_outer;
+                o.res_conv = r->res_conv;
+                stop = reference_exit(o, sl.it - 1, res, sl.res1, sl.res2, &conv);
+                sl.res2 = sl.res1;
+                sl.res1 = res;
             } else {
                 conv = res < r->rel_tol * sl.res0;
                 stop = conv || (sl.it - 1) >= r->max_outer;
@@ -845,13 +1116,24 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
     int rc = validate_params(&req->params, &why);
     if (rc) return srv_err(S, rc, why);
     const mgs_params& p = req->params;
-    if (p.model != MGS_MODEL_FR || p.precision != MGS_PREC_MIXED)
-        return srv_err(S, MGS_E_ARG, "the server solves f(R) requests in MGS_PREC_MIXED");
+    if ((p.model != MGS_MODEL_FR && p.model != MGS_MODEL_DGP) || p.precision != MGS_PREC_MIXED)
+        return srv_err(S, MGS_E_ARG, "the server solves f(R) and nDGP requests in MGS_PREC_MIXED");
     if (p.ngrid < 16 || p.ngrid > 1024) return srv_err(S, MGS_E_ARG, "the server takes 16 <= ngrid <= 1024");
+    if (p.model == MGS_MODEL_DGP && (p.ngrid & (p.ngrid - 1)))
+        return srv_err(S, MGS_E_ARG, "the server takes nDGP requests with a power-of-two ngrid");
     if (coarsest_of(p.ngrid) + 1 > MAXL) return srv_err(S, MGS_E_ARG, "too many levels");
     if (req->dtype != MGS_F32 && req->dtype != MGS_F64) return srv_err(S, MGS_E_ARG, "dtype");
-    if (!(req->rel_tol > 0.0) || req->max_outer < 1) return srv_err(S, MGS_E_ARG, "rel_tol > 0 and max_outer >= 1");
-    if (footprint(p.ngrid) > S->budget) return srv_err(S, MGS_E_NOMEM, "request larger than the server's HBM budget");
+    if (req->max_outer < 1) return srv_err(S, MGS_E_ARG, "max_outer >= 1");
+    if (req->exit_rule == MGS_EXIT_REL_TOL) {
+        if (!(req->rel_tol > 0.0)) return srv_err(S, MGS_E_ARG, "rel_tol > 0");
+    } else if (req->exit_rule == MGS_EXIT_REFERENCE) {
+        if (!(req->res_conv >= 0.0) || !std::isfinite(req->res_conv))
+            return srv_err(S, MGS_E_ARG, "res_conv must be finite and >= 0");
+    } else {
+        return srv_err(S, MGS_E_ARG, "exit_rule must be MGS_EXIT_REL_TOL or MGS_EXIT_REFERENCE");
+    }
+    if (footprint(p.ngrid, p.model) > S->budget)
+        return srv_err(S, MGS_E_NOMEM, "request larger than the server's HBM budget");
     const size_t n3 = (size_t)p.ngrid * p.ngrid * p.ngrid;
     const size_t es = req->dtype == MGS_F32 ? 4 : 8;
     for (size_t q = 0; q < n3; q++) {
@@ -864,6 +1146,8 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
     r->dtype = req->dtype;
     r->max_outer = req->max_outer;
     r->rel_tol = req->rel_tol;
+    r->exit_rule = req->exit_rule;
+    r->res_conv = req->res_conv;
     r->delta.assign((const char*)req->delta, (const char*)req->delta + n3 * es);
     {
         std::lock_guard<std::mutex> lk(S->mu);

@@ -36,6 +36,7 @@ FLAG_REFERENCE_ROOTS = 1
 PREC_FP64, PREC_MIXED = 0, 1
 F32, F64 = 0, 1
 HOST, DEVICE = 0, 1
+EXIT_REL_TOL, EXIT_REFERENCE = 0, 1
 ARR_U, ARR_RHO, ARR_F, ARR_RES, ARR_RRES, ARR_RU = range(6)
 KC_SMOOTH, KC_RESIDUAL, KC_RESTRICT, KC_CORRECT, KC_COARSE, KC_UPDATE = range(6)
 KC_NAMES = ["smooth", "residual", "restrict", "correct", "coarse", "update", "pm"]
@@ -77,7 +78,7 @@ class ServerOpts(C.Structure):
 class Request(C.Structure):
     _fields_ = [
         ("params", Params), ("delta", C.c_void_p), ("dtype", C.c_int32), ("max_outer", C.c_int32),
-        ("rel_tol", C.c_double),
+        ("rel_tol", C.c_double), ("exit_rule", C.c_int32), ("reserved", C.c_int32), ("res_conv", C.c_double),
     ]
 
 
@@ -442,13 +443,17 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, cfg: Config, delta, rel_tol=1e-8, max_outer=60):
-        """Queue a solve of cfg (f(R), PREC_MIXED) on density delta; returns a ticket."""
+    def submit(self, cfg: Config, delta, rel_tol=1e-8, max_outer=60, exit_rule=None, res_conv=0.0):
+        """Queue a solve of cfg (f(R) or nDGP, PREC_MIXED) on density delta;
+        returns a ticket.  exit_rule EXIT_REL_TOL (the default) stops as
+        solve(mode=1, rel_tol=...), EXIT_REFERENCE as solve(mode=0,
+        res_conv=...)."""
         a = np.ascontiguousarray(delta)
         dt = F32 if a.dtype == np.float32 else F64
         if dt == F64:
             a = np.ascontiguousarray(a, dtype=np.float64)
-        r = Request(cfg.c(), _np_ptr(a), dt, int(max_outer), float(rel_tol))
+        rule = EXIT_REL_TOL if exit_rule is None else int(exit_rule)
+        r = Request(cfg.c(), _np_ptr(a), dt, int(max_outer), float(rel_tol), rule, 0, float(res_conv))
         t = C.c_int64()
         rc = self.L.mgs_server_submit(self.h, C.byref(r), C.byref(t))
         if rc:

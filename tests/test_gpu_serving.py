@@ -1,9 +1,10 @@
 """The continuous-batching solve server (mgs_server_* in include/mgsolve.h).
 
 Each request's result must be the same alone, in a mix with other requests
-of other sizes and parameters, and from the single-request solve
-(mgs_solve, mixed precision, mode 1): the server runs the same per-request
-arithmetic, tiling and reduction order, so the bar is bitwise equality.
+of other sizes, parameters and models (f(R), nDGP), and from the
+single-request solve (mgs_solve, mixed precision, mode 1 or 0): the server
+runs the same per-request arithmetic, tiling and reduction order, so the bar
+is bitwise equality.
 Admission by HBM footprint is exercised with a budget that holds only one
 request at a time.  Error paths are checked without a GPU."""
 import numpy as np
@@ -92,6 +93,64 @@ def test_admission_by_footprint_and_graph_replay(use_graphs):
     assert out[2][1]["queued_ms"] >= out[0][1]["queued_ms"]
 
 
+def dgp_cfg(N, branch=1, box=64.0, a=1.0, h0rc=1.0):
+    return M.Config(N=N, model=M.MODEL_DGP, branch=branch, box=box, omega_m=0.3, omega_l=0.7, aexpn=a, h0rc=h0rc,
+                    precision=M.PREC_MIXED)
+
+
+# nDGP requests (power-of-two grids, both branches) mixed with f(R) ones
+DGP_MIX = [
+    (32, 1, 64.0, 1.0, 1.0, 1.0, 21),
+    (16, -1, 32.0, 1.0, 1.0, 0.8, 22),
+    (64, 1, 128.0, 0.8, 2.0, 1.1, 23),
+    (32, -1, 64.0, 1.0, 0.5, 1.2, 24),
+]
+DGP_EXIT = dict(exit_rule=M.EXIT_REFERENCE, res_conv=1e-30, max_outer=40)
+
+
+@pytest.mark.gpu
+def test_dgp_results_equal_alone_in_mix_and_standalone():
+    """nDGP requests stop by the reference's rule (the DGP residual of a
+    periodic box levels off, so the stagnation exits end the solve); in a
+    batch with f(R) requests each result is bitwise that of the request
+    alone and of mgs_solve (mixed, mode 0, one inner cycle)."""
+    tol = 1e-8
+    reqs = []
+    for k, (N, branch, box, a, h0rc, sigma, seed) in enumerate(DGP_MIX):
+        reqs.append(("dgp", dgp_cfg(N, branch, box, a, h0rc), density(N, box, sigma, seed, f32=(k % 2 == 1))))
+    for k, (N, fr0, frn, box, a, sigma, seed) in enumerate(MIX[:2]):
+        reqs.append(("fr", cfg(N, fr0, frn, box, a), density(N, box, sigma, seed, f32=True)))
+
+    def submit(srv, kind, c, d):
+        return srv.submit(c, d, **DGP_EXIT) if kind == "dgp" else srv.submit(c, d, rel_tol=tol)
+
+    print(f"[serving] {len(reqs)} nDGP + f(R) requests, mixed", flush=True)
+    with M.Server(max_active=6) as srv:
+        tickets = [submit(srv, k, c, d) for k, c, d in reqs]
+        mixed = [srv.wait(t, c.N) for t, (_, c, _) in zip(tickets, reqs)]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["max_batch"] >= 4, st
+    for (kind, c, d), (u_mix, r_mix) in zip(reqs, mixed):
+        print(f"[serving] {kind} N={c.N} outer={r_mix['outer']} res {r_mix['res0']:.3e} -> "
+              f"{r_mix['res_final']:.3e}", flush=True)
+        assert r_mix["status"] == 0, r_mix
+        assert r_mix["res_final"] < 0.05 * r_mix["res0"], r_mix
+        with M.Server(max_active=1) as srv:
+            u_alone, r_alone = srv.wait(submit(srv, kind, c, d), c.N)
+        assert np.array_equal(u_alone, u_mix)
+        assert r_alone["outer"] == r_mix["outer"] and r_alone["res_final"] == r_mix["res_final"]
+        with M.Solver(c) as S:
+            S.set_density(d)
+            if kind == "dgp":
+                rep = S.solve(mode=0, res_conv=1e-30, max_cycles=40, inner_cycles=1)
+            else:
+                rep = S.solve(mode=1, rel_tol=tol, max_cycles=60, inner_cycles=1)
+            u_ref = S.get_field()
+        assert np.array_equal(u_ref, u_mix)
+        assert rep["outer"] == r_mix["outer"] and rep["res_final"] == r_mix["res_final"]
+        assert rep["converged"] == r_mix["converged"]
+
+
 @pytest.mark.gpu
 def test_server_rejects_bad_requests():
     with M.Server(max_active=2) as srv:
@@ -100,6 +159,12 @@ def test_server_rejects_bad_requests():
             srv.submit(M.Config(N=16, precision=M.PREC_FP64), d)
         with pytest.raises(M.MgsError, match="ngrid"):
             srv.submit(cfg(8), np.zeros((8,) * 3, np.float32))
+        with pytest.raises(M.MgsError, match="power-of-two"):
+            srv.submit(dgp_cfg(24), np.zeros((24,) * 3, np.float32), **DGP_EXIT)
+        with pytest.raises(M.MgsError, match="exit_rule"):
+            srv.submit(dgp_cfg(16), d, exit_rule=7)
+        with pytest.raises(M.MgsError, match="res_conv"):
+            srv.submit(dgp_cfg(16), d, exit_rule=M.EXIT_REFERENCE, res_conv=-1.0)
         bad = d.copy()
         bad[1, 2, 3] = np.inf
         with pytest.raises(M.MgsError, match="finite"):
