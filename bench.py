@@ -215,48 +215,46 @@ def pm_step(N, seed, steps):
     return out
 
 
-def serving_stream(n_req, seed, max_active):
-    """The serving stream: n_req f(R) n=1 requests, ngrid uniform over
-    {64, 128, 192, 256} (cell 2 Mpc/h), |fR0| from {1e-4, 1e-5, 1e-6},
-    seeded lognormal densities with sigma_g ~ U[0.8, 1.6] (generated on the
-    GPU, kept on the host as float32), each solved to relative residual 1e-8.
-    First all requests at once (saturated throughput), then Poisson arrivals
-    at 0.8x that rate (latency).  Three requests are re-solved alone and
-    compared with their in-mix results."""
-    import threading
-
-    rng = np.random.default_rng(seed)
-    sizes = rng.choice([64, 128, 192, 256], n_req)
-    fr0s = -rng.choice([1e-4, 1e-5, 1e-6], n_req)
-    sigmas = rng.uniform(0.8, 1.6, n_req)
-    cfgs = [M.Config(N=int(N), model=M.MODEL_FR, fr_n=1, box=2.0 * int(N), omega_m=0.3, omega_l=0.7, aexpn=1.0,
-                     fr0=float(f), precision=M.PREC_MIXED) for N, f in zip(sizes, fr0s)]
+def lognormal_set(cfgs, sigmas, seed0):
+    """Seeded lognormal densities (request i: seed seed0 + i), generated on the
+    GPU and kept on the host as float32."""
     dens = []
     gens = {}
-    t0 = time.perf_counter()
     for i, c in enumerate(cfgs):
-        g = gens.get(c.N)
+        g = gens.get((c.N, c.box))
         if g is None:
-            g = gens[c.N] = M.Solver(M.Config(N=c.N, box=c.box, precision=M.PREC_FP64))
-        g.make_lognormal_density(2000 + i, float(sigmas[i]))
+            g = gens[(c.N, c.box)] = M.Solver(M.Config(N=c.N, box=c.box, precision=M.PREC_FP64))
+        g.make_lognormal_density(seed0 + i, float(sigmas[i]))
         dens.append(g.get_density(np.float32))
     for g in gens.values():
         g.close()
-    log(f"[serve] {n_req} densities generated in {time.perf_counter() - t0:.1f} s")
-    # saturated: everything queued at once
+    return dens
+
+
+def serve_run(tag, cfgs, kws, dens, max_active, rng):
+    """All requests at once (saturated throughput), then Poisson arrivals at
+    0.8x that rate (latency); three requests re-solved alone and compared with
+    their in-mix results.  kws[i]: submit arguments of request i (its exit
+    rule); success: status 0 and, for the rel_tol rule, converged."""
+    import threading
+
+    n_req = len(cfgs)
+
+    def ok(o, kw):
+        return o["status"] == 0 and (o["converged"] or kw.get("exit_rule") == M.EXIT_REFERENCE)
+
     with M.Server(max_active=max_active) as srv:
         ts = time.perf_counter()
-        tickets = [srv.submit(c, d) for c, d in zip(cfgs, dens)]
+        tickets = [srv.submit(c, d, **kw) for c, d, kw in zip(cfgs, dens, kws)]
         outs = [srv.wait(t, c.N, np.float32)[1] for t, c in zip(tickets, cfgs)]
         sat_s = time.perf_counter() - ts
         sat = srv.stats()
     rate = 0.8 * n_req / sat_s
-    log(f"[serve] saturated: {n_req / sat_s:.1f} solves/s, p50 {sat['p50_ms']:.1f} ms, batch <= {sat['max_batch']}")
-    bad = [o for o in outs if o["status"] != 0 or not o["converged"]]
-    # Poisson arrivals at 0.8x the saturated rate
+    log(f"[{tag}] saturated: {n_req / sat_s:.1f} solves/s, p50 {sat['p50_ms']:.1f} ms, batch <= {sat['max_batch']}")
+    bad = [o for o, kw in zip(outs, kws) if not ok(o, kw)]
     gaps = rng.exponential(1.0 / rate, n_req)
     fields = {}
-    keep = {0, n_req // 2, n_req - 1}
+    keep = {0, 1, n_req // 2, n_req - 1}
     with M.Server(max_active=max_active) as srv:
         tickets = [None] * n_req
 
@@ -268,7 +266,7 @@ def serving_stream(n_req, seed, max_active):
                 dt = start + at - time.perf_counter()
                 if dt > 0:
                     time.sleep(dt)
-                tickets[i] = srv.submit(cfgs[i], dens[i])
+                tickets[i] = srv.submit(cfgs[i], dens[i], **kws[i])
 
         th = threading.Thread(target=producer)
         tp = time.perf_counter()
@@ -279,22 +277,20 @@ def serving_stream(n_req, seed, max_active):
             u, o = srv.wait(tickets[i], cfgs[i].N, np.float32)
             if i in keep:
                 fields[i] = u
-            if o["status"] != 0 or not o["converged"]:
+            if not ok(o, kws[i]):
                 bad.append(o)
         th.join()
         pois_s = time.perf_counter() - tp
         st = srv.stats()
-    log(f"[serve] Poisson {rate:.1f}/s: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms, "
+    log(f"[{tag}] Poisson {rate:.1f}/s: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms, "
         f"{st['solves_per_s']:.1f} solves/s, batch <= {st['max_batch']}")
-    # the same requests alone
     diff = 0.0
     with M.Server(max_active=1) as srv:
         for i in sorted(keep):
-            u, _ = srv.wait(srv.submit(cfgs[i], dens[i]), cfgs[i].N, np.float32)
+            u, _ = srv.wait(srv.submit(cfgs[i], dens[i], **kws[i]), cfgs[i].N, np.float32)
             diff = max(diff, float(np.max(np.abs(u - fields[i]))))
     return {
         "requests": n_req,
-        "sizes": [64, 128, 192, 256],
         "max_active": max_active,
         "saturated_solves_per_s": n_req / sat_s,
         "arrival_rate_per_s": rate,
@@ -307,6 +303,57 @@ def serving_stream(n_req, seed, max_active):
         "failed_or_unconverged": len(bad),
         "alone_vs_mix_max_abs_diff": diff,
     }
+
+
+def serving_stream(n_req, seed, max_active):
+    """The serving stream: n_req f(R) n=1 requests, ngrid uniform over
+    {64, 128, 192, 256} (cell 2 Mpc/h), |fR0| from {1e-4, 1e-5, 1e-6},
+    seeded lognormal densities with sigma_g ~ U[0.8, 1.6] (request i: seed
+    2000 + i), each solved to relative residual 1e-8."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([64, 128, 192, 256], n_req)
+    fr0s = -rng.choice([1e-4, 1e-5, 1e-6], n_req)
+    sigmas = rng.uniform(0.8, 1.6, n_req)
+    cfgs = [M.Config(N=int(N), model=M.MODEL_FR, fr_n=1, box=2.0 * int(N), omega_m=0.3, omega_l=0.7, aexpn=1.0,
+                     fr0=float(f), precision=M.PREC_MIXED) for N, f in zip(sizes, fr0s)]
+    t0 = time.perf_counter()
+    dens = lognormal_set(cfgs, sigmas, 2000)
+    log(f"[serve] {n_req} densities generated in {time.perf_counter() - t0:.1f} s")
+    out = serve_run("serve", cfgs, [{}] * n_req, dens, max_active, rng)
+    out["sizes"] = [64, 128, 192, 256]
+    return out
+
+
+def serving_mix(n_req, seed, max_active):
+    """nDGP requests mixed with f(R) ones in the serving loop: request i is
+    nDGP (normal branch, H0rc from {0.5, 1, 2}) for even i, f(R) n=1 (|fR0|
+    from {1e-4, 1e-5, 1e-6}) for odd i; ngrid uniform over the power-of-two
+    sizes {64, 128, 256} (cell 2 Mpc/h); sigma_g ~ U[0.8, 1.6] (seed 6000 +
+    i).  f(R) requests stop at relative residual 1e-8, nDGP requests by the
+    reference's stagnation exits (their residual levels off)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([64, 128, 256], n_req)
+    fr0s = -rng.choice([1e-4, 1e-5, 1e-6], n_req)
+    h0rcs = rng.choice([0.5, 1.0, 2.0], n_req)
+    sigmas = rng.uniform(0.8, 1.6, n_req)
+    cfgs, kws = [], []
+    for i in range(n_req):
+        N = int(sizes[i])
+        if i % 2 == 0:
+            cfgs.append(M.Config(N=N, model=M.MODEL_DGP, branch=1, box=2.0 * N, omega_m=0.3, omega_l=0.7,
+                                 aexpn=1.0, h0rc=float(h0rcs[i]), precision=M.PREC_MIXED))
+            kws.append(dict(exit_rule=M.EXIT_REFERENCE, res_conv=1e-30, max_outer=40))
+        else:
+            cfgs.append(M.Config(N=N, model=M.MODEL_FR, fr_n=1, box=2.0 * N, omega_m=0.3, omega_l=0.7, aexpn=1.0,
+                                 fr0=float(fr0s[i]), precision=M.PREC_MIXED))
+            kws.append({})
+    t0 = time.perf_counter()
+    dens = lognormal_set(cfgs, sigmas, 6000)
+    log(f"[serve-mix] {n_req} densities generated in {time.perf_counter() - t0:.1f} s")
+    out = serve_run("serve-mix", cfgs, kws, dens, max_active, rng)
+    out["sizes"] = [64, 128, 256]
+    out["models"] = "nDGP (even i) + f(R) (odd i)"
+    return out
 
 
 def profiled_traffic():
@@ -348,6 +395,8 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0.0, help="only run a device copy of this many GiB")
     ap.add_argument("--serve-requests", type=int, default=512, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-mix-requests", type=int, default=96,
+                    help="requests of the nDGP + f(R) serving mix (0: skip)")
     ap.add_argument("--dgp-n", type=int, default=512, help="grid of the nDGP comparison (0: skip)")
     ap.add_argument("--pm-n", type=int, default=512, help="grid (and particles per side) of the PM step (0: skip)")
     args = ap.parse_args()
@@ -418,6 +467,9 @@ def main():
     serving = None
     if args.serve_requests > 0:
         serving = serving_stream(args.serve_requests, 4000, args.serve_slots)
+    serving_mixed = None
+    if args.serve_mix_requests > 0:
+        serving_mixed = serving_mix(args.serve_mix_requests, 4100, args.serve_slots)
 
     cpu = None
     if not args.no_cpu:
@@ -453,6 +505,7 @@ def main():
                         "source": "serving stream (Poisson arrivals), submit -> result"} if serving else None),
         "solve_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "serving": serving,
+        "serving_dgp_fr_mix": serving_mixed,
         "dgp": dgp,
         "pm_step": pm,
         "mixed": {
