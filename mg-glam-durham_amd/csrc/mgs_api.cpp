@@ -526,7 +526,7 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
         prof_end(h);
         return 0;
     }
-    if (l > 0 && v.n <= TAIL_LEVEL_MAX_N && h->use_tail) {
+    if (l > 0 && v.n <= h->tail_n && h->use_tail) {
         // the small levels in one launch
         TailLevelIn lv[16];
         const int nlev = h->lc - l + 1;
@@ -583,13 +583,14 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
 
 // FP64 outer residual; also writes the inner problem's g, xb and colour 0 of
 // d after its first colour-0 pass (reads u, delta; writes g, xb, d: 26 B/cell)
-static int outer_residual(mgs_handle* h, double* rms)
+// norm_only: the residual norm without the next inner problem's data
+static int outer_residual(mgs_handle* h, double* rms, bool norm_only = false)
 {
     LevelD& v = h->L[0];
     LevelF& m = h->M[0];
-    prof_begin(h, MGS_KC_RESIDUAL, 26.0 * n3d(v.n));
+    prof_begin(h, MGS_KC_RESIDUAL, (norm_only ? 16.0 : 26.0) * n3d(v.n));
     const int nb = launch_outer_residual(h->p.fr_n, v.u, v.rho, m.f, m.xb, m.d, h->partials, v.n, fr_coefs(h->p, 0),
-                                         df_coefs(h->p, 0), h->s);
+                                         df_coefs(h->p, 0), norm_only, h->s);
     prof_end(h);
     launch_sum_partials(h->partials, nb, h->dsum, h->s);
     HIPCHK(h, hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s));
@@ -721,14 +722,17 @@ static int outer_residual_dgp(mgs_handle* h, double* rms)
 // next one contracts up to 1.5 times less per cycle), between 1 and the cap;
 // the first outer iteration runs the cap.  Fewer outer iterations save their
 // FP64 residual, update and base restriction, about half an inner cycle each.
-static int auto_inner(const mgs_handle* h, double res, double target, double res_prev, int m_prev)
+// *reaches: the count is expected to reach the target (not capped).
+static int auto_inner(const mgs_handle* h, double res, double target, double res_prev, int m_prev, bool* reaches)
 {
+    *reaches = false;
     if (m_prev <= 0 || !(res_prev > 0.0) || !(res > 0.0) || !(target > 0.0)) return h->inner_max;
     const double rho = std::pow(res / res_prev, 1.0 / m_prev);
     if (!(rho > 0.0) || !(rho < 0.9)) return h->inner_max;
     const double m = std::ceil(std::log(0.7 * target / res) / std::log(rho));
-    if (!(m >= 1.0)) return 1;
-    return m >= (double)h->inner_max ? h->inner_max : (int)m;
+    if (m > (double)h->inner_max) return h->inner_max;
+    *reaches = true;
+    return m >= 1.0 ? (int)m : 1;
 }
 
 // FP64 defect correction around the FP32 inner FAS (f(R) and nDGP)
@@ -745,9 +749,14 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
     int m_prev = 0;
     bool conv = false;
     int cycles = 0;
+    // f(R), mode 1: when this outer residual is expected to end the solve
+    // (automatic inner count predicted to reach the target, or the cycle
+    // cap), take its norm alone; should the solve go on after all, the full
+    // kernel recomputes the same residual with the inner problem's data
+    bool last_expected = false;
     for (int it = 1;; it++) {
         double res = 0.0;
-        if ((rc = fr ? outer_residual(h, &res) : outer_residual_dgp(h, &res))) return rc;
+        if ((rc = fr ? outer_residual(h, &res, last_expected) : outer_residual_dgp(h, &res))) return rc;
         if (!std::isfinite(res)) return set_err(h, MGS_E_DOMAIN, "residual is not finite");
         if (it == 1) rep->res0 = res;
         rep->res_final = res;
@@ -768,11 +777,14 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
             can_exit = true;
         }
         if (can_exit) break;
+        if (last_expected && (rc = outer_residual(h, &res))) return rc;
+        bool reaches = false;
         if (automatic) {
-            inner = auto_inner(h, res, o.rel_tol * rep->res0, res_prev, m_prev);
+            inner = auto_inner(h, res, o.rel_tol * rep->res0, res_prev, m_prev, &reaches);
             res_prev = res;
             m_prev = inner;
         }
+        last_expected = fr && o.mode == 1 && (reaches || it >= o.max_cycles);
         // inner problem: base restricted to every level, increment from zero
         for (int l = 1; l <= h->lc; l++) {
             prof_begin(h, l == 1 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(h->M[l - 1].n) + n3d(h->M[l].n)) * 4.0);
@@ -877,6 +889,9 @@ int mgs_create(const mgs_params* params, mgs_handle** out)
     {
         const char* e = std::getenv("MGS_NO_TAIL");
         h->use_tail = !(e && e[0] == '1');
+        const char* tn = std::getenv("MGS_TAIL_N");
+        const int tv = tn ? std::atoi(tn) : 0;
+        if (tv == 8 || tv == 16 || tv == 32) h->tail_n = tv;
         // tuning overrides of the mixed inner cycle's shape
         const char* w = std::getenv("MGS_INNER_SWEEPS");
         int a = 0, b = 0;

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T
 }
 
 // correct_solution (PMP2extradof.f90:1361-1578), one coarse column per thread;
-// ONLY1: colour-1 sons only (correct64_sons)
+// ONLY1: colour-1 sons only (son_corrected)
 template <bool ONLY1>
 __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH, const double* __restrict__ ru,
                                                    double* __restrict__ uf, Geo gf, Geo gc, int guard, int TX, int TY,
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH
     const int I = cc.m, J = cc.j;
     Block9 B;
     B.init(gc, I, J);
-    double hs[9], rs[9], hk[9], rk[9];  // plane beside K (K-1, then K+1) and plane K
+    double hs[9], rs[9], hk[9], rk[9];  // planes K-1 and K
     const int Km = wrap_dec(cc.k0, gc.n);
 #pragma unroll
     for (int q = 0; q < 9; q++) {
@@ -150,21 +150,25 @@ __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH
         rk[q] = ru[B.at(q, cc.k0)];
     }
     for (int K = cc.k0; K < cc.k1; K++) {
-        correct64_sons<0, ONLY1>(hs, rs, hk, rk, uf, gf, I, J, K, guard);
+        // every load of this plane (sons of fine planes 2K and 2K+1, coarse
+        // plane K+1), then the updates
+        double o0[4], o1[4], hp[9], rp[9];
+        correct64_sons_load<0, ONLY1>(uf, gf, I, J, K, o0);
+        correct64_sons_load<1, ONLY1>(uf, gf, I, J, K, o1);
         const int Kp = wrap_inc(K, gc.n);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            hs[q] = uH[B.at(q, Kp)];
-            rs[q] = ru[B.at(q, Kp)];
+            hp[q] = uH[B.at(q, Kp)];
+            rp[q] = ru[B.at(q, Kp)];
         }
-        correct64_sons<1, ONLY1>(hs, rs, hk, rk, uf, gf, I, J, K, guard);
+        correct64_sons_apply<0, ONLY1>(hs, rs, hk, rk, uf, gf, I, J, K, guard, o0);
+        correct64_sons_apply<1, ONLY1>(hp, rp, hk, rk, uf, gf, I, J, K, guard, o1);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            const double th = hk[q], tr = rk[q];
-            hk[q] = hs[q];
-            rk[q] = rs[q];
-            hs[q] = th;
-            rs[q] = tr;
+            hs[q] = hk[q];
+            rs[q] = rk[q];
+            hk[q] = hp[q];
+            rk[q] = rp[q];
         }
     }
 }
@@ -462,13 +466,17 @@ void launch_sum_partials(const double* partials, int count, double* out, hipStre
 }
 
 int launch_outer_residual(int frn, const double* u, const double* delta, float* g32, float* xb, float* d,
-                          double* partials, int n, const FrCoef& c, const DfCoef& dc, hipStream_t s)
+                          double* partials, int n, const FrCoef& c, const DfCoef& dc, bool norm_only, hipStream_t s)
 {
     const Geo g = make_geo(n);
     const unsigned half = (unsigned)g.half;
-    if (frn == 0) return both_colours(FrOuterOp<0>{u, delta, g32, xb, d, c, dc, half}, partials, g, s);
-    if (frn == 1) return both_colours(FrOuterOp<1>{u, delta, g32, xb, d, c, dc, half}, partials, g, s);
-    return both_colours(FrOuterOp<2>{u, delta, g32, xb, d, c, dc, half}, partials, g, s);
+#define OUTER(FRN)                                                                                       \
+    return norm_only ? both_colours(FrOuterOp<FRN, true>{u, delta, g32, xb, d, c, dc, half}, partials, g, s) \
+                     : both_colours(FrOuterOp<FRN, false>{u, delta, g32, xb, d, c, dc, half}, partials, g, s)
+    if (frn == 0) OUTER(0);
+    if (frn == 1) OUTER(1);
+    OUTER(2);
+#undef OUTER
 }
 
 void launch_df_smooth_update(int frn, const float* d, const float* xb, const float* rhs, double* u, int n,

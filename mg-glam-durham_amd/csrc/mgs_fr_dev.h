@@ -309,17 +309,25 @@ __device__ __forceinline__ bool son_corrected(int t)
     return !ONLY1 || (((t & 1) + (t >> 1) + S3) & 1) == 1;
 }
 
+// the old values of the sons of fine plane 2K+S3 that are corrected (loaded
+// before any son of the coarse plane is stored: each son is read and written
+// by its own thread only)
 template <int S3, bool ONLY1>
-__device__ __forceinline__ void correct64_sons(const double* hz, const double* rz, const double* hk,
-                                               const double* rk, double* __restrict__ uf, const Geo& gf, int I, int J,
-                                               int K, int guard)
+__device__ __forceinline__ void correct64_sons_load(const double* __restrict__ uf, const Geo& gf, int I, int J, int K,
+                                                    double* old)
 {
-    const double a = 1.0 / 64.0, b = 3.0 * a, c9 = 9.0 * a, d = 27.0 * a;
-    const double wt[8] = {a, b, b, c9, b, c9, c9, d};
-    double old[4];
 #pragma unroll
     for (int t = 0; t < 4; t++)
         if (son_corrected<S3, ONLY1>(t)) old[t] = uf[son_index(gf, I, J, K, t & 1, t >> 1, S3)];
+}
+
+template <int S3, bool ONLY1>
+__device__ __forceinline__ void correct64_sons_apply(const double* hz, const double* rz, const double* hk,
+                                                     const double* rk, double* __restrict__ uf, const Geo& gf, int I,
+                                                     int J, int K, int guard, const double* old)
+{
+    const double a = 1.0 / 64.0, b = 3.0 * a, c9 = 9.0 * a, d = 27.0 * a;
+    const double wt[8] = {a, b, b, c9, b, c9, c9, d};
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         if (!son_corrected<S3, ONLY1>(t)) continue;
@@ -483,8 +491,9 @@ struct DfSmoothUpdateOp {
 // the inner increment after its first colour-0 pass from d = 0 (where every
 // neighbour sum vanishes): d = solve(0) on colour 0.  Colour 1 of d is not
 // written: the inner cycle's next pass is the colour-1 pass, which sets it
-// from colour 0 without reading it.
-template <int FRN>
+// from colour 0 without reading it.  NORM_ONLY: the same residual and
+// partial sums, without the inner problem's data (a solve's last residual).
+template <int FRN, bool NORM_ONLY = false>
 struct FrOuterOp {
     using T = double;
     const double* u;
@@ -513,10 +522,12 @@ struct FrOuterOp {
         OP = OP * c.fct3r;
         OP = OP + c.fct1r * w.delta;
         OP = OP + c.fct2r * (c.a_fR_rt * ru - 1.0);
-        const float gv = (float)(-OP), xv = (float)(uc * c.inv_ubg);
-        elem(g, o) = gv;
-        elem(xb, o) = xv;
-        if (o < half) elem(d, o) = df_solve<FRN>(0.0f, xv, gv, dc);
+        if constexpr (!NORM_ONLY) {
+            const float gv = (float)(-OP), xv = (float)(uc * c.inv_ubg);
+            elem(g, o) = gv;
+            elem(xb, o) = xv;
+            if (o < half) elem(d, o) = df_solve<FRN>(0.0f, xv, gv, dc);
+        }
         return OP;
     }
 };
@@ -619,7 +630,8 @@ __device__ __forceinline__ void df_restrict_residual_col(const float* __restrict
 #pragma unroll
         for (int q = 0; q < 4; q++) dl[q] = p1.d[q];
         p0 = p2;
-    }}
+    }
+}
 
 // restricted base of coarse cell (I, J, K): xb_c = (mean of the children's
 // xb^{n+1})^{1/(n+1)}
@@ -631,31 +643,38 @@ __device__ __forceinline__ void restrict_base_cell(const float* __restrict__ xf,
 #pragma unroll
     for (int q = 0; q < 8; q++)
         w += wpow<FRN>(elem(xf, sidx(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1))));
-    elem(xc, sidx(gc, I, J, K)) = xroot<FRN>(w * 0.125f);}
+    elem(xc, sidx(gc, I, J, K)) = xroot<FRN>(w * 0.125f);
+}
 
 // inner coarse-grid correction d_h += P(d_H - R d_h), kept only where the
 // corrected x stays positive (the delta-form image of the reference's guard);
-// z-marching like k_correct64, on e = d_H - R d_h
+// z-marching like k_correct64, on e = d_H - R d_h.  Colour-1 sons only: every
+// inner correction is followed by a colour-0 pass (see son_corrected).  The
+// sons' old d and base weight are loaded by df_sons_load before any son of
+// the plane is stored (each son is read and written by its own thread only),
+// so all loads of a coarse plane are in flight together.
 template <int FRN, int S3>
-__device__ __forceinline__ void df_correct_sons(const float* ez, const float* ek, float* __restrict__ df,
-                                                const float* __restrict__ xbf, const Geo& gf, int I, int J, int K)
+__device__ __forceinline__ void df_sons_load(const float* __restrict__ df, const float* __restrict__ xbf, const Geo& gf,
+                                             int I, int J, int K, float* old, float* wb)
 {
-    // colour-1 sons only: every inner correction is followed by a colour-0
-    // pass (see son_corrected)
-    constexpr bool ONLY1 = true;
-    const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
-    const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
-    float old[4], wb[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        if (!son_corrected<S3, ONLY1>(t)) continue;
+        if (!son_corrected<S3, true>(t)) continue;
         const unsigned fi = son_index(gf, I, J, K, t & 1, t >> 1, S3);
         old[t] = elem(df, fi);
         wb[t] = wpow<FRN>(elem(xbf, fi));
     }
+}
+
+template <int S3>
+__device__ __forceinline__ void df_sons_apply(const float* ez, const float* ek, float* __restrict__ df, const Geo& gf,
+                                              int I, int J, int K, const float* old, const float* wb)
+{
+    const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
+    const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        if (!son_corrected<S3, ONLY1>(t)) continue;
+        if (!son_corrected<S3, true>(t)) continue;
         const int s1 = t & 1, s2 = t >> 1;
         float P = 0.0f;
 #pragma unroll
@@ -714,20 +733,26 @@ __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, con
     const int I = cc.m, J = cc.j;
     Block9 B;
     B.init(gc, I, J);
-    float es[9], ek[9];  // plane beside K (K-1, then K+1) and plane K
-    e_block9<SHFL>(dH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, es);
+    float em[9], ek[9];  // planes K-1 and K
+    e_block9<SHFL>(dH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, em);
     e_block9<SHFL>(dH, rd, B, cc.k0, lok, rok, ek);
     for (int K = cc.k0; K < cc.k1; K++) {
-        df_correct_sons<FRN, 0>(es, ek, df, xbf, gf, I, J, K);
-        e_block9<SHFL>(dH, rd, B, wrap_inc(K, gc.n), lok, rok, es);
-        df_correct_sons<FRN, 1>(es, ek, df, xbf, gf, I, J, K);
+        // every load of this plane (sons of fine planes 2K and 2K+1, coarse
+        // plane K+1), then the updates
+        float o0[4], w0[4], o1[4], w1[4], ep[9];
+        df_sons_load<FRN, 0>(df, xbf, gf, I, J, K, o0, w0);
+        df_sons_load<FRN, 1>(df, xbf, gf, I, J, K, o1, w1);
+        e_block9<SHFL>(dH, rd, B, wrap_inc(K, gc.n), lok, rok, ep);
+        df_sons_apply<0>(em, ek, df, gf, I, J, K, o0, w0);
+        df_sons_apply<1>(ep, ek, df, gf, I, J, K, o1, w1);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            const float t = ek[q];
-            ek[q] = es[q];
-            es[q] = t;
+            em[q] = ek[q];
+            ek[q] = ep[q];
         }
-    }}
+    }
+}
+
 // coarsest level: red-black sweeps of the delta-form smoother in FP64 by one
 // workgroup (the level holds at most a few thousand cells)
 template <int FRN>

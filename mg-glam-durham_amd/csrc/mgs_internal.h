@@ -94,6 +94,7 @@ struct mgs_handle {
     hipGraphExec_t g_inner = nullptr;
     bool use_graphs = true;
     bool use_tail = true;  // mixed f(R): small inner levels in one launch (MGS_NO_TAIL=1 turns it off)
+    int tail_n = 16;  // the tail starts at the first level with n <= tail_n (MGS_TAIL_N: 8, 16 or 32)
     // mixed f(R): level-0 pre/post sweeps of the inner cycle (MGS_INNER_SWEEPS
     // "a,b" overrides them for mgs_solve only) and the automatic inner-cycle
     // cap (MGS_INNER_MAX)

@@ -40,9 +40,10 @@ void launch_sum_partials(const double* partials, int count, double* out, hipStre
 
 // ---- mixed precision f(R) path: FP64 outer loop, FP32 delta-form FAS ----
 // FP64 residual of u (partials for its norm), g = -r and xb = u/u_bg in FP32,
-// and the inner increment d after its first colour-0 pass from zero
+// and the inner increment d after its first colour-0 pass from zero;
+// norm_only: the partials alone (the same values)
 int launch_outer_residual(int frn, const double* u, const double* delta, float* g, float* xb, float* d,
-                          double* partials, int n, const FrCoef& c, const DfCoef& dc, hipStream_t s);
+                          double* partials, int n, const FrCoef& c, const DfCoef& dc, bool norm_only, hipStream_t s);
 // the last colour-1 inner pass fused with u += u_bg (x(d) - xb) on all cells
 void launch_df_smooth_update(int frn, const float* d, const float* xb, const float* rhs, double* u, int n,
                              const DfCoef& c, double ubg, hipStream_t s);
@@ -70,7 +71,6 @@ struct TailLevelIn {
     float *d, *xb, *f, *rres, *rd;
     DfCoef dc;
 };
-constexpr int TAIL_LEVEL_MAX_N = 16;
 bool launch_df_tail(int frn, int nlev, const TailLevelIn* lv, hipStream_t s);
 
 }  // namespace mgs
