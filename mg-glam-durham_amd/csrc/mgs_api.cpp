@@ -276,6 +276,35 @@ int op_smooth(mgs_handle* h, int l, int red)
     return 0;
 }
 
+// Nsmth = 2 red-black pairs (colour 0, then 1) on level l.  nDGP levels with
+// z-marching passes ping-pong between u and the second array: after two
+// pairs the field is back in u.
+static void op_smooth2(mgs_handle* h, int l)
+{
+    LevelD& v = h->L[l];
+    if (h->p.model == MGS_MODEL_DGP && h->dgp_spare64 && dgp_zmarch(v.n)) {
+        const DgpCoef c = dgp_coefs(h->p, l);
+        const size_t half = (size_t)v.n * v.n * (v.n / 2);
+        double* A = v.u;
+        double* B = h->dgp_spare64;
+        for (int s = 0; s < 2; s++) {
+            for (int colour = 0; colour < 2; colour++) {
+                if (l == 0) prof_begin(h, MGS_KC_SMOOTH, 2.0 * n3d(v.n) * 8.0);
+                else prof_begin(h, MGS_KC_COARSE, 2.0 * n3d(v.n) * 8.0);
+                launch_dgp_zpass64(l, colour ? B : A, A + half, B + (size_t)colour * half, v.rho, v.f, v.n, colour, c,
+                                   h->s);
+                prof_end(h);
+            }
+            std::swap(A, B);
+        }
+        return;
+    }
+    for (int s = 0; s < 2; s++) {
+        op_smooth(h, l, 1);
+        op_smooth(h, l, 0);
+    }
+}
+
 // level-l residual, stored in res_l when store; rms returned when rms != null
 int op_residual(mgs_handle* h, int l, bool store, double* rms)
 {
@@ -433,18 +462,11 @@ void drop_graphs(mgs_handle* h)
 // V_cycle(ilevel), PMP2extradof.f90:352-555 (Nsmth = 2, :361)
 int vcycle_plain(mgs_handle* h, int l)
 {
-    const int Nsmth = 2;
     if (l >= h->lc) {
-        for (int s = 0; s < Nsmth; s++) {
-            op_smooth(h, l, 1);
-            op_smooth(h, l, 0);
-        }
+        op_smooth2(h, l);
         return 0;
     }
-    for (int s = 0; s < Nsmth; s++) {
-        op_smooth(h, l, 1);
-        op_smooth(h, l, 0);
-    }
+    op_smooth2(h, l);
     if (l > 0) op_residual(h, l, true, nullptr);
     op_restrict_both(h, l + 1);
     if (l == 0) {
@@ -458,10 +480,7 @@ int vcycle_plain(mgs_handle* h, int l)
         vcycle_plain(h, l + 1);
     }
     op_correct(h, l + 1, true);
-    for (int s = 0; s < Nsmth; s++) {
-        op_smooth(h, l, 1);
-        op_smooth(h, l, 0);
-    }
+    op_smooth2(h, l);
     return 0;
 }
 
@@ -481,14 +500,7 @@ static int cgc_plain(mgs_handle* h, int l, int kind)
     return 0;
 }
 
-static void smooth2_plain(mgs_handle* h, int l)
-{
-    const int Nsmth = 2;
-    for (int s = 0; s < Nsmth; s++) {
-        op_smooth(h, l, 1);
-        op_smooth(h, l, 0);
-    }
-}
+static void smooth2_plain(mgs_handle* h, int l) { op_smooth2(h, l); }
 
 // F_cycle (PMP2extradof.f90:563-858): S2, CGC(F), S2, CGC(V), S2;
 // W_cycle (:867-1157): S2, CGC(W), S2, CGC(W), S2; S2 only on the coarsest
@@ -651,9 +663,39 @@ static int solve_plain(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
 }
 
 // ---- nDGP: inner V-cycle on the FP32 increment e (see mgs_dgp.hip) ----------
+// 2 x Nsmth colour passes (0, 1, 0, 1) of the inner smoother on level l;
+// z-marching levels ping-pong between e and the scratch array and end in e
+static void dgpm_smooth4(mgs_handle* h, int l, int cls)
+{
+    LevelF& v = h->M[l];
+    LevelD& V = h->L[l];
+    const DgpCoef dc = dgp_coefs(h->p, l);
+    float* tmp = (float*)h->dgp_tmp;
+    const double bytes = 3.0 * n3d(v.n) * 4.0;
+    if (dgp_zmarch(v.n)) {
+        const size_t half = (size_t)v.n * v.n * (v.n / 2);
+        float* A = v.d;
+        float* B = tmp;  // holds a level-0 array of floats
+        for (int s = 0; s < 2; s++) {
+            for (int colour = 0; colour < 2; colour++) {
+                prof_begin(h, cls, bytes);
+                launch_dgpm_zpass(colour ? B : A, A + half, B + (size_t)colour * half, v.xb, V.rho, v.f, v.n, colour,
+                                  dc, h->s);
+                prof_end(h);
+            }
+            std::swap(A, B);
+        }
+        return;
+    }
+    for (int it = 0; it < 4; it++) {
+        prof_begin(h, cls, bytes);
+        launch_dgpm_smooth(v.d, v.xb, V.rho, v.f, tmp, v.n, (it & 1) ? 1 : 0, dc, h->s);
+        prof_end(h);
+    }
+}
+
 static int vcycle_inner_dgp(mgs_handle* h, int l)
 {
-    const int Nsmth = 2;
     LevelF& v = h->M[l];
     LevelD& V = h->L[l];
     const DgpCoef dc = dgp_coefs(h->p, l);
@@ -666,11 +708,7 @@ static int vcycle_inner_dgp(mgs_handle* h, int l)
         prof_end(h);
         return 0;
     }
-    for (int it = 0; it < 2 * Nsmth; it++) {
-        prof_begin(h, cls, 3.0 * n3d(v.n) * s);
-        launch_dgpm_smooth(v.d, v.xb, V.rho, v.f, tmp, v.n, (it & 1) ? 1 : 0, dc, h->s);
-        prof_end(h);
-    }
+    dgpm_smooth4(h, l, cls);
     LevelF& c = h->M[l + 1];
     prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, 6.0 * n3d(v.n) * s);
     launch_dgpm_residual(v.d, v.xb, V.rho, v.f, v.res, v.n, dc, h->s);
@@ -692,11 +730,7 @@ static int vcycle_inner_dgp(mgs_handle* h, int l)
     prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * s);
     launch_dgpm_correct(c.d, c.rd, v.d, c.n, h->s);
     prof_end(h);
-    for (int it = 0; it < 2 * Nsmth; it++) {
-        prof_begin(h, cls, 3.0 * n3d(v.n) * s);
-        launch_dgpm_smooth(v.d, v.xb, V.rho, v.f, tmp, v.n, (it & 1) ? 1 : 0, dc, h->s);
-        prof_end(h);
-    }
+    dgpm_smooth4(h, l, cls);
     return 0;
 }
 

@@ -13,74 +13,108 @@
 
 namespace mgs {
 
-// Sigma's stencil sums and the Laplacian numerator at absolute cell (i,j,k)
-__device__ __forceinline__ void dgp_stencil(const double* __restrict__ u, const Geo& g, int i, int j, int k,
-                                            double& se0, double& se1, double& se2, double& lap, double& c0)
+// The 19 points of the stencil at a cell: centre, 6 faces and 12 edges
+// (the edges named by their offsets, e.g. pp_xy = (i+1, j+1, k)).  Both the
+// per-cell gathers and the z-marching register windows fill one, and the
+// operators below read it, so every path adds the same values in the same
+// order.
+template <typename T>
+struct S19 {
+    T c0, xp, xm, yp, ym, zp, zm;
+    T xy_pp, xy_mm, xy_pm, xy_mp;  // (i+-1, j+-1, k)
+    T xz_pp, xz_mm, xz_pm, xz_mp;  // (i+-1, j, k+-1)
+    T yz_pp, yz_mm, yz_pm, yz_mp;  // (i, j+-1, k+-1)
+};
+
+template <typename T>
+__device__ __forceinline__ S19<T> s19_at(const T* __restrict__ u, const Geo& g, int i, int j, int k)
 {
     const int n = g.n;
     const int im = wrap_dec(i, n), ip = wrap_inc(i, n);
     const int jm = wrap_dec(j, n), jp = wrap_inc(j, n);
     const int km = wrap_dec(k, n), kp = wrap_inc(k, n);
-    c0 = u[split_index(g, i, j, k)];
-    const double xp = u[split_index(g, ip, j, k)], xm = u[split_index(g, im, j, k)];
-    const double yp = u[split_index(g, i, jp, k)], ym = u[split_index(g, i, jm, k)];
-    const double zp = u[split_index(g, i, j, kp)], zm = u[split_index(g, i, j, km)];
-    const double dxx = xp + xm - 2.0 * c0;
-    const double dyy = yp + ym - 2.0 * c0;
-    const double dzz = zp + zm - 2.0 * c0;
-    const double cxy = u[split_index(g, ip, jp, k)] + u[split_index(g, im, jm, k)] - u[split_index(g, ip, jm, k)] -
-                       u[split_index(g, im, jp, k)];
-    const double cxz = u[split_index(g, ip, j, kp)] + u[split_index(g, im, j, km)] - u[split_index(g, ip, j, km)] -
-                       u[split_index(g, im, j, kp)];
-    const double cyz = u[split_index(g, i, jp, kp)] + u[split_index(g, i, jm, km)] - u[split_index(g, i, jp, km)] -
-                       u[split_index(g, i, jm, kp)];
+    S19<T> s;
+    s.c0 = u[split_index(g, i, j, k)];
+    s.xp = u[split_index(g, ip, j, k)];
+    s.xm = u[split_index(g, im, j, k)];
+    s.yp = u[split_index(g, i, jp, k)];
+    s.ym = u[split_index(g, i, jm, k)];
+    s.zp = u[split_index(g, i, j, kp)];
+    s.zm = u[split_index(g, i, j, km)];
+    s.xy_pp = u[split_index(g, ip, jp, k)];
+    s.xy_mm = u[split_index(g, im, jm, k)];
+    s.xy_pm = u[split_index(g, ip, jm, k)];
+    s.xy_mp = u[split_index(g, im, jp, k)];
+    s.xz_pp = u[split_index(g, ip, j, kp)];
+    s.xz_mm = u[split_index(g, im, j, km)];
+    s.xz_pm = u[split_index(g, ip, j, km)];
+    s.xz_mp = u[split_index(g, im, j, kp)];
+    s.yz_pp = u[split_index(g, i, jp, kp)];
+    s.yz_mm = u[split_index(g, i, jm, km)];
+    s.yz_pm = u[split_index(g, i, jp, km)];
+    s.yz_mp = u[split_index(g, i, jm, kp)];
+    return s;
+}
+
+// Sigma's stencil sums and the Laplacian numerator
+__device__ __forceinline__ void dgp_sums(const S19<double>& u, double& se0, double& se1, double& se2, double& lap)
+{
+    const double dxx = u.xp + u.xm - 2.0 * u.c0;
+    const double dyy = u.yp + u.ym - 2.0 * u.c0;
+    const double dzz = u.zp + u.zm - 2.0 * u.c0;
+    const double cxy = u.xy_pp + u.xy_mm - u.xy_pm - u.xy_mp;
+    const double cxz = u.xz_pp + u.xz_mm - u.xz_pm - u.xz_mp;
+    const double cyz = u.yz_pp + u.yz_mm - u.yz_pm - u.yz_mp;
     se0 = dxx * dxx + dyy * dyy + dzz * dzz;
     se1 = dxx * dyy + dxx * dzz + dyy * dzz;
     se2 = cxy * cxy + cxz * cxz + cyz * cyz;
-    const double s6 = xp + xm + yp + ym + zp + zm;
-    lap = s6 - c0 * 6.0;
+    const double s6 = u.xp + u.xm + u.yp + u.ym + u.zp + u.zm;
+    lap = s6 - u.c0 * 6.0;
 }
 
-// L(phi) at a cell (without the FAS right-hand side)
-__device__ __forceinline__ double dgp_L(const double* __restrict__ u, const Geo& g, int i, int j, int k, double rho,
-                                        const DgpCoef& c, double& c0)
+// L(phi) at a cell from its stencil values (without the FAS right-hand side)
+__device__ __forceinline__ double dgp_L19(const S19<double>& u, double rho, const DgpCoef& c)
 {
     const double TWOOVERTHREE = 2.0 / 3.0, ONEOVEREIGHT = 0.125, THREEOVERFOUR = 0.75, EIGHTOVERTHREE = 8.0 / 3.0;
     double se0, se1, se2, lap;
-    dgp_stencil(u, g, i, j, k, se0, se1, se2, lap, c0);
+    dgp_sums(u, se0, se1, se2, lap);
     double Sigma = c.fct1 * rho;
     Sigma = Sigma + se0 * TWOOVERTHREE / c.dx4 - se1 * TWOOVERTHREE / c.dx4 + se2 * ONEOVEREIGHT / c.dx4;
     return lap / c.dx2 - (copysign(1.0, c.alpha) * sqrt(c.alpha * c.alpha + EIGHTOVERTHREE * Sigma) - c.alpha) *
                              THREEOVERFOUR;
 }
 
-// second differences, face sum and centre of a field at (i, j, k)
+// L(phi) at absolute cell (i, j, k); c0: the cell's value
+__device__ __forceinline__ double dgp_L(const double* __restrict__ u, const Geo& g, int i, int j, int k, double rho,
+                                        const DgpCoef& c, double& c0)
+{
+    const S19<double> s = s19_at(u, g, i, j, k);
+    c0 = s.c0;
+    return dgp_L19(s, rho, c);
+}
+
+// second differences, face sum and centre of a field at a cell
 struct D2f {
     float xx, yy, zz, xy, xz, yz, s6, c0;
 };
 
+__device__ __forceinline__ D2f d2_of(const S19<float>& u)
+{
+    D2f d;
+    d.c0 = u.c0;
+    d.xx = u.xp + u.xm - 2.0f * d.c0;
+    d.yy = u.yp + u.ym - 2.0f * d.c0;
+    d.zz = u.zp + u.zm - 2.0f * d.c0;
+    d.xy = u.xy_pp + u.xy_mm - u.xy_pm - u.xy_mp;
+    d.xz = u.xz_pp + u.xz_mm - u.xz_pm - u.xz_mp;
+    d.yz = u.yz_pp + u.yz_mm - u.yz_pm - u.yz_mp;
+    d.s6 = u.xp + u.xm + u.yp + u.ym + u.zp + u.zm;
+    return d;
+}
+
 __device__ __forceinline__ D2f d2_at(const float* __restrict__ u, const Geo& g, int i, int j, int k)
 {
-    const int n = g.n;
-    const int im = wrap_dec(i, n), ip = wrap_inc(i, n);
-    const int jm = wrap_dec(j, n), jp = wrap_inc(j, n);
-    const int km = wrap_dec(k, n), kp = wrap_inc(k, n);
-    D2f d;
-    d.c0 = u[split_index(g, i, j, k)];
-    const float xp = u[split_index(g, ip, j, k)], xm = u[split_index(g, im, j, k)];
-    const float yp = u[split_index(g, i, jp, k)], ym = u[split_index(g, i, jm, k)];
-    const float zp = u[split_index(g, i, j, kp)], zm = u[split_index(g, i, j, km)];
-    d.xx = xp + xm - 2.0f * d.c0;
-    d.yy = yp + ym - 2.0f * d.c0;
-    d.zz = zp + zm - 2.0f * d.c0;
-    d.xy = u[split_index(g, ip, jp, k)] + u[split_index(g, im, jm, k)] - u[split_index(g, ip, jm, k)] -
-           u[split_index(g, im, jp, k)];
-    d.xz = u[split_index(g, ip, j, kp)] + u[split_index(g, im, j, km)] - u[split_index(g, ip, j, km)] -
-           u[split_index(g, im, j, kp)];
-    d.yz = u[split_index(g, i, jp, kp)] + u[split_index(g, i, jm, km)] - u[split_index(g, i, jp, km)] -
-           u[split_index(g, i, jm, kp)];
-    d.s6 = xp + xm + yp + ym + zp + zm;
-    return d;
+    return d2_of(s19_at(u, g, i, j, k));
 }
 
 struct DgpF {

@@ -82,8 +82,9 @@ struct mgs_handle {
     bool has_density = false;
     bool rho_cached = false;
     // DGP: out-of-place colour pass target and per-block partials
-    double* dgp_tmp = nullptr;
+    double* dgp_tmp = nullptr;     // a colour half of level 0 in FP64 (N^3 floats)
     double* dgp_rms = nullptr;
+    double* dgp_spare64 = nullptr;  // FP64 z-marching passes: second level-0 array
     // FFT plans (density generator, power spectrum)
     void* fft = nullptr;
     // particles and the potential of the PM step (mgs_pm.hip)
@@ -147,6 +148,13 @@ void launch_dgpm_coarsest(float* e, const float* pb, const double* rho, const fl
                           int sweeps, const DgpCoef& c, hipStream_t s);
 void launch_dgpm_correct(const float* eH, const float* rd, float* ef, int nc, hipStream_t s);
 void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s);
+// z-marching colour passes (levels with dgp_zmarch(n)): read the field's
+// colour halves h0/h1 (e0/e1), write the pass colour's new half to out
+bool dgp_zmarch(int n);
+void launch_dgp_zpass64(int level, const double* h0, const double* h1, double* out, const double* rho,
+                        const double* f, int n, int colour, const DgpCoef& c, hipStream_t s);
+void launch_dgpm_zpass(const float* e0, const float* e1, float* out, const float* pb, const double* rho,
+                       const float* rhs, int n, int colour, const DgpCoef& c, hipStream_t s);
 
 // mgs_fft.hip
 void fft_release(mgs_handle* h);

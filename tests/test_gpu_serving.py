@@ -104,6 +104,7 @@ DGP_MIX = [
     (16, -1, 32.0, 1.0, 1.0, 0.8, 22),
     (64, 1, 128.0, 0.8, 2.0, 1.1, 23),
     (32, -1, 64.0, 1.0, 0.5, 1.2, 24),
+    (128, 1, 256.0, 1.0, 1.0, 1.0, 25),  # the standalone solve's z-marching passes
 ]
 DGP_EXIT = dict(exit_rule=M.EXIT_REFERENCE, res_conv=1e-30, max_outer=40)
 
