@@ -48,6 +48,30 @@ DgpCoef dgp_coefs(const mgs_params& p, int level)
     return c;
 }
 
+// z-marching launches (the windows and per-column bodies are in mgs_dgp_dev.h)
+static ZLaunch dgp_zlaunch(int n) { return make_zlaunch(n / 2, n, DGP_KC); }
+
+// both colours of a level (residual-type OP), one block partial of the sum of
+// squares per block when partials is non-null
+template <class OP>
+__global__ __launch_bounds__(256) void k_dgp_both_z(OP op, double* __restrict__ partials, Geo g, int TX, int TY,
+                                                    int XB, int JB, unsigned NB)
+{
+    ZCol cc;
+    double acc = 0.0;
+    if (zcol(g, TX, TY, XB, JB, DGP_KC, NB, cc)) acc = dgp_both_zcol(op, g, cc.m, cc.j, cc.k0);
+    if (partials) store_partial(partials, NB, block_sum(acc));
+}
+
+template <class OP>
+static unsigned launch_both_z(const OP& op, double* partials, int n, hipStream_t s)
+{
+    const Geo g = make_geo(n);
+    const ZLaunch L = dgp_zlaunch(n);
+    hipLaunchKernelGGL((k_dgp_both_z<OP>), L.grid, L.block, 0, s, op, partials, g, L.TX, L.TY, L.XB, L.JB, L.NB);
+    return L.NB;
+}
+
 // relaxation_iterations_DGP, one colour, out of place into tmp (one colour half)
 template <bool COARSE>
 __global__ __launch_bounds__(256) void k_dgp_smooth64(const double* __restrict__ u, const double* __restrict__ rho,
@@ -193,6 +217,11 @@ int launch_dgp_residual64(int level, const double* u, const double* rho, const d
 {
     (void)scratch;
     const Geo g = make_geo(n);
+    if (dgp_zmarch(n)) {
+        const unsigned hf = (unsigned)g.half;
+        if (level == 0) return (int)launch_both_z(DgpRes64Op<false>{u, rho, f, res, c, hf}, partials, n, s);
+        return (int)launch_both_z(DgpRes64Op<true>{u, rho, f, res, c, hf}, partials, n, s);
+    }
     const Launch L = make_launch(g.h, 2L * n * n);
     if (level == 0)
         hipLaunchKernelGGL((k_dgp_residual64<false>), L.grid, L.block, 0, s, u, rho, f, res, partials, g, c, L.TX,
@@ -324,6 +353,7 @@ int launch_dgpm_outer(const double* u, const double* rho, float* g32, float* pb,
                       const DgpCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
+    if (dgp_zmarch(n)) return (int)launch_both_z(DgpmOuterOp{u, rho, g32, pb, e, c, (unsigned)g.half}, partials, n, s);
     const Launch L = make_launch(g.h, 2L * n * n);
     hipLaunchKernelGGL(k_dgpm_outer, L.grid, L.block, 0, s, u, rho, g32, pb, e, partials, g, c, L.TX, L.TY, L.XB,
                        L.NB);
@@ -344,6 +374,10 @@ void launch_dgpm_residual(const float* e, const float* pb, const double* rho, co
                           const DgpCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
+    if (dgp_zmarch(n)) {
+        launch_both_z(DgpmResOp{e, pb, rho, rhs, out, dgpf(c), (unsigned)g.half}, nullptr, n, s);
+        return;
+    }
     const Launch L = make_launch(g.h, 2L * n * n);
     hipLaunchKernelGGL(k_dgpm_residual, L.grid, L.block, 0, s, e, pb, rho, rhs, out, g, dgpf(c), L.TX, L.TY, L.XB,
                        L.NB);
@@ -375,167 +409,8 @@ void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s)
 }
 
 // ===========================================================================
-// z-marching colour passes with 19-point register windows (levels with
-// n/2 >= 64, so that a wave is 64 consecutive m of one row)
+// z-marching colour passes (the windows are in mgs_dgp_dev.h)
 // ===========================================================================
-// A thread owns column (m, j) and marches over a run of DGP_KC planes.  Its
-// window holds, for planes k-1, k, k+1 and rows j-1, j, j+1, the values at
-// x = 2m-1, 2m, 2m+1, 2m+2: every point of the 19-point stencil of the cell
-// of either colour at (m, j, k).  A step loads the pair at m of plane k+2 in
-// the three rows (both colour halves at one offset) and takes x = 2m-1 and
-// 2m+2 from the neighbouring lanes (the lanes at a wave's edges load them);
-// plane k+2 and the next cell's own data are loaded before plane k is
-// computed.  The stencil values go through S19 like the per-cell kernels',
-// so every result is bit-identical to theirs.
-//
-// The field is read through its two colour halves (h0, h1) and the pass
-// writes the new values of its colour to `out`, a half of a second array: a
-// pair of passes (colour 0, then colour 1 reading the new colour 0) moves
-// the field into the second array without the per-cell kernels' copy.
-constexpr int DGP_KC = 8;
-
-bool dgp_zmarch(int n)
-{
-    const int h = n / 2;
-    return h >= 64 && h % 64 == 0 && n % DGP_KC == 0;
-}
-
-static ZLaunch dgp_zlaunch(int n) { return make_zlaunch(n / 2, n, DGP_KC); }
-
-template <typename T>
-struct DWin {
-    T l[3][3], e[3][3], o[3][3], r[3][3];  // [plane k-1, k, k+1][row j-1, j, j+1]: x = 2m-1, 2m, 2m+1, 2m+2
-};
-
-template <typename T>
-struct DRaw {
-    T ev[3], ov[3], lv[3], rv[3];  // one plane, rows j-1, j, j+1
-};
-
-struct DCol {
-    unsigned plane, m, mm, mp;
-    unsigned rowoff[3];
-    int jr[3];
-    bool lload, rload;
-};
-
-__device__ __forceinline__ DCol dcol_of(const Geo& g, int m, int j)
-{
-    DCol d;
-    d.plane = (unsigned)g.n * (unsigned)g.h;
-    d.m = (unsigned)m;
-    d.mm = (unsigned)(m == 0 ? g.h - 1 : m - 1);
-    d.mp = (unsigned)(m + 1 == g.h ? 0 : m + 1);
-    d.jr[0] = wrap_dec(j, g.n);
-    d.jr[1] = j;
-    d.jr[2] = wrap_inc(j, g.n);
-#pragma unroll
-    for (int R = 0; R < 3; R++) d.rowoff[R] = (unsigned)d.jr[R] * (unsigned)g.h;
-    const unsigned lane = (threadIdx.x + threadIdx.y * blockDim.x) & 63u;
-    d.lload = lane == 0;
-    d.rload = lane == 63;
-    return d;
-}
-
-// plane K of rows j-1..j+1: the pair at m (x = 2m has colour (J+K)&1) and,
-// at a wave's edges, x = 2m-1 (the colour of x = 2m+1) and x = 2m+2
-template <typename T>
-__device__ __forceinline__ void draw_load(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d, int K,
-                                          DRaw<T>& r)
-{
-#pragma unroll
-    for (int R = 0; R < 3; R++) {
-        const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
-        const bool odd = ((d.jr[R] + K) & 1) != 0;
-        const T* he = odd ? h1 : h0;
-        const T* ho = odd ? h0 : h1;
-        r.ev[R] = he[base + d.m];
-        r.ov[R] = ho[base + d.m];
-        r.lv[R] = T(0);
-        r.rv[R] = T(0);
-        if (d.lload) r.lv[R] = ho[base + d.mm];
-        if (d.rload) r.rv[R] = he[base + d.mp];
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void dwin_set(DWin<T>& w, int P, const DRaw<T>& r, const DCol& d)
-{
-#pragma unroll
-    for (int R = 0; R < 3; R++) {
-        const T l = __shfl_up(r.ov[R], 1u, 64);
-        const T rr = __shfl_down(r.ev[R], 1u, 64);
-        w.l[P][R] = d.lload ? r.lv[R] : l;
-        w.e[P][R] = r.ev[R];
-        w.o[P][R] = r.ov[R];
-        w.r[P][R] = d.rload ? r.rv[R] : rr;
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void dwin_shift(DWin<T>& w)
-{
-#pragma unroll
-    for (int R = 0; R < 3; R++) {
-        w.l[0][R] = w.l[1][R];
-        w.e[0][R] = w.e[1][R];
-        w.o[0][R] = w.o[1][R];
-        w.r[0][R] = w.r[1][R];
-        w.l[1][R] = w.l[2][R];
-        w.e[1][R] = w.e[2][R];
-        w.o[1][R] = w.o[2][R];
-        w.r[1][R] = w.r[2][R];
-    }
-}
-
-// value at x = i + dx of plane slot P, row slot R, for i = 2m + s
-template <typename T>
-__device__ __forceinline__ T wx(const DWin<T>& w, int P, int R, int dx, bool s)
-{
-    if (dx < 0) return s ? w.e[P][R] : w.l[P][R];
-    if (dx == 0) return s ? w.o[P][R] : w.e[P][R];
-    return s ? w.r[P][R] : w.o[P][R];
-}
-
-template <typename T>
-__device__ __forceinline__ S19<T> s19_win(const DWin<T>& w, bool s)
-{
-    S19<T> u;
-    u.c0 = wx(w, 1, 1, 0, s);
-    u.xp = wx(w, 1, 1, 1, s);
-    u.xm = wx(w, 1, 1, -1, s);
-    u.yp = wx(w, 1, 2, 0, s);
-    u.ym = wx(w, 1, 0, 0, s);
-    u.zp = wx(w, 2, 1, 0, s);
-    u.zm = wx(w, 0, 1, 0, s);
-    u.xy_pp = wx(w, 1, 2, 1, s);
-    u.xy_mm = wx(w, 1, 0, -1, s);
-    u.xy_pm = wx(w, 1, 0, 1, s);
-    u.xy_mp = wx(w, 1, 2, -1, s);
-    u.xz_pp = wx(w, 2, 1, 1, s);
-    u.xz_mm = wx(w, 0, 1, -1, s);
-    u.xz_pm = wx(w, 0, 1, 1, s);
-    u.xz_mp = wx(w, 2, 1, -1, s);
-    u.yz_pp = wx(w, 2, 2, 0, s);
-    u.yz_mm = wx(w, 0, 0, 0, s);
-    u.yz_pm = wx(w, 0, 2, 0, s);
-    u.yz_mp = wx(w, 2, 0, 0, s);
-    return u;
-}
-
-// the window of planes k0-1, k0, k0+1 (all loads first)
-template <typename T>
-__device__ __forceinline__ void dwin_init(DWin<T>& w, const T* __restrict__ h0, const T* __restrict__ h1,
-                                          const DCol& d, int k0, int n)
-{
-    DRaw<T> r0, r1, r2;
-    draw_load(h0, h1, d, wrap_dec(k0, n), r0);
-    draw_load(h0, h1, d, k0, r1);
-    draw_load(h0, h1, d, wrap_inc(k0, n), r2);
-    dwin_set(w, 0, r0, d);
-    dwin_set(w, 1, r1, d);
-    dwin_set(w, 2, r2, d);
-}
 
 // relaxation_iterations_DGP (as k_dgp_smooth64), one colour
 template <bool COARSE>

@@ -10,6 +10,7 @@
 
 #include "mgs_device.h"
 #include "mgs_internal.h"
+#include "mgs_stencil.h"
 
 namespace mgs {
 
@@ -266,5 +267,291 @@ __device__ __forceinline__ void dgpm_coarsest_block(float* __restrict__ e, const
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// z-marching with 19-point register windows (levels with n/2 >= 64, so that
+// a wave is 64 consecutive m of one row)
+// ---------------------------------------------------------------------------
+// A thread owns column (m, j) and marches over a run of DGP_KC planes.  Its
+// window holds, for planes k-1, k, k+1 and rows j-1, j, j+1, the values at
+// x = 2m-1, 2m, 2m+1, 2m+2: every point of the 19-point stencil of the cell
+// of either colour at (m, j, k).  A step loads the pair at m of plane k+2 in
+// the three rows (both colour halves at one offset) and takes x = 2m-1 and
+// 2m+2 from the neighbouring lanes (the lanes at a wave's edges load them);
+// plane k+2 and the next cell's own data are loaded before plane k is
+// computed.  The stencil values go through S19 like the per-cell kernels',
+// so every result is bit-identical to theirs.
+//
+// The field is read through its two colour halves (h0, h1) and the pass
+// writes the new values of its colour to `out`, a half of a second array: a
+// pair of passes (colour 0, then colour 1 reading the new colour 0) moves
+// the field into the second array without the per-cell kernels' copy.
+template <typename T>
+struct DWin {
+    T l[3][3], e[3][3], o[3][3], r[3][3];  // [plane k-1, k, k+1][row j-1, j, j+1]: x = 2m-1, 2m, 2m+1, 2m+2
+};
+
+template <typename T>
+struct DRaw {
+    T ev[3], ov[3], lv[3], rv[3];  // one plane, rows j-1, j, j+1
+};
+
+struct DCol {
+    unsigned plane, m, mm, mp;
+    unsigned rowoff[3];
+    int jr[3];
+    bool lload, rload;
+};
+
+__device__ __forceinline__ DCol dcol_of(const Geo& g, int m, int j)
+{
+    DCol d;
+    d.plane = (unsigned)g.n * (unsigned)g.h;
+    d.m = (unsigned)m;
+    d.mm = (unsigned)(m == 0 ? g.h - 1 : m - 1);
+    d.mp = (unsigned)(m + 1 == g.h ? 0 : m + 1);
+    d.jr[0] = wrap_dec(j, g.n);
+    d.jr[1] = j;
+    d.jr[2] = wrap_inc(j, g.n);
+#pragma unroll
+    for (int R = 0; R < 3; R++) d.rowoff[R] = (unsigned)d.jr[R] * (unsigned)g.h;
+    const unsigned lane = (threadIdx.x + threadIdx.y * blockDim.x) & 63u;
+    d.lload = lane == 0;
+    d.rload = lane == 63;
+    return d;
+}
+
+// plane K of rows j-1..j+1: the pair at m (x = 2m has colour (J+K)&1) and,
+// at a wave's edges, x = 2m-1 (the colour of x = 2m+1) and x = 2m+2
+template <typename T>
+__device__ __forceinline__ void draw_load(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d, int K,
+                                          DRaw<T>& r)
+{
+#pragma unroll
+    for (int R = 0; R < 3; R++) {
+        const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
+        const bool odd = ((d.jr[R] + K) & 1) != 0;
+        const T* he = odd ? h1 : h0;
+        const T* ho = odd ? h0 : h1;
+        r.ev[R] = he[base + d.m];
+        r.ov[R] = ho[base + d.m];
+        r.lv[R] = T(0);
+        r.rv[R] = T(0);
+        if (d.lload) r.lv[R] = ho[base + d.mm];
+        if (d.rload) r.rv[R] = he[base + d.mp];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void dwin_set(DWin<T>& w, int P, const DRaw<T>& r, const DCol& d)
+{
+#pragma unroll
+    for (int R = 0; R < 3; R++) {
+        const T l = __shfl_up(r.ov[R], 1u, 64);
+        const T rr = __shfl_down(r.ev[R], 1u, 64);
+        w.l[P][R] = d.lload ? r.lv[R] : l;
+        w.e[P][R] = r.ev[R];
+        w.o[P][R] = r.ov[R];
+        w.r[P][R] = d.rload ? r.rv[R] : rr;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void dwin_shift(DWin<T>& w)
+{
+#pragma unroll
+    for (int R = 0; R < 3; R++) {
+        w.l[0][R] = w.l[1][R];
+        w.e[0][R] = w.e[1][R];
+        w.o[0][R] = w.o[1][R];
+        w.r[0][R] = w.r[1][R];
+        w.l[1][R] = w.l[2][R];
+        w.e[1][R] = w.e[2][R];
+        w.o[1][R] = w.o[2][R];
+        w.r[1][R] = w.r[2][R];
+    }
+}
+
+// value at x = i + dx of plane slot P, row slot R, for i = 2m + s
+template <typename T>
+__device__ __forceinline__ T wx(const DWin<T>& w, int P, int R, int dx, bool s)
+{
+    if (dx < 0) return s ? w.e[P][R] : w.l[P][R];
+    if (dx == 0) return s ? w.o[P][R] : w.e[P][R];
+    return s ? w.r[P][R] : w.o[P][R];
+}
+
+template <typename T>
+__device__ __forceinline__ S19<T> s19_win(const DWin<T>& w, bool s)
+{
+    S19<T> u;
+    u.c0 = wx(w, 1, 1, 0, s);
+    u.xp = wx(w, 1, 1, 1, s);
+    u.xm = wx(w, 1, 1, -1, s);
+    u.yp = wx(w, 1, 2, 0, s);
+    u.ym = wx(w, 1, 0, 0, s);
+    u.zp = wx(w, 2, 1, 0, s);
+    u.zm = wx(w, 0, 1, 0, s);
+    u.xy_pp = wx(w, 1, 2, 1, s);
+    u.xy_mm = wx(w, 1, 0, -1, s);
+    u.xy_pm = wx(w, 1, 0, 1, s);
+    u.xy_mp = wx(w, 1, 2, -1, s);
+    u.xz_pp = wx(w, 2, 1, 1, s);
+    u.xz_mm = wx(w, 0, 1, -1, s);
+    u.xz_pm = wx(w, 0, 1, 1, s);
+    u.xz_mp = wx(w, 2, 1, -1, s);
+    u.yz_pp = wx(w, 2, 2, 0, s);
+    u.yz_mm = wx(w, 0, 0, 0, s);
+    u.yz_pm = wx(w, 0, 2, 0, s);
+    u.yz_mp = wx(w, 2, 0, 0, s);
+    return u;
+}
+
+// the window of planes k0-1, k0, k0+1 (all loads first)
+template <typename T>
+__device__ __forceinline__ void dwin_init(DWin<T>& w, const T* __restrict__ h0, const T* __restrict__ h1,
+                                          const DCol& d, int k0, int n)
+{
+    DRaw<T> r0, r1, r2;
+    draw_load(h0, h1, d, wrap_dec(k0, n), r0);
+    draw_load(h0, h1, d, k0, r1);
+    draw_load(h0, h1, d, wrap_inc(k0, n), r2);
+    dwin_set(w, 0, r0, d);
+    dwin_set(w, 1, r1, d);
+    dwin_set(w, 2, r2, d);
+}
+
+// Both cells of the pair at (m, j) over the run [k0, k0 + DGP_KC), from one
+// window per field (OP::NF fields of type OP::T, colour halves
+// op.half(f, 0/1)).  Per plane the cell at x = 2m comes first, then x = 2m+1;
+// OP::load(index) loads a cell's own data, OP::cell(index, own, stencils)
+// returns a value whose square is accumulated (the sum is returned).  The
+// next plane's window values and own data are loaded before a plane is
+// evaluated.
+template <class OP>
+__device__ __forceinline__ double dgp_both_zcol(const OP& op, const Geo& g, int m, int j, int k0)
+{
+    using T = typename OP::T;
+    using Own = typename OP::Own;
+    constexpr int NF = OP::NF;
+    const DCol d = dcol_of(g, m, j);
+    const unsigned half = (unsigned)g.half;
+    DWin<T> w[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) dwin_init(w[f], op.half(f, 0), op.half(f, 1), d, k0, g.n);
+    unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
+    unsigned par = (unsigned)((j + k0) & 1);  // colour of x = 2m
+    Own o0 = op.load(par * half + off), o1 = op.load((1u - par) * half + off);
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < DGP_KC; t++) {
+        const int k = k0 + t;
+        DRaw<T> r[NF];
+        Own n0 = o0, n1 = o1;
+        const unsigned offn = off + d.plane, parn = par ^ 1u;
+        if (t + 1 < DGP_KC) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) draw_load(op.half(f, 0), op.half(f, 1), d, wrap_inc(k + 1, g.n), r[f]);
+            n0 = op.load(parn * half + offn);
+            n1 = op.load((1u - parn) * half + offn);
+        }
+        S19<T> s0[NF], s1[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            s0[f] = s19_win(w[f], false);
+            s1[f] = s19_win(w[f], true);
+        }
+        const double r0 = op.cell(par * half + off, o0, s0);
+        const double r1 = op.cell((1u - par) * half + off, o1, s1);
+        acc += r0 * r0 + r1 * r1;
+        if (t + 1 < DGP_KC) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                dwin_shift(w[f]);
+                dwin_set(w[f], 2, r[f], d);
+            }
+            off = offn;
+            par = parn;
+            o0 = n0;
+            o1 = n1;
+        }
+    }
+    return acc;
+}
+
+// FP64 residual L(phi) (- f on coarse levels), stored in res when non-null
+template <bool COARSE>
+struct DgpRes64Op {
+    using T = double;
+    static constexpr int NF = 1;
+    struct Own {
+        double rho, f;
+    };
+    const double* u;
+    const double* rho;
+    const double* f;
+    double* res;
+    DgpCoef c;
+    unsigned hf;
+    __device__ const double* half(int, int colour) const { return u + (colour ? hf : 0u); }
+    __device__ __forceinline__ Own load(unsigned i) const { return Own{rho[i], COARSE ? f[i] : 0.0}; }
+    __device__ __forceinline__ double cell(unsigned i, const Own& o, const S19<double>* s) const
+    {
+        double OP = dgp_L19(s[0], o.rho, c);
+        if constexpr (COARSE) OP = OP - o.f;
+        if (res) res[i] = OP;
+        return OP;
+    }
+};
+
+// the mixed solve's FP64 outer residual r = L(phi), with the inner problem's
+// data g = -r, phi_b = phi (FP32) and e = 0 (as dgpm_outer_cell)
+struct DgpmOuterOp {
+    using T = double;
+    static constexpr int NF = 1;
+    struct Own {
+        double rho;
+    };
+    const double* u;
+    const double* rho;
+    float* g32;
+    float* pb;
+    float* e;
+    DgpCoef c;
+    unsigned hf;
+    __device__ const double* half(int, int colour) const { return u + (colour ? hf : 0u); }
+    __device__ __forceinline__ Own load(unsigned i) const { return Own{rho[i]}; }
+    __device__ __forceinline__ double cell(unsigned i, const Own& o, const S19<double>* s) const
+    {
+        const double OP = dgp_L19(s[0], o.rho, c);
+        g32[i] = (float)(-OP);
+        pb[i] = (float)s[0].c0;
+        e[i] = 0.0f;
+        return OP;
+    }
+};
+
+// the mixed inner residual out = D(e) - rhs (as dgpm_residual_cell); no norm
+struct DgpmResOp {
+    using T = float;
+    static constexpr int NF = 2;  // e, phi_b
+    struct Own {
+        float rho, rhs;
+    };
+    const float* e;
+    const float* pb;
+    const double* rho;
+    const float* rhs;
+    float* out;
+    DgpF c;
+    unsigned hf;
+    __device__ const float* half(int f, int colour) const { return (f ? pb : e) + (colour ? hf : 0u); }
+    __device__ __forceinline__ Own load(unsigned i) const { return Own{(float)rho[i], rhs[i]}; }
+    __device__ __forceinline__ double cell(unsigned i, const Own& o, const S19<float>* s) const
+    {
+        out[i] = dgp_dop(d2_of(s[1]), d2_of(s[0]), o.rho, c) - o.rhs;
+        return 0.0;
+    }
+};
 
 }  // namespace mgs

@@ -148,9 +148,17 @@ void launch_dgpm_coarsest(float* e, const float* pb, const double* rho, const fl
                           int sweeps, const DgpCoef& c, hipStream_t s);
 void launch_dgpm_correct(const float* eH, const float* rd, float* ef, int nc, hipStream_t s);
 void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s);
-// z-marching colour passes (levels with dgp_zmarch(n)): read the field's
-// colour halves h0/h1 (e0/e1), write the pass colour's new half to out
-bool dgp_zmarch(int n);
+// nDGP levels whose passes and residuals z-march with 19-point register
+// windows (mgs_dgp_dev.h): whole waves along m, runs of DGP_KC planes
+constexpr int DGP_KC = 8;
+inline bool dgp_zmarch(int n)
+{
+    const int h = n / 2;
+    return h >= 64 && h % 64 == 0 && n % DGP_KC == 0;
+}
+// z-marching colour passes (levels with dgp_zmarch(n)): read
+// the field's colour halves h0/h1 (e0/e1), write the pass colour's new half
+// to out
 void launch_dgp_zpass64(int level, const double* h0, const double* h1, double* out, const double* rho,
                         const double* f, int n, int colour, const DgpCoef& c, hipStream_t s);
 void launch_dgpm_zpass(const float* e0, const float* e1, float* out, const float* pb, const double* rho,

@@ -68,14 +68,15 @@ struct LevelDesc {
 };
 
 struct SlotDesc {
-    int frn, lc, model, pad1;
+    int frn, lc, model, zdgp;  // zdgp: nDGP outer residual on z-marching tiles (dgp_zmarch(ngrid))
     double* u;
     const double* rho;
     double* partials;
     FrCoef fc;
     // nDGP: FP64 operator coefficients of level 0, the tiles of its outer
-    // residual (as make_launch(h, 2 n^2)), and scratch of ngrid^3 floats (the
-    // colour passes' out-of-place half, then the residual before restriction)
+    // residual (as launch_dgpm_outer: z-marching runs, else make_launch(h,
+    // 2 n^2)), and scratch of ngrid^3 floats (the colour passes'
+    // out-of-place half, then the residual before restriction)
     DgpCoef gc0;
     Shape zo;
     float* scr;
@@ -173,7 +174,12 @@ __global__ __launch_bounds__(256) void kb_outer_residual(const SlotDesc* __restr
         const LevelDesc& L = sd.L[0];
         const unsigned lb = t - P->res_pre[i];
         double acc = 0.0;
-        if (sd.model == MGS_MODEL_DGP) {
+        if (sd.model == MGS_MODEL_DGP && sd.zdgp) {
+            ZCol c;
+            if (tile_col(L.g, L.g.h, sd.zo, lb, c))
+                acc = dgp_both_zcol(DgpmOuterOp{sd.u, sd.rho, L.f, L.xb, L.d, sd.gc0, (unsigned)L.g.half}, L.g, c.m,
+                                    c.j, c.k0);
+        } else if (sd.model == MGS_MODEL_DGP) {
             int m;
             unsigned row;
             if (tile_xy(L.g.h, 2u * (unsigned)L.g.n * (unsigned)L.g.n, sd.zo, lb, m, row))
@@ -515,6 +521,7 @@ static size_t layout(Hier& h, int N, int model, char* base)
     h.rho = (double*)take(n3 * 8);
     const bool dgp = model == MGS_MODEL_DGP;
     const Geo g0 = make_geo(N);
+    // nDGP: the per-cell tiling's partial count bounds the z-marching one
     const size_t nb = dgp ? (size_t)make_launch(g0.h, 2L * N * N).NB : (size_t)make_zlaunch_both(g0).NB;
     h.partials = (double*)take((nb + 64) * 8);
     if (dgp) {
@@ -662,9 +669,14 @@ void fill_desc(SlotDesc& sd, const Hier& h, const mgs_params& p)
     sd.partials = h.partials;
     if (dgp) {
         const Geo g0 = make_geo(h.N);
-        const Launch lo = make_launch(g0.h, 2L * h.N * h.N);
         sd.gc0 = dgp_coefs(p, 0);
-        sd.zo = Shape{lo.TX, lo.TY, lo.XB, 0, 0, lo.NB};
+        sd.zdgp = dgp_zmarch(h.N) ? 1 : 0;
+        if (sd.zdgp) {
+            sd.zo = shape_of(make_zlaunch(g0.h, h.N, DGP_KC));
+        } else {
+            const Launch lo = make_launch(g0.h, 2L * h.N * h.N);
+            sd.zo = Shape{lo.TX, lo.TY, lo.XB, 0, 0, lo.NB};
+        }
         sd.scr = h.scr;
     } else {
         sd.fc = fr_coefs(p, 0);
