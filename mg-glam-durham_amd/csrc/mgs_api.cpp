@@ -182,6 +182,9 @@ static int alloc_hierarchy(mgs_handle* h)
             if (l > 0 || h->p.model == MGS_MODEL_DGP) {
                 if ((rc = dalloc(h, &v.res, n3))) return rc;
             }
+            if (h->p.model == MGS_MODEL_DGP && dgp_zmarch(v.n)) {
+                if ((rc = dalloc(h, &v.rho, n3))) return rc;
+            }
             if (l > 0) {
                 if ((rc = dalloc(h, &v.rres, n3))) return rc;
                 if ((rc = dalloc(h, &v.rd, n3))) return rc;
@@ -420,6 +423,9 @@ static int cache_densities(mgs_handle* h)
 {
     if (h->rho_cached) return 0;
     for (int l = 1; l <= h->lc; l++) op_restrict_density(h, l);
+    // the mixed nDGP kernels' FP32 densities
+    for (size_t l = 0; l < h->M.size(); l++)
+        if (h->M[l].rho) launch_round_f32(h->L[l].rho, h->M[l].rho, (size_t)n3d(h->M[l].n), h->s);
     h->rho_cached = true;
     return 0;
 }
@@ -672,14 +678,14 @@ static void dgpm_smooth4(mgs_handle* h, int l, int cls)
     const DgpCoef dc = dgp_coefs(h->p, l);
     float* tmp = (float*)h->dgp_tmp;
     const double bytes = 3.0 * n3d(v.n) * 4.0;
-    if (dgp_zmarch(v.n)) {
+    if (dgp_zmarch(v.n) && v.rho) {
         const size_t half = (size_t)v.n * v.n * (v.n / 2);
         float* A = v.d;
         float* B = tmp;  // holds a level-0 array of floats
         for (int s = 0; s < 2; s++) {
             for (int colour = 0; colour < 2; colour++) {
                 prof_begin(h, cls, bytes);
-                launch_dgpm_zpass(colour ? B : A, A + half, B + (size_t)colour * half, v.xb, V.rho, v.f, v.n, colour,
+                launch_dgpm_zpass(colour ? B : A, A + half, B + (size_t)colour * half, v.xb, v.rho, v.f, v.n, colour,
                                   dc, h->s);
                 prof_end(h);
             }
@@ -711,13 +717,13 @@ static int vcycle_inner_dgp(mgs_handle* h, int l)
     dgpm_smooth4(h, l, cls);
     LevelF& c = h->M[l + 1];
     prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, 6.0 * n3d(v.n) * s);
-    launch_dgpm_residual(v.d, v.xb, V.rho, v.f, v.res, v.n, dc, h->s);
+    launch_dgpm_residual(v.d, v.xb, V.rho, v.rho, v.f, v.res, v.n, dc, h->s);
     launch_restrict8_32(v.res, c.rres, nullptr, c.n, h->s);
     launch_restrict8_32(v.d, c.rd, c.d, c.n, h->s);
     prof_end(h);
     auto coarse = [h, l, &c]() {
         prof_begin(h, MGS_KC_COARSE, 5.0 * n3d(c.n) * 4.0);
-        launch_dgpm_residual(c.rd, c.xb, h->L[l + 1].rho, c.rres, c.f, c.n, dgp_coefs(h->p, l + 1), h->s);
+        launch_dgpm_residual(c.rd, c.xb, h->L[l + 1].rho, c.rho, c.rres, c.f, c.n, dgp_coefs(h->p, l + 1), h->s);
         prof_end(h);
         return vcycle_inner_dgp(h, l + 1);
     };

@@ -23,6 +23,7 @@
 #include "mgs_internal.h"
 #include "mgs_kernels.h"
 #include "mgs_dgp_dev.h"
+#include "mgs_fr_dev.h"
 #include "mgs_stencil.h"
 
 namespace mgs {
@@ -332,14 +333,68 @@ __global__ __launch_bounds__(1024) void k_dgpm_coarsest(float* __restrict__ e, c
     dgpm_coarsest_block(e, pb, rho, rhs, tmp, g, sweeps, c);
 }
 
-__global__ __launch_bounds__(256) void k_dgpm_correct(const float* __restrict__ eH, const float* __restrict__ rd,
-                                                      float* __restrict__ ef, Geo gf, Geo gc, int TX, int TY, int XB,
-                                                      unsigned NB)
+// The same correction z-marching over coarse columns, as k_df_correct: two
+// planes of e = e_H - R e_h in registers (x-neighbours from the adjacent
+// lanes), all 8 sons of a coarse cell, every load of a coarse plane issued
+// before its sons are stored.  Sums in dgpm_correct_cell's order.
+template <int S3>
+__device__ __forceinline__ void dgpm_sons_load(const float* __restrict__ ef, const Geo& gf, int I, int J, int K,
+                                               float* old)
 {
-    int I;
-    unsigned row;
-    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    dgpm_correct_cell(eH, rd, ef, gf, gc, I, (int)(row % (unsigned)gc.n), (int)(row / (unsigned)gc.n));
+#pragma unroll
+    for (int t = 0; t < 4; t++) old[t] = elem(ef, son_index(gf, I, J, K, t & 1, t >> 1, S3));
+}
+
+template <int S3>
+__device__ __forceinline__ void dgpm_sons_apply(const float* ez, const float* ek, float* __restrict__ ef,
+                                                const Geo& gf, int I, int J, int K, const float* old)
+{
+    const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
+    const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int s1 = t & 1, s2 = t >> 1;
+        float P = 0.0f;
+#pragma unroll
+        for (int ind = 0; ind < 8; ind++) {
+            const int x = (ind & 1) ? 1 : (s1 ? 2 : 0);
+            const int y = (ind & 2) ? 1 : (s2 ? 2 : 0);
+            P += wt[ind] * ((ind & 4) ? ek[x + 3 * y] : ez[x + 3 * y]);
+        }
+        elem(ef, son_index(gf, I, J, K, s1, s2, S3)) = old[t] + P;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dgpm_zcorrect(const float* __restrict__ eH, const float* __restrict__ rd,
+                                                       float* __restrict__ ef, Geo gf, Geo gc, int TX, int TY, int XB,
+                                                       int JB, int KC, unsigned NB)
+{
+    ZCol cc;
+    if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
+    // lanes t-1 / t+1 hold columns I-1 / I+1 of the same row and plane run
+    // when they are in the same block row, in this wave and inside the level
+    const unsigned t = threadIdx.y * (unsigned)TX + threadIdx.x;
+    const bool lok = threadIdx.x > 0 && (t & 63u) != 0;
+    const bool rok = (int)threadIdx.x + 1 < TX && cc.m + 1 < gc.n && ((t + 1) & 63u) != 0;
+    const int I = cc.m, J = cc.j;
+    Block9 B;
+    B.init(gc, I, J);
+    float em[9], ek[9];  // planes K-1 and K
+    e_block9<true>(eH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, em);
+    e_block9<true>(eH, rd, B, cc.k0, lok, rok, ek);
+    for (int K = cc.k0; K < cc.k1; K++) {
+        float o0[4], o1[4], ep[9];
+        dgpm_sons_load<0>(ef, gf, I, J, K, o0);
+        dgpm_sons_load<1>(ef, gf, I, J, K, o1);
+        e_block9<true>(eH, rd, B, wrap_inc(K, gc.n), lok, rok, ep);
+        dgpm_sons_apply<0>(em, ek, ef, gf, I, J, K, o0);
+        dgpm_sons_apply<1>(ep, ek, ef, gf, I, J, K, o1);
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            em[q] = ek[q];
+            ek[q] = ep[q];
+        }
+    }
 }
 
 // phi += e
@@ -370,12 +425,12 @@ void launch_dgpm_smooth(float* e, const float* pb, const double* rho, const floa
     launch_copy_f32(tmp, e + (size_t)colour * g.half, g.half, s);
 }
 
-void launch_dgpm_residual(const float* e, const float* pb, const double* rho, const float* rhs, float* out, int n,
-                          const DgpCoef& c, hipStream_t s)
+void launch_dgpm_residual(const float* e, const float* pb, const double* rho, const float* rho32, const float* rhs,
+                          float* out, int n, const DgpCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    if (dgp_zmarch(n)) {
-        launch_both_z(DgpmResOp{e, pb, rho, rhs, out, dgpf(c), (unsigned)g.half}, nullptr, n, s);
+    if (dgp_zmarch(n) && rho32) {
+        launch_both_z(DgpmResOp{e, pb, rho32, rhs, out, dgpf(c), (unsigned)g.half}, nullptr, n, s);
         return;
     }
     const Launch L = make_launch(g.h, 2L * n * n);
@@ -396,8 +451,8 @@ void launch_dgpm_coarsest(float* e, const float* pb, const double* rho, const fl
 void launch_dgpm_correct(const float* eH, const float* rd, float* ef, int nc, hipStream_t s)
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
-    const Launch L = make_launch(nc, (long)nc * nc);
-    hipLaunchKernelGGL(k_dgpm_correct, L.grid, L.block, 0, s, eH, rd, ef, gf, gc, L.TX, L.TY, L.XB, L.NB);
+    const ZLaunch L = make_zlaunch(nc, nc);
+    hipLaunchKernelGGL(k_dgpm_zcorrect, L.grid, L.block, 0, s, eH, rd, ef, gf, gc, L.TX, L.TY, L.XB, L.JB, L.KC, L.NB);
 }
 
 void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s)
@@ -458,7 +513,7 @@ __global__ __launch_bounds__(256) void k_dgp_zsmooth64(const double* __restrict_
 // one colour of the mixed inner smoother (as k_dgpm_smooth)
 __global__ __launch_bounds__(256) void k_dgpm_zsmooth(const float* __restrict__ e0, const float* __restrict__ e1,
                                                       float* __restrict__ out, const float* __restrict__ pb,
-                                                      const double* __restrict__ rho, const float* __restrict__ rhs,
+                                                      const float* __restrict__ rho32, const float* __restrict__ rhs,
                                                       Geo g, int colour, DgpF c, int TX, int TY, int XB, int JB,
                                                       unsigned NB)
 {
@@ -472,7 +527,7 @@ __global__ __launch_bounds__(256) void k_dgpm_zsmooth(const float* __restrict__ 
     dwin_init(we, e0, e1, d, k0, g.n);
     dwin_init(wb, b0, b1, d, k0, g.n);
     unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
-    float rh = (float)rho[(unsigned)colour * (unsigned)g.half + off];
+    float rh = rho32[(unsigned)colour * (unsigned)g.half + off];
     float fv = rhs[(unsigned)colour * (unsigned)g.half + off];
 #pragma unroll
     for (int t = 0; t < DGP_KC; t++) {
@@ -484,7 +539,7 @@ __global__ __launch_bounds__(256) void k_dgpm_zsmooth(const float* __restrict__ 
             const int kn = wrap_inc(k + 1, g.n);
             draw_load(e0, e1, d, kn, re);
             draw_load(b0, b1, d, kn, rb);
-            rhn = (float)rho[(unsigned)colour * (unsigned)g.half + offn];
+            rhn = rho32[(unsigned)colour * (unsigned)g.half + offn];
             fvn = rhs[(unsigned)colour * (unsigned)g.half + offn];
         }
         const bool s = ((colour + cc.j + k) & 1) != 0;
@@ -516,13 +571,26 @@ void launch_dgp_zpass64(int level, const double* h0, const double* h1, double* o
                            L.TY, L.XB, L.JB, L.NB);
 }
 
-void launch_dgpm_zpass(const float* e0, const float* e1, float* out, const float* pb, const double* rho,
+void launch_dgpm_zpass(const float* e0, const float* e1, float* out, const float* pb, const float* rho32,
                        const float* rhs, int n, int colour, const DgpCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
     const ZLaunch L = dgp_zlaunch(n);
-    hipLaunchKernelGGL(k_dgpm_zsmooth, L.grid, L.block, 0, s, e0, e1, out, pb, rho, rhs, g, colour, dgpf(c), L.TX,
+    hipLaunchKernelGGL(k_dgpm_zsmooth, L.grid, L.block, 0, s, e0, e1, out, pb, rho32, rhs, g, colour, dgpf(c), L.TX,
                        L.TY, L.XB, L.JB, L.NB);
+}
+
+__global__ __launch_bounds__(256) void k_round_f32(const double* __restrict__ a, float* __restrict__ b, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = (float)a[i];
+}
+
+void launch_round_f32(const double* a, float* b, size_t n, hipStream_t s)
+{
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_round_f32, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n);
 }
 
 }  // namespace mgs

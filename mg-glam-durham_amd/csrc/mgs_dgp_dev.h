@@ -531,7 +531,8 @@ struct DgpmOuterOp {
     }
 };
 
-// the mixed inner residual out = D(e) - rhs (as dgpm_residual_cell); no norm
+// the mixed inner residual out = D(e) - rhs (as dgpm_residual_cell, which
+// rounds the FP64 density to FP32: rho32 holds those values); no norm
 struct DgpmResOp {
     using T = float;
     static constexpr int NF = 2;  // e, phi_b
@@ -540,13 +541,13 @@ struct DgpmResOp {
     };
     const float* e;
     const float* pb;
-    const double* rho;
+    const float* rho32;
     const float* rhs;
     float* out;
     DgpF c;
     unsigned hf;
     __device__ const float* half(int f, int colour) const { return (f ? pb : e) + (colour ? hf : 0u); }
-    __device__ __forceinline__ Own load(unsigned i) const { return Own{(float)rho[i], rhs[i]}; }
+    __device__ __forceinline__ Own load(unsigned i) const { return Own{rho32[i], rhs[i]}; }
     __device__ __forceinline__ double cell(unsigned i, const Own& o, const S19<float>* s) const
     {
         out[i] = dgp_dop(d2_of(s[1]), d2_of(s[0]), o.rho, c) - o.rhs;

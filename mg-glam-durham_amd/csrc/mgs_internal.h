@@ -32,6 +32,7 @@ struct LevelF {
     float* res = nullptr;
     float* rres = nullptr;
     float* rd = nullptr;     // frozen R d
+    float* rho = nullptr;    // nDGP, z-marching levels: the level's density rounded to FP32
 };
 
 struct ProfRec {
@@ -142,8 +143,9 @@ int launch_dgpm_outer(const double* u, const double* rho, float* g32, float* pb,
                       const DgpCoef& c, hipStream_t s);
 void launch_dgpm_smooth(float* e, const float* pb, const double* rho, const float* rhs, float* tmp, int n, int colour,
                         const DgpCoef& c, hipStream_t s);
-void launch_dgpm_residual(const float* e, const float* pb, const double* rho, const float* rhs, float* out, int n,
-                          const DgpCoef& c, hipStream_t s);
+// rho32: the density rounded to FP32 (used on z-marching levels)
+void launch_dgpm_residual(const float* e, const float* pb, const double* rho, const float* rho32, const float* rhs,
+                          float* out, int n, const DgpCoef& c, hipStream_t s);
 void launch_dgpm_coarsest(float* e, const float* pb, const double* rho, const float* rhs, float* tmp, int n,
                           int sweeps, const DgpCoef& c, hipStream_t s);
 void launch_dgpm_correct(const float* eH, const float* rd, float* ef, int nc, hipStream_t s);
@@ -161,8 +163,9 @@ inline bool dgp_zmarch(int n)
 // to out
 void launch_dgp_zpass64(int level, const double* h0, const double* h1, double* out, const double* rho,
                         const double* f, int n, int colour, const DgpCoef& c, hipStream_t s);
-void launch_dgpm_zpass(const float* e0, const float* e1, float* out, const float* pb, const double* rho,
+void launch_dgpm_zpass(const float* e0, const float* e1, float* out, const float* pb, const float* rho32,
                        const float* rhs, int n, int colour, const DgpCoef& c, hipStream_t s);
+void launch_round_f32(const double* a, float* b, size_t n, hipStream_t s);
 
 // mgs_fft.hip
 void fft_release(mgs_handle* h);
