@@ -35,6 +35,8 @@ DgpCoef dgp_coefs(const mgs_params& p, int level)
     const double dx = (double)(1 << level);
     c.dx2 = dx * dx;
     c.dx4 = c.dx2 * c.dx2;
+    c.inv_dx2 = 1.0 / c.dx2;
+    c.inv_dx4 = 1.0 / c.dx4;
     const double Orc = 1.0 / (4.0 * p.h0rc * p.h0rc);
     const double a3 = a * a * a;
     if (p.branch >= 0)

@@ -80,9 +80,11 @@ __device__ __forceinline__ double dgp_L19(const S19<double>& u, double rho, cons
     double se0, se1, se2, lap;
     dgp_sums(u, se0, se1, se2, lap);
     double Sigma = c.fct1 * rho;
-    Sigma = Sigma + se0 * TWOOVERTHREE / c.dx4 - se1 * TWOOVERTHREE / c.dx4 + se2 * ONEOVEREIGHT / c.dx4;
-    return lap / c.dx2 - (copysign(1.0, c.alpha) * sqrt(c.alpha * c.alpha + EIGHTOVERTHREE * Sigma) - c.alpha) *
-                             THREEOVERFOUR;
+    // the reference divides by dx^4 and dx^2; dx is a power of two, so the
+    // products with the exact reciprocals are the same numbers
+    Sigma = Sigma + se0 * TWOOVERTHREE * c.inv_dx4 - se1 * TWOOVERTHREE * c.inv_dx4 + se2 * ONEOVEREIGHT * c.inv_dx4;
+    return lap * c.inv_dx2 - (copysign(1.0, c.alpha) * sqrt(c.alpha * c.alpha + EIGHTOVERTHREE * Sigma) - c.alpha) *
+                                 THREEOVERFOUR;
 }
 
 // L(phi) at absolute cell (i, j, k); c0: the cell's value

@@ -62,7 +62,8 @@ constexpr int MIXED_INNER_MAX = 4;
 // DGP constants of one level, PMP2MGsolver_DGP.f90:36-48
 struct DgpCoef {
     double dx2, dx4, alpha, beta, fct1;
-    double fct2_1, fct3_1;   // fused level-1 restriction
+    double fct2_1, fct3_1;     // fused level-1 restriction
+    double inv_dx2, inv_dx4;   // 1/dx2, 1/dx4: powers of two, so x * inv == x / dx exactly
 };
 
 }  // namespace mgs
