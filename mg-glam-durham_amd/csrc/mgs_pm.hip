@@ -6,11 +6,16 @@
 // in its convention: positions in grid units in [1, N+1), cell I = INT(X).
 // They live on the device in six float arrays owned by the handle.
 //
-// Deposit: one thread per particle adds its 8 CIC weights with hardware FP64
-// atomics straight into the solver's level-0 density (colour-split FP64, the
-// array mgs_set_density fills), so a solve can follow without a host trip.
-// FP64 accumulation makes the sum independent of the atomic order to ~1e-16
-// (the reference adds into REAL*4 FI).
+// Deposit into the solver's level-0 density (colour-split FP64, the array
+// mgs_set_density fills), so a solve can follow without a host trip.  The
+// particles are binned by the 8^3-cell tile of their CIC base cell, in runs
+// of consecutive particles of one tile (a counting sort of the runs: per-tile
+// counts whose atomics also rank the runs, an exclusive scan, a scatter of
+// (first particle, length) into tile order); one workgroup per tile then
+// accumulates its particles' 8 weights in a 9^3 FP64 buffer in LDS and adds
+// each touched cell to the grid with one hardware FP64 atomic, instead of
+// eight global atomics per particle.  FP64 accumulation makes the sum independent of the atomic order
+// to ~1e-16 (the reference adds into REAL*4 FI).
 //
 // Move: one thread per particle gathers the 32 potential values of its 2x2x2
 // node block and their outer neighbours (PMP2main.f90:496-558), forms the
@@ -21,6 +26,7 @@
 // weights are formed in FP64 from it, as there.
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <string>
@@ -39,6 +45,18 @@ struct PmState {
     double* part = nullptr;
     int* bad = nullptr;    // particles outside [1, N+1) seen by the last kernel
     int npart_blocks = 0;
+    // deposit binning (runs of particles, see k_bin_count): per particle its
+    // key and rank, the run descriptors in tile order; per tile its count of
+    // runs and first run
+    int64_t nbin = 0;
+    int* key = nullptr;
+    int* rank = nullptr;
+    int64_t* runs = nullptr;  // nbin (at most one run per particle)
+    int ntiles = 0;
+    int* count = nullptr;
+    int64_t* first = nullptr;
+    void* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
 };
 
 static PmState* pm_of(mgs_handle* h) { return (PmState*)h->pm; }
@@ -60,6 +78,8 @@ void pm_release(mgs_handle* h)
     if (p->phi) (void)hipFree(p->phi);
     if (p->part) (void)hipFree(p->part);
     if (p->bad) (void)hipFree(p->bad);
+    for (void* q : {(void*)p->key, (void*)p->rank, (void*)p->runs, (void*)p->count, (void*)p->first, p->scan_tmp})
+        if (q) (void)hipFree(q);
     delete p;
     h->pm = nullptr;
 }
@@ -88,34 +108,186 @@ __device__ __forceinline__ int cell_of(double X, int N)
     return (int)X;
 }
 
-// DENSIT's CIC loop, PMP2mod_density.f90:235-281 (0-based cells here)
-__global__ __launch_bounds__(PM_BLOCK) void k_cic_deposit(const float* __restrict__ xs, const float* __restrict__ ys,
-                                                          const float* __restrict__ zs, int64_t np,
-                                                          double* __restrict__ rho, Geo g, double wpar,
-                                                          int* __restrict__ bad)
+// DENSIT's CIC loop, PMP2mod_density.f90:235-281 (0-based cells here), over
+// the particles binned by tile.
+constexpr int PM_TILE = 8;                                             // cells per tile side
+constexpr int PM_TBUF = PM_TILE + 1;                                   // with the +1 CIC reach
+constexpr int PM_TCELLS = PM_TBUF * PM_TBUF * PM_TBUF;                 // 729
+
+// tile of 0-based base cell (i0, j0, k0); TT tiles per side
+__device__ __forceinline__ int tile_of(int i0, int j0, int k0, int TT)
 {
-    const int N = g.n;
+    return ((k0 / PM_TILE) * TT + j0 / PM_TILE) * TT + i0 / PM_TILE;
+}
+
+// The binning passes take PM_RUN consecutive particles per thread (particles
+// in array order are usually neighbours in space, so consecutive ones usually
+// share a tile) and bin *runs*: maximal sequences of a thread's particles in
+// one tile, described by (first particle, length).
+constexpr int PM_RUN = 8;
+constexpr int KEY_OUTSIDE = -1;  // particle outside [1, N+1)
+constexpr int KEY_CONT = -2;     // particle continuing the run before it
+
+// per run: its tile (in key[] at the run's first particle; the others KEY_CONT)
+// and its rank among the runs of that tile (in rank[] there); per tile: the
+// count of runs
+__global__ __launch_bounds__(PM_BLOCK) void k_bin_count(const float* __restrict__ xs, const float* __restrict__ ys,
+                                                        const float* __restrict__ zs, int64_t np, int N, int TT,
+                                                        int* __restrict__ key, int* __restrict__ rank,
+                                                        int* __restrict__ count, int* __restrict__ bad)
+{
+    const int64_t nchunk = (np + PM_RUN - 1) / PM_RUN;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += stride) {
-        const double X = xs[p], Y = ys[p], Z = zs[p];
-        const int I = cell_of(X, N), J = cell_of(Y, N), K = cell_of(Z, N);
-        if (I < 0 || J < 0 || K < 0) {
-            atomicAdd(bad, 1);
-            continue;
+    for (int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ch < nchunk; ch += stride) {
+        const int64_t p0 = ch * PM_RUN;
+        // a whole chunk with 16-byte aligned rows: two 16-byte loads per array
+        const bool vec = (np & 3) == 0 && p0 + PM_RUN <= np;
+        float x[PM_RUN], y[PM_RUN], z[PM_RUN];
+        if (vec) {
+#pragma unroll
+            for (int v = 0; v < PM_RUN / 4; v++) {
+                const float4 a = ((const float4*)(xs + p0))[v], b = ((const float4*)(ys + p0))[v],
+                             c = ((const float4*)(zs + p0))[v];
+                x[4 * v] = a.x, x[4 * v + 1] = a.y, x[4 * v + 2] = a.z, x[4 * v + 3] = a.w;
+                y[4 * v] = b.x, y[4 * v + 1] = b.y, y[4 * v + 2] = b.z, y[4 * v + 3] = b.w;
+                z[4 * v] = c.x, z[4 * v + 1] = c.y, z[4 * v + 2] = c.z, z[4 * v + 3] = c.w;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < PM_RUN; r++) {
+                const int64_t p = p0 + r < np ? p0 + r : np - 1;
+                x[r] = xs[p];
+                y[r] = ys[p];
+                z[r] = zs[p];
+            }
         }
-        const double D1 = X - (double)I, D2 = Y - (double)J, D3 = Z - (double)K;
-        const double T1 = 1.0 - D1, T2 = 1.0 - D2, T3 = 1.0 - D3;
-        const double T2W = T2 * wpar, D2W = D2 * wpar;
-        const int i0 = I - 1, j0 = J - 1, k0 = K - 1;
-        const int i1 = I == N ? 0 : I, j1 = J == N ? 0 : J, k1 = K == N ? 0 : K;
-        unsafeAtomicAdd(&rho[split_index(g, i0, j0, k0)], T3 * T1 * T2W);
-        unsafeAtomicAdd(&rho[split_index(g, i1, j0, k0)], T3 * D1 * T2W);
-        unsafeAtomicAdd(&rho[split_index(g, i0, j1, k0)], T3 * T1 * D2W);
-        unsafeAtomicAdd(&rho[split_index(g, i1, j1, k0)], T3 * D1 * D2W);
-        unsafeAtomicAdd(&rho[split_index(g, i0, j0, k1)], D3 * T1 * T2W);
-        unsafeAtomicAdd(&rho[split_index(g, i1, j0, k1)], D3 * D1 * T2W);
-        unsafeAtomicAdd(&rho[split_index(g, i0, j1, k1)], D3 * T1 * D2W);
-        unsafeAtomicAdd(&rho[split_index(g, i1, j1, k1)], D3 * D1 * D2W);
+        int t[PM_RUN];
+#pragma unroll
+        for (int r = 0; r < PM_RUN; r++) {
+            t[r] = KEY_OUTSIDE;
+            if (p0 + r < np) {
+                const int I = cell_of(x[r], N), J = cell_of(y[r], N), K = cell_of(z[r], N);
+                if (I < 0 || J < 0 || K < 0)
+                    atomicAdd(bad, 1);
+                else
+                    t[r] = tile_of(I - 1, J - 1, K - 1, TT);
+            }
+        }
+        int k[PM_RUN], rk[PM_RUN];
+#pragma unroll
+        for (int r = 0; r < PM_RUN; r++) {
+            const bool start = t[r] >= 0 && (r == 0 || t[r] != t[r - 1]);
+            k[r] = start || t[r] < 0 ? t[r] : KEY_CONT;
+            rk[r] = 0;
+            if (start) rk[r] = atomicAdd(&count[t[r]], 1);
+        }
+        if (p0 + PM_RUN <= np) {  // key + p0 is 32-byte aligned (p0 a multiple of 8)
+#pragma unroll
+            for (int v = 0; v < PM_RUN / 4; v++) {
+                ((int4*)(key + p0))[v] = make_int4(k[4 * v], k[4 * v + 1], k[4 * v + 2], k[4 * v + 3]);
+                ((int4*)(rank + p0))[v] = make_int4(rk[4 * v], rk[4 * v + 1], rk[4 * v + 2], rk[4 * v + 3]);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < PM_RUN; r++)
+                if (p0 + r < np) {
+                    key[p0 + r] = k[r];
+                    rank[p0 + r] = rk[r];
+                }
+        }
+    }
+}
+
+// run descriptors into tile order: (first particle) * PM_RUN + (length - 1)
+__global__ __launch_bounds__(PM_BLOCK) void k_bin_scatter(int64_t np, const int* __restrict__ key,
+                                                          const int* __restrict__ rank,
+                                                          const int64_t* __restrict__ first,
+                                                          int64_t* __restrict__ runs)
+{
+    const int64_t nchunk = (np + PM_RUN - 1) / PM_RUN;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ch < nchunk; ch += stride) {
+        const int64_t p0 = ch * PM_RUN;
+        int k[PM_RUN];
+        if (p0 + PM_RUN <= np) {
+#pragma unroll
+            for (int v = 0; v < PM_RUN / 4; v++) {
+                const int4 a = ((const int4*)(key + p0))[v];
+                k[4 * v] = a.x, k[4 * v + 1] = a.y, k[4 * v + 2] = a.z, k[4 * v + 3] = a.w;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < PM_RUN; r++) k[r] = p0 + r < np ? key[p0 + r] : KEY_OUTSIDE;
+        }
+#pragma unroll
+        for (int r = 0; r < PM_RUN; r++) {
+            if (k[r] < 0) continue;
+            int len = 1;
+#pragma unroll
+            for (int q = r + 1; q < PM_RUN; q++) {
+                bool cont = true;
+#pragma unroll
+                for (int u = r + 1; u <= q; u++) cont = cont && k[u] == KEY_CONT;
+                len += cont ? 1 : 0;
+            }
+            runs[first[k[r]] + rank[p0 + r]] = (p0 + r) * PM_RUN + (len - 1);
+        }
+    }
+}
+
+// one workgroup per tile: the CIC weights of the particles of its runs
+// (DENSIT's arithmetic) accumulated in LDS over the tile's cells and their +1
+// reach, then one atomic per touched cell into rho (a plain add of the cells
+// no other tile reaches measured slower: it waits for the loaded value)
+__global__ __launch_bounds__(PM_BLOCK) void k_tile_deposit(const float* __restrict__ xs, const float* __restrict__ ys,
+                                                           const float* __restrict__ zs,
+                                                           const int64_t* __restrict__ runs,
+                                                           const int* __restrict__ count,
+                                                           const int64_t* __restrict__ first, int ntiles, int TT,
+                                                           double* __restrict__ rho, Geo g, double wpar)
+{
+    __shared__ double acc[PM_TCELLS];
+    const int N = g.n;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int c = count[t];
+        if (c == 0) continue;  // uniform over the block
+        for (int i = threadIdx.x; i < PM_TCELLS; i += blockDim.x) acc[i] = 0.0;
+        __syncthreads();
+        const int i00 = (t % TT) * PM_TILE, j00 = ((t / TT) % TT) * PM_TILE, k00 = (t / (TT * TT)) * PM_TILE;
+        const int64_t o = first[t];
+        // thread q: particle q % PM_RUN of run q / PM_RUN (idle past its length)
+        for (int64_t q = threadIdx.x; q < (int64_t)c * PM_RUN; q += blockDim.x) {
+            const int64_t d = runs[o + q / PM_RUN];
+            const int u = (int)(q % PM_RUN);
+            if (u > (int)(d % PM_RUN)) continue;
+            const int64_t pr = d / PM_RUN;
+            const double X = xs[pr + u], Y = ys[pr + u], Z = zs[pr + u];
+            const int I = (int)X, J = (int)Y, K = (int)Z;  // in [1, N]: checked when binned
+            const double D1 = X - (double)I, D2 = Y - (double)J, D3 = Z - (double)K;
+            const double T1 = 1.0 - D1, T2 = 1.0 - D2, T3 = 1.0 - D3;
+            const double T2W = T2 * wpar, D2W = D2 * wpar;
+            const int b = ((K - 1 - k00) * PM_TBUF + (J - 1 - j00)) * PM_TBUF + (I - 1 - i00);
+            atomicAdd(&acc[b], T3 * T1 * T2W);
+            atomicAdd(&acc[b + 1], T3 * D1 * T2W);
+            atomicAdd(&acc[b + PM_TBUF], T3 * T1 * D2W);
+            atomicAdd(&acc[b + PM_TBUF + 1], T3 * D1 * D2W);
+            atomicAdd(&acc[b + PM_TBUF * PM_TBUF], D3 * T1 * T2W);
+            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + 1], D3 * D1 * T2W);
+            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + PM_TBUF], D3 * T1 * D2W);
+            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + PM_TBUF + 1], D3 * D1 * D2W);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < PM_TCELLS; i += blockDim.x) {
+            const double v = acc[i];
+            if (v == 0.0) continue;
+            int gi = i00 + i % PM_TBUF, gj = j00 + (i / PM_TBUF) % PM_TBUF, gk = k00 + i / (PM_TBUF * PM_TBUF);
+            gi = gi >= N ? gi - N : gi;
+            gj = gj >= N ? gj - N : gj;
+            gk = gk >= N ? gk - N : gk;
+            // no-return atomics: the stores do not wait for the old values
+            unsafeAtomicAdd(&rho[split_index(g, gi, gj, gk)], v);
+        }
+        __syncthreads();
     }
 }
 
@@ -297,20 +469,70 @@ int mgs_particles_get(mgs_handle* h, float* x, float* y, float* z, float* vx, fl
     return 0;
 }
 
+// the deposit's binning buffers for the current particle count and ntiles
+static int pm_ensure_bins(mgs_handle* h, PmState* p, int ntiles)
+{
+    if (p->nbin != p->np) {
+        for (void* q : {(void*)p->key, (void*)p->rank, (void*)p->runs})
+            if (q) (void)hipFree(q);
+        p->key = p->rank = nullptr;
+        p->runs = nullptr;
+        p->nbin = 0;
+        const size_t n = (size_t)p->np;
+        if (hipMalloc((void**)&p->key, n * sizeof(int)) != hipSuccess ||
+            hipMalloc((void**)&p->rank, n * sizeof(int)) != hipSuccess ||
+            hipMalloc((void**)&p->runs, n * sizeof(int64_t)) != hipSuccess)
+            return set_err(h, MGS_E_NOMEM, "deposit binning arrays");
+        p->nbin = p->np;
+    }
+    if (p->ntiles != ntiles) {
+        for (void* q : {(void*)p->count, (void*)p->first, p->scan_tmp})
+            if (q) (void)hipFree(q);
+        p->count = nullptr;
+        p->first = nullptr;
+        p->scan_tmp = nullptr;
+        p->ntiles = 0;
+        size_t bytes = 0;
+        if (hipMalloc((void**)&p->count, (size_t)ntiles * sizeof(int)) != hipSuccess ||
+            hipMalloc((void**)&p->first, (size_t)ntiles * sizeof(int64_t)) != hipSuccess ||
+            hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, p->count, p->first, ntiles, h->s) != hipSuccess ||
+            hipMalloc(&p->scan_tmp, bytes + 256) != hipSuccess)
+            return set_err(h, MGS_E_NOMEM, "deposit tile arrays");
+        p->scan_bytes = bytes + 256;
+        p->ntiles = ntiles;
+    }
+    return 0;
+}
+
 int mgs_cic_deposit(mgs_handle* h)
 {
     if (!h) return MGS_E_ARG;
     PmState* p = pm_of(h);
     if (!p || !p->buf) return set_err(h, MGS_E_STATE, "no particles set");
+    if (p->np >= ((int64_t)1 << 31)) return set_err(h, MGS_E_ARG, "the deposit takes fewer than 2^31 particles");
     const int N = h->p.ngrid;
     const Geo g = make_geo(N);
-    prof_begin(h, MGS_KC_PM, (double)p->np * 12.0 + 8.0 * 2.0 * (double)N * N * N);
+    const int TT = (N + PM_TILE - 1) / PM_TILE;
+    int rc = pm_ensure_bins(h, p, TT * TT * TT);
+    if (rc) return rc;
+    const int64_t np = p->np;
+    const float* b = p->buf;
+    prof_begin(h, MGS_KC_PM, (double)np * 56.0 + 16.0 * (double)N * N * N);
     launch_fill_f64(h->L[0].rho, (size_t)N * N * N, -1.0, h->s);
-    if (hipMemsetAsync(p->bad, 0, sizeof(int), h->s) != hipSuccess) return set_err(h, MGS_E_HIP, "deposit");
-    hipLaunchKernelGGL(k_cic_deposit, dim3(pm_grid(p->np)), dim3(PM_BLOCK), 0, h->s, p->buf, p->buf + p->np,
-                       p->buf + 2 * p->np, p->np, h->L[0].rho, g, wpar_of(N, p->np), p->bad);
+    if (hipMemsetAsync(p->bad, 0, sizeof(int), h->s) != hipSuccess ||
+        hipMemsetAsync(p->count, 0, (size_t)p->ntiles * sizeof(int), h->s) != hipSuccess)
+        return set_err(h, MGS_E_HIP, "deposit");
+    const int gbin = pm_grid((np + PM_RUN - 1) / PM_RUN);
+    hipLaunchKernelGGL(k_bin_count, dim3(gbin), dim3(PM_BLOCK), 0, h->s, b, b + np, b + 2 * np, np, N, TT,
+                       p->key, p->rank, p->count, p->bad);
+    if (hipcub::DeviceScan::ExclusiveSum(p->scan_tmp, p->scan_bytes, p->count, p->first, p->ntiles, h->s) !=
+        hipSuccess)
+        return set_err(h, MGS_E_HIP, "deposit scan");
+    hipLaunchKernelGGL(k_bin_scatter, dim3(gbin), dim3(PM_BLOCK), 0, h->s, np, p->key, p->rank, p->first, p->runs);
+    hipLaunchKernelGGL(k_tile_deposit, dim3((unsigned)p->ntiles), dim3(PM_BLOCK), 0, h->s, b, b + np, b + 2 * np,
+                       p->runs, p->count, p->first, p->ntiles, TT, h->L[0].rho, g, wpar_of(N, np));
     prof_end(h);
-    int rc = pm_check_bad(h, "CIC deposit");
+    rc = pm_check_bad(h, "CIC deposit");
     h->has_density = true;
     h->rho_cached = false;
     return rc;

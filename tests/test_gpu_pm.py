@@ -31,6 +31,39 @@ def fr_cfg(N, prec=M.PREC_FP64):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("N", [20, 40])
+def test_binned_deposit_clustered_and_partial_tiles(N):
+    """The deposit bins particles by 8^3-cell tile: grids that are not a
+    multiple of 8 (a partial last tile on each axis), particles piled into a
+    few cells (long runs of one tile, LDS accumulation contended), positions
+    on both edges of [1, N+1) (the +1 reach wraps), and a shuffled order."""
+    rng = np.random.default_rng(N)
+    n = 30000
+    x, y, z, _ = particles(N, n, N + 1)
+    # a third in 3 cells, a third in one tile near the upper corner
+    k = n // 3
+    for a in (x, y, z):
+        a[3:k] = np.float32(rng.choice([1.25, 7.5, N + 0.75], k - 3))
+        a[k:2 * k] = np.float32(rng.uniform(N - 6.0, N + 1.0, k))
+        np.clip(a, 1.0, np.nextafter(np.float32(N + 1), np.float32(0)), out=a)
+    perm = rng.permutation(n)
+    x, y, z = x[perm], y[perm], z[perm]
+    print(f"[pm] binned deposit N={N} np={n}", flush=True)
+    with M.Solver(fr_cfg(N)) as S:
+        S.set_particles(x, y, z)
+        S.cic_deposit()
+        d = S.get_density(np.float64)
+        S.cic_deposit()  # reuses the binning buffers
+        d2 = S.get_density(np.float64)
+    d_o = P.cic_deposit(x, y, z, N)
+    w = P.wpar(N, n)
+    # sums of up to ~1e4 weights per cell in varying order
+    assert np.allclose(d, d_o, rtol=0, atol=1e-11 * w * max(1.0, np.max(d_o + 1.0) / w)), np.max(np.abs(d - d_o))
+    assert np.allclose(d2, d_o, rtol=0, atol=1e-11 * w * max(1.0, np.max(d_o + 1.0) / w))
+    assert abs(np.sum(d + 1.0) - N ** 3) <= 1e-9 * N ** 3
+
+
+@pytest.mark.gpu
 def test_deposit_and_potential_match_oracle():
     N = 32
     x, y, z, _ = particles(N, 40000, 7)
