@@ -212,7 +212,7 @@ int ensure_staging(mgs_handle* h, size_t bytes)
 {
     if (h->staging_bytes >= bytes) return 0;
     if (h->staging) {
-        hipFree(h->staging);
+        (void)hipFree(h->staging);
         h->staging = nullptr;
         h->staging_bytes = 0;
     }
@@ -225,36 +225,39 @@ int ensure_staging(mgs_handle* h, size_t bytes)
 // ---------------------------------------------------------------------------
 // profiling
 // ---------------------------------------------------------------------------
+// a launch whose events cannot be created or recorded goes unmeasured
 void prof_begin(mgs_handle* h, int kc, double bytes)
 {
     if (!h->prof) return;
     ProfRec r;
     r.kc = kc;
     r.bytes = bytes;
-    hipEventCreate(&r.a);
-    hipEventCreate(&r.b);
-    hipEventRecord(r.a, h->s);
+    r.a = r.b = nullptr;
+    r.ok = hipEventCreate(&r.a) == hipSuccess && hipEventCreate(&r.b) == hipSuccess &&
+           hipEventRecord(r.a, h->s) == hipSuccess;
     h->recs.push_back(r);
 }
 
 void prof_end(mgs_handle* h)
 {
-    if (!h->prof) return;
-    hipEventRecord(h->recs.back().b, h->s);
+    if (!h->prof || h->recs.empty()) return;
+    ProfRec& r = h->recs.back();
+    r.ok = r.ok && hipEventRecord(r.b, h->s) == hipSuccess;
 }
 
 static void prof_flush(mgs_handle* h)
 {
     if (h->recs.empty()) return;
-    hipStreamSynchronize(h->s);
+    const bool synced = hipStreamSynchronize(h->s) == hipSuccess;
     for (auto& r : h->recs) {
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, r.a, r.b);
-        h->pc[r.kc].launches += 1;
-        h->pc[r.kc].ms += ms;
-        h->pc[r.kc].bytes += r.bytes;
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        if (synced && r.ok && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            h->pc[r.kc].launches += 1;
+            h->pc[r.kc].ms += ms;
+            h->pc[r.kc].bytes += r.bytes;
+        }
+        if (r.a) (void)hipEventDestroy(r.a);
+        if (r.b) (void)hipEventDestroy(r.b);
     }
     h->recs.clear();
 }
@@ -1285,10 +1288,18 @@ int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* repor
     if (opts->cycle < 0 || opts->cycle > 2) return set_err(h, MGS_E_ARG, "cycle must be 0 (V), 1 (F) or 2 (W)");
     if (opts->cycle != 0 && h->p.precision == MGS_PREC_MIXED)
         return set_err(h, MGS_E_ARG, "the mixed-precision solve uses V-cycles (cycle = 0)");
-    hipEvent_t e0, e1;
-    HIPCHK(h, hipEventCreate(&e0));
-    HIPCHK(h, hipEventCreate(&e1));
-    HIPCHK(h, hipEventRecord(e0, h->s));
+    // the solve's wall time on the stream; destroyed on every return path
+    struct EventPair {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~EventPair()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    HIPCHK(h, hipEventCreate(&ev.a));
+    HIPCHK(h, hipEventCreate(&ev.b));
+    HIPCHK(h, hipEventRecord(ev.a, h->s));
     if (opts->init_guess) {
         double ubg = 0.0;
         fr_background(h->p, &ubg);
@@ -1299,12 +1310,10 @@ int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* repor
         rc = solve_mixed(h, *opts, report);
     else
         rc = solve_plain(h, *opts, report);
-    HIPCHK(h, hipEventRecord(e1, h->s));
-    HIPCHK(h, hipEventSynchronize(e1));
+    HIPCHK(h, hipEventRecord(ev.b, h->s));
+    HIPCHK(h, hipEventSynchronize(ev.b));
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (hipEventElapsedTime(&ms, ev.a, ev.b) != hipSuccess) ms = 0.0f;
     report->seconds = ms * 1e-3;
     if (rc) return rc;
     HIPCHK(h, hipGetLastError());

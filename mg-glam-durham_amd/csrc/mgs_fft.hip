@@ -67,10 +67,10 @@ void fft_release(mgs_handle* h)
     if (!f) return;
     if (f->have_fwd) hipfftDestroy(f->fwd);
     if (f->have_inv) hipfftDestroy(f->inv);
-    if (f->real) hipFree(f->real);
-    if (f->cplx) hipFree(f->cplx);
-    if (f->bins) hipFree(f->bins);
-    if (f->binpart) hipFree(f->binpart);
+    if (f->real) (void)hipFree(f->real);
+    if (f->cplx) (void)hipFree(f->cplx);
+    if (f->bins) (void)hipFree(f->bins);
+    if (f->binpart) (void)hipFree(f->binpart);
     delete f;
     h->fft = nullptr;
 }
@@ -390,7 +390,7 @@ int mgs_fifth_force(mgs_handle* h, void* psi, void* gx, void* gy, void* gz, int3
         } else {
             void* q = nullptr;
             if (hipMalloc(&q, n3 * es) != hipSuccess) {
-                for (void* t : tmp) hipFree(t);
+                for (void* t : tmp) (void)hipFree(t);
                 return set_err(h, MGS_E_NOMEM, "fifth-force staging");
             }
             tmp.push_back(q);
@@ -417,7 +417,7 @@ int mgs_fifth_force(mgs_handle* h, void* psi, void* gx, void* gy, void* gz, int3
             if (outs[a] && hipMemcpyAsync(outs[a], dev[a], n3 * es, hipMemcpyDeviceToHost, h->s) != hipSuccess)
                 rc = set_err(h, MGS_E_HIP, "fifth-force copy");
     if (hipStreamSynchronize(h->s) != hipSuccess) rc = set_err(h, MGS_E_HIP, "fifth force");
-    for (void* t : tmp) hipFree(t);
+    for (void* t : tmp) (void)hipFree(t);
     return rc;
 }
 

@@ -39,6 +39,7 @@ struct ProfRec {
     int kc;
     double bytes;
     hipEvent_t a, b;
+    bool ok;  // both events recorded
 };
 
 struct ProfClass {
