@@ -1,7 +1,10 @@
 """The headline workloads at their full sizes, checked through properties
 that do not depend on size (the oracle cannot run whole solves at 512^3 in
-seconds, but its FP64 operators can evaluate a 512^3 field):
+seconds, but its FP64 operators can evaluate a 512^3 field), and the
+reference's CPU case at 128^3, where the oracle's whole solve is quick:
 
+  * f(R) 128^3 (n=1, |fR0|=1e-5, sigma_g=1): device FP64 and mixed solves
+    against the oracle's FP64 solve;
   * f(R) 512^3 (n=1, |fR0|=1e-6, sigma_g=1.5, seed 1002): the mixed solve's
     field satisfies the oracle's FP64 operator to relative residual 1e-8,
     and agrees with the plain FP64 solve's field;
@@ -9,8 +12,8 @@ seconds, but its FP64 operators can evaluate a 512^3 field):
     solve reaches the FP64 solve's residual floor, measured by the oracle's
     operator, and the same field up to the free constant;
   * PM step at 512^3 with 512^3 particles: the deposit conserves mass
-    (sum of 1 + delta = N^3), the density is >= -1, the potential has no
-    mean mode, and the push keeps every particle inside [1, N+1).
+    (sum of 1 + delta = N^3), the density is >= -1, the potential with the
+    fifth force is finite, and the push keeps every particle inside [1, N+1).
 """
 import numpy as np
 import pytest
@@ -24,6 +27,37 @@ BOX = 512.0
 
 def rel_linf(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.gpu
+def test_config1_fr128_solves_match_the_oracle_solve():
+    """The reference's CPU case (128^3, box 256, n=1, |fR0|=1e-5, sigma_g=1,
+    FP64 V(2,2) to relative residual 1e-8): the oracle's whole solve against
+    the device's FP64 solve (same cycles, histories to 1e-6 per cycle, field
+    to 1e-9) and mixed solve (field to 1e-6)."""
+    n, box = 128, 256.0
+    op = O.Params(N=n, model=1, fr_n=1, branch=1, box=box, om=0.3, oml=0.7, a=1.0, fr0=-1e-5, h0rc=1.0, polish=1)
+    delta = O.lognormal_density(n, box, 1.0, seed=1001)
+    u_o, rep_o = O.solve(op, delta, mode=1, rel_tol=1e-8, max_cycles=60, prec="f64")
+    print(f"[fullsize] config 1: oracle {rep_o['cycles']} cycles, rel. residual "
+          f"{rep_o['res_final'] / rep_o['res0']:.3e}", flush=True)
+    for prec in (M.PREC_FP64, M.PREC_MIXED):
+        c = M.Config(N=n, model=M.MODEL_FR, fr_n=1, box=box, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=-1e-5,
+                     precision=prec)
+        with M.Solver(c) as S:
+            S.set_density(delta)
+            rep = S.solve(mode=1, rel_tol=1e-8, max_cycles=60)
+            u = S.get_field()
+        err = rel_linf(u, u_o)
+        print(f"  precision={prec}: rel. residual {rep['res_final'] / rep['res0']:.3e}, field vs oracle {err:.2e}",
+              flush=True)
+        assert rep["converged"]
+        if prec == M.PREC_FP64:
+            assert rep["cycles"] == rep_o["cycles"]
+            assert np.allclose(rep["history"], rep_o["history"], rtol=1e-6, atol=0)
+            assert err <= 1e-9
+        else:
+            assert err <= 1e-6
 
 
 @pytest.mark.gpu
@@ -122,8 +156,6 @@ def test_pm512_step_invariants():
         S.potential_add_fifth_force()
         phi = S.get_potential(np.float32)
         assert np.all(np.isfinite(phi))
-        # the Poisson solve sets the mean mode to zero; the fifth force's mean
-        # is what it adds
         print(f"  potential mean {float(np.mean(phi, dtype=np.float64)):.3e}, max |phi| "
               f"{float(np.max(np.abs(phi))):.3e}", flush=True)
         del phi
