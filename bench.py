@@ -257,6 +257,8 @@ def serve_run(tag, cfgs, kws, dens, max_active, rng):
     keep = {0, 1, n_req // 2, n_req - 1}
     with M.Server(max_active=max_active) as srv:
         tickets = [None] * n_req
+        t_sub = [0.0] * n_req
+        t_got = [0.0] * n_req
 
         def producer():
             start = time.perf_counter()
@@ -266,6 +268,7 @@ def serve_run(tag, cfgs, kws, dens, max_active, rng):
                 dt = start + at - time.perf_counter()
                 if dt > 0:
                     time.sleep(dt)
+                t_sub[i] = time.perf_counter()
                 tickets[i] = srv.submit(cfgs[i], dens[i], **kws[i])
 
         th = threading.Thread(target=producer)
@@ -275,6 +278,7 @@ def serve_run(tag, cfgs, kws, dens, max_active, rng):
             while tickets[i] is None:
                 time.sleep(0.0005)
             u, o = srv.wait(tickets[i], cfgs[i].N, np.float32)
+            t_got[i] = time.perf_counter()
             if i in keep:
                 fields[i] = u
             if not ok(o, kws[i]):
@@ -282,7 +286,11 @@ def serve_run(tag, cfgs, kws, dens, max_active, rng):
         th.join()
         pois_s = time.perf_counter() - tp
         st = srv.stats()
-    log(f"[{tag}] Poisson {rate:.1f}/s: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms, "
+    # delivered: submit call -> the field in the caller's array (wait() copies
+    # it out of HBM; the client claims results in submission order)
+    dl = 1e3 * (np.array(t_got) - np.array(t_sub))
+    log(f"[{tag}] Poisson {rate:.1f}/s: ready in HBM p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms; "
+        f"delivered p50 {np.percentile(dl, 50):.1f} ms p99 {np.percentile(dl, 99):.1f} ms; "
         f"{st['solves_per_s']:.1f} solves/s, batch <= {st['max_batch']}")
     diff = 0.0
     with M.Server(max_active=1) as srv:
@@ -297,6 +305,9 @@ def serve_run(tag, cfgs, kws, dens, max_active, rng):
         "p50_ms": st["p50_ms"],
         "p99_ms": st["p99_ms"],
         "mean_ms": st["mean_ms"],
+        "latency_is": "submit -> result ready in HBM (server clock)",
+        "delivered_p50_ms": float(np.percentile(dl, 50)),
+        "delivered_p99_ms": float(np.percentile(dl, 99)),
         "solves_per_s": st["solves_per_s"],
         "wall_s": pois_s,
         "max_batch": st["max_batch"],
@@ -501,8 +512,9 @@ def main():
             "N": N,
             "inner_cycles_per_outer": args.inner if args.inner > 0 else "auto",
         },
-        "latency_ms": ({"p50": serving["p50_ms"], "p99": serving["p99_ms"],
-                        "source": "serving stream (Poisson arrivals), submit -> result"} if serving else None),
+        "latency_ms": ({"p50": serving["delivered_p50_ms"], "p99": serving["delivered_p99_ms"],
+                        "source": "serving stream (Poisson arrivals), submit -> field in the caller's array"}
+                       if serving else None),
         "solve_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "serving": serving,
         "serving_dgp_fr_mix": serving_mixed,

@@ -304,6 +304,9 @@ typedef struct mgs_server_stats {
     double solves_per_s;     /* completed / (last completion - first submission) */
     int32_t max_batch;       /* most requests advanced by one batched step */
     int32_t reserved;
+    double step_ms;          /* worker time in batched steps (launch to results on the host) */
+    double admit_ms;         /* worker time admitting requests (density upload included) */
+    double finish_ms;        /* worker time handing results back (field download included) */
 } mgs_server_stats;
 
 int mgs_server_create(const mgs_server_opts* opts, mgs_server** out);

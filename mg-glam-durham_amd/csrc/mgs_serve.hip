@@ -26,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -33,6 +34,8 @@
 #include <string>
 #include <thread>
 #include <vector>
+
+#include <sys/mman.h>
 
 #include "../../include/mgsolve.h"
 #include "mgs_dgp_dev.h"
@@ -546,7 +549,8 @@ static size_t layout(Hier& h, int N, int model, char* base)
 struct Request {
     int64_t ticket = 0;
     mgs_params p{};
-    std::vector<char> delta;
+    char* delta = nullptr;  // host staging (take_stage), until admission
+    size_t delta_bytes = 0;
     int32_t dtype = MGS_F32;
     int32_t max_outer = 60;
     double rel_tol = 1e-8;
@@ -555,7 +559,13 @@ struct Request {
     Clock::time_point t_submit, t_admit, t_done;
     bool done = false;
     mgs_request_result res{};
-    std::vector<double> u;  // raw field, reference order
+    // The result stays in HBM, in the request's hierarchy (hold), until
+    // mgs_server_wait copies it out; when admission needs the memory first,
+    // the raw field (reference order) is spilled to u.
+    Hier* hold = nullptr;
+    bool spilling = false;
+    std::vector<double> u;
+    ~Request() { std::free(delta); }
 };
 
 struct Slot {
@@ -592,9 +602,17 @@ struct mgs_server {
     double *dsum = nullptr, *hsum = nullptr;
     std::map<int, hipGraphExec_t> graphs;  // by the batch's f(R) and nDGP depths
     bool desc_dirty = true;
+    // done requests whose result is held in HBM, oldest first, and the
+    // hierarchies waiters are copying out (both under mu)
+    std::deque<Request*> held;
+    int claimed = 0;
+    // idle host staging buffers by size (under mu)
+    std::map<size_t, std::vector<char*>> stages;
+    size_t stage_idle = 0;
     // statistics
     std::vector<double> lat;
     int64_t submitted = 0, rejected = 0, steps = 0;
+    double step_ms = 0.0, admit_ms = 0.0, finish_ms = 0.0;  // worker time
     int max_batch = 0;
     bool any_submit = false;
     Clock::time_point first_submit, last_done;
@@ -602,6 +620,59 @@ struct mgs_server {
 };
 
 static thread_local std::string g_server_create_error;
+
+// no NaN or Inf among n values: a branch-free reduction the compiler
+// vectorizes (|v| <= max is false for both)
+static bool all_finite(const void* a, int32_t dtype, size_t n)
+{
+    unsigned bad = 0;
+    if (dtype == MGS_F32) {
+        const float* v = (const float*)a;
+        for (size_t q = 0; q < n; q++) bad |= !(std::fabs(v[q]) <= 3.4028234663852886e38f);
+    } else {
+        const double* v = (const double*)a;
+        for (size_t q = 0; q < n; q++) bad |= !(std::fabs(v[q]) <= 1.7976931348623157e308);
+    }
+    return bad == 0;
+}
+
+// Host staging for submitted densities, recycled by size so that steady
+// serving copies into resident pages (a fresh buffer pays a page fault per
+// 4 KiB on first touch): 2 MiB-aligned with transparent huge pages asked for.
+constexpr size_t STAGE_IDLE_MAX = size_t(4) << 30;
+
+static char* take_stage(mgs_server* S, size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->stages.find(bytes);
+        if (it != S->stages.end() && !it->second.empty()) {
+            char* p = it->second.back();
+            it->second.pop_back();
+            S->stage_idle -= bytes;
+            return p;
+        }
+    }
+    const size_t huge = size_t(2) << 20;
+    const size_t len = (bytes + huge - 1) / huge * huge;
+    char* p = (char*)std::aligned_alloc(huge, len);
+    if (p) (void)madvise(p, len, MADV_HUGEPAGE);
+    return p;
+}
+
+static void give_stage(mgs_server* S, char* p, size_t bytes)
+{
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stage_idle + bytes <= STAGE_IDLE_MAX) {
+            S->stages[bytes].push_back(p);
+            S->stage_idle += bytes;
+            return;
+        }
+    }
+    std::free(p);
+}
 
 static int srv_err(mgs_server* S, int code, const std::string& msg)
 {
@@ -615,28 +686,42 @@ static int srv_err(mgs_server* S, int code, const std::string& msg)
 namespace {
 
 // free idle pooled hierarchies (other sizes first) until `need` more bytes fit
+// free idle pooled hierarchies (other sizes first) until `need` more bytes
+// fit.  The pool is shared with the waiters (under mu); S->used is the
+// worker's.
 void trim_pool(mgs_server* S, size_t need)
 {
-    for (auto it = S->pool.begin(); it != S->pool.end() && S->used + need > S->budget;) {
-        auto& v = it->second;
-        while (!v.empty() && S->used + need > S->budget) {
-            Hier* h = v.back();
-            v.pop_back();
-            (void)hipFree(h->base);
-            S->used -= h->bytes;
-            delete h;
+    std::vector<Hier*> victims;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        size_t freed = 0;
+        for (auto it = S->pool.begin(); it != S->pool.end() && S->used - freed + need > S->budget;) {
+            auto& v = it->second;
+            while (!v.empty() && S->used - freed + need > S->budget) {
+                victims.push_back(v.back());
+                freed += v.back()->bytes;
+                v.pop_back();
+            }
+            it = v.empty() ? S->pool.erase(it) : std::next(it);
         }
-        it = v.empty() ? S->pool.erase(it) : std::next(it);
+    }
+    for (Hier* h : victims) {
+        (void)hipFree(h->base);
+        S->used -= h->bytes;
+        delete h;
     }
 }
 
 Hier* take_hier(mgs_server* S, int N, int model)
 {
-    auto it = S->pool.find(pool_key(N, model));
-    if (it != S->pool.end() && !it->second.empty()) {
-        Hier* h = it->second.back();
-        it->second.pop_back();
-        return h;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->pool.find(pool_key(N, model));
+        if (it != S->pool.end() && !it->second.empty()) {
+            Hier* h = it->second.back();
+            it->second.pop_back();
+            return h;
+        }
     }
     Hier probe;
     const size_t bytes = layout(probe, N, model, nullptr);
@@ -710,11 +795,11 @@ int start_request(mgs_server* S, Slot& sl)
     const int N = r->p.ngrid;
     const size_t n3 = (size_t)N * N * N;
     if (r->dtype == MGS_F32) {
-        if (hipMemcpyAsync(h->d[0], r->delta.data(), n3 * 4, hipMemcpyHostToDevice, S->s) != hipSuccess)
+        if (hipMemcpyAsync(h->d[0], r->delta, n3 * 4, hipMemcpyHostToDevice, S->s) != hipSuccess)
             return MGS_E_HIP;
         launch_pack_f64_from_f32(h->d[0], h->rho, N, S->s);
     } else {
-        if (hipMemcpyAsync(h->u, r->delta.data(), n3 * 8, hipMemcpyHostToDevice, S->s) != hipSuccess)
+        if (hipMemcpyAsync(h->u, r->delta, n3 * 8, hipMemcpyHostToDevice, S->s) != hipSuccess)
             return MGS_E_HIP;
         launch_pack_f64_from_f64(h->u, h->rho, N, S->s);
     }
@@ -726,25 +811,15 @@ int start_request(mgs_server* S, Slot& sl)
     }
     launch_fill_f64(h->u, n3, ubg, S->s);
     if (hipStreamSynchronize(S->s) != hipSuccess) return MGS_E_HIP;
-    std::vector<char>().swap(r->delta);
+    give_stage(S, r->delta, r->delta_bytes);
+    r->delta = nullptr;
     return 0;
 }
 
-// copy the converged field out and hand the hierarchy back to the pool
-int finish_request(mgs_server* S, Slot& sl, int status)
+// the request is done: record it; a failed request's hierarchy goes back to
+// the pool, a solved one's stays with the request (its result)
+void complete_request(mgs_server* S, Request* r, Hier* h, int rc)
 {
-    Request* r = sl.r;
-    Hier* h = sl.h;
-    int rc = status;
-    if (!rc) {
-        const int N = r->p.ngrid;
-        const size_t n3 = (size_t)N * N * N;
-        r->u.resize(n3);
-        launch_unpack_f64_to_f64(h->u, h->rho, N, 1.0, S->s);
-        if (hipMemcpyAsync(r->u.data(), h->rho, n3 * 8, hipMemcpyDeviceToHost, S->s) != hipSuccess ||
-            hipStreamSynchronize(S->s) != hipSuccess)
-            rc = MGS_E_HIP;
-    }
     {
         std::lock_guard<std::mutex> lk(S->mu);
         r->t_done = Clock::now();
@@ -752,16 +827,60 @@ int finish_request(mgs_server* S, Slot& sl, int status)
         r->res.queued_ms = ms_between(r->t_submit, r->t_admit);
         r->res.latency_ms = ms_between(r->t_submit, r->t_done);
         r->done = true;
-        S->pool[pool_key(h->N, h->model)].push_back(h);
-        if (!rc) {
+        if (rc) {
+            S->pool[pool_key(h->N, h->model)].push_back(h);
+        } else {
+            r->hold = h;
+            S->held.push_back(r);
             S->lat.push_back(r->res.latency_ms);
             S->last_done = r->t_done;
         }
     }
+    S->cv_done.notify_all();
+}
+
+// Hand a finished request back: its slot is free at once, and its field
+// (complete: every step is synchronized before the stopping rule runs)
+// stays in HBM until mgs_server_wait copies it out.
+int finish_request(mgs_server* S, Slot& sl, int status)
+{
+    Request* r = sl.r;
+    Hier* h = sl.h;
     sl = Slot();
     S->desc_dirty = true;
+    complete_request(S, r, h, status);
+    return status;
+}
+
+// Admission needs memory that unclaimed results hold: copy the oldest one's
+// field to the host and pool its hierarchy.  False when none is held.
+bool spill_oldest(mgs_server* S)
+{
+    Request* r = nullptr;
+    Hier* h = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->held.empty()) return false;
+        r = S->held.front();
+        S->held.pop_front();
+        h = r->hold;
+        r->hold = nullptr;
+        r->spilling = true;
+    }
+    const int N = h->N;
+    const size_t n3 = (size_t)N * N * N;
+    r->u.resize(n3);
+    launch_unpack_f64_to_f64(h->u, h->rho, N, 1.0, S->s);
+    const bool ok = hipMemcpyAsync(r->u.data(), h->rho, n3 * 8, hipMemcpyDeviceToHost, S->s) == hipSuccess &&
+                    hipStreamSynchronize(S->s) == hipSuccess;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!ok) r->res.status = MGS_E_HIP;
+        r->spilling = false;
+        S->pool[pool_key(h->N, h->model)].push_back(h);
+    }
     S->cv_done.notify_all();
-    return rc;
+    return true;
 }
 
 // move waiting requests into free slots, FIFO, while their hierarchies fit
@@ -785,8 +904,14 @@ int admit(mgs_server* S)
         }
         if (free_slot < 0) break;
         Hier* h = take_hier(S, r->p.ngrid, r->p.model);
+        while (!h && spill_oldest(S)) h = take_hier(S, r->p.ngrid, r->p.model);
+        bool claimed = false;
         if (!h) {
-            if (active == 0) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            claimed = S->claimed > 0;
+        }
+        if (!h) {
+            if (active == 0 && !claimed) {
                 // it cannot fit even alone: fail it rather than wait forever
                 std::lock_guard<std::mutex> lk(S->mu);
                 S->waiting.pop_front();
@@ -963,22 +1088,21 @@ void worker_main(mgs_server* S)
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            bool any_active = false;
-            for (int i = 0; i < S->o.max_active; i++) any_active = any_active || S->slots[i].r;
             S->cv_work.wait(lk, [&] {
                 bool act = false;
                 for (int i = 0; i < S->o.max_active; i++) act = act || S->slots[i].r;
                 return S->stop || act || !S->waiting.empty();
             });
             if (S->stop) break;
-            (void)any_active;
         }
+        const Clock::time_point ta = Clock::now();
         admit(S);
+        S->admit_ms += ms_between(ta, Clock::now());
         int nact = 0;
         for (int i = 0; i < S->o.max_active; i++) nact += S->slots[i].r ? 1 : 0;
         if (!nact) {
-            // the head of the queue is waiting for memory held by nobody: it
-            // was failed by admit(); sleep until new work arrives
+            // the head of the queue waits for a hierarchy a waiter is copying
+            // out (or was failed by admit()): sleep until one is returned
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_work.wait_for(lk, std::chrono::milliseconds(5));
             continue;
@@ -986,6 +1110,7 @@ void worker_main(mgs_server* S)
         // descriptors and plan for this batch
         for (int i = 0; i < S->o.max_active; i++)
             if (S->slots[i].r && S->desc_dirty) fill_desc(S->hS[i], *S->slots[i].h, S->slots[i].r->p);
+        const Clock::time_point ts = Clock::now();
         int dfr = -1, ddgp = -1;
         build_plan(S, &dfr, &ddgp);
         int rc = 0;
@@ -999,6 +1124,8 @@ void worker_main(mgs_server* S)
             rc = MGS_E_HIP;
         S->steps++;
         S->max_batch = std::max(S->max_batch, nact);
+        const Clock::time_point tf = Clock::now();
+        S->step_ms += ms_between(ts, tf);
         // per request: the stopping rule of solve_mixed (mode 1 or 0)
         for (int i = 0; i < S->o.max_active; i++) {
             Slot& sl = S->slots[i];
@@ -1040,7 +1167,31 @@ void worker_main(mgs_server* S)
             }
             if (stop) finish_request(S, sl, 0);
         }
+        S->finish_ms += ms_between(tf, Clock::now());
     }
+}
+
+// a result held in HBM into the caller's host array: unpacked into the
+// reference order in the requested type and scale (into the hierarchy's
+// level-0 scratch) and copied, on a stream of the calling thread
+int copy_out(mgs_server* S, Hier* h, void* u, int32_t dtype, double scale)
+{
+    if (hipSetDevice(S->device) != hipSuccess) return MGS_E_HIP;
+    hipStream_t ws = nullptr;
+    if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) return MGS_E_HIP;
+    const int N = h->N;
+    const size_t n3 = (size_t)N * N * N;
+    hipError_t e;
+    if (dtype == MGS_F32) {
+        launch_unpack_f64_to_f32(h->u, h->d[0], N, scale, ws);
+        e = hipMemcpyAsync(u, h->d[0], n3 * sizeof(float), hipMemcpyDeviceToHost, ws);
+    } else {
+        launch_unpack_f64_to_f64(h->u, h->rho, N, scale, ws);
+        e = hipMemcpyAsync(u, h->rho, n3 * sizeof(double), hipMemcpyDeviceToHost, ws);
+    }
+    const hipError_t es = hipStreamSynchronize(ws);
+    (void)hipStreamDestroy(ws);
+    return e == hipSuccess && es == hipSuccess ? 0 : MGS_E_HIP;
 }
 
 }  // namespace
@@ -1093,6 +1244,7 @@ int mgs_server_destroy(mgs_server* S)
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
     if (S->s) (void)hipStreamSynchronize(S->s);
+    S->held.clear();  // their hierarchies are freed with the requests below
     for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
     for (int i = 0; i < MAXS; i++)
         if (S->slots[i].h) {
@@ -1104,7 +1256,15 @@ int mgs_server_destroy(mgs_server* S)
             (void)hipFree(h->base);
             delete h;
         }
-    for (auto& kv : S->reqs) delete kv.second;
+    for (auto& kv : S->reqs) {
+        if (kv.second->hold) {
+            (void)hipFree(kv.second->hold->base);
+            delete kv.second->hold;
+        }
+        delete kv.second;
+    }
+    for (auto& kv : S->stages)
+        for (char* p : kv.second) std::free(p);
     if (S->dS) (void)hipFree(S->dS);
     if (S->dP) (void)hipFree(S->dP);
     if (S->dsum) (void)hipFree(S->dsum);
@@ -1148,10 +1308,13 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
         return srv_err(S, MGS_E_NOMEM, "request larger than the server's HBM budget");
     const size_t n3 = (size_t)p.ngrid * p.ngrid * p.ngrid;
     const size_t es = req->dtype == MGS_F32 ? 4 : 8;
-    for (size_t q = 0; q < n3; q++) {
-        const double v = req->dtype == MGS_F32 ? (double)((const float*)req->delta)[q] : ((const double*)req->delta)[q];
-        if (!std::isfinite(v))
-            return srv_err(S, MGS_E_DOMAIN, "density contrast must be finite (cell " + std::to_string(q) + ")");
+    if (!all_finite(req->delta, req->dtype, n3)) {
+        for (size_t q = 0; q < n3; q++) {
+            const double v =
+                req->dtype == MGS_F32 ? (double)((const float*)req->delta)[q] : ((const double*)req->delta)[q];
+            if (!std::isfinite(v))
+                return srv_err(S, MGS_E_DOMAIN, "density contrast must be finite (cell " + std::to_string(q) + ")");
+        }
     }
     Request* r = new Request();
     r->p = p;
@@ -1160,7 +1323,13 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
     r->rel_tol = req->rel_tol;
     r->exit_rule = req->exit_rule;
     r->res_conv = req->res_conv;
-    r->delta.assign((const char*)req->delta, (const char*)req->delta + n3 * es);
+    r->delta_bytes = n3 * es;
+    r->delta = take_stage(S, r->delta_bytes);
+    if (!r->delta) {
+        delete r;
+        return srv_err(S, MGS_E_NOMEM, "host staging buffer");
+    }
+    std::memcpy(r->delta, req->delta, r->delta_bytes);
     {
         std::lock_guard<std::mutex> lk(S->mu);
         if ((int)S->waiting.size() >= S->o.queue_capacity) {
@@ -1189,17 +1358,31 @@ int mgs_server_wait(mgs_server* S, int64_t ticket, void* u, int32_t dtype, doubl
     if (!S) return MGS_E_ARG;
     if (u && dtype != MGS_F32 && dtype != MGS_F64) return srv_err(S, MGS_E_ARG, "dtype");
     Request* r = nullptr;
+    Hier* h = nullptr;
     {
         std::unique_lock<std::mutex> lk(S->mu);
         auto it = S->reqs.find(ticket);
         if (it == S->reqs.end()) return srv_err(S, MGS_E_ARG, "unknown ticket");
         r = it->second;
-        S->cv_done.wait(lk, [&] { return r->done; });
+        S->cv_done.wait(lk, [&] { return r->done && !r->spilling; });
         S->reqs.erase(it);
+        if (r->hold) {  // the result is still in HBM: claim it
+            h = r->hold;
+            r->hold = nullptr;
+            S->held.erase(std::find(S->held.begin(), S->held.end(), r));
+            S->claimed++;
+        }
     }
     int rc = r->res.status;
-    if (res) *res = r->res;
-    if (!rc && u) {
+    if (h) {
+        if (!rc && u) rc = copy_out(S, h, u, dtype, scale);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->pool[pool_key(h->N, h->model)].push_back(h);
+            S->claimed--;
+        }
+        S->cv_work.notify_all();  // its memory is free for admission
+    } else if (!rc && u) {
         const size_t n3 = r->u.size();
         for (size_t q = 0; q < n3; q++) {
             const double v = scale == 1.0 ? r->u[q] : r->u[q] * scale;
@@ -1208,6 +1391,10 @@ int mgs_server_wait(mgs_server* S, int64_t ticket, void* u, int32_t dtype, doubl
             else
                 ((double*)u)[q] = v;
         }
+    }
+    if (res) {
+        *res = r->res;
+        res->status = rc;
     }
     if (rc) S->err = "request failed (code " + std::to_string(rc) + ")";
     delete r;
@@ -1224,6 +1411,9 @@ int mgs_server_get_stats(mgs_server* S, mgs_server_stats* st)
     st->rejected = S->rejected;
     st->steps = S->steps;
     st->max_batch = S->max_batch;
+    st->step_ms = S->step_ms;
+    st->admit_ms = S->admit_ms;
+    st->finish_ms = S->finish_ms;
     if (!S->lat.empty()) {
         std::vector<double> v = S->lat;
         std::sort(v.begin(), v.end());
