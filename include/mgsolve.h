@@ -300,13 +300,14 @@ typedef struct mgs_request_result {
 typedef struct mgs_server_stats {
     int64_t submitted, completed, rejected;
     int64_t steps;           /* batched outer-iteration steps executed */
-    double p50_ms, p99_ms, mean_ms;   /* latency submit -> result of completed requests */
+    double p50_ms, p99_ms, mean_ms;   /* latency submit -> result ready in HBM, completed requests */
     double solves_per_s;     /* completed / (last completion - first submission) */
     int32_t max_batch;       /* most requests advanced by one batched step */
     int32_t reserved;
     double step_ms;          /* worker time in batched steps (launch to results on the host) */
     double admit_ms;         /* worker time admitting requests (density upload included) */
-    double finish_ms;        /* worker time handing results back (field download included) */
+    double finish_ms;        /* worker time applying the stopping rule and freeing slots */
+    int64_t spilled;         /* unclaimed results copied to the host to admit others */
 } mgs_server_stats;
 
 int mgs_server_create(const mgs_server_opts* opts, mgs_server** out);
@@ -316,7 +317,9 @@ const char* mgs_server_last_error(const mgs_server* srv);   /* also valid for sr
 int mgs_server_submit(mgs_server* srv, const mgs_request* req, int64_t* ticket);
 /* block until the request is done; copy its field u (raw, x scale) to the
  * host array `u` (may be NULL) and its report to `res` (may be NULL); a
- * ticket can be waited for once */
+ * ticket can be waited for once.  A finished request's field stays in HBM
+ * (counted against hbm_budget) until this call copies it out, or until
+ * admission needs the memory and copies it to the host (stats.spilled). */
 int mgs_server_wait(mgs_server* srv, int64_t ticket, void* u, int32_t dtype, double scale, mgs_request_result* res);
 int mgs_server_get_stats(mgs_server* srv, mgs_server_stats* st);
 

@@ -611,7 +611,7 @@ struct mgs_server {
     size_t stage_idle = 0;
     // statistics
     std::vector<double> lat;
-    int64_t submitted = 0, rejected = 0, steps = 0;
+    int64_t submitted = 0, rejected = 0, steps = 0, spilled = 0;
     double step_ms = 0.0, admit_ms = 0.0, finish_ms = 0.0;  // worker time
     int max_batch = 0;
     bool any_submit = false;
@@ -877,6 +877,7 @@ bool spill_oldest(mgs_server* S)
         std::lock_guard<std::mutex> lk(S->mu);
         if (!ok) r->res.status = MGS_E_HIP;
         r->spilling = false;
+        S->spilled++;
         S->pool[pool_key(h->N, h->model)].push_back(h);
     }
     S->cv_done.notify_all();
@@ -1414,6 +1415,7 @@ int mgs_server_get_stats(mgs_server* S, mgs_server_stats* st)
     st->step_ms = S->step_ms;
     st->admit_ms = S->admit_ms;
     st->finish_ms = S->finish_ms;
+    st->spilled = S->spilled;
     if (!S->lat.empty()) {
         std::vector<double> v = S->lat;
         std::sort(v.begin(), v.end());

@@ -94,7 +94,7 @@ class ServerStats(C.Structure):
         ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64), ("steps", C.c_int64),
         ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("solves_per_s", C.c_double),
         ("max_batch", C.c_int32), ("reserved", C.c_int32), ("step_ms", C.c_double), ("admit_ms", C.c_double),
-        ("finish_ms", C.c_double),
+        ("finish_ms", C.c_double), ("spilled", C.c_int64),
     ]
 
 

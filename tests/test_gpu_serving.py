@@ -73,7 +73,13 @@ def test_results_equal_alone_in_mix_and_standalone():
 @pytest.mark.parametrize("use_graphs", [True, False])
 def test_admission_by_footprint_and_graph_replay(use_graphs):
     """A budget that holds one 32^3 hierarchy: requests queue and are
-    admitted one after another; results do not depend on graph replay."""
+    admitted one after another; results do not depend on graph replay.
+    Results wait in HBM until claimed, so admitting the next request spills
+    the unclaimed one to the host.  Waiting for the last request first
+    makes the path of each certain: request 2 is claimed from HBM, requests
+    0 and 1 (spilled to admit 1 and 2) are read from their host copies; as
+    FP64 or FP32 and scaled, every path gives the field the unconstrained
+    server copies out of HBM."""
     N = 32
     c = cfg(N)
     ds = [density(N, 64.0, 1.0, 40 + i, f32=True) for i in range(3)]
@@ -81,14 +87,17 @@ def test_admission_by_footprint_and_graph_replay(use_graphs):
     with M.Server(max_active=8, use_graphs=True) as big:
         ref = [big.wait(big.submit(c, d), N)[0] for d in ds]
     budget = 40 * N ** 3  # one hierarchy (about 35 bytes per cell), not two
+    reads = [(np.float64, 1.0), (np.float32, 1.0), (np.float32, 0.25)]
     with M.Server(max_active=8, hbm_budget=budget, use_graphs=use_graphs) as srv:
         ts = [srv.submit(c, d) for d in ds]
-        out = [srv.wait(t, N) for t in ts]
+        last = srv.wait(ts[2], N, *reads[2])
+        out = [srv.wait(t, N, dt, sc) for t, (dt, sc) in zip(ts[:2], reads[:2])] + [last]
         st = srv.stats()
-    assert st["max_batch"] == 1, st
-    for (u, r), u_ref in zip(out, ref):
+    assert st["max_batch"] == 1 and st["spilled"] == 2, st
+    for (u, r), u_ref, (dt, sc) in zip(out, ref, reads):
         assert r["status"] == 0 and r["converged"]
-        assert np.array_equal(u, u_ref)
+        assert u.dtype == dt
+        assert np.array_equal(u, (u_ref * sc).astype(dt))
     # requests queued behind the first waited for admission
     assert out[2][1]["queued_ms"] >= out[0][1]["queued_ms"]
 
