@@ -119,66 +119,109 @@ __global__ __launch_bounds__(256) void k_dgp_residual64(const double* __restrict
     store_partial(partials, NB, block_sum(acc));
 }
 
-// restrict_residual_DGP(1), PMP2MGsolver_DGP.f90:459-763 (rearranged Sigma)
+// restrict_residual_DGP(1), PMP2MGsolver_DGP.f90:459-763 (rearranged Sigma).
+// A coarse cell (I, J, K) reads the 4x4x4 block of fine u with corner
+// (2I-1, 2J-1, 2K-1): Sigma of each of its 8 children takes the 18
+// neighbours around it, and the 8 children and their 24 outer face
+// neighbours make P.  A thread owns coarse column (I, J) and marches over a
+// run of K, holding one 4x4 in-plane patch per fine plane: the patches of
+// fine planes 2K+1 and 2K+2 are planes 2K'-1 and 2K' of K' = K+1, so each
+// step loads two fine planes (32 values; the x-neighbour columns hit the
+// cache lines of the adjacent lanes) and the children's densities.  Every
+// sum keeps the reference's operation order.
+struct DPatch {
+    double v[4][4];  // [y][x]: fine x = 2I-1+xx, y = 2J-1+yy (periodic)
+};
+
+__device__ __forceinline__ double dgp_sigma1(const DPatch& A, const DPatch& B, const DPatch& C, int yy, int xx,
+                                             double dens, const DgpCoef& c)
+{
+    const double ONEOVEREIGHT = 1.0 / 8.0, ONEOVERTHREE = 1.0 / 3.0;
+    const double dx4 = 1.0;
+    const double Sx = B.v[yy][xx + 1] + B.v[yy][xx - 1];
+    const double Sy = B.v[yy + 1][xx] + B.v[yy - 1][xx];
+    const double Sz = C.v[yy][xx] + A.v[yy][xx];
+    const double Dxy = (B.v[yy + 1][xx + 1] + B.v[yy - 1][xx - 1]) - (B.v[yy - 1][xx + 1] + B.v[yy + 1][xx - 1]);
+    const double Dxz = (C.v[yy][xx + 1] + A.v[yy][xx - 1]) - (A.v[yy][xx + 1] + C.v[yy][xx - 1]);
+    const double Dyz = (C.v[yy + 1][xx] + A.v[yy - 1][xx]) - (A.v[yy + 1][xx] + C.v[yy - 1][xx]);
+    double s = c.fct1 * dens;
+    s = s + Sx * (2.0 * Sx - Sy - Sz) * ONEOVERTHREE / dx4 + Sy * (2.0 * Sy - Sx - Sz) * ONEOVERTHREE / dx4 +
+        Sz * (2.0 * Sz - Sx - Sy) * ONEOVERTHREE / dx4 + (Dxy * Dxy) * ONEOVEREIGHT / dx4 +
+        (Dxz * Dxz) * ONEOVEREIGHT / dx4 + (Dyz * Dyz) * ONEOVEREIGHT / dx4;
+    return s;
+}
+
 __global__ __launch_bounds__(256) void k_dgp_restrict_residual1_64(const double* __restrict__ u0,
                                                                    const double* __restrict__ delta,
                                                                    double* __restrict__ rres1, Geo gf, Geo gc,
-                                                                   DgpCoef c, int TX, int TY, int XB, unsigned NB)
+                                                                   DgpCoef c, int TX, int TY, int XB, int JB, int KC,
+                                                                   unsigned NB)
 {
-    int I;
-    unsigned row;
-    if (!map_xy(gc.n, (long)gc.n * gc.n, TX, TY, XB, NB, I, row)) return;
-    const int J = (int)(row % (unsigned)gc.n), K = (int)(row / (unsigned)gc.n);
-    const int N = gf.n;
-    const double ONEOVEREIGHT = 1.0 / 8.0, ONEOVERTHREE = 1.0 / 3.0, EIGHTOVERTHREE = 8.0 / 3.0;
-    const double dx2 = 1.0, dx4 = 1.0;
-    double Q = -8.0 * c.fct2_1;
-    double sig[8];
+    ZCol cc;
+    if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
+    const int I = cc.m, J = cc.j, N = gf.n;
+    const unsigned plane = (unsigned)gf.n * (unsigned)gf.h, half = (unsigned)gf.half;
+    // fine columns and rows of the patch: m = x/2 with its parity, row j*h
+    const int xs[4] = {I == 0 ? N - 1 : 2 * I - 1, 2 * I, 2 * I + 1, 2 * I + 2 == N ? 0 : 2 * I + 2};
+    const int ys[4] = {J == 0 ? N - 1 : 2 * J - 1, 2 * J, 2 * J + 1, 2 * J + 2 == N ? 0 : 2 * J + 2};
+    auto at = [&](int yy, int xx, int z) -> unsigned {
+        const unsigned colour = (unsigned)((xs[xx] + ys[yy] + z) & 1);
+        return colour * half + (unsigned)z * plane + (unsigned)ys[yy] * (unsigned)gf.h + (unsigned)(xs[xx] >> 1);
+    };
+    auto load = [&](int z, DPatch& P) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int i = 2 * I + (q & 1), j = 2 * J + ((q >> 1) & 1), k = 2 * K + ((q >> 2) & 1);
-        const int im = wrap_dec(i, N), ip = wrap_inc(i, N);
-        const int jm = wrap_dec(j, N), jp = wrap_inc(j, N);
-        const int km = wrap_dec(k, N), kp = wrap_inc(k, N);
-#define U_(x, y, z) u0[split_index(gf, x, y, z)]
-        const double Sx = U_(ip, j, k) + U_(im, j, k);
-        const double Sy = U_(i, jp, k) + U_(i, jm, k);
-        const double Sz = U_(i, j, kp) + U_(i, j, km);
-        const double Dxy = (U_(ip, jp, k) + U_(im, jm, k)) - (U_(ip, jm, k) + U_(im, jp, k));
-        const double Dxz = (U_(ip, j, kp) + U_(im, j, km)) - (U_(ip, j, km) + U_(im, j, kp));
-        const double Dyz = (U_(i, jp, kp) + U_(i, jm, km)) - (U_(i, jp, km) + U_(i, jm, kp));
-#undef U_
-        double s = c.fct1 * delta[split_index(gf, i, j, k)];
-        s = s + Sx * (2.0 * Sx - Sy - Sz) * ONEOVERTHREE / dx4 + Sy * (2.0 * Sy - Sx - Sz) * ONEOVERTHREE / dx4 +
-            Sz * (2.0 * Sz - Sx - Sy) * ONEOVERTHREE / dx4 + (Dxy * Dxy) * ONEOVEREIGHT / dx4 +
-            (Dxz * Dxz) * ONEOVEREIGHT / dx4 + (Dyz * Dyz) * ONEOVEREIGHT / dx4;
-        sig[q] = s;
+        for (int yy = 0; yy < 4; yy++)
+#pragma unroll
+            for (int xx = 0; xx < 4; xx++) P.v[yy][xx] = u0[at(yy, xx, z)];
+    };
+    const double EIGHTOVERTHREE = 8.0 / 3.0;
+    const double dx2 = 1.0;
+    // B: fine plane 2K, C: 2K+1; Sigma of B's children before plane 2K+2 is
+    // loaded, so of plane 2K-1 only the 4 centre values (z-1 faces) stay live
+    DPatch A, B, C;
+    load(cc.k0 == 0 ? N - 1 : 2 * cc.k0 - 1, A);
+    load(2 * cc.k0, B);
+    for (int K = cc.k0; K < cc.k1; K++) {
+        const int z0 = 2 * K, z1 = 2 * K + 1, zu = 2 * K + 2 == N ? 0 : 2 * K + 2;
+        load(z1, C);
+        double sig[8];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int yy = 1 + ((q >> 1) & 1), xx = 1 + (q & 1);
+            sig[q] = dgp_sigma1(A, B, C, yy, xx, delta[at(yy, xx, z0)], c);
+        }
+        const double zl0 = A.v[1][1], zl1 = A.v[1][2], zl2 = A.v[2][1], zl3 = A.v[2][2];
+        load(zu, A);  // A now holds fine plane 2K+2
+#pragma unroll
+        for (int q = 4; q < 8; q++) {
+            const int yy = 1 + ((q >> 1) & 1), xx = 1 + (q & 1);
+            sig[q] = dgp_sigma1(B, C, A, yy, xx, delta[at(yy, xx, z1)], c);
+        }
+        double Q = -8.0 * c.fct2_1;
+#pragma unroll
+        for (int q = 0; q < 8; q++) Q = Q - c.fct3_1 * sqrt(c.alpha * c.alpha + EIGHTOVERTHREE * sig[q]);
+        double P = 0.0;
+        P = P - 3.0 * B.v[1][1];
+        P = P - 3.0 * B.v[1][2];
+        P = P - 3.0 * B.v[2][1];
+        P = P - 3.0 * B.v[2][2];
+        P = P - 3.0 * C.v[1][1];
+        P = P - 3.0 * C.v[1][2];
+        P = P - 3.0 * C.v[2][1];
+        P = P - 3.0 * C.v[2][2];
+        P = P + B.v[1][0]; P = P + B.v[2][0]; P = P + C.v[1][0]; P = P + C.v[2][0];  // x-1 faces
+        P = P + B.v[1][3]; P = P + B.v[2][3]; P = P + C.v[1][3]; P = P + C.v[2][3];  // x+2 faces
+        P = P + B.v[0][1]; P = P + B.v[0][2]; P = P + C.v[0][1]; P = P + C.v[0][2];  // y-1 faces
+        P = P + B.v[3][1]; P = P + B.v[3][2]; P = P + C.v[3][1]; P = P + C.v[3][2];  // y+2 faces
+        P = P + zl0; P = P + zl1; P = P + zl2; P = P + zl3;                          // z-1 faces
+        P = P + A.v[1][1]; P = P + A.v[1][2]; P = P + A.v[2][1]; P = P + A.v[2][2];  // z+2 faces
+        P = P / dx2 + Q;
+        rres1[split_index(gc, I, J, K)] = P / 8.0;
+        // planes 2K+1 and 2K+2 are 2K'-1 and 2K' of the next coarse plane
+        const DPatch t = A;
+        A = C;
+        B = t;
     }
-#pragma unroll
-    for (int q = 0; q < 8; q++) Q = Q - c.fct3_1 * sqrt(c.alpha * c.alpha + EIGHTOVERTHREE * sig[q]);
-    const int x0 = 2 * I, x1 = 2 * I + 1, y0 = 2 * J, y1 = 2 * J + 1, z0 = 2 * K, z1 = 2 * K + 1;
-    const int xl = (I == 0) ? N - 1 : 2 * I - 1, xu = (2 * I + 2 == N) ? 0 : 2 * I + 2;
-    const int yl = (J == 0) ? N - 1 : 2 * J - 1, yu = (2 * J + 2 == N) ? 0 : 2 * J + 2;
-    const int zl = (K == 0) ? N - 1 : 2 * K - 1, zu = (2 * K + 2 == N) ? 0 : 2 * K + 2;
-    double P = 0.0;
-#define V_(x, y, z) u0[split_index(gf, x, y, z)]
-    P = P - 3.0 * V_(x0, y0, z0);
-    P = P - 3.0 * V_(x1, y0, z0);
-    P = P - 3.0 * V_(x0, y1, z0);
-    P = P - 3.0 * V_(x1, y1, z0);
-    P = P - 3.0 * V_(x0, y0, z1);
-    P = P - 3.0 * V_(x1, y0, z1);
-    P = P - 3.0 * V_(x0, y1, z1);
-    P = P - 3.0 * V_(x1, y1, z1);
-    P = P + V_(xl, y0, z0); P = P + V_(xl, y1, z0); P = P + V_(xl, y0, z1); P = P + V_(xl, y1, z1);
-    P = P + V_(xu, y0, z0); P = P + V_(xu, y1, z0); P = P + V_(xu, y0, z1); P = P + V_(xu, y1, z1);
-    P = P + V_(x0, yl, z0); P = P + V_(x1, yl, z0); P = P + V_(x0, yl, z1); P = P + V_(x1, yl, z1);
-    P = P + V_(x0, yu, z0); P = P + V_(x1, yu, z0); P = P + V_(x0, yu, z1); P = P + V_(x1, yu, z1);
-    P = P + V_(x0, y0, zl); P = P + V_(x1, y0, zl); P = P + V_(x0, y1, zl); P = P + V_(x1, y1, zl);
-    P = P + V_(x0, y0, zu); P = P + V_(x1, y0, zu); P = P + V_(x0, y1, zu); P = P + V_(x1, y1, zu);
-#undef V_
-    P = P / dx2 + Q;
-    rres1[split_index(gc, I, J, K)] = P / 8.0;
 }
 
 // calculate_physical_right_hand_side_DGP, PMP2MGsolver_DGP.f90:793-885
@@ -239,9 +282,9 @@ void launch_dgp_restrict_residual1_64(const double* u0, const double* delta, dou
                                       hipStream_t s)
 {
     const Geo gf = make_geo(N), gc = make_geo(N / 2);
-    const Launch L = make_launch(gc.n, (long)gc.n * gc.n);
+    const ZLaunch L = make_zlaunch(gc.n, gc.n);
     hipLaunchKernelGGL(k_dgp_restrict_residual1_64, L.grid, L.block, 0, s, u0, delta, rres1, gf, gc, c0, L.TX, L.TY,
-                       L.XB, L.NB);
+                       L.XB, L.JB, L.KC, L.NB);
 }
 
 void launch_dgp_rhs64(const double* ru, const double* rho, const double* rres, double* f, int n, const DgpCoef& c,
