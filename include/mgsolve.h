@@ -312,7 +312,9 @@ typedef struct mgs_server_stats {
 
 int mgs_server_create(const mgs_server_opts* opts, mgs_server** out);
 int mgs_server_destroy(mgs_server* srv);
-const char* mgs_server_last_error(const mgs_server* srv);   /* also valid for srv == NULL */
+/* the calling thread's last server error message (srv may be NULL; submit
+ * and wait may be called from different threads) */
+const char* mgs_server_last_error(const mgs_server* srv);
 /* enqueue a request; *ticket identifies it for mgs_server_wait */
 int mgs_server_submit(mgs_server* srv, const mgs_request* req, int64_t* ticket);
 /* block until the request is done; copy its field u (raw, x scale) to the

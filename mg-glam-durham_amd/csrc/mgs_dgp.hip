@@ -512,7 +512,53 @@ void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s)
 // z-marching colour passes (the windows are in mgs_dgp_dev.h)
 // ===========================================================================
 
-// relaxation_iterations_DGP (as k_dgp_smooth64), one colour
+// relaxation_iterations_DGP (as k_dgp_smooth64), one colour.  S0: parity of
+// the column's first cell (the same for a whole wave); with the run unrolled
+// the parity of every step is a constant, so the stencil values are the
+// window's registers themselves.
+template <bool COARSE, bool S0>
+__device__ __forceinline__ void dgp_zsmooth64_run(const double* __restrict__ h0, const double* __restrict__ h1,
+                                                  double* __restrict__ out, const double* __restrict__ rho,
+                                                  const double* __restrict__ f, const Geo& g, int colour,
+                                                  const DgpCoef& c, const DCol& d, int k0)
+{
+    DWin2<double> w;
+    dwin2_init(w, h0, h1, d, k0, g.n);
+    DSide<double> sd;
+    dside_load(h0, h1, d, k0, g.n, S0, sd);
+    const double dLphi = -6.0 / c.dx2;
+    unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
+    double rh = rho[(unsigned)colour * (unsigned)g.half + off];
+    double fv = COARSE ? f[(unsigned)colour * (unsigned)g.half + off] : 0.0;
+#pragma unroll
+    for (int t = 0; t < DGP_KC; t++) {
+        const int k = k0 + t;
+        const bool s = S0 ^ ((t & 1) != 0);
+        DRaw2<double> r;
+        DSide<double> sdn;
+        double rhn = 0.0, fvn = 0.0;
+        const unsigned offn = off + d.plane;
+        if (t + 1 < DGP_KC) {
+            draw2_load(h0, h1, d, wrap_inc(k + 1, g.n), r);
+            dside_load(h0, h1, d, wrap_inc(k, g.n), g.n, !s, sdn);
+            rhn = rho[(unsigned)colour * (unsigned)g.half + offn];
+            if constexpr (COARSE) fvn = f[(unsigned)colour * (unsigned)g.half + offn];
+        }
+        const S19<double> u = s19_win2(w, s, sd, d);
+        double L = dgp_L19(u, rh, c);
+        if constexpr (COARSE) L = L - fv;
+        out[off] = u.c0 - L / dLphi;
+        if (t + 1 < DGP_KC) {
+            dwin2_shift(w);
+            dwin2_set(w, 2, r);
+            sd = sdn;
+            off = offn;
+            rh = rhn;
+            fv = fvn;
+        }
+    }
+}
+
 template <bool COARSE>
 __global__ __launch_bounds__(256) void k_dgp_zsmooth64(const double* __restrict__ h0, const double* __restrict__ h1,
                                                        double* __restrict__ out, const double* __restrict__ rho,
@@ -522,32 +568,57 @@ __global__ __launch_bounds__(256) void k_dgp_zsmooth64(const double* __restrict_
     ZCol cc;
     if (!zcol(g, TX, TY, XB, JB, DGP_KC, NB, cc)) return;
     const DCol d = dcol_of(g, cc.m, cc.j);
-    const int k0 = cc.k0;
-    DWin<double> w;
-    dwin_init(w, h0, h1, d, k0, g.n);
-    const double dLphi = -6.0 / c.dx2;
+    if ((colour + cc.j + cc.k0) & 1)
+        dgp_zsmooth64_run<COARSE, true>(h0, h1, out, rho, f, g, colour, c, d, cc.k0);
+    else
+        dgp_zsmooth64_run<COARSE, false>(h0, h1, out, rho, f, g, colour, c, d, cc.k0);
+}
+
+// one colour of the mixed inner smoother (as k_dgpm_smooth); S0 as above
+template <bool S0>
+__device__ __forceinline__ void dgpm_zsmooth_run(const float* __restrict__ e0, const float* __restrict__ e1,
+                                                 float* __restrict__ out, const float* __restrict__ pb,
+                                                 const float* __restrict__ rho32, const float* __restrict__ rhs,
+                                                 const Geo& g, int colour, const DgpF& c, const DCol& d, int k0)
+{
+    const float* b0 = pb;
+    const float* b1 = pb + g.half;
+    DWin2<float> we, wb;
+    dwin2_init(we, e0, e1, d, k0, g.n);
+    dwin2_init(wb, b0, b1, d, k0, g.n);
+    DSide<float> se, sb;
+    dside_load(e0, e1, d, k0, g.n, S0, se);
+    dside_load(b0, b1, d, k0, g.n, S0, sb);
     unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
-    double rh = rho[(unsigned)colour * (unsigned)g.half + off];
-    double fv = COARSE ? f[(unsigned)colour * (unsigned)g.half + off] : 0.0;
+    float rh = rho32[(unsigned)colour * (unsigned)g.half + off];
+    float fv = rhs[(unsigned)colour * (unsigned)g.half + off];
 #pragma unroll
     for (int t = 0; t < DGP_KC; t++) {
         const int k = k0 + t;
-        DRaw<double> r;
-        double rhn = 0.0, fvn = 0.0;
+        const bool s = S0 ^ ((t & 1) != 0);
+        DRaw2<float> re, rb;
+        DSide<float> sen, sbn;
+        float rhn = 0.0f, fvn = 0.0f;
         const unsigned offn = off + d.plane;
         if (t + 1 < DGP_KC) {
-            draw_load(h0, h1, d, wrap_inc(k + 1, g.n), r);
-            rhn = rho[(unsigned)colour * (unsigned)g.half + offn];
-            if constexpr (COARSE) fvn = f[(unsigned)colour * (unsigned)g.half + offn];
+            const int kn = wrap_inc(k + 1, g.n);
+            draw2_load(e0, e1, d, kn, re);
+            draw2_load(b0, b1, d, kn, rb);
+            dside_load(e0, e1, d, wrap_inc(k, g.n), g.n, !s, sen);
+            dside_load(b0, b1, d, wrap_inc(k, g.n), g.n, !s, sbn);
+            rhn = rho32[(unsigned)colour * (unsigned)g.half + offn];
+            fvn = rhs[(unsigned)colour * (unsigned)g.half + offn];
         }
-        const bool s = ((colour + cc.j + k) & 1) != 0;
-        const S19<double> u = s19_win(w, s);
-        double L = dgp_L19(u, rh, c);
-        if constexpr (COARSE) L = L - fv;
-        out[off] = u.c0 - L / dLphi;
+        const D2f de = d2_of(s19_win2(we, s, se, d)), db = d2_of(s19_win2(wb, s, sb, d));
+        const float D = dgp_dop(db, de, rh, c) - fv;
+        out[off] = de.c0 + D / (6.0f * c.inv_dx2);
         if (t + 1 < DGP_KC) {
-            dwin_shift(w);
-            dwin_set(w, 2, r, d);
+            dwin2_shift(we);
+            dwin2_shift(wb);
+            dwin2_set(we, 2, re);
+            dwin2_set(wb, 2, rb);
+            se = sen;
+            sb = sbn;
             off = offn;
             rh = rhn;
             fv = fvn;
@@ -555,7 +626,6 @@ __global__ __launch_bounds__(256) void k_dgp_zsmooth64(const double* __restrict_
     }
 }
 
-// one colour of the mixed inner smoother (as k_dgpm_smooth)
 __global__ __launch_bounds__(256) void k_dgpm_zsmooth(const float* __restrict__ e0, const float* __restrict__ e1,
                                                       float* __restrict__ out, const float* __restrict__ pb,
                                                       const float* __restrict__ rho32, const float* __restrict__ rhs,
@@ -565,42 +635,10 @@ __global__ __launch_bounds__(256) void k_dgpm_zsmooth(const float* __restrict__ 
     ZCol cc;
     if (!zcol(g, TX, TY, XB, JB, DGP_KC, NB, cc)) return;
     const DCol d = dcol_of(g, cc.m, cc.j);
-    const int k0 = cc.k0;
-    const float* b0 = pb;
-    const float* b1 = pb + g.half;
-    DWin<float> we, wb;
-    dwin_init(we, e0, e1, d, k0, g.n);
-    dwin_init(wb, b0, b1, d, k0, g.n);
-    unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
-    float rh = rho32[(unsigned)colour * (unsigned)g.half + off];
-    float fv = rhs[(unsigned)colour * (unsigned)g.half + off];
-#pragma unroll
-    for (int t = 0; t < DGP_KC; t++) {
-        const int k = k0 + t;
-        DRaw<float> re, rb;
-        float rhn = 0.0f, fvn = 0.0f;
-        const unsigned offn = off + d.plane;
-        if (t + 1 < DGP_KC) {
-            const int kn = wrap_inc(k + 1, g.n);
-            draw_load(e0, e1, d, kn, re);
-            draw_load(b0, b1, d, kn, rb);
-            rhn = rho32[(unsigned)colour * (unsigned)g.half + offn];
-            fvn = rhs[(unsigned)colour * (unsigned)g.half + offn];
-        }
-        const bool s = ((colour + cc.j + k) & 1) != 0;
-        const D2f de = d2_of(s19_win(we, s)), db = d2_of(s19_win(wb, s));
-        const float D = dgp_dop(db, de, rh, c) - fv;
-        out[off] = de.c0 + D / (6.0f * c.inv_dx2);
-        if (t + 1 < DGP_KC) {
-            dwin_shift(we);
-            dwin_shift(wb);
-            dwin_set(we, 2, re, d);
-            dwin_set(wb, 2, rb, d);
-            off = offn;
-            rh = rhn;
-            fv = fvn;
-        }
-    }
+    if ((colour + cc.j + cc.k0) & 1)
+        dgpm_zsmooth_run<true>(e0, e1, out, pb, rho32, rhs, g, colour, c, d, cc.k0);
+    else
+        dgpm_zsmooth_run<false>(e0, e1, out, pb, rho32, rhs, g, colour, c, d, cc.k0);
 }
 
 void launch_dgp_zpass64(int level, const double* h0, const double* h1, double* out, const double* rho,

@@ -409,6 +409,144 @@ __device__ __forceinline__ S19<T> s19_win(const DWin<T>& w, bool s)
     return u;
 }
 
+// Single-colour passes hold only x = 2m and 2m+1 of the window (DWin2).
+// The cell of parity s needs the third x at five (plane, row) slots, q =
+// 0..4: (1,1), (1,0), (1,2), (0,1), (2,1): x = 2m-1 when s = 0 (the o value
+// of lane m-1) and x = 2m+2 when s = 1 (the e value of lane m+1), taken by a
+// shuffle at use.  A wave is 64 consecutive m of one row and plane, so s is
+// the same for all its lanes; only the edge lane on that side loads the five
+// values itself (DSide, one step ahead).  The stencil values are the same
+// numbers as DWin's, so results are bit-identical to its passes.
+template <typename T>
+struct DWin2 {
+    T e[3][3], o[3][3];  // [plane k-1, k, k+1][row j-1, j, j+1]: x = 2m, 2m+1
+};
+
+template <typename T>
+struct DRaw2 {
+    T ev[3], ov[3];
+};
+
+template <typename T>
+struct DSide {
+    T v[5];
+};
+
+__device__ __forceinline__ int side_plane(int q) { return q == 3 ? 0 : (q == 4 ? 2 : 1); }
+__device__ __forceinline__ int side_row(int q) { return q == 1 ? 0 : (q == 2 ? 2 : 1); }
+
+template <typename T>
+__device__ __forceinline__ void draw2_load(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d, int K,
+                                           DRaw2<T>& r)
+{
+#pragma unroll
+    for (int R = 0; R < 3; R++) {
+        const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
+        const bool odd = ((d.jr[R] + K) & 1) != 0;
+        r.ev[R] = (odd ? h1 : h0)[base + d.m];
+        r.ov[R] = (odd ? h0 : h1)[base + d.m];
+    }
+}
+
+// the five side values of the cell at plane k with parity s (edge lane only)
+template <typename T>
+__device__ __forceinline__ void dside_load(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d, int k,
+                                           int n, bool s, DSide<T>& sd)
+{
+#pragma unroll
+    for (int q = 0; q < 5; q++) sd.v[q] = T(0);
+    if (s ? d.rload : d.lload) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            const int P = side_plane(q), R = side_row(q);
+            const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
+            const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
+            const bool odd = ((d.jr[R] + K) & 1) != 0;
+            // x = 2m+2 has the colour of x = 2m, x = 2m-1 that of 2m+1
+            sd.v[q] = s ? (odd ? h1 : h0)[base + d.mp] : (odd ? h0 : h1)[base + d.mm];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void dwin2_set(DWin2<T>& w, int P, const DRaw2<T>& r)
+{
+#pragma unroll
+    for (int R = 0; R < 3; R++) {
+        w.e[P][R] = r.ev[R];
+        w.o[P][R] = r.ov[R];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void dwin2_shift(DWin2<T>& w)
+{
+#pragma unroll
+    for (int R = 0; R < 3; R++) {
+        w.e[0][R] = w.e[1][R];
+        w.o[0][R] = w.o[1][R];
+        w.e[1][R] = w.e[2][R];
+        w.o[1][R] = w.o[2][R];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void dwin2_init(DWin2<T>& w, const T* __restrict__ h0, const T* __restrict__ h1,
+                                           const DCol& d, int k0, int n)
+{
+    DRaw2<T> r0, r1, r2;
+    draw2_load(h0, h1, d, wrap_dec(k0, n), r0);
+    draw2_load(h0, h1, d, k0, r1);
+    draw2_load(h0, h1, d, wrap_inc(k0, n), r2);
+    dwin2_set(w, 0, r0);
+    dwin2_set(w, 1, r1);
+    dwin2_set(w, 2, r2);
+}
+
+// the 19 stencil values of the cell of parity s (all lanes execute the
+// shuffles)
+template <typename T>
+__device__ __forceinline__ S19<T> s19_win2(const DWin2<T>& w, bool s, const DSide<T>& sd, const DCol& d)
+{
+    const bool edge = s ? d.rload : d.lload;
+    const int lane = (int)((threadIdx.x + threadIdx.y * blockDim.x) & 63u);
+    const int src = s ? lane + 1 : lane - 1;  // out of the wave only on the edge lane, which keeps sd
+    T v[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const int P = side_plane(q), R = side_row(q);
+        const T sh = __shfl(s ? w.e[P][R] : w.o[P][R], src, 64);
+        v[q] = edge ? sd.v[q] : sh;
+    }
+    // x = i + dx at slot (P, R), i = 2m + s; q: the side slot of (P, R)
+    auto X = [&](int P, int R, int dx, int q) -> T {
+        if (dx == 0) return s ? w.o[P][R] : w.e[P][R];
+        if (dx < 0) return s ? w.e[P][R] : v[q];
+        return s ? v[q] : w.o[P][R];
+    };
+    S19<T> u;
+    u.c0 = X(1, 1, 0, 0);
+    u.xp = X(1, 1, 1, 0);
+    u.xm = X(1, 1, -1, 0);
+    u.yp = X(1, 2, 0, 0);
+    u.ym = X(1, 0, 0, 0);
+    u.zp = X(2, 1, 0, 0);
+    u.zm = X(0, 1, 0, 0);
+    u.xy_pp = X(1, 2, 1, 2);
+    u.xy_mm = X(1, 0, -1, 1);
+    u.xy_pm = X(1, 0, 1, 1);
+    u.xy_mp = X(1, 2, -1, 2);
+    u.xz_pp = X(2, 1, 1, 4);
+    u.xz_mm = X(0, 1, -1, 3);
+    u.xz_pm = X(0, 1, 1, 3);
+    u.xz_mp = X(2, 1, -1, 4);
+    u.yz_pp = X(2, 2, 0, 0);
+    u.yz_mm = X(0, 0, 0, 0);
+    u.yz_pm = X(0, 2, 0, 0);
+    u.yz_mp = X(2, 0, 0, 0);
+    return u;
+}
+
 // the window of planes k0-1, k0, k0+1 (all loads first)
 template <typename T>
 __device__ __forceinline__ void dwin_init(DWin<T>& w, const T* __restrict__ h0, const T* __restrict__ h1,

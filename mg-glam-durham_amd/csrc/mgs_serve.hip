@@ -616,10 +616,11 @@ struct mgs_server {
     int max_batch = 0;
     bool any_submit = false;
     Clock::time_point first_submit, last_done;
-    std::string err;
 };
 
-static thread_local std::string g_server_create_error;
+// the calling thread's last server error (submit and wait may run on
+// different client threads; a message per thread keeps them apart)
+static thread_local std::string g_server_error;
 
 // no NaN or Inf among n values: a branch-free reduction the compiler
 // vectorizes (|v| <= max is false for both)
@@ -676,10 +677,8 @@ static void give_stage(mgs_server* S, char* p, size_t bytes)
 
 static int srv_err(mgs_server* S, int code, const std::string& msg)
 {
-    if (S)
-        S->err = msg;
-    else
-        g_server_create_error = msg;
+    (void)S;
+    g_server_error = msg;
     return code;
 }
 
@@ -1098,7 +1097,11 @@ void worker_main(mgs_server* S)
         }
         const Clock::time_point ta = Clock::now();
         admit(S);
-        S->admit_ms += ms_between(ta, Clock::now());
+        {
+            const double dt = ms_between(ta, Clock::now());
+            std::lock_guard<std::mutex> lk(S->mu);  // statistics are read under mu
+            S->admit_ms += dt;
+        }
         int nact = 0;
         for (int i = 0; i < S->o.max_active; i++) nact += S->slots[i].r ? 1 : 0;
         if (!nact) {
@@ -1123,10 +1126,13 @@ void worker_main(mgs_server* S)
         if (!rc && (hipMemcpyAsync(S->hsum, S->dsum, sizeof(double) * MAXS, hipMemcpyDeviceToHost, S->s) != hipSuccess ||
                     hipStreamSynchronize(S->s) != hipSuccess))
             rc = MGS_E_HIP;
-        S->steps++;
-        S->max_batch = std::max(S->max_batch, nact);
         const Clock::time_point tf = Clock::now();
-        S->step_ms += ms_between(ts, tf);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->steps++;
+            S->max_batch = std::max(S->max_batch, nact);
+            S->step_ms += ms_between(ts, tf);
+        }
         // per request: the stopping rule of solve_mixed (mode 1 or 0)
         for (int i = 0; i < S->o.max_active; i++) {
             Slot& sl = S->slots[i];
@@ -1168,7 +1174,11 @@ void worker_main(mgs_server* S)
             }
             if (stop) finish_request(S, sl, 0);
         }
-        S->finish_ms += ms_between(tf, Clock::now());
+        {
+            const double dt = ms_between(tf, Clock::now());
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->finish_ms += dt;
+        }
     }
 }
 
@@ -1279,7 +1289,8 @@ int mgs_server_destroy(mgs_server* S)
 
 const char* mgs_server_last_error(const mgs_server* S)
 {
-    return S ? S->err.c_str() : g_server_create_error.c_str();
+    (void)S;
+    return g_server_error.c_str();
 }
 
 int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
@@ -1336,7 +1347,7 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
         if ((int)S->waiting.size() >= S->o.queue_capacity) {
             S->rejected++;
             delete r;
-            S->err = "queue full";
+            g_server_error = "queue full";
             return MGS_E_BUSY;
         }
         r->ticket = S->next_ticket++;
@@ -1397,7 +1408,7 @@ int mgs_server_wait(mgs_server* S, int64_t ticket, void* u, int32_t dtype, doubl
         *res = r->res;
         res->status = rc;
     }
-    if (rc) S->err = "request failed (code " + std::to_string(rc) + ")";
+    if (rc) g_server_error = "request failed (code " + std::to_string(rc) + ")";
     delete r;
     return rc;
 }
