@@ -275,29 +275,21 @@ __device__ __forceinline__ void dgpm_coarsest_block(float* __restrict__ e, const
 // a wave is 64 consecutive m of one row)
 // ---------------------------------------------------------------------------
 // A thread owns column (m, j) and marches over a run of DGP_KC planes.  Its
-// window holds, for planes k-1, k, k+1 and rows j-1, j, j+1, the values at
-// x = 2m-1, 2m, 2m+1, 2m+2: every point of the 19-point stencil of the cell
-// of either colour at (m, j, k).  A step loads the pair at m of plane k+2 in
-// the three rows (both colour halves at one offset) and takes x = 2m-1 and
-// 2m+2 from the neighbouring lanes (the lanes at a wave's edges load them);
-// plane k+2 and the next cell's own data are loaded before plane k is
-// computed.  The stencil values go through S19 like the per-cell kernels',
-// so every result is bit-identical to theirs.
+// window (DWin2) holds, for planes k-1, k, k+1 and rows j-1, j, j+1, the
+// pair at x = 2m, 2m+1; with x = 2m-1 and 2m+2 at the five slots the
+// stencil reaches off the column (taken from the neighbouring lanes by a
+// shuffle, or loaded by the lane at the wave's edge), that is every point of
+// the 19-point stencil of the cell of either colour at (m, j, k).  A step
+// loads the pair at m of plane k+2 in the three rows (both colour halves at
+// one offset); plane k+2, the edge lanes' side values and the next cell's
+// own data are loaded before plane k is computed.  The stencil values go
+// through S19 like the per-cell kernels', so every result is bit-identical
+// to theirs.
 //
 // The field is read through its two colour halves (h0, h1) and the pass
 // writes the new values of its colour to `out`, a half of a second array: a
 // pair of passes (colour 0, then colour 1 reading the new colour 0) moves
 // the field into the second array without the per-cell kernels' copy.
-template <typename T>
-struct DWin {
-    T l[3][3], e[3][3], o[3][3], r[3][3];  // [plane k-1, k, k+1][row j-1, j, j+1]: x = 2m-1, 2m, 2m+1, 2m+2
-};
-
-template <typename T>
-struct DRaw {
-    T ev[3], ov[3], lv[3], rv[3];  // one plane, rows j-1, j, j+1
-};
-
 struct DCol {
     unsigned plane, m, mm, mp;
     unsigned rowoff[3];
@@ -321,92 +313,6 @@ __device__ __forceinline__ DCol dcol_of(const Geo& g, int m, int j)
     d.lload = lane == 0;
     d.rload = lane == 63;
     return d;
-}
-
-// plane K of rows j-1..j+1: the pair at m (x = 2m has colour (J+K)&1) and,
-// at a wave's edges, x = 2m-1 (the colour of x = 2m+1) and x = 2m+2
-template <typename T>
-__device__ __forceinline__ void draw_load(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d, int K,
-                                          DRaw<T>& r)
-{
-#pragma unroll
-    for (int R = 0; R < 3; R++) {
-        const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
-        const bool odd = ((d.jr[R] + K) & 1) != 0;
-        const T* he = odd ? h1 : h0;
-        const T* ho = odd ? h0 : h1;
-        r.ev[R] = he[base + d.m];
-        r.ov[R] = ho[base + d.m];
-        r.lv[R] = T(0);
-        r.rv[R] = T(0);
-        if (d.lload) r.lv[R] = ho[base + d.mm];
-        if (d.rload) r.rv[R] = he[base + d.mp];
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void dwin_set(DWin<T>& w, int P, const DRaw<T>& r, const DCol& d)
-{
-#pragma unroll
-    for (int R = 0; R < 3; R++) {
-        const T l = __shfl_up(r.ov[R], 1u, 64);
-        const T rr = __shfl_down(r.ev[R], 1u, 64);
-        w.l[P][R] = d.lload ? r.lv[R] : l;
-        w.e[P][R] = r.ev[R];
-        w.o[P][R] = r.ov[R];
-        w.r[P][R] = d.rload ? r.rv[R] : rr;
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void dwin_shift(DWin<T>& w)
-{
-#pragma unroll
-    for (int R = 0; R < 3; R++) {
-        w.l[0][R] = w.l[1][R];
-        w.e[0][R] = w.e[1][R];
-        w.o[0][R] = w.o[1][R];
-        w.r[0][R] = w.r[1][R];
-        w.l[1][R] = w.l[2][R];
-        w.e[1][R] = w.e[2][R];
-        w.o[1][R] = w.o[2][R];
-        w.r[1][R] = w.r[2][R];
-    }
-}
-
-// value at x = i + dx of plane slot P, row slot R, for i = 2m + s
-template <typename T>
-__device__ __forceinline__ T wx(const DWin<T>& w, int P, int R, int dx, bool s)
-{
-    if (dx < 0) return s ? w.e[P][R] : w.l[P][R];
-    if (dx == 0) return s ? w.o[P][R] : w.e[P][R];
-    return s ? w.r[P][R] : w.o[P][R];
-}
-
-template <typename T>
-__device__ __forceinline__ S19<T> s19_win(const DWin<T>& w, bool s)
-{
-    S19<T> u;
-    u.c0 = wx(w, 1, 1, 0, s);
-    u.xp = wx(w, 1, 1, 1, s);
-    u.xm = wx(w, 1, 1, -1, s);
-    u.yp = wx(w, 1, 2, 0, s);
-    u.ym = wx(w, 1, 0, 0, s);
-    u.zp = wx(w, 2, 1, 0, s);
-    u.zm = wx(w, 0, 1, 0, s);
-    u.xy_pp = wx(w, 1, 2, 1, s);
-    u.xy_mm = wx(w, 1, 0, -1, s);
-    u.xy_pm = wx(w, 1, 0, 1, s);
-    u.xy_mp = wx(w, 1, 2, -1, s);
-    u.xz_pp = wx(w, 2, 1, 1, s);
-    u.xz_mm = wx(w, 0, 1, -1, s);
-    u.xz_pm = wx(w, 0, 1, 1, s);
-    u.xz_mp = wx(w, 2, 1, -1, s);
-    u.yz_pp = wx(w, 2, 2, 0, s);
-    u.yz_mm = wx(w, 0, 0, 0, s);
-    u.yz_pm = wx(w, 0, 2, 0, s);
-    u.yz_mp = wx(w, 2, 0, 0, s);
-    return u;
 }
 
 // Single-colour passes hold only x = 2m and 2m+1 of the window (DWin2).
@@ -464,6 +370,27 @@ __device__ __forceinline__ void dside_load(const T* __restrict__ h0, const T* __
             const bool odd = ((d.jr[R] + K) & 1) != 0;
             // x = 2m+2 has the colour of x = 2m, x = 2m-1 that of 2m+1
             sd.v[q] = s ? (odd ? h1 : h0)[base + d.mp] : (odd ? h0 : h1)[base + d.mm];
+        }
+    }
+}
+
+// both sides at once, for passes that evaluate both cells of the pair: the
+// left edge lane loads x = 2m-1 (the side of the cell at 2m), the right edge
+// lane x = 2m+2 (the side of the cell at 2m+1), into the same five slots
+template <typename T>
+__device__ __forceinline__ void dside_load_edges(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d,
+                                                 int k, int n, DSide<T>& sd)
+{
+#pragma unroll
+    for (int q = 0; q < 5; q++) sd.v[q] = T(0);
+    if (d.lload || d.rload) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            const int P = side_plane(q), R = side_row(q);
+            const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
+            const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
+            const bool odd = ((d.jr[R] + K) & 1) != 0;
+            sd.v[q] = d.rload ? (odd ? h1 : h0)[base + d.mp] : (odd ? h0 : h1)[base + d.mm];
         }
     }
 }
@@ -547,20 +474,6 @@ __device__ __forceinline__ S19<T> s19_win2(const DWin2<T>& w, bool s, const DSid
     return u;
 }
 
-// the window of planes k0-1, k0, k0+1 (all loads first)
-template <typename T>
-__device__ __forceinline__ void dwin_init(DWin<T>& w, const T* __restrict__ h0, const T* __restrict__ h1,
-                                          const DCol& d, int k0, int n)
-{
-    DRaw<T> r0, r1, r2;
-    draw_load(h0, h1, d, wrap_dec(k0, n), r0);
-    draw_load(h0, h1, d, k0, r1);
-    draw_load(h0, h1, d, wrap_inc(k0, n), r2);
-    dwin_set(w, 0, r0, d);
-    dwin_set(w, 1, r1, d);
-    dwin_set(w, 2, r2, d);
-}
-
 // Both cells of the pair at (m, j) over the run [k0, k0 + DGP_KC), from one
 // window per field (OP::NF fields of type OP::T, colour halves
 // op.half(f, 0/1)).  Per plane the cell at x = 2m comes first, then x = 2m+1;
@@ -576,9 +489,13 @@ __device__ __forceinline__ double dgp_both_zcol(const OP& op, const Geo& g, int 
     constexpr int NF = OP::NF;
     const DCol d = dcol_of(g, m, j);
     const unsigned half = (unsigned)g.half;
-    DWin<T> w[NF];
+    DWin2<T> w[NF];
+    DSide<T> sd[NF];
 #pragma unroll
-    for (int f = 0; f < NF; f++) dwin_init(w[f], op.half(f, 0), op.half(f, 1), d, k0, g.n);
+    for (int f = 0; f < NF; f++) {
+        dwin2_init(w[f], op.half(f, 0), op.half(f, 1), d, k0, g.n);
+        dside_load_edges(op.half(f, 0), op.half(f, 1), d, k0, g.n, sd[f]);
+    }
     unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
     unsigned par = (unsigned)((j + k0) & 1);  // colour of x = 2m
     Own o0 = op.load(par * half + off), o1 = op.load((1u - par) * half + off);
@@ -586,20 +503,24 @@ __device__ __forceinline__ double dgp_both_zcol(const OP& op, const Geo& g, int 
 #pragma unroll
     for (int t = 0; t < DGP_KC; t++) {
         const int k = k0 + t;
-        DRaw<T> r[NF];
+        DRaw2<T> r[NF];
+        DSide<T> sdn[NF];
         Own n0 = o0, n1 = o1;
         const unsigned offn = off + d.plane, parn = par ^ 1u;
         if (t + 1 < DGP_KC) {
 #pragma unroll
-            for (int f = 0; f < NF; f++) draw_load(op.half(f, 0), op.half(f, 1), d, wrap_inc(k + 1, g.n), r[f]);
+            for (int f = 0; f < NF; f++) {
+                draw2_load(op.half(f, 0), op.half(f, 1), d, wrap_inc(k + 1, g.n), r[f]);
+                dside_load_edges(op.half(f, 0), op.half(f, 1), d, wrap_inc(k, g.n), g.n, sdn[f]);
+            }
             n0 = op.load(parn * half + offn);
             n1 = op.load((1u - parn) * half + offn);
         }
         S19<T> s0[NF], s1[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            s0[f] = s19_win(w[f], false);
-            s1[f] = s19_win(w[f], true);
+            s0[f] = s19_win2(w[f], false, sd[f], d);
+            s1[f] = s19_win2(w[f], true, sd[f], d);
         }
         const double r0 = op.cell(par * half + off, o0, s0);
         const double r1 = op.cell((1u - par) * half + off, o1, s1);
@@ -607,8 +528,9 @@ __device__ __forceinline__ double dgp_both_zcol(const OP& op, const Geo& g, int 
         if (t + 1 < DGP_KC) {
 #pragma unroll
             for (int f = 0; f < NF; f++) {
-                dwin_shift(w[f]);
-                dwin_set(w[f], 2, r[f], d);
+                dwin2_shift(w[f]);
+                dwin2_set(w[f], 2, r[f]);
+                sd[f] = sdn[f];
             }
             off = offn;
             par = parn;
