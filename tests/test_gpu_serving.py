@@ -162,6 +162,52 @@ def test_dgp_results_equal_alone_in_mix_and_standalone():
 
 
 @pytest.mark.gpu
+def test_spill_and_claim_with_concurrent_submit_and_wait():
+    """One thread submits while another waits in shuffled order, under a
+    budget of about three 48^3 hierarchies: finished results wait in HBM, so
+    admission keeps spilling unclaimed ones to the host while waiters claim
+    others from HBM.  Waiting for the last ticket first makes spills certain
+    (every earlier request had to be admitted before it).  Every field, read
+    as FP64 or FP32, equals the one an unconstrained server returns."""
+    import threading
+    import time
+
+    sizes = [48, 32, 48, 48, 32, 48, 48, 48]
+    base = [(cfg(N, -1e-5, 1, 2.0 * N), density(N, 2.0 * N, 1.0 + 0.05 * q, 60 + q, f32=(q % 2 == 0)))
+            for q, N in enumerate(sizes)]
+    reqs = [base[q % len(base)] for q in range(3 * len(base))]
+    print(f"[serving] concurrent submit/wait, {len(reqs)} requests", flush=True)
+    with M.Server(max_active=8) as big:
+        ref = [big.wait(big.submit(c, d), c.N)[0] for c, d in base]
+    budget = 3 * 40 * 48 ** 3
+    rng = np.random.default_rng(3)
+    tickets = [None] * len(reqs)
+    with M.Server(max_active=4, hbm_budget=budget) as srv:
+
+        def producer():
+            for q, (c, d) in enumerate(reqs):
+                tickets[q] = srv.submit(c, d)
+                time.sleep(float(rng.uniform(0.0, 0.002)))
+
+        th = threading.Thread(target=producer)
+        th.start()
+        order = [len(reqs) - 1] + [int(q) for q in rng.permutation(len(reqs) - 1)]
+        out = {}
+        for q in order:
+            while tickets[q] is None:
+                time.sleep(0.0005)
+            dt = np.float64 if q % 2 else np.float32
+            out[q] = srv.wait(tickets[q], reqs[q][0].N, dt)
+        th.join()
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["spilled"] > 0, st
+    for q in range(len(reqs)):
+        u, r = out[q]
+        assert r["status"] == 0 and r["converged"], (q, r)
+        assert np.array_equal(u, ref[q % len(base)].astype(u.dtype)), q
+
+
+@pytest.mark.gpu
 def test_server_rejects_bad_requests():
     with M.Server(max_active=2) as srv:
         d = np.zeros((16,) * 3, np.float32)
