@@ -274,20 +274,32 @@ def serve_run(tag, cfgs, kws, dens, max_active, rng):
         th = threading.Thread(target=producer)
         tp = time.perf_counter()
         th.start()
-        for i in range(n_req):
-            while tickets[i] is None:
-                time.sleep(0.0005)
-            u, o = srv.wait(tickets[i], cfgs[i].N, np.float32)
-            t_got[i] = time.perf_counter()
-            if i in keep:
-                fields[i] = u
-            if not ok(o, kws[i]):
-                bad.append(o)
+        # the client claims results in the order they finish, into one
+        # reused output array per grid size
+        pending = list(range(n_req))
+        outbuf = {}
+        while pending:
+            ready = [i for i in pending if tickets[i] is not None and srv.test(tickets[i])]
+            if not ready:
+                time.sleep(0.0002)
+                continue
+            for i in ready:
+                N = cfgs[i].N
+                if N not in outbuf:
+                    outbuf[N] = np.empty((N, N, N), np.float32)
+                u, o = srv.wait(tickets[i], N, np.float32, out=outbuf[N])
+                t_got[i] = time.perf_counter()
+                if i in keep:
+                    fields[i] = u.copy()
+                if not ok(o, kws[i]):
+                    bad.append(o)
+            done = set(ready)
+            pending = [i for i in pending if i not in done]
         th.join()
         pois_s = time.perf_counter() - tp
         st = srv.stats()
     # delivered: submit call -> the field in the caller's array (wait() copies
-    # it out of HBM; the client claims results in submission order)
+    # it out of HBM; the client claims results as they finish)
     dl = 1e3 * (np.array(t_got) - np.array(t_sub))
     log(f"[{tag}] Poisson {rate:.1f}/s: ready in HBM p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms; "
         f"delivered p50 {np.percentile(dl, 50):.1f} ms p99 {np.percentile(dl, 99):.1f} ms; "

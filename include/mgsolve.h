@@ -323,6 +323,10 @@ int mgs_server_submit(mgs_server* srv, const mgs_request* req, int64_t* ticket);
  * (counted against hbm_budget) until this call copies it out, or until
  * admission needs the memory and copies it to the host (stats.spilled). */
 int mgs_server_wait(mgs_server* srv, int64_t ticket, void* u, int32_t dtype, double scale, mgs_request_result* res);
+/* *done = 1 when the request has finished (mgs_server_wait then returns
+ * without waiting for a solve), else 0; never blocks.  Lets a client claim
+ * results in the order they finish. */
+int mgs_server_test(mgs_server* srv, int64_t ticket, int32_t* done);
 int mgs_server_get_stats(mgs_server* srv, mgs_server_stats* st);
 
 #ifdef __cplusplus

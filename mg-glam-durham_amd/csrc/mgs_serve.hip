@@ -1365,6 +1365,16 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
     return 0;
 }
 
+int mgs_server_test(mgs_server* S, int64_t ticket, int32_t* done)
+{
+    if (!S || !done) return MGS_E_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return srv_err(S, MGS_E_ARG, "unknown ticket");
+    *done = it->second->done ? 1 : 0;
+    return 0;
+}
+
 int mgs_server_wait(mgs_server* S, int64_t ticket, void* u, int32_t dtype, double scale, mgs_request_result* res)
 {
     if (!S) return MGS_E_ARG;

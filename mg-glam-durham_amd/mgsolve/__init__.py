@@ -159,6 +159,7 @@ def load():
         "mgs_server_submit": ([H, C.POINTER(Request), C.POINTER(C.c_int64)], C.c_int),
         "mgs_server_wait": ([H, C.c_int64, vp, i32, C.c_double, C.POINTER(RequestResult)], C.c_int),
         "mgs_server_get_stats": ([H, C.POINTER(ServerStats)], C.c_int),
+        "mgs_server_test": ([H, C.c_int64, C.POINTER(i32)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -461,9 +462,16 @@ class Server:
             raise MgsError(f"mgs_server_submit failed ({rc}): {self.L.mgs_server_last_error(self.h).decode()}")
         return t.value
 
-    def wait(self, ticket, N, dtype=np.float64, scale=1.0):
-        """Block until the ticket is done; returns (field u, result dict)."""
-        u = np.empty((N, N, N), dtype=dtype)
+    def wait(self, ticket, N, dtype=np.float64, scale=1.0, out=None):
+        """Block until the ticket is done; returns (field u, result dict).
+        out: an (N, N, N) C-contiguous array of dtype to fill (reusing one
+        avoids first-touch page faults on every result)."""
+        if out is None:
+            u = np.empty((N, N, N), dtype=dtype)
+        else:
+            if out.shape != (N, N, N) or out.dtype != np.dtype(dtype) or not out.flags["C_CONTIGUOUS"]:
+                raise MgsError("out must be a C-contiguous (N, N, N) array of the requested dtype")
+            u = out
         res = RequestResult()
         rc = self.L.mgs_server_wait(self.h, int(ticket), _np_ptr(u), F32 if dtype == np.float32 else F64,
                                     float(scale), C.byref(res))
@@ -471,6 +479,14 @@ class Server:
         if rc:
             raise MgsError(f"mgs_server_wait failed ({rc}): {self.L.mgs_server_last_error(self.h).decode()}")
         return u, out
+
+    def test(self, ticket):
+        """True when the ticket's request has finished (wait() will not block on a solve)."""
+        d = C.c_int32(0)
+        rc = self.L.mgs_server_test(self.h, int(ticket), C.byref(d))
+        if rc:
+            raise MgsError(f"mgs_server_test failed ({rc}): {self.L.mgs_server_last_error(self.h).decode()}")
+        return bool(d.value)
 
     def stats(self):
         st = ServerStats()

@@ -7,6 +7,8 @@ runs the same per-request arithmetic, tiling and reduction order, so the bar
 is bitwise equality.
 Admission by HBM footprint is exercised with a budget that holds only one
 request at a time.  Error paths are checked without a GPU."""
+import time
+
 import numpy as np
 import pytest
 
@@ -90,9 +92,13 @@ def test_admission_by_footprint_and_graph_replay(use_graphs):
     reads = [(np.float64, 1.0), (np.float32, 1.0), (np.float32, 0.25)]
     with M.Server(max_active=8, hbm_budget=budget, use_graphs=use_graphs) as srv:
         ts = [srv.submit(c, d) for d in ds]
+        while not srv.test(ts[2]):  # non-blocking completion check
+            time.sleep(0.001)
         last = srv.wait(ts[2], N, *reads[2])
         out = [srv.wait(t, N, dt, sc) for t, (dt, sc) in zip(ts[:2], reads[:2])] + [last]
         st = srv.stats()
+        with pytest.raises(M.MgsError, match="unknown ticket"):
+            srv.test(ts[0])  # claimed tickets are gone
     assert st["max_batch"] == 1 and st["spilled"] == 2, st
     for (u, r), u_ref, (dt, sc) in zip(out, ref, reads):
         assert r["status"] == 0 and r["converged"]
