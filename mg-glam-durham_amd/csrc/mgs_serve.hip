@@ -609,6 +609,9 @@ struct mgs_server {
     // idle host staging buffers by size (under mu)
     std::map<size_t, std::vector<char*>> stages;
     size_t stage_idle = 0;
+    // pinned copy-out staging: idle buffers by size, bytes pinned in all (under mu)
+    std::map<size_t, std::vector<void*>> pins;
+    size_t pinned = 0;
     // statistics
     std::vector<double> lat;
     int64_t submitted = 0, rejected = 0, steps = 0, spilled = 0;
@@ -1185,6 +1188,65 @@ void worker_main(mgs_server* S)
 // a result held in HBM into the caller's host array: unpacked into the
 // reference order in the requested type and scale (into the hierarchy's
 // level-0 scratch) and copied, on a stream of the calling thread
+// Pinned staging for copy-outs, pooled by size and kept until the server is
+// destroyed (a buffer per claim would cost a pinning per result); at most
+// PIN_MAX bytes in all, beyond which a claim copies through the runtime's
+// pageable path.
+constexpr size_t PIN_MAX = size_t(2) << 30;
+constexpr size_t PIN_MIN = size_t(1) << 20;  // smaller fields go direct
+
+void* take_pin(mgs_server* S, size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->pins.find(bytes);
+        if (it != S->pins.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+        if (S->pinned + bytes > PIN_MAX) return nullptr;
+        S->pinned += bytes;
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->pinned -= bytes;
+        return nullptr;
+    }
+    return p;
+}
+
+void give_pin(mgs_server* S, void* p, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->pins[bytes].push_back(p);
+}
+
+// host copy of a large field with a few threads (one slice each)
+void copy_host(void* dst, const void* src, size_t bytes)
+{
+    const int nt = bytes >= (size_t(16) << 20) ? 4 : 1;
+    if (nt == 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t slice = (bytes / nt + 4095) & ~size_t(4095);
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) {
+        const size_t o = slice * t;
+        if (o >= bytes) break;
+        const size_t len = std::min(slice, bytes - o);
+        th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, len); });
+    }
+    std::memcpy(dst, src, std::min(slice, bytes));
+    for (auto& x : th) x.join();
+}
+
+// a result held in HBM into the caller's host array: unpacked into the
+// reference order in the requested type and scale (into the hierarchy's
+// level-0 scratch), copied to pinned staging and from there into the array,
+// on a stream of the calling thread
 int copy_out(mgs_server* S, Hier* h, void* u, int32_t dtype, double scale)
 {
     if (hipSetDevice(S->device) != hipSuccess) return MGS_E_HIP;
@@ -1192,17 +1254,25 @@ int copy_out(mgs_server* S, Hier* h, void* u, int32_t dtype, double scale)
     if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) return MGS_E_HIP;
     const int N = h->N;
     const size_t n3 = (size_t)N * N * N;
-    hipError_t e;
+    const size_t bytes = n3 * (dtype == MGS_F32 ? sizeof(float) : sizeof(double));
+    const void* dev;
     if (dtype == MGS_F32) {
         launch_unpack_f64_to_f32(h->u, h->d[0], N, scale, ws);
-        e = hipMemcpyAsync(u, h->d[0], n3 * sizeof(float), hipMemcpyDeviceToHost, ws);
+        dev = h->d[0];
     } else {
         launch_unpack_f64_to_f64(h->u, h->rho, N, scale, ws);
-        e = hipMemcpyAsync(u, h->rho, n3 * sizeof(double), hipMemcpyDeviceToHost, ws);
+        dev = h->rho;
     }
+    void* pin = bytes >= PIN_MIN ? take_pin(S, bytes) : nullptr;
+    const hipError_t e = hipMemcpyAsync(pin ? pin : u, dev, bytes, hipMemcpyDeviceToHost, ws);
     const hipError_t es = hipStreamSynchronize(ws);
     (void)hipStreamDestroy(ws);
-    return e == hipSuccess && es == hipSuccess ? 0 : MGS_E_HIP;
+    const bool ok = e == hipSuccess && es == hipSuccess;
+    if (pin) {
+        if (ok) copy_host(u, pin, bytes);
+        give_pin(S, pin, bytes);
+    }
+    return ok ? 0 : MGS_E_HIP;
 }
 
 }  // namespace
@@ -1276,6 +1346,8 @@ int mgs_server_destroy(mgs_server* S)
     }
     for (auto& kv : S->stages)
         for (char* p : kv.second) std::free(p);
+    for (auto& kv : S->pins)
+        for (void* p : kv.second) (void)hipHostFree(p);
     if (S->dS) (void)hipFree(S->dS);
     if (S->dP) (void)hipFree(S->dP);
     if (S->dsum) (void)hipFree(S->dsum);
