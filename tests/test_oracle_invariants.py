@@ -33,7 +33,9 @@ def P(N=16, **kw):
 
 
 def consts(p):
-    """Model constants from their definitions (SURVEY notation)."""
+    """Model constants from their definitions: c~ = c N / Box
+    (PMP2MGsolver_fR.f90:54), the background curvatures R and R0, f_R of
+    the background and u_bg = |f_R|^(1/(n+1)) (PMP2MGsolver_fR.f90:59-65)."""
     ctilde = 2.99792458e3 * p.N / p.box
     Rbg = 3.0 * (p.om / p.a ** 3 + 4.0 * p.oml)
     R0 = 3.0 * (p.om + 4.0 * p.oml)
