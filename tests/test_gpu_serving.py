@@ -229,8 +229,15 @@ def test_server_rejects_bad_requests():
             srv.submit(dgp_cfg(16), d, exit_rule=M.EXIT_REFERENCE, res_conv=-1.0)
         bad = d.copy()
         bad[1, 2, 3] = np.inf
-        with pytest.raises(M.MgsError, match="finite"):
+        with pytest.raises(M.MgsError, match=r"finite \(cell 291\)"):  # (1 * 16 + 2) * 16 + 3
             srv.submit(cfg(16), bad)
+        # a field copied by several threads: the bad value in the last slice
+        big = np.zeros((192,) * 3, np.float32)
+        big[191, 190, 7] = np.nan
+        cell = (191 * 192 + 190) * 192 + 7
+        with pytest.raises(M.MgsError, match=rf"finite \(cell {cell}\)"):
+            srv.submit(cfg(192, box=384.0), big)
+        assert srv.stats()["submitted"] == 0
         with pytest.raises(M.MgsError, match="budget"):
             with M.Server(max_active=1, hbm_budget=1024) as tiny:
                 tiny.submit(cfg(16), d)
