@@ -240,23 +240,34 @@ __global__ __launch_bounds__(256) void k_bin_reduce(const double* __restrict__ p
 }
 
 // POTENTfft5's Green's function, PMP2main.f90:736-742, 805-816: the discrete
-// Laplacian's eigenvalue A = (G(ky) - 6 + G(kx)) + G(kz), G(k) = 2cos(P16 k/N)
-// with the reference's P16 = 6.28318530718, and A = 1 for the mean mode
-// (|A| < 1e-7).  The half spectrum (x fastest, N/2+1 of it) is multiplied by
-// trfi / A and by 1/N^3, hipFFT's transforms being unnormalised where
-// FFTPACK5's forward one divides by N per dimension.
+// Laplacian's eigenvalue A = (G(ky) - 6 + G(kx)) + G(kz), A = 1 for the mean
+// mode (|A| < 1e-7).  G is the reference's GREENf table over FFTPACK5's
+// half-complex wavenumbers i = 0 .. N/2: G(0) = 2, G(N/2) = -2 and
+// G(i) = 2cos(P16 i/N) between, with the reference's P16 (PMP2main.f90:711, a
+// REAL*8 parameter set from the default REAL literal 6.28318530718, so it
+// holds that literal's REAL*4 rounding); a complex wavenumber k > N/2 is the
+// half-complex i = N - k.  The half spectrum (x fastest, N/2+1 of it) is
+// multiplied by trfi / A and by 1/N^3, hipFFT's transforms being
+// unnormalised where FFTPACK5's forward one divides by N per dimension.
+__device__ __forceinline__ double green_g(int k, int N)
+{
+    const double P16 = (double)6.28318530718f;
+    const int i = k <= N / 2 ? k : N - k;
+    if (2 * i == N) return -2.0;
+    return 2.0 * cos(P16 * i / N);
+}
+
 __global__ __launch_bounds__(256) void k_green(hipfftDoubleComplex* __restrict__ c, int N, double trfi,
                                                double inv_n3)
 {
     const size_t nh = (size_t)(N / 2 + 1);
     const size_t total = nh * N * N;
-    const double P16 = 6.28318530718;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
         const int kx = (int)(q % nh);
         const size_t r = q / nh;
         const int ky = (int)(r % N), kz = (int)(r / N);
-        const double gx = 2.0 * cos(P16 * kx / N), gy = 2.0 * cos(P16 * ky / N), gz = 2.0 * cos(P16 * kz / N);
+        const double gx = green_g(kx, N), gy = green_g(ky, N), gz = green_g(kz, N);
         double A = gy - 6.0;
         A = gx + A;
         A = A + gz;

@@ -323,23 +323,27 @@ __global__ __launch_bounds__(PM_BLOCK) void k_move(float* __restrict__ xs, float
         const int x1 = I - 1, x2 = I == N ? 0 : I, x3 = I + 1 >= N ? I + 1 - N : I + 1, x0 = I == 1 ? N - 1 : I - 2;
         const int y1 = J - 1, y2 = J == N ? 0 : J, y3 = J + 1 >= N ? J + 1 - N : J + 1, y0 = J == 1 ? N - 1 : J - 2;
         const int z1 = K - 1, z2 = K == N ? 0 : K, z3 = K + 1 >= N ? K + 1 - N : K + 1, z0 = K == 1 ? N - 1 : K - 2;
-#define PHI(a, b, c) (double)phi[(size_t)(a) + n * ((size_t)(b) + n * (size_t)(c))]
-        const double F111 = PHI(x1, y1, z1), F211 = PHI(x2, y1, z1), F121 = PHI(x1, y2, z1), F221 = PHI(x2, y2, z1);
-        const double F112 = PHI(x1, y1, z2), F212 = PHI(x2, y1, z2), F122 = PHI(x1, y2, z2), F222 = PHI(x2, y2, z2);
-        const double F113 = PHI(x1, y1, z3), F213 = PHI(x2, y1, z3), F123 = PHI(x1, y2, z3), F223 = PHI(x2, y2, z3);
-        const double F110 = PHI(x1, y1, z0), F210 = PHI(x2, y1, z0), F120 = PHI(x1, y2, z0), F220 = PHI(x2, y2, z0);
-        const double F311 = PHI(x3, y1, z1), F321 = PHI(x3, y2, z1), F131 = PHI(x1, y3, z1), F231 = PHI(x2, y3, z1);
-        const double F312 = PHI(x3, y1, z2), F322 = PHI(x3, y2, z2), F132 = PHI(x1, y3, z2), F232 = PHI(x2, y3, z2);
-        const double F011 = PHI(x0, y1, z1), F021 = PHI(x0, y2, z1), F101 = PHI(x1, y0, z1), F201 = PHI(x2, y0, z1);
-        const double F012 = PHI(x0, y1, z2), F022 = PHI(x0, y2, z2), F102 = PHI(x1, y0, z2), F202 = PHI(x2, y0, z2);
+#define PHI(a, b, c) phi[(size_t)(a) + n * ((size_t)(b) + n * (size_t)(c))]
+        const float F111 = PHI(x1, y1, z1), F211 = PHI(x2, y1, z1), F121 = PHI(x1, y2, z1), F221 = PHI(x2, y2, z1);
+        const float F112 = PHI(x1, y1, z2), F212 = PHI(x2, y1, z2), F122 = PHI(x1, y2, z2), F222 = PHI(x2, y2, z2);
+        const float F113 = PHI(x1, y1, z3), F213 = PHI(x2, y1, z3), F123 = PHI(x1, y2, z3), F223 = PHI(x2, y2, z3);
+        const float F110 = PHI(x1, y1, z0), F210 = PHI(x2, y1, z0), F120 = PHI(x1, y2, z0), F220 = PHI(x2, y2, z0);
+        const float F311 = PHI(x3, y1, z1), F321 = PHI(x3, y2, z1), F131 = PHI(x1, y3, z1), F231 = PHI(x2, y3, z1);
+        const float F312 = PHI(x3, y1, z2), F322 = PHI(x3, y2, z2), F132 = PHI(x1, y3, z2), F232 = PHI(x2, y3, z2);
+        const float F011 = PHI(x0, y1, z1), F021 = PHI(x0, y2, z1), F101 = PHI(x1, y0, z1), F201 = PHI(x2, y0, z1);
+        const float F012 = PHI(x0, y1, z2), F022 = PHI(x0, y2, z2), F102 = PHI(x1, y0, z2), F202 = PHI(x2, y0, z2);
 #undef PHI
-        // doubled gradients at the 8 nodes (the 1/2 is in pconst)
-        const double GX111 = F211 - F011, GX211 = F311 - F111, GX121 = F221 - F021, GX221 = F321 - F121;
-        const double GX112 = F212 - F012, GX212 = F312 - F112, GX122 = F222 - F022, GX222 = F322 - F122;
-        const double GY111 = F121 - F101, GY211 = F221 - F201, GY121 = F131 - F111, GY221 = F231 - F211;
-        const double GY112 = F122 - F102, GY212 = F222 - F202, GY122 = F132 - F112, GY222 = F232 - F212;
-        const double GZ111 = F112 - F110, GZ211 = F212 - F210, GZ121 = F122 - F120, GZ221 = F222 - F220;
-        const double GZ112 = F113 - F111, GZ212 = F213 - F211, GZ122 = F123 - F121, GZ222 = F223 - F221;
+        // doubled gradients at the 8 nodes (the 1/2 is in pconst); the
+        // reference's F's are default REAL, so each difference is a REAL*4
+        // subtraction (PMP2main.f90:580-602), widened afterwards
+#define G2(a, b) (double)((a) - (b))
+        const double GX111 = G2(F211, F011), GX211 = G2(F311, F111), GX121 = G2(F221, F021), GX221 = G2(F321, F121);
+        const double GX112 = G2(F212, F012), GX212 = G2(F312, F112), GX122 = G2(F222, F022), GX222 = G2(F322, F122);
+        const double GY111 = G2(F121, F101), GY211 = G2(F221, F201), GY121 = G2(F131, F111), GY221 = G2(F231, F211);
+        const double GY112 = G2(F122, F102), GY212 = G2(F222, F202), GY122 = G2(F132, F112), GY222 = G2(F232, F212);
+        const double GZ111 = G2(F112, F110), GZ211 = G2(F212, F210), GZ121 = G2(F122, F120), GZ221 = G2(F222, F220);
+        const double GZ112 = G2(F113, F111), GZ212 = G2(F213, F211), GZ122 = G2(F123, F121), GZ222 = G2(F223, F221);
+#undef G2
         const double GX = pconst * (T3 * (T2T1 * GX111 + T2D1 * GX211 + D2T1 * GX121 + D2D1 * GX221) +
                                     D3 * (T2T1 * GX112 + T2D1 * GX212 + D2T1 * GX122 + D2D1 * GX222));
         const double GY = pconst * (T3 * (T2T1 * GY111 + T2D1 * GY211 + D2T1 * GY121 + D2D1 * GY221) +
@@ -377,10 +381,13 @@ __global__ __launch_bounds__(PM_BLOCK) void k_move(float* __restrict__ xs, float
 
 // WPAR = YN**3/FLOAT(Nparticles) (PMP2mod_density.f90:40, PMP2main.f90:470):
 // the particle count is converted to default REAL first, as there
+// WPAR = YN**3/FLOAT(Nparticles) (PMP2mod_density.f90:40, PMP2main.f90:470):
+// the count through default REAL, the quotient in REAL*8, stored in WPAR, an
+// implicitly typed default REAL
 static double wpar_of(int N, int64_t np)
 {
     const double yn = (double)N;
-    return yn * yn * yn / (double)(float)np;
+    return (double)(float)(yn * yn * yn / (double)(float)np);
 }
 
 static int pm_ensure_particles(mgs_handle* h, int64_t np)

@@ -363,6 +363,75 @@ _PHILOX_W0 = np.uint32(0x9E3779B9)
 _PHILOX_W1 = np.uint32(0xBB67AE85)
 
 
+class RefPmLib:
+    """The reference's PM-step routines (DENSIT, POTENTfft5, POTENT_FOR_MG,
+    MOVE, POWERfft5) on the Tools module state, through
+    oracle/ref_pm_harness.f90 (oracle/_ref/libmgref_pm.so).  Golden-vector
+    generation only."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "_ref", "libmgref_pm.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = C.CDLL(path)
+        L.ref_pm_init.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64] + [C.c_double] * 7
+        fp = C.POINTER(C.c_float)
+        dp = C.POINTER(C.c_double)
+        L.ref_pm_put_particles.argtypes = [fp] * 6
+        L.ref_pm_get_particles.argtypes = [fp] * 6
+        L.ref_pm_put.argtypes = [C.c_int, fp]
+        L.ref_pm_get.argtypes = [C.c_int, fp]
+        L.ref_pm_move.argtypes = [C.c_double]
+        L.ref_pm_move.restype = C.c_double
+        L.ref_pm_power.argtypes = [C.c_int, C.c_int, C.c_int] + [dp] * 6
+        self.L = L
+        self.N = None
+        self.np = 0
+
+    def init(self, N, model=1, fr_n=1, npart=1, box=64.0, om=0.3, oml=0.7, a=1.0, astep=0.01, fr0=-1e-5,
+             h0rc=1.0):
+        self.N, self.np = N, int(npart)
+        self.L.ref_pm_init(N, model, fr_n, int(npart), box, om, oml, a, astep, fr0, h0rc)
+
+    def put_particles(self, *arrs):
+        bufs = [np.ascontiguousarray(a, np.float32) for a in arrs]
+        assert all(b.size == self.np for b in bufs) and len(bufs) == 6
+        self.L.ref_pm_put_particles(*[b.ctypes.data_as(C.POINTER(C.c_float)) for b in bufs])
+
+    def get_particles(self):
+        bufs = [np.empty(self.np, np.float32) for _ in range(6)]
+        self.L.ref_pm_get_particles(*[b.ctypes.data_as(C.POINTER(C.c_float)) for b in bufs])
+        return bufs
+
+    def put(self, which, arr):
+        a = np.ascontiguousarray(arr, np.float32)
+        assert a.size == self.N ** 3
+        self.L.ref_pm_put(which, a.ctypes.data_as(C.POINTER(C.c_float)))
+
+    def get(self, which):
+        a = np.empty((self.N,) * 3, np.float32)
+        self.L.ref_pm_get(which, a.ctypes.data_as(C.POINTER(C.c_float)))
+        return a
+
+    def densit(self):
+        self.L.ref_pm_densit()
+
+    def potentfft5(self):
+        self.L.ref_pm_potentfft5()
+
+    def potent_for_mg(self):
+        self.L.ref_pm_potent_for_mg()
+
+    def move(self, factv=1.0):
+        return self.L.ref_pm_move(factv)
+
+    def power(self, ipol=0, iswitch=1, icic=0):
+        h = self.N // 2
+        out = [np.zeros(h) for _ in range(6)]
+        self.L.ref_pm_power(ipol, iswitch, icic, *[o.ctypes.data_as(C.POINTER(C.c_double)) for o in out])
+        return out
+
+
 def philox4x32(c0, c1, c2, c3, k0, k1, rounds=10):
     """Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) on uint32 arrays."""
     c0 = c0.astype(np.uint32); c1 = c1.astype(np.uint32)

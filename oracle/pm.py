@@ -7,21 +7,29 @@ the product.  Arrays are indexed [k, j, i] (x fastest in memory, the
 reference's FI order); particle positions are REAL*4 in grid units in
 [1, N+1) with cell I = INT(X), as in the reference.
 
-Parity pins (tests/test_oracle_pm.py): mass conservation and exact node /
-lattice / wrap cases of the deposit, the single-Fourier-mode closed form and
-the discrete-Laplacian inverse of the Poisson solve, and exact node gradients
-of a sinusoidal potential, the wrap and the REAL*4 edge rule of the push.
-The reference's own PM routines are not built (PMP2main.f90 is the main
-program with the rest of the simulation around it), so no golden vectors
-exist for them.
+Parity pins:
+* golden vectors from the reference's own DENSIT, POTENTfft5, POTENT_FOR_MG
+  and MOVE, compiled from its sources by oracle/Makefile (libmgref_pm.so)
+  and written by tests/golden/make_golden.py (tests/test_oracle_pm_golden.py):
+  MOVE and POTENT_FOR_MG bit for bit, the deposit and the Poisson solve to
+  the REAL*4 rounding the reference's FI storage adds;
+* invariants (tests/test_oracle_pm.py): mass conservation and exact node /
+  lattice / wrap cases of the deposit, the single-Fourier-mode closed form
+  and the discrete-Laplacian inverse of the Poisson solve, exact node
+  gradients of a sinusoidal potential, the wrap and the REAL*4 edge rule of
+  the push.
 """
 import numpy as np
 
+import oracle as O
+
 
 def wpar(N, npart):
-    """WPAR = YN**3/FLOAT(Nparticles): the count goes through default REAL
-    (PMP2mod_density.f90:40, PMP2main.f90:470)."""
-    return float(N) ** 3 / float(np.float32(npart))
+    """WPAR = YN**3/FLOAT(Nparticles) (PMP2mod_density.f90:40,
+    PMP2main.f90:470): the count goes through default REAL, the quotient is
+    formed in REAL*8 (YN is REAL*8) and stored in WPAR, an implicitly typed
+    default REAL."""
+    return float(np.float32(float(N) ** 3 / float(np.float32(npart))))
 
 
 def _cells(X, N):
@@ -51,16 +59,31 @@ def cic_deposit(x, y, z, N):
     return FI
 
 
+# PMP2main.f90:711: real*8, parameter :: P16 = 6.28318530718 -- a default
+# REAL literal, so the parameter holds its REAL*4 rounding
+P16 = float(np.float32(6.28318530718))
+
+
+def green_table(N):
+    """POTENTfft5's GREENf (PMP2main.f90:736-742) indexed by the complex
+    wavenumber k = 0 .. N-1: FFTPACK5's half-complex wavenumber is
+    i = min(k, N - k), G(0) = 2, G(N/2) = -2, G(i) = 2 cos(P16 i / N) between
+    (even N)."""
+    k = np.arange(N)
+    i = np.minimum(k, N - k)
+    G = 2.0 * np.cos(P16 * i / N)
+    G[0] = 2.0
+    G[N // 2] = -2.0
+    return G
+
+
 def green_eigen(N):
     """POTENTfft5's discrete-Laplacian eigenvalues on the half spectrum
     [kz, ky, kx] (PMP2main.f90:736-742, 805-816): A = (G(ky) - 6 + G(kx)) +
-    G(kz), G(k) = 2 cos(P16 k / N) with the reference's P16 = 6.28318530718,
-    and A = 1 where |A| < 1e-7 (the mean mode)."""
-    P16 = 6.28318530718
-    kf = np.arange(N)
-    kh = np.arange(N // 2 + 1)
-    G = 2.0 * np.cos(P16 * kf / N)
-    Gh = 2.0 * np.cos(P16 * kh / N)
+    G(kz) with the reference's GREENf table (green_table) and A = 1 where
+    |A| < 1e-7 (the mean mode)."""
+    G = green_table(N)
+    Gh = G[: N // 2 + 1]
     A = ((G[None, :, None] - 6.0) + Gh[None, None, :]) + G[:, None, None]
     return np.where(np.abs(A) < 1.0e-7, 1.0, A)
 
@@ -86,6 +109,28 @@ def potential_fi(delta, trfi, psi=None):
     return fi
 
 
+def trfi_of(om, aexpn):
+    """POTENTfft5's trfi = 1.5*Om/aexpn (PMP2main.f90:727-731, standard LCDM
+    branch): implicitly typed default REAL, evaluated in REAL*4"""
+    return float(np.float32(np.float32(1.5) * np.float32(om)) / np.float32(aexpn))
+
+
+def potent_for_mg(p, fi, fi2):
+    """POTENT_FOR_MG, PMP2main.f90:869-935 (f(R) and nDGP branches): FI = FI
+    + sf with sf = -fct1 FI2**(n+1) (f(R), fct1 = c~^2 fR_bg / 2) or
+    0.5 FI2 (nDGP); FI2 is REAL*4, so FI2**2 and FI2**3 are REAL*4 products,
+    and the sum is formed in REAL*8 and stored in the REAL*4 FI."""
+    fi2 = np.asarray(fi2, np.float32)
+    if p.model == 1:
+        ctilde = 2.99792458e3 * float(p.N) / float(np.float32(p.box))
+        fct1 = ctilde ** 2 * 0.5 * O.fr_fbg(p)
+        w = fi2 if p.fr_n == 0 else (fi2 * fi2 if p.fr_n == 1 else (fi2 * fi2) * fi2)
+        sf = -fct1 * w.astype(np.float64)
+    else:
+        sf = 0.5 * fi2.astype(np.float64)
+    return (np.asarray(fi, np.float32).astype(np.float64) + sf).astype(np.float32)
+
+
 def laplacian7(phi):
     """7-point discrete Laplacian in grid units, periodic"""
     out = -6.0 * phi
@@ -96,11 +141,11 @@ def laplacian7(phi):
 
 def move(phi, x, y, z, vx, vy, vz, pconst, xconst):
     """MOVE's particle loop for the f(R) and nDGP models, PMP2main.f90:
-    488-684 (no friction or varying-mass terms, Linear_nDGP false): FP64
-    arithmetic on REAL*4 particles and the potential phi [k, j, i].
+    488-684 (no friction or varying-mass terms, Linear_nDGP false): REAL*8
+    arithmetic on REAL*4 particles, node gradients of the REAL*4 potential
+    phi [k, j, i] differenced in REAL*4.
     Returns (x, y, z, vx, vy, vz) as float32 and SVEL."""
-    phi = np.asarray(phi, np.float64)
-    N = phi.shape[0]
+    N = np.asarray(phi).shape[0]
     X = np.asarray(x, np.float32).astype(np.float64)
     Y = np.asarray(y, np.float32).astype(np.float64)
     Z = np.asarray(z, np.float32).astype(np.float64)
@@ -120,8 +165,10 @@ def move(phi, x, y, z, vx, vy, vz, pconst, xconst):
     y0, y1, y2, y3 = idx(J)
     z0, z1, z2, z3 = idx(K)
 
+    phi32 = np.asarray(phi, np.float32)
+
     def F(a, b, c):
-        return phi[c, b, a]
+        return phi32[c, b, a]
 
     F111, F211, F121, F221 = F(x1, y1, z1), F(x2, y1, z1), F(x1, y2, z1), F(x2, y2, z1)
     F112, F212, F122, F222 = F(x1, y1, z2), F(x2, y1, z2), F(x1, y2, z2), F(x2, y2, z2)
@@ -131,12 +178,18 @@ def move(phi, x, y, z, vx, vy, vz, pconst, xconst):
     F312, F322, F132, F232 = F(x3, y1, z2), F(x3, y2, z2), F(x1, y3, z2), F(x2, y3, z2)
     F011, F021, F101, F201 = F(x0, y1, z1), F(x0, y2, z1), F(x1, y0, z1), F(x2, y0, z1)
     F012, F022, F102, F202 = F(x0, y1, z2), F(x0, y2, z2), F(x1, y0, z2), F(x2, y0, z2)
-    GX = pconst * (T3 * (T2T1 * (F211 - F011) + T2D1 * (F311 - F111) + D2T1 * (F221 - F021) + D2D1 * (F321 - F121)) +
-                   D3 * (T2T1 * (F212 - F012) + T2D1 * (F312 - F112) + D2T1 * (F222 - F022) + D2D1 * (F322 - F122)))
-    GY = pconst * (T3 * (T2T1 * (F121 - F101) + T2D1 * (F221 - F201) + D2T1 * (F131 - F111) + D2D1 * (F231 - F211)) +
-                   D3 * (T2T1 * (F122 - F102) + T2D1 * (F222 - F202) + D2T1 * (F132 - F112) + D2D1 * (F232 - F212)))
-    GZ = pconst * (T3 * (T2T1 * (F112 - F110) + T2D1 * (F212 - F210) + D2T1 * (F122 - F120) + D2D1 * (F222 - F220)) +
-                   D3 * (T2T1 * (F113 - F111) + T2D1 * (F213 - F211) + D2T1 * (F123 - F121) + D2D1 * (F223 - F221)))
+    # the doubled node gradients GX111 = F211 - F011, ... (:580-602): the
+    # F's are implicitly typed default REAL, so each difference is rounded
+    # to REAL*4 before it meets the REAL*8 weights
+    def G(a, b):
+        return (a - b).astype(np.float64)
+
+    GX = pconst * (T3 * (T2T1 * G(F211, F011) + T2D1 * G(F311, F111) + D2T1 * G(F221, F021) + D2D1 * G(F321, F121)) +
+                   D3 * (T2T1 * G(F212, F012) + T2D1 * G(F312, F112) + D2T1 * G(F222, F022) + D2D1 * G(F322, F122)))
+    GY = pconst * (T3 * (T2T1 * G(F121, F101) + T2D1 * G(F221, F201) + D2T1 * G(F131, F111) + D2D1 * G(F231, F211)) +
+                   D3 * (T2T1 * G(F122, F102) + T2D1 * G(F222, F202) + D2T1 * G(F132, F112) + D2D1 * G(F232, F212)))
+    GZ = pconst * (T3 * (T2T1 * G(F112, F110) + T2D1 * G(F212, F210) + D2T1 * G(F122, F120) + D2D1 * G(F222, F220)) +
+                   D3 * (T2T1 * G(F113, F111) + T2D1 * G(F213, F211) + D2T1 * G(F123, F121) + D2D1 * G(F223, F221)))
     VVX, VVY, VVZ = VVX + GX, VVY + GY, VVZ + GZ
     XN = float(np.float32(np.float32(N) + np.float32(1.0) - np.float32(1.0e-8)))
     YN = float(N)
@@ -153,8 +206,15 @@ def move(phi, x, y, z, vx, vy, vz, pconst, xconst):
 
 
 def pconst_xconst(aexpn, astep, om, oml, factv=1.0):
-    """MOVE's PCONST and XCONST for the LCDM background (PMP2main.f90:390-393)"""
-    pconst = -np.sqrt(aexpn / (om + oml * aexpn ** 3)) * astep * 0.5 / factv
-    ahalf = aexpn + astep / 2.0
-    xconst = astep / np.sqrt(ahalf * (om + oml * ahalf ** 3)) / ahalf
-    return float(pconst), float(xconst)
+    """MOVE's PCONST and XCONST for the LCDM background (PMP2main.f90:333-335):
+    every operand is default REAL (AEXPN, ASTEP, Om, OmL, FactV and the
+    implicitly typed Ahalf), so both are evaluated in REAL*4, left to right,
+    x**3 as (x*x)*x, and only stored in REAL*8."""
+    f = np.float32
+    a, st, om, oml, fv = f(aexpn), f(astep), f(om), f(oml), f(factv)
+    a3 = (a * a) * a
+    pconst = ((-np.sqrt(a / (om + oml * a3))) * st) * f(0.5) / fv
+    ahalf = a + st / f(2.0)
+    h3 = (ahalf * ahalf) * ahalf
+    xconst = st / np.sqrt(ahalf * (om + oml * h3)) / ahalf
+    return float(f(pconst)), float(f(xconst))

@@ -21,7 +21,13 @@ Fixtures (all little-endian numpy .npz, float32 fields):
                                 exits, 32^3: final normalised field
   cycles_{F,W}_{fr,dgp}.npz     3 F_cycle / W_cycle calls, 16^3: residual
                                 history + final field
-(`python make_golden.py cycles` writes only the last group.)
+  pm_*.npz                      the PM-step routines around the solve, 16^3
+                                and 12^3: DENSIT's CIC deposit, POTENTfft5,
+                                POTENT_FOR_MG (f(R) n = 0, 1, 2 and nDGP),
+                                MOVE(FactV), POWERfft5 (NGP and CIC k-space
+                                bins, monopole and quadrupole, linear and log
+                                bins), from oracle/_ref/libmgref_pm.so
+(`python make_golden.py cycles` or `... pm` writes only that group.)
 """
 import os
 import sys
@@ -198,9 +204,99 @@ def cycle_fixtures():
             print(f"cycles_{cname}_{name}", hist)
 
 
+def _particles(rng, N, npart):
+    """REAL*4 positions in [1, N+1): uniform, plus exact nodes, the last
+    cell (I = N, wrapping to cell 1) and values just below N+1 and at 1"""
+    pos = rng.uniform(1.0, N + 1.0, (3, npart)).astype(np.float32)
+    top = np.nextafter(np.float32(N + 1), np.float32(0))
+    special = np.array([1.0, float(N), float(N) + 0.5, top, 1.0 + 1e-6, N / 2 + 1.0], np.float32)
+    for ax in range(3):
+        pos[ax, : len(special)] = np.roll(special, ax)
+        pos[ax, len(special): 2 * len(special)] = special[::-1]
+    pos = np.clip(pos, np.float32(1.0), top)
+    return pos
+
+
+def pm_fixtures():
+    """The reference's DENSIT (CIC), POTENTfft5, POTENT_FOR_MG, MOVE and
+    POWERfft5 on seeded inputs (PMP2mod_density.f90:235-281,
+    PMP2main.f90:304-935, PMP2mod_power.f90:209-463)."""
+    R = O.RefPmLib()
+    rng = np.random.default_rng(20261203)
+    par = np.array
+    # DENSIT: a particle count that makes WPAR inexact
+    N, npart = 16, 3001
+    R.init(N, npart=npart, box=64.0, om=0.3, oml=0.7, a=0.5)
+    pos = _particles(rng, N, npart)
+    zero = np.zeros(npart, np.float32)
+    R.put_particles(pos[0], pos[1], pos[2], zero, zero, zero)
+    R.densit()
+    np.savez_compressed(os.path.join(HERE, "pm_deposit.npz"), x=pos[0], y=pos[1], z=pos[2], fi=R.get(1),
+                        N=par([N]))
+    print("pm_deposit", float(R.get(1).sum()))
+    # POTENTfft5 (trfi = 1.5 Om / AEXPN in default REAL), two grid sizes
+    for N, a in ((16, 0.5), (12, 1.0)):
+        R.init(N, box=64.0, om=0.3, oml=0.7, a=a)
+        delta = (rng.lognormal(0.0, 1.0, (N, N, N)) - 1.4).astype(np.float32)
+        R.put(1, delta)
+        R.potentfft5()
+        np.savez_compressed(os.path.join(HERE, f"pm_potential_{N}.npz"), delta=delta, phi=R.get(1),
+                            params=par([N, 64.0, O.f32(0.3), O.f32(0.7), a]))
+        print(f"pm_potential_{N}", float(np.abs(R.get(1)).max()))
+    # POTENT_FOR_MG: f(R) n = 0, 1, 2 and nDGP
+    N = 16
+    for model, frn, tag in ((1, 0, "fr_n0"), (1, 1, "fr_n1"), (1, 2, "fr_n2"), (2, 1, "dgp")):
+        R.init(N, model=model, fr_n=frn, box=64.0, om=0.3, oml=0.7, a=0.5, fr0=-1e-5, h0rc=1.0)
+        fi = (50.0 * rng.standard_normal((N, N, N))).astype(np.float32)
+        fi2 = (1.0 + 0.3 * rng.standard_normal((N, N, N))).astype(np.float32)
+        if model == 1:
+            fi2 = np.abs(fi2).astype(np.float32)
+        R.put(1, fi)
+        R.put(2, fi2)
+        R.potent_for_mg()
+        np.savez_compressed(os.path.join(HERE, f"pm_potent_for_mg_{tag}.npz"), fi_in=fi, fi2=fi2, fi=R.get(1),
+                            params=par([N, model, frn, 64.0, O.f32(0.3), O.f32(0.7), 0.5, -1e-5, 1.0]))
+        print(f"pm_potent_for_mg_{tag}", float(np.abs(R.get(1) - fi).max()))
+    # MOVE(FactV): a smooth potential, particles with velocities
+    N, npart = 16, 2001
+    for factv, tag in ((1.0, "f1"), (1.2, "f12")):
+        R.init(N, npart=npart, box=64.0, om=0.3, oml=0.7, a=0.5, astep=0.02)
+        kz, ky, kx = np.meshgrid(*(np.arange(N),) * 3, indexing="ij")
+        phi = (40.0 * np.sin(2 * np.pi * (kx + 2 * ky) / N + 0.3) * np.cos(2 * np.pi * kz / N)
+               + 5.0 * rng.standard_normal((N, N, N))).astype(np.float32)
+        pos = _particles(rng, N, npart)
+        vel = (3.0 * rng.standard_normal((3, npart))).astype(np.float32)
+        R.put(1, phi)
+        R.put_particles(pos[0], pos[1], pos[2], vel[0], vel[1], vel[2])
+        enkin = R.move(factv)
+        x, y, z, vx, vy, vz = R.get_particles()
+        np.savez_compressed(os.path.join(HERE, f"pm_move_{tag}.npz"), phi=phi, x_in=pos[0], y_in=pos[1],
+                            z_in=pos[2], vx_in=vel[0], vy_in=vel[1], vz_in=vel[2], x=x, y=y, z=z, vx=vx, vy=vy,
+                            vz=vz, enkin=par([enkin]),
+                            params=par([N, 64.0, O.f32(0.3), O.f32(0.7), 0.5, O.f32(0.02), O.f32(factv)]))
+        print(f"pm_move_{tag}", enkin)
+    # POWERfft5: NGP (iCIC = 0) and CIC (iCIC = 1) k-space bins, monopole
+    # and quadrupole, on one random field per grid size
+    for N in (16, 12):
+        R.init(N, npart=N ** 3, box=100.0)
+        f = rng.standard_normal((N, N, N)).astype(np.float32)
+        out = {"field": f, "params": par([N, 100.0])}
+        for ipol, icic in ((0, 0), (0, 1), (1, 0)):
+            R.put(1, f)
+            k, P, cnt, klog, plog, cntlog = R.power(ipol=ipol, iswitch=1, icic=icic)
+            tag = f"p{ipol}_c{icic}"
+            out.update({f"k_{tag}": k, f"pk_{tag}": P, f"cnt_{tag}": cnt, f"klog_{tag}": klog,
+                        f"pklog_{tag}": plog, f"cntlog_{tag}": cntlog})
+        np.savez_compressed(os.path.join(HERE, f"pm_power_{N}.npz"), **out)
+        print(f"pm_power_{N}", out["cnt_p0_c0"][:4], out["pk_p0_c0"][:3])
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["cycles"]:
         cycle_fixtures()
+    elif sys.argv[1:] == ["pm"]:
+        pm_fixtures()
     else:
         main()
         cycle_fixtures()
+        pm_fixtures()

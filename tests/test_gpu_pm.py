@@ -5,15 +5,18 @@ restatement in oracle/pm.py.
 Bars: the deposit to FP64 rounding (atomic order), the REAL*4 potential to
 one REAL*4 ulp (its FP64 value differs from numpy's FFT only at rounding
 level, so the stored values are equal but for ties); the push bit-exact
-(FP64 arithmetic in the reference's operation order on REAL*4 particles, no
-contraction on either side); SVEL to FP64 rounding (block order)."""
+(the reference's operation order and REAL*4 typing on REAL*4 particles, no
+contraction on either side); SVEL to FP64 rounding (block order).  The same
+kernels also run on the reference's own golden vectors (pm_*.npz)."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle as O
 import pm as P
 import mgsolve as M
-from conftest import gpu_available
+from conftest import GOLDEN, gpu_available
 
 
 def particles(N, n, seed):
@@ -142,6 +145,62 @@ def test_move_matches_oracle_bitwise():
         assert np.array_equal(a, b), (name, int(np.sum(a != b)))
     assert svel == pytest.approx(expect[6], rel=1e-12)
     assert np.all(got[0] >= 1.0) and np.all(got[0] < N + 1.0)
+
+
+def _golden(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+@pytest.mark.gpu
+def test_pm_kernels_match_the_reference_golden_vectors():
+    """The device's deposit, Poisson solve, push and P(k) against the
+    reference's own DENSIT, POTENTfft5, MOVE and POWERfft5 outputs
+    (tests/golden/pm_*.npz): push bit for bit, the rest within the REAL*4
+    rounding of the reference's FI storage (bars as in
+    tests/test_oracle_pm_golden.py)."""
+    f = _golden("pm_deposit")
+    N = int(f["N"][0])
+    print(f"[pm-golden] deposit N={N} np={f['x'].size}", flush=True)
+    with M.Solver(fr_cfg(N)) as S:
+        S.set_particles(f["x"], f["y"], f["z"])
+        S.cic_deposit()
+        d = S.get_density()
+    assert np.max(np.abs(d - f["fi"].astype(np.float64))) < 2e-6
+    for N in (16, 12):
+        f = _golden(f"pm_potential_{N}")
+        _, box, om, oml, a = [float(v) for v in f["params"]]
+        print(f"[pm-golden] potential N={N}", flush=True)
+        with M.Solver(fr_cfg(N)) as S:
+            S.set_density(f["delta"])
+            S.potential(P.trfi_of(om, a))
+            phi = S.get_potential(np.float32).astype(np.float64)
+        ref = f["phi"].astype(np.float64)
+        assert np.max(np.abs(phi - ref)) < 2e-6 * np.max(np.abs(ref))
+    for tag in ("f1", "f12"):
+        f = _golden(f"pm_move_{tag}")
+        N, box, om, oml, a, astep, factv = [float(v) for v in f["params"]]
+        N = int(N)
+        pc, xc = P.pconst_xconst(a, astep, om, oml, factv)
+        print(f"[pm-golden] move {tag}", flush=True)
+        with M.Solver(fr_cfg(N)) as S:
+            S.set_potential(f["phi"])
+            S.set_particles(f["x_in"], f["y_in"], f["z_in"], f["vx_in"], f["vy_in"], f["vz_in"])
+            S.move(pc, xc)
+            got = S.get_particles()
+        for arr, nm in zip(got, ("x", "y", "z", "vx", "vy", "vz")):
+            assert np.array_equal(arr.view(np.int32), f[nm].view(np.int32)), (tag, nm)
+    for N in (16, 12):
+        f = _golden(f"pm_power_{N}")
+        box = float(f["params"][1])
+        print(f"[pm-golden] power N={N}", flush=True)
+        cfg = fr_cfg(N)
+        cfg.box = box
+        with M.Solver(cfg) as S:
+            S.set_density(f["field"])
+            k, Pk, cnt = S.power_spectrum(which=0)
+        assert np.array_equal(cnt, f["cnt_p0_c0"])
+        assert np.max(np.abs(Pk / f["pk_p0_c0"] - 1.0)) < 1e-6
+        assert np.max(np.abs(k / f["k_p0_c0"] - 1.0)) < 1e-6
 
 
 @pytest.mark.gpu

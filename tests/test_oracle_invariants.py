@@ -238,6 +238,48 @@ def test_lognormal_density_statistics():
     assert not np.array_equal(d, O.lognormal_density(N, 64.0, 1.2, seed=6))
 
 
+def _binned_model(N, box, slope, cutoff_power):
+    """per-|m|-bin average over the bin's modes of k^slope exp(-k^2/(cutoff_power k_c^2))"""
+    m = O.kgrid(N)
+    kz, ky, kx = np.meshgrid(m, m, m, indexing="ij")
+    rm = np.sqrt(kx * kx + ky * ky + kz * kz).ravel()
+    sel = rm >= 0.5
+    k = 2 * np.pi / box * rm[sel]
+    kc = 0.5 * np.pi * N / box
+    model = k ** slope * np.exp(-k * k / (cutoff_power * kc * kc))
+    b = np.maximum((rm[sel] + 0.5).astype(np.int64), 1)
+    s = np.bincount(b, weights=model, minlength=N + 1)
+    c = np.bincount(b, minlength=N + 1)
+    return (s / np.maximum(c, 1))[1: N // 2 + 1], c[1: N // 2 + 1]
+
+
+def test_lognormal_gaussian_field_has_the_stated_spectrum():
+    """g = log(1 + delta) + sigma^2/2 has P_g(k) proportional to
+    k^-1.5 exp(-k^2 / (2 k_c^2)), k_c = k_Nyq / 2, bin by bin (6 seeds, bins
+    of >= 50 modes, 20 % scatter bound); a filter of the power instead of its
+    square root (slope -3) or a cutoff on the amplitude's scale
+    (exp(-k^2 / k_c^2)) leaves ratios off by more than 50 %."""
+    N, box, sg = 32, 64.0, 1.1
+    acc = np.zeros(N // 2)
+    seeds = range(40, 46)
+    for sd in seeds:
+        d = O.lognormal_density(N, box, sg, seed=sd)
+        g = np.log1p(d) + 0.5 * sg ** 2
+        _, Pk, cnt = O.power_spectrum(g, box)
+        acc += Pk
+    acc /= len(seeds)
+
+    def flatness(slope, cutoff_power):
+        model, c = _binned_model(N, box, slope, cutoff_power)
+        use = c >= 50
+        r = acc[use] / model[use]
+        return np.max(np.abs(r / np.median(r) - 1.0))
+
+    assert flatness(-1.5, 2.0) < 0.2
+    assert flatness(-3.0, 2.0) > 0.5
+    assert flatness(-1.5, 1.0) > 0.5
+
+
 def test_power_spectrum_single_mode():
     N, box = 16, 100.0
     x = np.arange(N)

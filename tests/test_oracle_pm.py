@@ -12,7 +12,10 @@ def test_deposit_conserves_mass_and_is_zero_on_a_lattice():
     rng = np.random.default_rng(3)
     x, y, z = (rng.uniform(1.0, N + 1.0, 500).astype(np.float32) for _ in range(3))
     d = P.cic_deposit(x, y, z, N)
-    assert abs(np.sum(d + 1.0) - N ** 3) <= 1e-9 * N ** 3
+    # each particle deposits WPAR, which the reference holds in REAL*4: the
+    # total is npart * WPAR, within a REAL*4 rounding of N^3
+    assert abs(np.sum(d + 1.0) - x.size * P.wpar(N, x.size)) <= 1e-9 * N ** 3
+    assert abs(x.size * P.wpar(N, x.size) - N ** 3) <= 6e-8 * N ** 3
     # one particle on every node: delta = 0 exactly
     g = np.arange(1, N + 1, dtype=np.float32)
     X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
@@ -49,7 +52,8 @@ def test_poisson_single_mode_closed_form():
     K, J, I = np.meshgrid(i, i, i, indexing="ij")
     for m in ((1, 0, 0), (2, 3, 1), (0, 5, 7)):
         delta = np.cos(2 * np.pi * (m[0] * I + m[1] * J + m[2] * K) / N)
-        A = sum(2 * np.cos(2 * np.pi * mm / N) - 2.0 for mm in m)
+        # the eigenvalue with the reference's P16 (the REAL*4 value of 2 pi)
+        A = sum(2 * np.cos(P.P16 * mm / N) - 2.0 for mm in m)
         phi = P.poisson_fft(delta, 1.7)
         assert np.allclose(phi, 1.7 * delta / A, rtol=0, atol=1e-10 * np.max(np.abs(delta / A)))
 
@@ -61,7 +65,10 @@ def test_poisson_inverts_the_discrete_laplacian():
     trfi = 0.45
     phi = P.poisson_fft(delta, trfi)
     lap = P.laplacian7(phi)
-    assert np.allclose(lap, trfi * (delta - delta.mean()), rtol=0, atol=1e-10)
+    # exact up to the reference's REAL*4 P16 in the Green's function (a
+    # relative 3e-8 change of 2 pi)
+    assert np.allclose(lap, trfi * (delta - delta.mean()), rtol=0, atol=1e-6)
+    assert P.P16 == float(np.float32(2 * np.pi)) and abs(P.P16 / (2 * np.pi) - 1) < 5e-8
     assert phi.mean() == pytest.approx(trfi * delta.mean(), rel=1e-9, abs=1e-14)
 
 
