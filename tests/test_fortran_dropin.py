@@ -1,9 +1,15 @@
-"""The Fortran drop-in drivers (mg-glam-durham_amd/fortran/mgs_extradof.f90):
-the reference's routine names and argument lists, with bodies that call
-libmgsolve.  A Fortran program (tests/fortran/dropin_test.f90) drives them as
-the reference's time-step loop does (scalar_field at two steps, then one
-V-cycle's worth of per-level routines by hand); the same calls made through
-the Python binding must give bit-identical fields and residuals."""
+"""The Fortran drop-in (mg-glam-durham_amd/fortran/mgs_tools_dropin*.f90):
+external procedures with the reference's names and argument types, compiled
+against the reference's own module Tools.
+
+oracle/Makefile's `dropin` target compiles the reference's sources (its main
+program PMP2main.f90 and every module it uses, without PMP2extradof.f90,
+PMP2MGsolver_fR.f90 and PMP2MGsolver_DGP.f90) and links them with the drop-in
+archive and libmgsolve.so, unedited (oracle/_ref/dropin/pmp2main_dropin.exe);
+tests/fortran/ref_caller.f90, a main program over the same Tools module,
+drives the routines as the reference's time-step loop does.  Without a GPU it
+must stop with the library's message; on the GPU the same calls made through
+the Python binding must give bit-identical fields, residuals and particles."""
 import os
 import subprocess
 
@@ -11,77 +17,123 @@ import numpy as np
 import pytest
 
 import oracle as O
+import pm as P
 import mgsolve as M
 from conftest import ROOT, gpu_available
 
-FDIR = os.path.join(ROOT, "mg-glam-durham_amd", "fortran")
-DRIVER = os.path.join(FDIR, "_build", "dropin_test")
-DROPIN = os.path.join(FDIR, "_build", "libmgs_dropin.so")
+DDIR = os.path.join(ROOT, "oracle", "_ref", "dropin")
+MAIN = os.path.join(DDIR, "pmp2main_dropin.exe")
+CALLER = os.path.join(DDIR, "ref_caller")
+ARCHIVE = os.path.join(DDIR, "libmgs_tools_dropin.a")
 
-# the reference routines the drop-in module must provide (module procedures
-# of mgs_extradof; flang mangles them as _QMmgs_extradofP<name>)
-ROUTINES = ["scalar_field", "initialise_extradof", "v_cycle", "restrict_density", "restrict_scalar_field",
-            "correct_solution", "relaxation_iterations_fr", "calculate_residual_fr", "restrict_residual_fr",
-            "calculate_physical_right_hand_side_fr", "relaxation_iterations_dgp", "calculate_residual_dgp",
-            "restrict_residual_dgp", "calculate_physical_right_hand_side_dgp", "densit", "potentfft5",
-            "potent_for_mg", "move"]
+# what PMP2extradof.f90, PMP2MGsolver_fR.f90 and PMP2MGsolver_DGP.f90 define
+# for the rest of the program (external procedures: lower case + underscore)
+SOLVER_ROUTINES = ["scalar_field", "initialise_extradof", "v_cycle", "f_cycle", "w_cycle", "restrict_density",
+                   "restrict_scalar_field", "correct_solution", "relaxation_iterations_fr",
+                   "calculate_residual_fr", "restrict_residual_fr", "calculate_physical_right_hand_side_fr",
+                   "relaxation_iterations_dgp", "calculate_residual_dgp", "restrict_residual_dgp",
+                   "calculate_physical_right_hand_side_dgp"]
+PM_ROUTINES = ["potentfft5", "potent_for_mg", "move", "mgs_densit"]
 
-
-def test_dropin_library_exports_reference_routines():
-    assert os.path.exists(DROPIN), "build() compiles the Fortran drivers"
-    out = subprocess.run(["nm", "-D", "--defined-only", DROPIN], capture_output=True, text=True, check=True).stdout
-    names = {line.split()[-1] for line in out.splitlines() if line.strip()}
-    for r in ROUTINES:
-        assert f"_QMmgs_extradofP{r}" in names, r
+needs_build = pytest.mark.skipif(not os.path.exists(CALLER),
+                                 reason="oracle/_ref/dropin is built where the reference sources are")
 
 
+def _defined(path):
+    out = subprocess.run(["nm", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if len(line.split()) == 3 and line.split()[1] == "T"}
+
+
+@needs_build
+def test_archive_exports_the_reference_routines_as_external_procedures():
+    names = _defined(ARCHIVE)
+    for r in SOLVER_ROUTINES + PM_ROUTINES:
+        assert f"{r}_" in names, r
+    # no module procedure stands in for an external one
+    assert not any(n.startswith("_QMmgs_extradof") for n in names)
+
+
+@needs_build
+def test_reference_main_program_links_against_the_dropin():
+    """PMP2main.f90 and its modules, unedited, resolve every solver routine in
+    the drop-in; the PM routines the main program defines itself stay its
+    own (the drop-in's PM member is not pulled in, so nothing is defined
+    twice)."""
+    names = _defined(MAIN)
+    for r in SOLVER_ROUTINES:
+        assert f"{r}_" in names, r
+    assert "move_" in names and "potentfft5_" in names and "potent_for_mg_" in names
+    assert "mgs_densit_" not in names
+    out = subprocess.run(["nm", "-u", MAIN], capture_output=True, text=True, check=True).stdout
+    assert "mgs_solve" in out and "mgs_create" in out  # the solve goes through libmgsolve
+
+
+@needs_build
 @pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
 def test_dropin_stops_with_library_message_without_gpu(tmp_path):
-    (tmp_path / "params.nml").write_text("&run ngrid=16 /\n")
+    (tmp_path / "params.nml").write_text("&run ngrid_in=16 /\n")
     for n in ("delta1", "delta2", "field0"):
         np.zeros(16 ** 3, np.float32).tofile(tmp_path / f"{n}.bin")
-    p = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    p = subprocess.run([CALLER, str(tmp_path)], capture_output=True, text=True, timeout=60)
     assert p.returncode != 0
-    assert "no CPU fallback" in p.stdout + p.stderr
+    assert "no HIP device available" in p.stdout + p.stderr
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def _fr_bg(om, oml, a, fr0, n):
+    """fR_bg with the reference's typing (Om/AEXPN**3 a REAL*4 quotient)"""
+    f = np.float32
+    q = float(f(om) / ((f(a) * f(a)) * f(a)))
+    R = 3.0 * (q + 4.0 * _f32(oml))
+    R0 = 3.0 * (_f32(om) + 4.0 * _f32(oml))
+    return fr0 * (R0 / R) ** (n + 1)
 
 
 CASES = [(M.MODEL_FR, M.PREC_FP64), (M.MODEL_FR, M.PREC_MIXED), (M.MODEL_DGP, M.PREC_FP64)]
 
 
+@needs_build
 @pytest.mark.gpu
 @pytest.mark.parametrize("model,precision", CASES)
 def test_dropin_matches_python_binding(tmp_path, model, precision):
     N, box, iters, conv = 32, 64.0, 25, 1e-9
-    npart, astep = 20000, 0.02
+    om, oml, a = 0.3, 0.7, 1.0
+    npart, astep, factv = 20000, 0.02, 1.2
     print(f"[dropin] model={model} precision={precision} N={N}", flush=True)
     fr0, h0rc = -1e-5, 1.0
+
     def d(x):  # a Fortran double literal
         return f"{x:.17e}".replace("e", "d")
 
     (tmp_path / "params.nml").write_text(
-        f"&run ngrid={N}, mg_model={model}, fr_n=1, n_branch=1, precision={precision}, iter_max={iters}, "
-        f"box={d(box)}, om={d(0.3)}, oml={d(0.7)}, aexpn={d(1.0)}, fr0={d(fr0)}, h0rc={d(h0rc)}, "
-        f"conv={d(conv)}, npart={npart}, astep_in={d(astep)} /\n")
+        f"&run ngrid_in={N}, mg_model_in={model}, fr_n_in=1, n_branch_in=1, precision={precision}, "
+        f"iter_max={iters}, box_in={d(box)}, om_in={d(om)}, oml_in={d(oml)}, aexpn_in={d(a)}, fr0_in={d(fr0)}, "
+        f"h0rc_in={d(h0rc)}, conv={d(conv)}, npart={npart}, astep_in={d(astep)}, factv_in={d(factv)} /\n")
     d1 = O.lognormal_density(N, box, 1.0, seed=31).astype(np.float32)
     d2 = O.lognormal_density(N, box, 1.1, seed=32).astype(np.float32)
-    cfg = M.Config(N=N, model=model, fr_n=1, box=box, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=fr0, h0rc=h0rc,
-                   precision=precision)
+    # Tools holds the cosmology in REAL*4
+    cfg = M.Config(N=N, model=model, fr_n=1, box=_f32(box), omega_m=_f32(om), omega_l=_f32(oml), aexpn=_f32(a),
+                   fr0=fr0, h0rc=h0rc, precision=precision)
+    fbg = _fr_bg(om, oml, a, fr0, 1)
+    ubg = abs(fbg) ** (1.0 / 2.0)
     rng = np.random.default_rng(5)
+    if model == M.MODEL_FR:
+        f0 = (ubg * (1.0 + 0.1 * rng.random((N,) * 3))).astype(np.float32)
+    else:
+        f0 = (1e-3 * rng.standard_normal((N,) * 3)).astype(np.float32)
+    for name, arr in (("delta1", d1), ("delta2", d2), ("field0", f0)):
+        arr.tofile(tmp_path / f"{name}.bin")
+    part = [rng.uniform(1.0, N + 1.0, npart).astype(np.float32) for _ in range(3)] + \
+           [rng.normal(0.0, 0.2, npart).astype(np.float32) for _ in range(3)]
+    np.concatenate(part).tofile(tmp_path / "particles.bin")
+    p = subprocess.run([CALLER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "The residual is equal to" in p.stdout
     with M.Solver(cfg) as S:
-        _, ubg = S.background()
-        if model == M.MODEL_FR:
-            f0 = (ubg * (1.0 + 0.1 * rng.random((N,) * 3))).astype(np.float32)
-        else:
-            f0 = (1e-3 * rng.standard_normal((N,) * 3)).astype(np.float32)
-        for name, a in (("delta1", d1), ("delta2", d2), ("field0", f0)):
-            a.tofile(tmp_path / f"{name}.bin")
-        part = [rng.uniform(1.0, N + 1.0, npart).astype(np.float32) for _ in range(3)] + \
-               [rng.normal(0.0, 0.2, npart).astype(np.float32) for _ in range(3)]
-        np.concatenate(part).tofile(tmp_path / "particles.bin")
-        p = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
-        assert p.returncode == 0, p.stdout + p.stderr
-        assert "The residual is equal to" in p.stdout
-        scale = 1.0 / ubg if model == M.MODEL_FR else 1.0
+        scale = abs(fbg) ** (-1.0 / 2.0) if model == M.MODEL_FR else 1.0
         # scalar_field(1): initialise, solve from the background
         S.initialise()
         S.set_density(d1)
@@ -89,7 +141,7 @@ def test_dropin_matches_python_binding(tmp_path, model, precision):
         s1 = S.get_field(np.float32, scale=scale)
         got1 = np.fromfile(tmp_path / "solve1.bin", np.float32).reshape(N, N, N)
         assert np.array_equal(got1, s1)
-        # scalar_field(2): FI2 de-normalised in FP32 (PMP2extradof.f90:47-62)
+        # scalar_field(2): FI2 de-normalised in REAL*4 (PMP2extradof.f90:44-60)
         start = (s1.astype(np.float64) * ubg).astype(np.float32) if model == M.MODEL_FR else s1
         S.set_density(d2)
         S.set_field(start)
@@ -119,24 +171,24 @@ def test_dropin_matches_python_binding(tmp_path, model, precision):
         S.cic_deposit()
         dens = S.get_density(np.float32)
         S.set_density(dens)
-        S.potential(1.5 * 0.3 / 1.0)
+        S.potential(P.trfi_of(om, a))
         phi = S.get_potential(np.float32)
         S.set_potential(phi)
         S.potential_add_fifth_force()
         phimg = S.get_potential(np.float32)
         S.set_particles(*part)
         S.set_potential(phimg)
-        pc = -np.sqrt(1.0 / (0.3 + 0.7 * 1.0)) * astep * 0.5 / 1.0
-        ah = 1.0 + astep / 2.0
-        xc = astep / np.sqrt(ah * (0.3 + 0.7 * (ah * ah * ah))) / ah  # ahalf**3 as two products
+        pc, xc = P.pconst_xconst(a, astep, om, oml, factv)
         svel = S.move(pc, xc)
         moved = np.concatenate(S.get_particles())
     got = np.fromfile(tmp_path / "ops.bin", np.float32).reshape(N, N, N)
     assert np.array_equal(got, ops)
     rf = [float(x) for x in (tmp_path / "ops.txt").read_text().split()]
     assert rf == [r0, r1, r2]
-    for name, a in (("pm_density", dens), ("pm_phi", phi), ("pm_phimg", phimg)):
-        assert np.array_equal(np.fromfile(tmp_path / f"{name}.bin", np.float32).reshape(N, N, N), a), name
+    for name, arr in (("pm_density", dens), ("pm_phi", phi), ("pm_phimg", phimg)):
+        assert np.array_equal(np.fromfile(tmp_path / f"{name}.bin", np.float32).reshape(N, N, N), arr), name
     assert np.array_equal(np.fromfile(tmp_path / "pm_part.bin", np.float32), moved)
+    # ENKIN = SVEL/2./(AEXPN+ASTEP/2.)**2 in REAL*4 (PMP2main.f90:689)
+    ah = np.float32(np.float32(a) + np.float32(astep) / np.float32(2.0))
     enkin = float((tmp_path / "pm.txt").read_text())
-    assert enkin == pytest.approx(svel / 2.0 / (1.0 + astep / 2.0) ** 2, rel=1e-15)
+    assert np.float32(enkin) == np.float32(svel / 2.0 / float(ah * ah))
