@@ -283,7 +283,10 @@ typedef struct mgs_request {
     double rel_tol;          /* MGS_EXIT_REL_TOL: stop when RMS(r)/RMS(r0) < rel_tol */
     int32_t exit_rule;       /* MGS_EXIT_REL_TOL (mgs_solve's mode 1) or MGS_EXIT_REFERENCE (mode 0: res_conv
                                 and the stagnation exits of PMP2extradof.f90:107-122) */
-    int32_t reserved;
+    int32_t verify_fp64;     /* 1: mgs_server_wait also solves the request with the plain FP64 method
+                                (a handle of its own, same exit rule) and reports the served field's
+                                difference from that solve in the result; the density stays in host
+                                memory until then */
     double res_conv;         /* MGS_EXIT_REFERENCE: converged when RMS(r) <= res_conv */
 } mgs_request;
 
@@ -291,10 +294,14 @@ typedef struct mgs_request_result {
     int32_t status;          /* 0 or the MGS_E_* code the request failed with */
     int32_t outer;           /* outer iterations */
     int32_t converged;
-    int32_t reserved;
-    double res0, res_final;  /* FP64 RMS residuals */
+    int32_t verified;        /* 1 when the fp64_* fields hold a comparison (verify_fp64) */
+    double res0, res_final;  /* FP64 RMS residuals under the FP64 operator: res_final / res0 is the
+                                served field's relative residual, which every result reports */
     double queued_ms;        /* submit -> admission */
     double latency_ms;       /* submit -> result ready */
+    double fp64_rel_linf;    /* verify_fp64: max |u - u64| / max |u64| against the plain FP64 solve */
+    double fp64_rel_l2;      /* verify_fp64: ||u - u64||_2 / ||u64||_2 */
+    double fp64_res_final;   /* verify_fp64: the FP64 solve's final RMS residual */
 } mgs_request_result;
 
 typedef struct mgs_server_stats {
@@ -318,11 +325,16 @@ const char* mgs_server_last_error(const mgs_server* srv);
 /* enqueue a request; *ticket identifies it for mgs_server_wait */
 int mgs_server_submit(mgs_server* srv, const mgs_request* req, int64_t* ticket);
 /* block until the request is done; copy its field u (raw, x scale) to the
- * host array `u` (may be NULL) and its report to `res` (may be NULL); a
- * ticket can be waited for once.  A finished request's field stays in HBM
- * (counted against hbm_budget) until this call copies it out, or until
- * admission needs the memory and copies it to the host (stats.spilled). */
-int mgs_server_wait(mgs_server* srv, int64_t ticket, void* u, int32_t dtype, double scale, mgs_request_result* res);
+ * host array `u` of u_count elements (may be NULL; MGS_E_ARG, with nothing
+ * written, when u_count is below the request's ngrid^3) and its report to
+ * `res` (may be NULL).  A ticket can be waited for once: a second wait on it,
+ * also one started while the first still blocks, returns MGS_E_STATE.  A
+ * finished request's field stays in HBM (counted against hbm_budget) until
+ * this call copies it out, or until admission needs the memory and copies it
+ * to the host (stats.spilled).  mgs_server_destroy wakes blocked waiters,
+ * which return MGS_E_STATE, and returns once none is left inside. */
+int mgs_server_wait(mgs_server* srv, int64_t ticket, void* u, size_t u_count, int32_t dtype, double scale,
+                    mgs_request_result* res);
 /* *done = 1 when the request has finished (mgs_server_wait then returns
  * without waiting for a solve), else 0; never blocks.  Lets a client claim
  * results in the order they finish. */

@@ -140,15 +140,33 @@ int levelmin_of(int N)
     return l + 1;
 }
 
+int dev_alloc(mgs_handle* h, size_t bytes, void** out, const char* what)
+{
+    *out = nullptr;
+    if (h->use_arena) {
+        const size_t off = (h->arena_used + 255) & ~(size_t)255;
+        if (h->arena && off + bytes > h->arena_cap)
+            return set_err(h, MGS_E_NOMEM, std::string("scratch arena too small for ") + what);
+        h->arena_used = off + bytes;
+        if (h->arena) *out = h->arena + off;
+        return 0;
+    }
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return set_err(h, MGS_E_NOMEM, std::string("hipMalloc ") + what + ": " + hipGetErrorString(e));
+    *out = q;
+    h->allocs.push_back(q);
+    return 0;
+}
+
 template <typename T>
 static int dalloc(mgs_handle* h, T** p, size_t count)
 {
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, count * sizeof(T) + 256);
-    if (e != hipSuccess) return set_err(h, MGS_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    int rc = dev_alloc(h, count * sizeof(T) + 256, &q, "hierarchy");
     *p = (T*)q;
-    h->allocs.push_back(q);
-    return 0;
+    return rc;
 }
 
 static int alloc_hierarchy(mgs_handle* h)
@@ -197,7 +215,10 @@ static int alloc_hierarchy(mgs_handle* h)
     int rc;
     if ((rc = dalloc(h, &h->partials, h->npart))) return rc;
     if ((rc = dalloc(h, &h->dsum, 16))) return rc;
-    HIPCHK(h, hipHostMalloc((void**)&h->hsum, 16 * sizeof(double), hipHostMallocDefault));
+    if (h->use_arena)
+        h->hsum = h->hsum_host;
+    else
+        HIPCHK(h, hipHostMalloc((void**)&h->hsum, 16 * sizeof(double), hipHostMallocDefault));
     return 0;
 }
 
@@ -859,6 +880,66 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
     return 0;
 }
 
+int init_handle(mgs_handle* h, const mgs_params* params, bool use_arena, unsigned char* arena, size_t arena_cap)
+{
+    h->p = *params;
+    h->use_arena = use_arena;
+    h->arena = arena;
+    h->arena_cap = arena_cap;
+    h->arena_used = 0;
+#ifdef MGS_TUNING
+    // tuning builds only (make EXTRA=-DMGS_TUNING): overrides of the mixed
+    // inner cycle's shape; the shipped library's numerics depend on no
+    // environment variable
+    if (!use_arena) {
+        const char* e = std::getenv("MGS_NO_TAIL");
+        h->use_tail = !(e && e[0] == '1');
+        const char* tn = std::getenv("MGS_TAIL_N");
+        const int tv = tn ? std::atoi(tn) : 0;
+        if (tv == 8 || tv == 16 || tv == 32) h->tail_n = tv;
+        const char* w = std::getenv("MGS_INNER_SWEEPS");
+        int a = 0, b = 0;
+        if (w && std::sscanf(w, "%d,%d", &a, &b) == 2 && a >= 1 && b >= 1 && a <= 8 && b <= 8) {
+            h->sweeps0[0] = a;
+            h->sweeps0[1] = b;
+        }
+        const char* im = std::getenv("MGS_INNER_MAX");
+        if (im && std::atoi(im) >= 1 && std::atoi(im) <= 64) h->inner_max = std::atoi(im);
+    }
+#endif
+    h->lmax = levelmax_of(params->ngrid);
+    h->lmin = levelmin_of(params->ngrid);
+    h->lc = h->lmax - h->lmin < 0 ? 0 : h->lmax - h->lmin;
+    int rc = alloc_hierarchy(h);
+    if (rc) return rc;
+    if (h->p.model == MGS_MODEL_DGP) rc = dgp_alloc(h);
+    return rc;
+}
+
+void release_handle(mgs_handle* h)
+{
+    prof_flush(h);
+    drop_graphs(h);
+    for (void* p : h->allocs) (void)hipFree(p);
+    h->allocs.clear();
+    if (h->staging) (void)hipFree(h->staging);
+    h->staging = nullptr;
+    if (h->hsum && h->hsum != h->hsum_host) (void)hipHostFree(h->hsum);
+    h->hsum = nullptr;
+    fft_release(h);
+    pm_release(h);
+}
+
+int solve_on_handle(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep)
+{
+    if (o.init_guess) {
+        double ubg = 0.0;
+        fr_background(h->p, &ubg);
+        launch_fill_f64(h->L[0].u, (size_t)n3d(h->p.ngrid), h->p.model == MGS_MODEL_FR ? ubg : 0.0, h->s);
+    }
+    return h->p.precision == MGS_PREC_MIXED ? solve_mixed(h, o, rep) : solve_plain(h, o, rep);
+}
+
 }  // namespace mgs
 
 // ---------------------------------------------------------------------------
@@ -928,43 +1009,15 @@ int mgs_create(const mgs_params* params, mgs_handle** out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return set_err(nullptr, MGS_E_HIP, "no HIP device available: libmgsolve has no CPU fallback");
     mgs_handle* h = new mgs_handle();
-    h->p = *params;
-    {
-        const char* e = std::getenv("MGS_NO_TAIL");
-        h->use_tail = !(e && e[0] == '1');
-        const char* tn = std::getenv("MGS_TAIL_N");
-        const int tv = tn ? std::atoi(tn) : 0;
-        if (tv == 8 || tv == 16 || tv == 32) h->tail_n = tv;
-        // tuning overrides of the mixed inner cycle's shape
-        const char* w = std::getenv("MGS_INNER_SWEEPS");
-        int a = 0, b = 0;
-        if (w && std::sscanf(w, "%d,%d", &a, &b) == 2 && a >= 1 && b >= 1 && a <= 8 && b <= 8) {
-            h->sweeps0[0] = a;
-            h->sweeps0[1] = b;
-        }
-        const char* im = std::getenv("MGS_INNER_MAX");
-        if (im && std::atoi(im) >= 1 && std::atoi(im) <= 64) h->inner_max = std::atoi(im);
-    }
-    h->lmax = levelmax_of(params->ngrid);
-    h->lmin = levelmin_of(params->ngrid);
-    h->lc = h->lmax - h->lmin < 0 ? 0 : h->lmax - h->lmin;
     if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return set_err(nullptr, MGS_E_HIP, "hipStreamCreate failed");
     }
-    rc = alloc_hierarchy(h);
+    rc = mgs::init_handle(h, params, false, nullptr, 0);
     if (rc) {
         g_create_error = h->err;
         mgs_destroy(h);
         return rc;
-    }
-    if (h->p.model == MGS_MODEL_DGP) {
-        rc = dgp_alloc(h);
-        if (rc) {
-            g_create_error = h->err;
-            mgs_destroy(h);
-            return rc;
-        }
     }
     *out = h;
     return 0;
@@ -974,13 +1027,7 @@ int mgs_destroy(mgs_handle* h)
 {
     if (!h) return 0;
     if (h->s) (void)hipStreamSynchronize(h->s);
-    prof_flush(h);
-    drop_graphs(h);
-    for (void* p : h->allocs) (void)hipFree(p);
-    if (h->staging) (void)hipFree(h->staging);
-    if (h->hsum) (void)hipHostFree(h->hsum);
-    fft_release(h);
-    pm_release(h);
+    mgs::release_handle(h);
     if (h->s) (void)hipStreamDestroy(h->s);
     delete h;
     return 0;
@@ -1300,16 +1347,7 @@ int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* repor
     HIPCHK(h, hipEventCreate(&ev.a));
     HIPCHK(h, hipEventCreate(&ev.b));
     HIPCHK(h, hipEventRecord(ev.a, h->s));
-    if (opts->init_guess) {
-        double ubg = 0.0;
-        fr_background(h->p, &ubg);
-        launch_fill_f64(h->L[0].u, (size_t)n3d(h->p.ngrid), h->p.model == MGS_MODEL_FR ? ubg : 0.0, h->s);
-    }
-    int rc;
-    if (h->p.precision == MGS_PREC_MIXED)
-        rc = solve_mixed(h, *opts, report);
-    else
-        rc = solve_plain(h, *opts, report);
+    const int rc = mgs::solve_on_handle(h, *opts, report);
     HIPCHK(h, hipEventRecord(ev.b, h->s));
     HIPCHK(h, hipEventSynchronize(ev.b));
     float ms = 0.0f;

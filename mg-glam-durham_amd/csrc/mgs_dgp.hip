@@ -299,16 +299,13 @@ int dgp_alloc(mgs_handle* h)
 {
     const size_t half = (size_t)h->p.ngrid * h->p.ngrid * (h->p.ngrid / 2);
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, half * sizeof(double) + 256);
-    if (e != hipSuccess) return set_err(h, MGS_E_NOMEM, "hipMalloc DGP scratch");
-    h->allocs.push_back(q);
+    int rc = dev_alloc(h, half * sizeof(double) + 256, &q, "DGP scratch");
+    if (rc) return rc;
     h->dgp_tmp = (double*)q;
     // FP64 z-marching passes: the second array of the field's ping-pong
     if (h->p.precision == MGS_PREC_FP64 && dgp_zmarch(h->p.ngrid)) {
-        q = nullptr;
-        e = hipMalloc(&q, 2 * half * sizeof(double) + 256);
-        if (e != hipSuccess) return set_err(h, MGS_E_NOMEM, "hipMalloc DGP second array");
-        h->allocs.push_back(q);
+        rc = dev_alloc(h, 2 * half * sizeof(double) + 256, &q, "DGP second array");
+        if (rc) return rc;
         h->dgp_spare64 = (double*)q;
     }
     return 0;

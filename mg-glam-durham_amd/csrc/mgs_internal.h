@@ -76,6 +76,13 @@ struct mgs_handle {
     std::vector<mgs::LevelD> L;
     std::vector<mgs::LevelF> M;
     std::vector<void*> allocs;
+    // arena mode (mgs_solve_noalloc): device memory comes from [arena,
+    // arena + arena_cap) instead of hipMalloc; with arena null it only counts
+    // the bytes a hierarchy needs (arena_used)
+    bool use_arena = false;
+    unsigned char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    double hsum_host[16] = {};  // arena mode: host copy of the norm (no pinned allocation)
     double* partials = nullptr;
     size_t npart = 0;
     double* dsum = nullptr;
@@ -97,11 +104,13 @@ struct mgs_handle {
     hipGraphExec_t g_plain = nullptr;
     hipGraphExec_t g_inner = nullptr;
     bool use_graphs = true;
-    bool use_tail = true;  // mixed f(R): small inner levels in one launch (MGS_NO_TAIL=1 turns it off)
-    int tail_n = 16;  // the tail starts at the first level with n <= tail_n (MGS_TAIL_N: 8, 16 or 32)
-    // mixed f(R): level-0 pre/post sweeps of the inner cycle (MGS_INNER_SWEEPS
-    // "a,b" overrides them for mgs_solve only) and the automatic inner-cycle
-    // cap (MGS_INNER_MAX)
+    // mixed f(R): small inner levels in one launch; tuning builds
+    // (-DMGS_TUNING) read MGS_NO_TAIL=1 to turn it off
+    bool use_tail = true;
+    int tail_n = 16;  // the tail starts at the first level with n <= tail_n (tuning builds: MGS_TAIL_N 8, 16 or 32)
+    // mixed f(R): level-0 pre/post sweeps of the inner cycle and the
+    // automatic inner-cycle cap (tuning builds: MGS_INNER_SWEEPS "a,b" and
+    // MGS_INNER_MAX override them for mgs_solve only)
     int sweeps0[2] = {mgs::MIXED_PRE0, mgs::MIXED_POST0};
     int inner_max = mgs::MIXED_INNER_MAX;
     // profiling
@@ -117,6 +126,16 @@ FrCoef fr_coefs(const mgs_params& p, int level);
 DfCoef df_coefs(const mgs_params& p, int level);
 double fr_background(const mgs_params& p, double* ubg);
 int ensure_staging(mgs_handle* h, size_t bytes);
+// device memory for the handle: hipMalloc, or the next 256-byte aligned piece
+// of the handle's arena (use_arena); recorded for mgs_destroy when allocated
+int dev_alloc(mgs_handle* h, size_t bytes, void** out, const char* what);
+// mgs_create's set-up of a handle without a stream of its own; arena mode when
+// use_arena (arena null: count the bytes only)
+int init_handle(mgs_handle* h, const mgs_params* params, bool use_arena, unsigned char* arena, size_t arena_cap);
+// mgs_solve's body on a prepared handle (no timing events)
+int solve_on_handle(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report* rep);
+// release what init_handle allocated (not the stream)
+void release_handle(mgs_handle* h);
 void prof_begin(mgs_handle* h, int kc, double bytes);
 void prof_end(mgs_handle* h);
 int vcycle_plain(mgs_handle* h, int l);

@@ -235,23 +235,36 @@ __global__ __launch_bounds__(PM_BLOCK) void k_bin_scatter(int64_t np, const int*
     }
 }
 
+// The deposit sums in fixed point so that its result does not depend on the
+// order in which particles meet a cell: every CIC contribution (DENSIT's
+// arithmetic in FP64, a value in [0, WPAR]) is rounded once to an integer
+// multiple of 2^-40 and the multiples are added as 64-bit integers, which is
+// exact and associative; a cell then holds -1 + (its integer) * 2^-40.  The
+// result is bitwise the same run to run, within 2^-41 per contribution of
+// the FP64 sum; a cell's integer holds 1 + delta up to 2^24 and converts
+// exactly up to 2^13.
+constexpr double PM_FIX = 1099511627776.0;          // 2^40
+constexpr double PM_UNFIX = 1.0 / 1099511627776.0;  // 2^-40
+
+__device__ __forceinline__ unsigned long long pm_fix(double v) { return __double2ull_rn(v * PM_FIX); }
+
 // one workgroup per tile: the CIC weights of the particles of its runs
-// (DENSIT's arithmetic) accumulated in LDS over the tile's cells and their +1
-// reach, then one atomic per touched cell into rho (a plain add of the cells
-// no other tile reaches measured slower: it waits for the loaded value)
+// accumulated in LDS over the tile's cells and their +1 reach, then one
+// atomic per touched cell into the level's density storage, which holds the
+// fixed-point sums until k_deposit_finish converts them
 __global__ __launch_bounds__(PM_BLOCK) void k_tile_deposit(const float* __restrict__ xs, const float* __restrict__ ys,
                                                            const float* __restrict__ zs,
                                                            const int64_t* __restrict__ runs,
                                                            const int* __restrict__ count,
                                                            const int64_t* __restrict__ first, int ntiles, int TT,
-                                                           double* __restrict__ rho, Geo g, double wpar)
+                                                           unsigned long long* __restrict__ sums, Geo g, double wpar)
 {
-    __shared__ double acc[PM_TCELLS];
+    __shared__ unsigned long long acc[PM_TCELLS];
     const int N = g.n;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int c = count[t];
         if (c == 0) continue;  // uniform over the block
-        for (int i = threadIdx.x; i < PM_TCELLS; i += blockDim.x) acc[i] = 0.0;
+        for (int i = threadIdx.x; i < PM_TCELLS; i += blockDim.x) acc[i] = 0ull;
         __syncthreads();
         const int i00 = (t % TT) * PM_TILE, j00 = ((t / TT) % TT) * PM_TILE, k00 = (t / (TT * TT)) * PM_TILE;
         const int64_t o = first[t];
@@ -267,27 +280,36 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tile_deposit(const float* __restri
             const double T1 = 1.0 - D1, T2 = 1.0 - D2, T3 = 1.0 - D3;
             const double T2W = T2 * wpar, D2W = D2 * wpar;
             const int b = ((K - 1 - k00) * PM_TBUF + (J - 1 - j00)) * PM_TBUF + (I - 1 - i00);
-            atomicAdd(&acc[b], T3 * T1 * T2W);
-            atomicAdd(&acc[b + 1], T3 * D1 * T2W);
-            atomicAdd(&acc[b + PM_TBUF], T3 * T1 * D2W);
-            atomicAdd(&acc[b + PM_TBUF + 1], T3 * D1 * D2W);
-            atomicAdd(&acc[b + PM_TBUF * PM_TBUF], D3 * T1 * T2W);
-            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + 1], D3 * D1 * T2W);
-            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + PM_TBUF], D3 * T1 * D2W);
-            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + PM_TBUF + 1], D3 * D1 * D2W);
+            atomicAdd(&acc[b], pm_fix(T3 * T1 * T2W));
+            atomicAdd(&acc[b + 1], pm_fix(T3 * D1 * T2W));
+            atomicAdd(&acc[b + PM_TBUF], pm_fix(T3 * T1 * D2W));
+            atomicAdd(&acc[b + PM_TBUF + 1], pm_fix(T3 * D1 * D2W));
+            atomicAdd(&acc[b + PM_TBUF * PM_TBUF], pm_fix(D3 * T1 * T2W));
+            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + 1], pm_fix(D3 * D1 * T2W));
+            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + PM_TBUF], pm_fix(D3 * T1 * D2W));
+            atomicAdd(&acc[b + PM_TBUF * PM_TBUF + PM_TBUF + 1], pm_fix(D3 * D1 * D2W));
         }
         __syncthreads();
         for (int i = threadIdx.x; i < PM_TCELLS; i += blockDim.x) {
-            const double v = acc[i];
-            if (v == 0.0) continue;
+            const unsigned long long v = acc[i];
+            if (v == 0ull) continue;
             int gi = i00 + i % PM_TBUF, gj = j00 + (i / PM_TBUF) % PM_TBUF, gk = k00 + i / (PM_TBUF * PM_TBUF);
             gi = gi >= N ? gi - N : gi;
             gj = gj >= N ? gj - N : gj;
             gk = gk >= N ? gk - N : gk;
-            // no-return atomics: the stores do not wait for the old values
-            unsafeAtomicAdd(&rho[split_index(g, gi, gj, gk)], v);
+            atomicAdd(&sums[split_index(g, gi, gj, gk)], v);
         }
         __syncthreads();
+    }
+}
+
+// the fixed-point sums (in place) -> delta = -1 + sum * 2^-40
+__global__ __launch_bounds__(256) void k_deposit_finish(double* __restrict__ rho, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const unsigned long long v = reinterpret_cast<const unsigned long long*>(rho)[i];
+        rho[i] = -1.0 + (double)v * PM_UNFIX;
     }
 }
 
@@ -524,8 +546,9 @@ int mgs_cic_deposit(mgs_handle* h)
     if (rc) return rc;
     const int64_t np = p->np;
     const float* b = p->buf;
-    prof_begin(h, MGS_KC_PM, (double)np * 56.0 + 16.0 * (double)N * N * N);
-    launch_fill_f64(h->L[0].rho, (size_t)N * N * N, -1.0, h->s);
+    prof_begin(h, MGS_KC_PM, (double)np * 56.0 + 32.0 * (double)N * N * N);
+    // the density storage holds the fixed-point sums (zero) until k_deposit_finish
+    launch_fill_f64(h->L[0].rho, (size_t)N * N * N, 0.0, h->s);
     if (hipMemsetAsync(p->bad, 0, sizeof(int), h->s) != hipSuccess ||
         hipMemsetAsync(p->count, 0, (size_t)p->ntiles * sizeof(int), h->s) != hipSuccess)
         return set_err(h, MGS_E_HIP, "deposit");
@@ -537,7 +560,13 @@ int mgs_cic_deposit(mgs_handle* h)
         return set_err(h, MGS_E_HIP, "deposit scan");
     hipLaunchKernelGGL(k_bin_scatter, dim3(gbin), dim3(PM_BLOCK), 0, h->s, np, p->key, p->rank, p->first, p->runs);
     hipLaunchKernelGGL(k_tile_deposit, dim3((unsigned)p->ntiles), dim3(PM_BLOCK), 0, h->s, b, b + np, b + 2 * np,
-                       p->runs, p->count, p->first, p->ntiles, TT, h->L[0].rho, g, wpar_of(N, np));
+                       p->runs, p->count, p->first, p->ntiles, TT, reinterpret_cast<unsigned long long*>(h->L[0].rho),
+                       g, wpar_of(N, np));
+    {
+        const size_t n3 = (size_t)N * N * N;
+        const unsigned gfin = (unsigned)std::min<size_t>((n3 + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_deposit_finish, dim3(gfin), dim3(256), 0, h->s, h->L[0].rho, n3);
+    }
     prof_end(h);
     rc = pm_check_bad(h, "CIC deposit");
     h->has_density = true;

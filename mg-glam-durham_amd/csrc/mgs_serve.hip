@@ -556,6 +556,8 @@ struct Request {
     double rel_tol = 1e-8;
     int32_t exit_rule = MGS_EXIT_REL_TOL;
     double res_conv = 0.0;
+    bool verify = false;    // keep delta until wait, which runs the FP64 comparison solve
+    bool waited = false;    // a waiter has taken the ticket (under mu)
     Clock::time_point t_submit, t_admit, t_done;
     bool done = false;
     mgs_request_result res{};
@@ -606,6 +608,8 @@ struct mgs_server {
     // hierarchies waiters are copying out (both under mu)
     std::deque<Request*> held;
     int claimed = 0;
+    int waiters = 0;  // threads inside mgs_server_wait (under mu); destroy waits for none
+
     // idle host staging buffers by size (under mu)
     std::map<size_t, std::vector<char*>> stages;
     size_t stage_idle = 0;
@@ -844,8 +848,10 @@ int start_request(mgs_server* S, Slot& sl)
     }
     launch_fill_f64(h->u, n3, ubg, S->s);
     if (hipStreamSynchronize(S->s) != hipSuccess) return MGS_E_HIP;
-    give_stage(S, r->delta, r->delta_bytes);
-    r->delta = nullptr;
+    if (!r->verify) {  // a verified request's density stays for the FP64 solve in wait
+        give_stage(S, r->delta, r->delta_bytes);
+        r->delta = nullptr;
+    }
     return 0;
 }
 
@@ -1308,6 +1314,46 @@ int copy_out(mgs_server* S, Hier* h, void* u, int32_t dtype, double scale)
 
 }  // namespace
 
+// The served field's difference from the plain FP64 method on the same
+// request: a handle of its own (mgs_create, FP64), the request's density and
+// exit rule, mgs_solve from the background, the raw fields compared on the
+// host.  Runs in the waiting client's thread, beside the server's stream.
+static int verify_fp64(const Request* r, const std::vector<double>& served, mgs_request_result* res)
+{
+    mgs_params p = r->p;
+    p.precision = MGS_PREC_FP64;
+    mgs_handle* h = nullptr;
+    int rc = mgs_create(&p, &h);
+    if (rc) return rc;
+    const size_t n3 = served.size();
+    std::vector<double> u64(n3);
+    mgs_solve_opts o{};
+    o.mode = r->exit_rule == MGS_EXIT_REL_TOL ? 1 : 0;
+    o.max_cycles = r->exit_rule == MGS_EXIT_REL_TOL ? std::max(100, r->max_outer) : r->max_outer;
+    o.rel_tol = r->rel_tol;
+    o.res_conv = r->res_conv;
+    o.init_guess = 1;
+    mgs_solve_report rep{};
+    rc = mgs_set_density(h, r->delta, r->dtype, MGS_HOST);
+    if (!rc) rc = mgs_solve(h, &o, &rep);
+    if (!rc) rc = mgs_get_field(h, u64.data(), MGS_F64, MGS_HOST, 1.0);
+    mgs_destroy(h);
+    if (rc) return rc;
+    double dmax = 0.0, umax = 0.0, d2 = 0.0, u2 = 0.0;
+    for (size_t q = 0; q < n3; q++) {
+        const double d = served[q] - u64[q];
+        dmax = std::max(dmax, std::fabs(d));
+        umax = std::max(umax, std::fabs(u64[q]));
+        d2 += d * d;
+        u2 += u64[q] * u64[q];
+    }
+    res->verified = 1;
+    res->fp64_rel_linf = umax > 0.0 ? dmax / umax : dmax;
+    res->fp64_rel_l2 = u2 > 0.0 ? std::sqrt(d2 / u2) : std::sqrt(d2);
+    res->fp64_res_final = rep.res_final;
+    return 0;
+}
+
 extern "C" {
 
 int mgs_server_create(const mgs_server_opts* opts, mgs_server** out)
@@ -1331,7 +1377,11 @@ int mgs_server_create(const mgs_server_opts* opts, mgs_server** out)
         return srv_err(nullptr, MGS_E_HIP, "hipMemGetInfo");
     }
     S->budget = o.hbm_budget ? (size_t)o.hbm_budget : (size_t)(0.6 * (double)freeb);
-    if (hipStreamCreate(&S->s) != hipSuccess || hipMalloc((void**)&S->dS, sizeof(SlotDesc) * MAXS) != hipSuccess ||
+    // a non-blocking stream: no null-stream call of any other thread (a
+    // client's, or a waiter's FP64 comparison solve) synchronizes with the
+    // worker's stream, which is being captured into graphs between steps
+    if (hipStreamCreateWithFlags(&S->s, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&S->dS, sizeof(SlotDesc) * MAXS) != hipSuccess ||
         hipMalloc((void**)&S->dP, sizeof(Plan)) != hipSuccess ||
         hipMalloc((void**)&S->dsum, sizeof(double) * MAXS) != hipSuccess ||
         hipHostMalloc((void**)&S->hS, sizeof(SlotDesc) * MAXS, hipHostMallocDefault) != hipSuccess ||
@@ -1355,6 +1405,13 @@ int mgs_server_destroy(mgs_server* S)
     }
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
+    {
+        // blocked waiters wake, see stop and leave with MGS_E_STATE; a waiter
+        // copying a result out finishes first
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv_done.notify_all();
+        S->cv_done.wait(lk, [&] { return S->waiters == 0; });
+    }
     if (S->s) (void)hipStreamSynchronize(S->s);
     S->held.clear();  // their hierarchies are freed with the requests below
     for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -1430,6 +1487,7 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
     r->rel_tol = req->rel_tol;
     r->exit_rule = req->exit_rule;
     r->res_conv = req->res_conv;
+    r->verify = req->verify_fp64 != 0;
     r->delta_bytes = n3 * es;
     r->delta = take_stage(S, r->delta_bytes);
     if (!r->delta) {
@@ -1437,6 +1495,8 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
         return srv_err(S, MGS_E_NOMEM, "host staging buffer");
     }
     if (!copy_finite(r->delta, req->delta, req->dtype, n3)) {
+        give_stage(S, r->delta, r->delta_bytes);
+        r->delta = nullptr;
         delete r;
         for (size_t q = 0; q < n3; q++) {
             const double v =
@@ -1450,6 +1510,12 @@ int mgs_server_submit(mgs_server* S, const mgs_request* req, int64_t* ticket)
         std::lock_guard<std::mutex> lk(S->mu);
         if ((int)S->waiting.size() >= S->o.queue_capacity) {
             S->rejected++;
+            // back to the idle staging (mu is held: the pool is appended to here)
+            if (S->stage_idle + r->delta_bytes <= STAGE_IDLE_MAX) {
+                S->stages[r->delta_bytes].push_back(r->delta);
+                S->stage_idle += r->delta_bytes;
+                r->delta = nullptr;
+            }
             delete r;
             g_server_error = "queue full";
             return MGS_E_BUSY;
@@ -1479,7 +1545,8 @@ int mgs_server_test(mgs_server* S, int64_t ticket, int32_t* done)
     return 0;
 }
 
-int mgs_server_wait(mgs_server* S, int64_t ticket, void* u, int32_t dtype, double scale, mgs_request_result* res)
+int mgs_server_wait(mgs_server* S, int64_t ticket, void* u, size_t u_count, int32_t dtype, double scale,
+                    mgs_request_result* res)
 {
     if (!S) return MGS_E_ARG;
     if (u && dtype != MGS_F32 && dtype != MGS_F64) return srv_err(S, MGS_E_ARG, "dtype");
@@ -1487,10 +1554,24 @@ int mgs_server_wait(mgs_server* S, int64_t ticket, void* u, int32_t dtype, doubl
     Hier* h = nullptr;
     {
         std::unique_lock<std::mutex> lk(S->mu);
+        if (S->stop) return srv_err(S, MGS_E_STATE, "the server is being destroyed");
         auto it = S->reqs.find(ticket);
         if (it == S->reqs.end()) return srv_err(S, MGS_E_ARG, "unknown ticket");
         r = it->second;
-        S->cv_done.wait(lk, [&] { return r->done && !r->spilling; });
+        if (r->waited) return srv_err(S, MGS_E_STATE, "ticket already waited for");
+        const size_t n3 = (size_t)r->p.ngrid * r->p.ngrid * r->p.ngrid;
+        if (u && u_count < n3)
+            return srv_err(S, MGS_E_ARG, "output array of " + std::to_string(u_count) + " elements for a field of " +
+                                             std::to_string(n3));
+        r->waited = true;
+        S->waiters++;
+        S->cv_done.wait(lk, [&] { return (r->done && !r->spilling) || S->stop; });
+        if (!(r->done && !r->spilling)) {
+            // destroy began first: the request (and its memory) is destroy's
+            S->waiters--;
+            S->cv_done.notify_all();
+            return srv_err(S, MGS_E_STATE, "the server was destroyed before the request finished");
+        }
         S->reqs.erase(it);
         if (r->hold) {  // the result is still in HBM: claim it
             h = r->hold;
@@ -1500,30 +1581,46 @@ int mgs_server_wait(mgs_server* S, int64_t ticket, void* u, int32_t dtype, doubl
         }
     }
     int rc = r->res.status;
+    const size_t n3 = (size_t)r->p.ngrid * r->p.ngrid * r->p.ngrid;
+    // a verified request's raw field in FP64, from HBM or its spill
+    std::vector<double> raw;
     if (h) {
-        if (!rc && u) rc = copy_out(S, h, u, dtype, scale);
+        if (!rc && r->verify) {
+            raw.resize(n3);
+            rc = copy_out(S, h, raw.data(), MGS_F64, 1.0);
+        } else if (!rc && u) {
+            rc = copy_out(S, h, u, dtype, scale);
+        }
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->pool[pool_key(h->N, h->model)].push_back(h);
             S->claimed--;
         }
         S->cv_work.notify_all();  // its memory is free for admission
-    } else if (!rc && u) {
-        const size_t n3 = r->u.size();
+    } else if (!rc) {
+        raw.swap(r->u);
+    }
+    if (!rc && u && !raw.empty()) {
         for (size_t q = 0; q < n3; q++) {
-            const double v = scale == 1.0 ? r->u[q] : r->u[q] * scale;
+            const double v = scale == 1.0 ? raw[q] : raw[q] * scale;
             if (dtype == MGS_F32)
                 ((float*)u)[q] = (float)v;
             else
                 ((double*)u)[q] = v;
         }
     }
-    if (res) {
-        *res = r->res;
-        res->status = rc;
-    }
+    if (res) *res = r->res;
+    if (!rc && r->verify && res) rc = verify_fp64(r, raw, res);
+    if (res) res->status = rc;
     if (rc) g_server_error = "request failed (code " + std::to_string(rc) + ")";
+    if (r->delta) give_stage(S, r->delta, r->delta_bytes);
+    r->delta = nullptr;
     delete r;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->waiters--;
+    }
+    S->cv_done.notify_all();
     return rc;
 }
 

@@ -78,14 +78,15 @@ class ServerOpts(C.Structure):
 class Request(C.Structure):
     _fields_ = [
         ("params", Params), ("delta", C.c_void_p), ("dtype", C.c_int32), ("max_outer", C.c_int32),
-        ("rel_tol", C.c_double), ("exit_rule", C.c_int32), ("reserved", C.c_int32), ("res_conv", C.c_double),
+        ("rel_tol", C.c_double), ("exit_rule", C.c_int32), ("verify_fp64", C.c_int32), ("res_conv", C.c_double),
     ]
 
 
 class RequestResult(C.Structure):
     _fields_ = [
-        ("status", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32), ("reserved", C.c_int32),
+        ("status", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32), ("verified", C.c_int32),
         ("res0", C.c_double), ("res_final", C.c_double), ("queued_ms", C.c_double), ("latency_ms", C.c_double),
+        ("fp64_rel_linf", C.c_double), ("fp64_rel_l2", C.c_double), ("fp64_res_final", C.c_double),
     ]
 
 
@@ -157,7 +158,7 @@ def load():
         "mgs_server_destroy": ([H], C.c_int),
         "mgs_server_last_error": ([H], C.c_char_p),
         "mgs_server_submit": ([H, C.POINTER(Request), C.POINTER(C.c_int64)], C.c_int),
-        "mgs_server_wait": ([H, C.c_int64, vp, i32, C.c_double, C.POINTER(RequestResult)], C.c_int),
+        "mgs_server_wait": ([H, C.c_int64, vp, C.c_size_t, i32, C.c_double, C.POINTER(RequestResult)], C.c_int),
         "mgs_server_get_stats": ([H, C.POINTER(ServerStats)], C.c_int),
         "mgs_server_test": ([H, C.c_int64, C.POINTER(i32)], C.c_int),
     }
@@ -254,7 +255,8 @@ class Solver:
 
     def set_density(self, delta):
         a = np.ascontiguousarray(delta)
-        assert a.size == self.N ** 3
+        if a.size != self.N ** 3:
+            raise MgsError(f"density has {a.size} values for a grid of {self.N}^3")
         self._chk(self.L.mgs_set_density(self.h, _np_ptr(a), self._dt(a), HOST), "mgs_set_density")
 
     def get_density(self, dtype=np.float64):
@@ -264,7 +266,8 @@ class Solver:
 
     def set_field(self, u):
         a = np.ascontiguousarray(u)
-        assert a.size == self.N ** 3
+        if a.size != self.N ** 3:
+            raise MgsError(f"field has {a.size} values for a grid of {self.N}^3")
         self._chk(self.L.mgs_set_field(self.h, _np_ptr(a), self._dt(a), HOST), "mgs_set_field")
 
     def get_field(self, dtype=np.float64, scale=1.0):
@@ -274,6 +277,9 @@ class Solver:
 
     def set_level(self, level, which, arr):
         a = np.ascontiguousarray(arr)
+        n = self.N >> level
+        if a.size != n ** 3:
+            raise MgsError(f"level {level} holds {n}^3 values, not {a.size}")
         self._chk(self.L.mgs_set_level_array(self.h, level, which, _np_ptr(a), self._dt(a), HOST),
                   "mgs_set_level_array")
 
@@ -384,7 +390,8 @@ class Solver:
 
     def set_potential(self, phi):
         a = np.ascontiguousarray(phi)
-        assert a.size == self.N ** 3
+        if a.size != self.N ** 3:
+            raise MgsError(f"potential has {a.size} values for a grid of {self.N}^3")
         self._chk(self.L.mgs_set_potential(self.h, _np_ptr(a), self._dt(a), HOST), "mgs_set_potential")
 
     def move(self, pconst, xconst):
@@ -423,6 +430,7 @@ class Server:
     def __init__(self, max_active=16, hbm_budget=0, queue_capacity=0, use_graphs=True):
         self.L = load()
         self.h = C.c_void_p()
+        self._n = {}  # ticket -> ngrid of the submitted request
         o = ServerOpts(int(hbm_budget), int(max_active), int(queue_capacity), 1 if use_graphs else 0, 0)
         rc = self.L.mgs_server_create(C.byref(o), C.byref(self.h))
         if rc:
@@ -445,36 +453,54 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, cfg: Config, delta, rel_tol=1e-8, max_outer=60, exit_rule=None, res_conv=0.0):
+    def submit(self, cfg: Config, delta, rel_tol=1e-8, max_outer=60, exit_rule=None, res_conv=0.0,
+               verify_fp64=False):
         """Queue a solve of cfg (f(R) or nDGP, PREC_MIXED) on density delta;
         returns a ticket.  exit_rule EXIT_REL_TOL (the default) stops as
         solve(mode=1, rel_tol=...), EXIT_REFERENCE as solve(mode=0,
-        res_conv=...)."""
+        res_conv=...).  verify_fp64: wait() also solves the request with the
+        plain FP64 method and reports the difference (fp64_rel_linf/_l2)."""
         a = np.ascontiguousarray(delta)
+        if a.dtype not in (np.float32, np.float64):
+            raise MgsError("delta must be float32 or float64")
+        if a.size != cfg.N ** 3:
+            raise MgsError(f"delta has {a.size} values for a grid of {cfg.N}^3")
         dt = F32 if a.dtype == np.float32 else F64
-        if dt == F64:
-            a = np.ascontiguousarray(a, dtype=np.float64)
         rule = EXIT_REL_TOL if exit_rule is None else int(exit_rule)
-        r = Request(cfg.c(), _np_ptr(a), dt, int(max_outer), float(rel_tol), rule, 0, float(res_conv))
+        r = Request(cfg.c(), _np_ptr(a), dt, int(max_outer), float(rel_tol), rule, 1 if verify_fp64 else 0,
+                    float(res_conv))
         t = C.c_int64()
         rc = self.L.mgs_server_submit(self.h, C.byref(r), C.byref(t))
         if rc:
             raise MgsError(f"mgs_server_submit failed ({rc}): {self.L.mgs_server_last_error(self.h).decode()}")
+        self._n[t.value] = cfg.N
         return t.value
 
-    def wait(self, ticket, N, dtype=np.float64, scale=1.0, out=None):
+    def wait(self, ticket, N=None, dtype=np.float64, scale=1.0, out=None):
         """Block until the ticket is done; returns (field u, result dict).
-        out: an (N, N, N) C-contiguous array of dtype to fill (reusing one
-        avoids first-touch page faults on every result)."""
+        N, when given, must be the submitted grid's; dtype float32 or
+        float64.  out: an (N, N, N) C-contiguous array of dtype to fill
+        (reusing one avoids first-touch page faults on every result)."""
+        n_sub = self._n.get(int(ticket))
+        if N is None:
+            N = n_sub
+        if N is None:
+            raise MgsError("unknown ticket: give N")
+        if n_sub is not None and N != n_sub:
+            raise MgsError(f"ticket {ticket} was submitted with N = {n_sub}, not {N}")
+        dtype = np.dtype(dtype)
+        if dtype not in (np.float32, np.float64):
+            raise MgsError("dtype must be float32 or float64")
         if out is None:
             u = np.empty((N, N, N), dtype=dtype)
         else:
-            if out.shape != (N, N, N) or out.dtype != np.dtype(dtype) or not out.flags["C_CONTIGUOUS"]:
+            if out.shape != (N, N, N) or out.dtype != dtype or not out.flags["C_CONTIGUOUS"]:
                 raise MgsError("out must be a C-contiguous (N, N, N) array of the requested dtype")
             u = out
         res = RequestResult()
-        rc = self.L.mgs_server_wait(self.h, int(ticket), _np_ptr(u), F32 if dtype == np.float32 else F64,
+        rc = self.L.mgs_server_wait(self.h, int(ticket), _np_ptr(u), u.size, F32 if dtype == np.float32 else F64,
                                     float(scale), C.byref(res))
+        self._n.pop(int(ticket), None)
         out = {k: getattr(res, k) for k, _ in RequestResult._fields_}
         if rc:
             raise MgsError(f"mgs_server_wait failed ({rc}): {self.L.mgs_server_last_error(self.h).decode()}")

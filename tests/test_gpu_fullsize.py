@@ -11,6 +11,9 @@ reference's CPU case at 128^3, where the oracle's whole solve is quick:
   * nDGP 512^3 (normal branch, H0rc=1, sigma_g=1.2, seed 1005): the mixed
     solve reaches the FP64 solve's residual floor, measured by the oracle's
     operator, and the same field up to the free constant;
+  * f(R) 1024^3 (n=1, |fR0|=1e-5, sigma_g=1.2, seed 1003), the single
+    large solve: mixed and FP64 fields both satisfy the oracle's FP64
+    operator to relative residual 1e-8 and agree with each other;
   * PM step at 512^3 with 512^3 particles: the deposit conserves mass
     (sum of 1 + delta = N^3), the density is >= -1, the potential with the
     fifth force is finite, and the push keeps every particle inside [1, N+1).
@@ -89,6 +92,36 @@ def test_fr512_mixed_solve_meets_the_oracle_operator():
         assert r < 1.05e-8 * r0
         assert abs(rep["res0"] - r0) <= 1e-12 * r0
     err = rel_linf(out[M.PREC_MIXED][0], out[M.PREC_FP64][0])
+    print(f"  mixed vs FP64 field rel. L_inf {err:.2e}", flush=True)
+    assert err < 1e-6
+
+
+@pytest.mark.gpu
+def test_config3_fr1024_mixed_and_fp64_meet_the_oracle_operator():
+    n, box, fr0, sigma, seed = 1024, 1024.0, -1e-5, 1.2, 1003
+    op = O.Params(N=n, model=1, fr_n=1, branch=1, box=box, om=0.3, oml=0.7, a=1.0, fr0=fr0, h0rc=1.0, polish=1)
+    delta = None
+    fields = {}
+    for prec in (M.PREC_MIXED, M.PREC_FP64):
+        c = M.Config(N=n, model=M.MODEL_FR, fr_n=1, box=box, omega_m=0.3, omega_l=0.7, aexpn=1.0, fr0=fr0,
+                     precision=prec)
+        print(f"[fullsize] f(R) 1024^3 precision={prec}", flush=True)
+        with M.Solver(c) as S:
+            if delta is None:
+                S.make_lognormal_density(seed, sigma)
+                delta = S.get_density(np.float64)
+            else:
+                S.set_density(delta)
+            rep = S.solve(mode=1, rel_tol=1e-8, max_cycles=60)
+            assert rep["converged"], rep
+            fields[prec] = S.get_field()
+        # the residual of u = u_bg is (Om/a) delta exactly
+        r0 = 0.3 * float(np.sqrt(np.mean(delta * delta)))
+        assert abs(rep["res0"] / r0 - 1.0) < 1e-10
+        r = O.fr_residual(op, 0, fields[prec], delta, None, None, "f64")
+        print(f"  {rep['seconds'] * 1e3:.1f} ms, oracle rel. residual {r / r0:.3e}", flush=True)
+        assert r < 1.05e-8 * r0
+    err = rel_linf(fields[M.PREC_MIXED], fields[M.PREC_FP64])
     print(f"  mixed vs FP64 field rel. L_inf {err:.2e}", flush=True)
     assert err < 1e-6
 

@@ -214,6 +214,131 @@ def test_spill_and_claim_with_concurrent_submit_and_wait():
 
 
 @pytest.mark.gpu
+def test_config4_stream_meets_the_oracle_operator_alone_equals_mix():
+    """The serving stream's workload: f(R) n=1 requests with N drawn from
+    {64, 128, 192, 256}, |fR0| from {1e-4, 1e-5, 1e-6}, sigma_g from
+    U[0.8, 1.6], seeded lognormal densities, arriving while others run.
+    Every served field meets the oracle's FP64 operator at relative residual
+    1e-8 (+5 % for the norm's summation order); four of them (one per size)
+    are bitwise the same served alone; two carry the FP64 comparison solve
+    (verify_fp64), whose field they match to 1e-6."""
+    rng = np.random.default_rng(404)
+    sizes = [64, 128, 192, 256] * 6
+    rng.shuffle(sizes)
+    reqs = []
+    for i, N in enumerate(sizes):
+        fr0 = -float(rng.choice([1e-4, 1e-5, 1e-6]))
+        sg = float(rng.uniform(0.8, 1.6))
+        c = cfg(N, fr0=fr0, box=2.0 * N)
+        with M.Solver(M.Config(N=N, model=M.MODEL_FR, box=2.0 * N, precision=M.PREC_MIXED)) as G:
+            G.make_lognormal_density(2000 + i, sg)
+            d = G.get_density(np.float64)
+        reqs.append((c, d))
+    print(f"[serving] config-4 stream: {len(reqs)} requests", flush=True)
+    tol = 1e-8
+    verify = {3, 17}
+    with M.Server(max_active=16) as srv:
+        tickets = []
+        for i, (c, d) in enumerate(reqs):
+            tickets.append(srv.submit(c, d, rel_tol=tol, verify_fp64=(i in verify)))
+            time.sleep(float(rng.exponential(0.002)))  # Poisson arrivals
+        out = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["max_batch"] >= 4, st
+    for i, ((c, d), (u, r)) in enumerate(zip(reqs, out)):
+        assert r["status"] == 0 and r["converged"], (i, r)
+        op = O.Params(N=c.N, model=1, fr_n=1, branch=1, box=c.box, om=0.3, oml=0.7, a=1.0, fr0=c.fr0, h0rc=1.0)
+        res = O.fr_residual(op, 0, u, d, None, None, "f64")
+        r0 = 0.3 * float(np.sqrt(np.mean(d * d)))  # residual of u = u_bg: (Om/a) delta
+        assert abs(r["res0"] / r0 - 1.0) < 1e-10, (i, r["res0"], r0)
+        assert res < 1.05 * tol * r0, (i, c.N, res / r0)
+        if i in verify:
+            assert r["verified"] == 1 and r["fp64_rel_linf"] < 1e-6, (i, r)
+            assert r["fp64_res_final"] < tol * r["res0"], (i, r)
+        else:
+            assert r["verified"] == 0
+    seen = {}
+    for i, (c, _) in enumerate(reqs):
+        seen.setdefault(c.N, i)
+    with M.Server(max_active=16) as alone:
+        for N, i in sorted(seen.items()):
+            c, d = reqs[i]
+            u_alone, _ = alone.wait(alone.submit(c, d, rel_tol=tol))
+            assert np.array_equal(u_alone, out[i][0]), (N, i)
+
+
+@pytest.mark.gpu
+def test_wait_misuse_is_an_error_not_a_fault():
+    """A second wait on one ticket, an output array smaller than the field,
+    and the wrong N or dtype are refused; destroying a server with a waiter
+    blocked on an unfinished request wakes it with an error."""
+    import ctypes as C
+    import threading
+
+    d = density(32, 64.0, 1.0, 3, f32=True)
+    with M.Server(max_active=2) as srv:
+        t = srv.submit(cfg(32), d)
+        with pytest.raises(M.MgsError, match="submitted with N"):
+            srv.wait(t, 16)
+        with pytest.raises(M.MgsError, match="dtype"):
+            srv.wait(t, 32, np.int32)
+        small = np.empty(16 ** 3, np.float64)
+        res = M.RequestResult()
+        rc = srv.L.mgs_server_wait(srv.h, t, small.ctypes.data_as(C.c_void_p), small.size, M.F64, 1.0,
+                                   C.byref(res))
+        assert rc == -1 and "elements" in srv.L.mgs_server_last_error(srv.h).decode()
+        u, r = srv.wait(t)
+        assert r["status"] == 0 and u.shape == (32, 32, 32)
+        with pytest.raises(M.MgsError, match="unknown ticket"):
+            srv.wait(t, 32)
+        # two waiters on one ticket: the second is refused while the first blocks
+        t2 = srv.submit(cfg(64, box=128.0), density(64, 128.0, 1.2, 4, f32=True))
+        first = {}
+        th = threading.Thread(target=lambda: first.setdefault("r", srv.wait(t2)))
+        th.start()
+        time.sleep(0.01)
+        rc2 = srv.L.mgs_server_wait(srv.h, t2, None, 0, M.F64, 1.0, None)
+        th.join()
+        assert rc2 in (-4, -1), rc2  # already waited for (-4) or, if the first finished, unknown (-1)
+        assert first["r"][1]["status"] == 0
+    # a full queue refuses a submission (its staging goes back to the pool
+    # without re-taking the server's lock) and the accepted requests finish
+    with M.Server(max_active=1, queue_capacity=1) as srv:
+        dd = density(128, 256.0, 1.5, 6, f32=True)
+        accepted, busy = [], 0
+        for _ in range(4):
+            try:
+                accepted.append(srv.submit(cfg(128, box=256.0), dd, rel_tol=1e-10))
+            except M.MgsError as e:
+                assert "queue full" in str(e)
+                busy += 1
+        assert busy >= 1 and accepted
+        for t in accepted:
+            assert srv.wait(t)[1]["status"] == 0
+        assert srv.stats()["rejected"] == busy
+    # destroy with a blocked waiter
+    srv = M.Server(max_active=1)
+    t3 = srv.submit(cfg(128, box=256.0), density(128, 256.0, 1.5, 5, f32=True), rel_tol=1e-12, max_outer=200)
+    box = {}
+    handle = srv.h
+    entering = threading.Event()
+
+    def waiter():
+        r = M.RequestResult()
+        entering.set()
+        box["rc"] = srv.L.mgs_server_wait(handle, t3, None, 0, M.F64, 1.0, C.byref(r))
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    entering.wait()
+    time.sleep(0.05)  # the waiter is inside mgs_server_wait well before this
+    srv.close()
+    th.join(timeout=60)
+    assert not th.is_alive()
+    assert box["rc"] in (0, -4), box
+
+
+@pytest.mark.gpu
 def test_server_rejects_bad_requests():
     with M.Server(max_active=2) as srv:
         d = np.zeros((16,) * 3, np.float32)

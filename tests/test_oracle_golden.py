@@ -95,6 +95,25 @@ def test_vcycles_match_reference(name, model):
     same(u, fx["u_out"], "field after 6 V-cycles")
 
 
+def test_dgp_jacobi_order_within_stated_tolerance_of_reference():
+    """The device's nDGP smoother updates each colour as a Jacobi sweep; the
+    reference updates in place in loop order within a colour
+    (PMP2MGsolver_DGP.f90:148-233).  The oracle's Jacobi mode, which the
+    device matches bit for bit, stays within the stated tolerance of the
+    reference's six V-cycles: residual history 2e-3 relative at cycle 1 and
+    1e-4 from cycle 2 on, field 5e-5 of its maximum (the same bar the GPU
+    test applies)."""
+    fx = np.load(os.path.join(GOLDEN, "vcycles_dgp.npz"))
+    p = params_of(fx)
+    u = fx["u_in"].astype(np.float64).copy()
+    hist = O.vcycles(p, u, fx["delta"].astype(np.float64).copy(), len(fx["hist"]), dgp_lexi=False, prec="f64")
+    rel = np.abs(hist / fx["hist"] - 1.0)
+    assert rel[0] <= 2e-3 and np.all(rel[1:] <= 1e-4), rel
+    assert np.max(np.abs(u - fx["u_out"])) <= 5e-5 * np.max(np.abs(fx["u_out"]))
+    # the orders do differ (the Jacobi sweep is not the reference's order)
+    assert rel[0] > 1e-5
+
+
 @pytest.mark.parametrize("name", ["cycles_F_fr", "cycles_W_fr", "cycles_F_dgp", "cycles_W_dgp"])
 def test_f_and_w_cycles_match_reference(name):
     """F_cycle and W_cycle (PMP2extradof.f90:563-1157): three calls from the
