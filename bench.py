@@ -68,6 +68,64 @@ def timed_solves(S, steps, warmup, profile, inner):
     return reps, walls, total, prof
 
 
+def host_cpu_info():
+    """CPU model, sockets and the logical CPUs this process may run on"""
+    model, sockets = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("physical id"):
+                    sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count() or 1
+    return {"model": model, "sockets": max(len(sockets), 1), "logical_cpus_visible": visible}
+
+
+def cpu_fp64_solve_baseline(delta64, N, box, fr0, budget_s=60.0, max_repeats=3):
+    """The oracle's FP64 solve (a C restatement of the reference's V(2,2)
+    cycle with OpenMP loops, refined closed-form roots) of the same density
+    to relative residual 1e-8 on the host cores, OMP_PROC_BIND=close,
+    OMP_PLACES=cores; median over up to max_repeats solves within budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import oracle as O
+        O.lib()
+    except Exception as e:
+        log(f"cpu fp64 baseline unavailable: {e}")
+        return None
+    p = O.Params(N=N, model=1, fr_n=1, branch=1, box=box, om=0.3, oml=0.7, a=1.0, fr0=fr0, h0rc=1.0, polish=1)
+    times, rep = [], None
+    t_all = time.perf_counter()
+    while len(times) < max_repeats:
+        t0 = time.perf_counter()
+        _, rep = O.solve(p, delta64, mode=1, rel_tol=1e-8, max_cycles=60, prec="f64")
+        times.append(time.perf_counter() - t0)
+        log(f"[bench] cpu fp64 solve {len(times)}: {times[-1]:.2f} s, {rep['cycles']} cycles, "
+            f"rel. residual {rep['res_final'] / rep['res0']:.2e}")
+        if time.perf_counter() - t_all + times[-1] > budget_s:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    info = host_cpu_info()
+    return {
+        "value": float(np.median(times)),
+        "unit": "s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"whole FP64 solve of the same {N}^3 density to rel. residual 1e-8 by the oracle (C restatement "
+                   f"of the reference's V(2,2) cycle, OpenMP, {threads} threads, OMP_PROC_BIND=close, "
+                   f"OMP_PLACES=cores): {rep['cycles']} cycles, rel. residual "
+                   f"{rep['res_final'] / rep['res0']:.2e}; median of {len(times)}"),
+        "solves_s": times,
+        "cpu": info,
+    }
+
+
 def cpu_reference_baseline(delta32, N, box, fr0, gpu_cycles, ncycles):
     """The reference's own V_cycle (flang -O2 -fopenmp, FP32 storage) on the
     host cores: a bounded sample of `ncycles` V-cycles on the same density,
@@ -423,6 +481,9 @@ def main():
     ap.add_argument("--dgp-n", type=int, default=512, help="grid of the nDGP comparison (0: skip)")
     ap.add_argument("--pm-n", type=int, default=512, help="grid (and particles per side) of the PM step (0: skip)")
     args = ap.parse_args()
+    # the CPU baseline's OpenMP runtimes read these when they load
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
         return
@@ -474,10 +535,10 @@ def main():
             "smoother_avg_ms": prof64["smooth"][1] / max(prof64["smooth"][0], 1),
             "smoother_gbs": prof64["smooth"][2] / max(prof64["smooth"][1], 1e-9) / 1e6,
         }
-        delta32 = delta.astype(np.float32)
+        delta64 = delta
         del S64
     else:
-        delta32 = Smix.get_density(np.float32)
+        delta64 = Smix.get_density(np.float64)
 
     dgp = None
     if args.dgp_n > 0:
@@ -496,8 +557,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_reference_baseline(delta32, N, box, fr0, fp64["cycles"] if fp64 else reps[-1]["outer"],
-                                     args.cpu_cycles)
+        cpu = cpu_fp64_solve_baseline(delta64, N, box, fr0)
+        ref = cpu_reference_baseline(delta64.astype(np.float32), N, box, fr0,
+                                     fp64["cycles"] if fp64 else reps[-1]["outer"], args.cpu_cycles)
+        if cpu is None:
+            cpu = ref
+        elif ref is not None:
+            cpu["reference_fp32_cycles"] = ref
 
     sm = prof["smooth"]
     avg_ms = sm[1] / max(sm[0], 1)

@@ -19,20 +19,31 @@
 
 namespace mgs {
 
+// Block order flag in ZLaunch::JB (row-block counts stay below 2^16): each
+// XCD owns a slab of JB/8 block rows and marches it plane run by plane run,
+// rows fastest.  A plane run's halo planes are then the neighbouring runs of
+// the same slab, which the same XCD's L2 still holds, and the rows above and
+// below are its own (but for the slab edges): the z-halo re-reads of a
+// rows-fastest order over the whole grid (whose plane-run footprint exceeds
+// an XCD's L2) stay on chip.
+constexpr int ZSLAB = 1 << 20;
+
 struct ZLaunch {
     dim3 grid, block;
     int TX, TY, XB, JB, KC;
     unsigned NB;
 };
 
-// planes marched per thread; MGS_ZRUN overrides it (launch-shape tuning)
+// planes marched per thread; in tuning builds (-DMGS_TUNING) MGS_ZRUN overrides it
 inline int zrun_length(int n)
 {
+#ifdef MGS_TUNING
     static const int forced = [] {
         const char* e = std::getenv("MGS_ZRUN");
         return e ? std::atoi(e) : 0;
     }();
     if (forced > 0) return forced < n ? forced : n;
+#endif
     // measured at 512^3 (runs of 2, 4, 6, 8, 16, 32 planes): 4 is best; the
     // window's two prologue loads per run cost less than the lost
     // parallelism of longer runs
@@ -43,12 +54,16 @@ inline int zrun_length(int n)
 // a run of KC planes; k-run-major block order
 inline ZLaunch make_zlaunch(int width, int n, int kc)
 {
-    // widest block row; MGS_ZTX overrides it (launch-shape tuning)
+    // widest block row; in tuning builds MGS_ZTX overrides it
+#ifdef MGS_TUNING
     static const int max_tx = [] {
         const char* e = std::getenv("MGS_ZTX");
         const int v = e ? std::atoi(e) : 0;
         return (v == 32 || v == 64 || v == 128) ? v : 256;
     }();
+#else
+    constexpr int max_tx = 256;
+#endif
     ZLaunch L;
     L.TX = width < max_tx ? width : max_tx;
     int ty = 256 / L.TX;
@@ -57,18 +72,25 @@ inline ZLaunch make_zlaunch(int width, int n, int kc)
     L.XB = (width + L.TX - 1) / L.TX;
     L.JB = (n + L.TY - 1) / L.TY;
     L.KC = kc < 1 ? 1 : (kc > n ? n : kc);
-    // block order: MGS_ZORDER=k makes the plane runs of one column block
+    // block order: in tuning builds MGS_ZORDER=k makes the plane runs of one column block
     // consecutive (flagged to the kernels by a negative JB)
+#ifdef MGS_TUNING
     static const bool kfast = [] {
         const char* e = std::getenv("MGS_ZORDER");
         return e && e[0] == 'k';
     }();
+#else
+    constexpr bool kfast = false;
+#endif
     const int kb = (n + L.KC - 1) / L.KC;
     L.NB = (unsigned)(L.XB * L.JB * kb);
     const unsigned per = (L.NB + 7) / 8;
     L.grid = dim3(per * 8);
     L.block = dim3(L.TX, L.TY);
-    if (kfast) L.JB = -L.JB;
+    if (kfast)
+        L.JB = -L.JB;
+    else if (L.JB % 8 == 0 && kb > 1)
+        L.JB |= ZSLAB;  // row slabs per XCD (zcol_at)
     return L;
 }
 
@@ -78,13 +100,17 @@ inline ZLaunch make_zlaunch(int width, int n) { return make_zlaunch(width, n, zr
 inline ZLaunch make_zlaunch(const Geo& g) { return make_zlaunch(g.h, g.n); }
 
 // passes over both colours (residual-type kernels, two windows per column);
-// MGS_ZRUN_BOTH overrides their run length (launch-shape tuning)
+// in tuning builds MGS_ZRUN_BOTH overrides their run length
 inline ZLaunch make_zlaunch_both(const Geo& g)
 {
+#ifdef MGS_TUNING
     static const int forced = [] {
         const char* e = std::getenv("MGS_ZRUN_BOTH");
         return e ? std::atoi(e) : 0;
     }();
+#else
+    constexpr int forced = 0;
+#endif
     return make_zlaunch(g.h, g.n, forced > 0 ? forced : zrun_length(g.n));
 }
 
@@ -98,7 +124,16 @@ __device__ __forceinline__ bool zcol_at(int width, int n, int TX, int TY, int XB
                                         int ty, ZCol& c)
 {
     unsigned xb, jb, kb;
-    if (JB > 0) {  // x blocks fastest, then rows, then plane runs
+    if (JB & ZSLAB) {  // x blocks fastest, then the slab's rows, then plane runs, then slabs
+        const unsigned JS = (unsigned)(JB & (ZSLAB - 1)) >> 3;
+        const unsigned KB = (unsigned)((n + KC - 1) / KC);
+        xb = XB == 1 ? 0u : lb % (unsigned)XB;
+        unsigned t = XB == 1 ? lb : lb / (unsigned)XB;
+        const unsigned jl = t % JS;
+        t /= JS;
+        kb = t % KB;
+        jb = (t / KB) * JS + jl;
+    } else if (JB > 0) {  // x blocks fastest, then rows, then plane runs
         xb = XB == 1 ? 0u : lb % (unsigned)XB;
         const unsigned t = XB == 1 ? lb : lb / (unsigned)XB;
         jb = t % (unsigned)JB;
