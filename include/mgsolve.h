@@ -248,6 +248,24 @@ int mgs_use_graphs(mgs_handle* h, int32_t enable);
  * (for zero-copy consumers on the same device) */
 int mgs_device_field(mgs_handle* h, void** ptr, size_t* bytes);
 
+/* ---- whole solve with no device allocation ---------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mgs_create + mgs_set_density + mgs_solve + mgs_get_field
+ * in one call that calls no hipMalloc/hipFree and no allocating library
+ * routine.  delta (input) and u (input when opts->init_guess == 0, output)
+ * are DEVICE arrays of ngrid^3 values in the reference's order; u comes back
+ * raw (scale 1).  The hierarchy lives in a fixed device array of
+ * MGS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends (enough for
+ * ngrid <= 64 in every model and precision); a grid that needs more is
+ * rejected with MGS_E_ARG.  Runs on the device's null stream and returns when
+ * the field is in u.  Calls on one GPU share that GPU's array: calls that
+ * overlap on one GPU race.  Bitwise the field of the handle-based path;
+ * report->seconds is host wall time.  Errors: mgs_last_error(NULL). */
+#define MGS_NOALLOC_SCRATCH_BYTES (16u << 20)
+int mgs_solve_noalloc(const mgs_params* params, const mgs_solve_opts* opts, const void* delta, int32_t delta_dtype,
+                      void* u, int32_t u_dtype, mgs_solve_report* report);
+
 /* ---- serving: continuous-batching solve server ------------------------
  * No counterpart in the reference (a build-only component).  A server owns a
  * worker thread and one HIP stream on the current device.  Requests (f(R) or

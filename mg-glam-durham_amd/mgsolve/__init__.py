@@ -161,6 +161,8 @@ def load():
         "mgs_server_wait": ([H, C.c_int64, vp, C.c_size_t, i32, C.c_double, C.POINTER(RequestResult)], C.c_int),
         "mgs_server_get_stats": ([H, C.POINTER(ServerStats)], C.c_int),
         "mgs_server_test": ([H, C.c_int64, C.POINTER(i32)], C.c_int),
+        "mgs_solve_noalloc": ([C.POINTER(Params), C.POINTER(SolveOpts), vp, i32, vp, i32, C.POINTER(SolveReport)],
+                              C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
