@@ -180,6 +180,19 @@ int mgs_fifth_force(mgs_handle* h, void* psi, void* gx, void* gy, void* gz, int3
 /* NGP-binned power spectrum of psi (POWERfft5 binning): nbins = N/2 bins,
  * k [h/Mpc], P [(Mpc/h)^3], count of modes */
 int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, double* count);
+/* POWERfft5(iPol, .) with its k-space binning iCIC (PMP2mod_power.f90:
+ * 209-463; the reference's module Power sets iCIC = 0 and dLog = 0.02):
+ * which as above; ipol 0 monopole, 1 quadrupole (weights P2(mu) = 1.5 mu^2 -
+ * 0.5, mu = kz/|k|, z the reference's line of sight after its transposition,
+ * times 5/2); icic 0 nearest bin int(|m| + 0.5) plus the log bins
+ * int(log10|m| / 0.02) (their power monopole only: zero for ipol 1), icic 1
+ * linear weights over bins int(|m| + 1e-4) and the next one (no log bins:
+ * zeros).  |m| and the bin rules in the reference's REAL*4 arithmetic.
+ * N/2 values each: k [h/Mpc], P [(Mpc/h)^3], count of modes (summed
+ * weights for icic 1); a bin without a positive count gives zeros.  klog,
+ * pklog, countlog and count may be NULL. */
+int mgs_power_spectrum_ex(mgs_handle* h, int32_t which, int32_t ipol, int32_t icic, double* k, double* pk,
+                          double* count, double* klog, double* pklog, double* countlog);
 
 /* ---- PM step around the solve: particles, CIC deposit, Poisson FFT, push -- */
 /* Particles are the reference's REAL*4 XPAR, YPAR, ZPAR, VX, VY, VZ

@@ -4,8 +4,9 @@
 //     hipFFT D2Z, k-space filter, Z2D, normalisation to sigma_g, exp map;
 //   * the fifth-force potential psi and its central-difference gradient
 //     (POTENT_FOR_MG, PMP2main.f90:884-950, and MOVE's stencil :565-594);
-//   * the NGP-binned power spectrum with POWERfft5's bins
-//     (PMP2mod_power.f90:323-418);
+//   * the power spectrum with POWERfft5's options: NGP or linear (CIC)
+//     k-space bins, monopole or quadrupole, and its log bins
+//     (PMP2mod_power.f90:323-456);
 //   * the PM step's potential: POTENTfft5's discrete-Green's-function
 //     Poisson solve (PMP2main.f90:705-867) and POTENT_FOR_MG's addition of
 //     the scalar field (:869-935), into the handle's REAL*4 potential.
@@ -27,8 +28,9 @@ struct FftState {
     bool have_fwd = false, have_inv = false;
     double* real = nullptr;          // N^3 lexicographic
     hipfftDoubleComplex* cplx = nullptr;  // N*N*(N/2+1)
-    double* bins = nullptr;          // 3 * nb: power, count, sum|m|
-    double* binpart = nullptr;       // BIN_BLOCKS * 3 * nb per-block histograms
+    double* bins = nullptr;          // 6 (N/2 + 1): power, count, sum|m| of the linear and the log bins
+    double* binpart = nullptr;       // BIN_BLOCKS such histograms, one per block
+    int* logbin = nullptr;           // log bin of every |m|^2 = 0 .. 3 (N/2)^2
 };
 
 // blocks of the binning kernel: each accumulates a private histogram in LDS
@@ -71,6 +73,7 @@ void fft_release(mgs_handle* h)
     if (f->cplx) (void)hipFree(f->cplx);
     if (f->bins) (void)hipFree(f->bins);
     if (f->binpart) (void)hipFree(f->binpart);
+    if (f->logbin) (void)hipFree(f->logbin);
     delete f;
     h->fft = nullptr;
 }
@@ -196,13 +199,23 @@ __global__ __launch_bounds__(256) void k_fifth_force(const double* __restrict__ 
     }
 }
 
-// power per half-spectrum entry into NGP bins, one private LDS histogram per
-// block (3 nb doubles: power, mode count, sum of |m|) written to part[block]
+// POWERfft5's binning (PMP2mod_power.f90:359-410) of the half spectrum (x
+// fastest, N/2+1 of it; an entry stands for itself and, but at i = 0 and
+// i = N/2, its conjugate mirror), one private LDS histogram per block written
+// to part[block]: power, mode count and sum of |m| for the linear bins 1..N/2
+// (hist[0 .. nb)), then the same for the log bins (hist[3 nb .. 6 nb)).  |m|,
+// the bin boundaries and the weights are the reference's REAL*4 values (RK =
+// sqrt(REAL(|m|^2)) correctly rounded, as the compiled reference's is); the
+// log bin of each |m|^2 comes from a table the host fills with the C
+// library's log10f, the function the compiled reference calls.  The
+// quadrupole's line of sight is z: the reference fills its bins after the
+// I <-> K transposition (:307-316), so its MI is the z wavenumber.
+template <int IPOL, int ICIC>
 __global__ __launch_bounds__(256) void k_bin_power(const hipfftDoubleComplex* __restrict__ c, int N, double norm,
-                                                   double* __restrict__ part, int nb)
+                                                   const int* __restrict__ logbin, double* __restrict__ part, int nb)
 {
     extern __shared__ double hist[];
-    for (int t = threadIdx.x; t < 3 * nb; t += blockDim.x) hist[t] = 0.0;
+    for (int t = threadIdx.x; t < 6 * nb; t += blockDim.x) hist[t] = 0.0;
     __syncthreads();
     const int nh = N / 2 + 1;
     const size_t total = (size_t)N * N * nh;
@@ -211,21 +224,52 @@ __global__ __launch_bounds__(256) void k_bin_power(const hipfftDoubleComplex* __
         const int i = (int)(q % nh);
         const size_t r = q / nh;
         const int j = (int)(r % N), k = (int)(r / N);
-        const double mx = i, my = fold(j, N), mz = fold(k, N);
-        const double rk = sqrt(mx * mx + my * my + mz * mz);
-        if (rk < 0.5) continue;
-        int mp = (int)(rk + 0.5);
-        if (mp < 1) mp = 1;
-        if (mp >= nb) continue;
+        const int my = fold(j, N), mz = fold(k, N);
+        const int n2 = i * i + my * my + mz * mz;
+        if (n2 == 0) continue;  // RK < 0.5: the mean
+        const float rk = sqrtf((float)n2);
         const double w = (i == 0 || 2 * i == N) ? 1.0 : 2.0;  // conjugate half
-        const double pw = (c[q].x * c[q].x + c[q].y * c[q].y) * norm;
-        atomicAdd(&hist[mp], w * pw);
-        atomicAdd(&hist[nb + mp], w);
-        atomicAdd(&hist[2 * nb + mp], w * rk);
+        double amp = (c[q].x * c[q].x + c[q].y * c[q].y) * norm;
+        if (IPOL) {
+            const float cosmu2 = (float)(mz * mz) / (rk * rk);
+            amp *= (double)(1.5f * cosmu2 - 0.5f);
+        }
+        if (ICIC) {
+            int mp = (int)(rk + 0.0001f);
+            if (mp < 1) mp = 1;
+            const int mp2 = mp + 1 < N ? mp + 1 : N;
+            const float w1 = (float)mp - rk + 1.0f;
+            const float w2 = 1.0f - w1;
+            const float d1 = (float)mp, d2 = d1 + 1.0f;
+            if (mp < nb) {
+                atomicAdd(&hist[mp], w * amp * (double)w1);
+                atomicAdd(&hist[nb + mp], w * (double)w1);
+                atomicAdd(&hist[2 * nb + mp], w * (double)(d1 * w1));
+            }
+            if (mp2 < nb) {
+                atomicAdd(&hist[mp2], w * amp * (double)w2);
+                atomicAdd(&hist[nb + mp2], w * (double)w2);
+                atomicAdd(&hist[2 * nb + mp2], w * (double)(d2 * w2));
+            }
+        } else {
+            int mp = (int)(rk + 0.5f);
+            if (mp < 1) mp = 1;
+            if (mp < nb) {
+                atomicAdd(&hist[mp], w * amp);
+                atomicAdd(&hist[nb + mp], w);
+                atomicAdd(&hist[2 * nb + mp], w * (double)rk);
+            }
+            const int ml = logbin[n2];
+            if (ml < nb) {
+                if (!IPOL) atomicAdd(&hist[3 * nb + ml], w * amp);
+                atomicAdd(&hist[4 * nb + ml], w);
+                atomicAdd(&hist[5 * nb + ml], w * (double)rk);
+            }
+        }
     }
     __syncthreads();
-    double* out = part + (size_t)blockIdx.x * 3 * nb;
-    for (int t = threadIdx.x; t < 3 * nb; t += blockDim.x) out[t] = hist[t];
+    double* out = part + (size_t)blockIdx.x * 6 * nb;
+    for (int t = threadIdx.x; t < 6 * nb; t += blockDim.x) out[t] = hist[t];
 }
 
 // bins[t] = sum over blocks of part[block][t], in block order
@@ -432,17 +476,45 @@ int mgs_fifth_force(mgs_handle* h, void* psi, void* gx, void* gy, void* gz, int3
     return rc;
 }
 
+// the log bin (PMP2mod_power.f90:396, dLog = 0.02 a REAL*4 parameter, :12)
+// of every |m|^2 = 0 .. n2max, in the reference's REAL*4 arithmetic with the C
+// library's log10f; entry 0 is never used
+static std::vector<int> log_bin_table(int n2max)
+{
+    std::vector<int> t((size_t)n2max + 1, 1);
+    const float dlog = 0.02f;
+    for (int n2 = 1; n2 <= n2max; n2++) {
+        const float rk = std::sqrt((float)n2);
+        const volatile float v = std::log10(rk) / dlog + 1000.0f;  // rounded to REAL*4 as stored
+        const int b = (int)v - 1000;
+        t[n2] = b < 1 ? 1 : b;
+    }
+    return t;
+}
+
 // which: 0 density delta, 1 field u, 2 fifth-force potential psi
-int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, double* count)
+int mgs_power_spectrum_ex(mgs_handle* h, int32_t which, int32_t ipol, int32_t icic, double* k, double* pk,
+                          double* count, double* klog, double* pklog, double* countlog)
 {
     if (!h || !k || !pk) return set_err(h, MGS_E_ARG, "NULL argument");
     if (which < 0 || which > 2) return set_err(h, MGS_E_ARG, "which must be 0 (delta), 1 (u) or 2 (psi)");
+    if (ipol != 0 && ipol != 1) return set_err(h, MGS_E_ARG, "ipol must be 0 (monopole) or 1 (quadrupole)");
+    if (icic != 0 && icic != 1) return set_err(h, MGS_E_ARG, "icic must be 0 (NGP bins) or 1 (linear bins)");
     int rc = fft_ensure(h);
     if (rc) return rc;
     FftState* f = fft_of(h);
     const int N = h->p.ngrid;
     const size_t n3 = (size_t)N * N * N;
     const Geo g = make_geo(N);
+    const int n2max = 3 * (N / 2) * (N / 2);
+    if (!f->logbin) {
+        const std::vector<int> t = log_bin_table(n2max);
+        if (hipMalloc((void**)&f->logbin, t.size() * sizeof(int)) != hipSuccess)
+            return set_err(h, MGS_E_NOMEM, "log bin table");
+        if (hipMemcpyAsync(f->logbin, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, h->s) != hipSuccess ||
+            hipStreamSynchronize(h->s) != hipSuccess)
+            return set_err(h, MGS_E_HIP, "log bin table copy");
+    }
     if (which == 0)
         launch_unpack_f64_to_f64(h->L[0].rho, f->real, N, 1.0, h->s);
     else if (which == 1)
@@ -456,24 +528,53 @@ int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, doub
                            ctilde * ctilde * 0.5 * fbg, 1.0 / ubg);
     }
     if (hipfftExecD2Z(f->fwd, f->real, f->cplx) != HIPFFT_SUCCESS) return set_err(h, MGS_E_HIP, "hipfftExecD2Z");
-    const int nb = N + 1;
+    const int nb = N / 2 + 1;  // bins 1..N/2 are returned
     const double norm = 1.0 / ((double)n3 * (double)n3);
-    hipLaunchKernelGGL(k_bin_power, dim3(BIN_BLOCKS), dim3(256), 3 * (size_t)nb * sizeof(double), h->s, f->cplx, N,
-                       norm, f->binpart, nb);
-    hipLaunchKernelGGL(k_bin_reduce, dim3((3 * nb + 255) / 256), dim3(256), 0, h->s, (const double*)f->binpart,
-                       BIN_BLOCKS, 3 * nb, f->bins);
-    std::vector<double> hb(3 * (size_t)nb);
+    const size_t lds = 6 * (size_t)nb * sizeof(double);
+    const dim3 G(BIN_BLOCKS), B(256);
+    if (icic)
+        hipLaunchKernelGGL((ipol ? k_bin_power<1, 1> : k_bin_power<0, 1>), G, B, lds, h->s, f->cplx, N, norm,
+                           (const int*)f->logbin, f->binpart, nb);
+    else
+        hipLaunchKernelGGL((ipol ? k_bin_power<1, 0> : k_bin_power<0, 0>), G, B, lds, h->s, f->cplx, N, norm,
+                           (const int*)f->logbin, f->binpart, nb);
+    hipLaunchKernelGGL(k_bin_reduce, dim3((6 * nb + 255) / 256), dim3(256), 0, h->s, (const double*)f->binpart,
+                       BIN_BLOCKS, 6 * nb, f->bins);
+    std::vector<double> hb(6 * (size_t)nb);
     if (hipMemcpyAsync(hb.data(), f->bins, hb.size() * sizeof(double), hipMemcpyDeviceToHost, h->s) != hipSuccess ||
         hipStreamSynchronize(h->s) != hipSuccess)
         return set_err(h, MGS_E_HIP, "bins copy");
-    const double box = h->p.box;
-    for (int b = 1; b <= N / 2; b++) {
-        const double c = hb[nb + b];
-        k[b - 1] = c > 0 ? hb[2 * nb + b] / c * 2.0 * M_PI / box : 0.0;
-        pk[b - 1] = c > 0 ? box * box * box * hb[b] / c : 0.0;
-        if (count) count[b - 1] = c;
+    // PMP2mod_power.f90:420-456: the quadrupole times (2L+1)/2; a bin with no
+    // (positive) count keeps zeros
+    const double box = h->p.box, scale = 2.0 * M_PI / box, pfac = ipol ? 2.5 : 1.0;
+    auto emit = [&](const double* hs, double* kk, double* pp, double* cc) {
+        for (int b = 1; b <= N / 2; b++) {
+            const double c = hs[nb + b];
+            if (kk) kk[b - 1] = c > 0 ? hs[2 * nb + b] / c * scale : 0.0;
+            if (pp) pp[b - 1] = c > 0 ? box * box * box * pfac * hs[b] / c : 0.0;
+            if (cc) cc[b - 1] = c > 0 ? c : 0.0;
+        }
+    };
+    emit(hb.data(), k, pk, count);
+    if (icic) {
+        for (int b = 0; b < N / 2; b++) {
+            if (klog) klog[b] = 0.0;
+            if (pklog) pklog[b] = 0.0;
+            if (countlog) countlog[b] = 0.0;
+        }
+    } else {
+        // the log power takes the monopole only (:398-402)
+        std::vector<double> lg(hb.begin() + 3 * nb, hb.end());
+        emit(lg.data(), klog, pklog, countlog);
+        if (ipol && pklog)
+            for (int b = 0; b < N / 2; b++) pklog[b] = 0.0;
     }
     return 0;
+}
+
+int mgs_power_spectrum(mgs_handle* h, int32_t which, double* k, double* pk, double* count)
+{
+    return mgs_power_spectrum_ex(h, which, 0, 0, k, pk, count, nullptr, nullptr, nullptr);
 }
 
 int mgs_potential(mgs_handle* h, double trfi)

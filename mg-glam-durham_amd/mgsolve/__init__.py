@@ -140,6 +140,7 @@ def load():
         "mgs_solve": ([H, C.POINTER(SolveOpts), C.POINTER(SolveReport)], C.c_int),
         "mgs_fifth_force": ([H, vp, vp, vp, vp, i32, i32], C.c_int),
         "mgs_power_spectrum": ([H, i32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+        "mgs_power_spectrum_ex": ([H, i32, i32, i32] + [C.POINTER(C.c_double)] * 6, C.c_int),
         "mgs_make_lognormal_density": ([H, C.c_uint64, C.c_double, C.c_double, C.c_double], C.c_int),
         "mgs_particles_set": ([H, vp, vp, vp, vp, vp, vp, C.c_int64, i32], C.c_int),
         "mgs_particles_get": ([H, vp, vp, vp, vp, vp, vp, i32], C.c_int),
@@ -355,6 +356,15 @@ class Solver:
         self._chk(self.L.mgs_power_spectrum(self.h, which, k.ctypes.data_as(dp), p.ctypes.data_as(dp),
                                             c.ctypes.data_as(dp)), "mgs_power_spectrum")
         return k, p, c
+
+    def power_spectrum_ex(self, which=2, ipol=0, icic=0):
+        """POWERfft5(ipol, .) with k-space binning icic: (k, P, count, klog,
+        Plog, countlog), N/2 bins each (mgs_power_spectrum_ex)."""
+        out = [np.zeros(self.N // 2) for _ in range(6)]
+        dp = C.POINTER(C.c_double)
+        self._chk(self.L.mgs_power_spectrum_ex(self.h, which, ipol, icic, *[a.ctypes.data_as(dp) for a in out]),
+                  "mgs_power_spectrum_ex")
+        return tuple(out)
 
     def make_lognormal_density(self, seed, sigma_g, slope=-1.5, kcut_frac=0.5):
         self._chk(self.L.mgs_make_lognormal_density(self.h, seed, sigma_g, slope, kcut_frac),

@@ -511,27 +511,103 @@ def central_gradient(psi):
     return gx, gy, gz
 
 
-def power_spectrum(field, box):
-    """NGP-binned P(k) with POWERfft5's bins (PMP2mod_power.f90:323-372,
-    407-418): bin MP = max(int(|m| + 0.5), 1) over integer wavevectors m,
-    k = 0 excluded; P = box^3 <|c_m|^2>, c_m = (1/N^3) sum f e^{-i k.x};
-    k_bin = <|m|> 2 pi / box.  Returns (k, P, count) for bins 1..N/2."""
+# dLog of POWERfft5's log bins, a REAL*4 parameter (PMP2mod_power.f90:12)
+DLOG32 = np.float32(0.02)
+_LIBM = None
+
+
+def log_bins(n2max):
+    """MPlog for every integer |m|^2 = 0 .. n2max (PMP2mod_power.f90:396):
+    INT(LOG10(RK)/dLog + 1000.) - 1000 clamped below at 1, evaluated as the
+    compiled reference does, in REAL*4 with the C library's log10f (RK =
+    SQRT(REAL(MI+MJ+MK)), a REAL*4 square root); entry 0 is unused."""
+    global _LIBM
+    if _LIBM is None:
+        import ctypes.util
+
+        _LIBM = C.CDLL(ctypes.util.find_library("m"))
+        _LIBM.log10f.restype = C.c_float
+        _LIBM.log10f.argtypes = [C.c_float]
+    rk = np.sqrt(np.arange(n2max + 1, dtype=np.float32))
+    out = np.ones(n2max + 1, np.int64)
+    for n2 in range(1, n2max + 1):
+        v = np.float32(np.float32(_LIBM.log10f(float(rk[n2]))) / DLOG32) + np.float32(1000.0)
+        out[n2] = max(1, int(v) - 1000)
+    return out
+
+
+def power_spectrum_ex(field, box, ipol=0, icic=0):
+    """POWERfft5(iPol, .) (PMP2mod_power.f90:209-463) over the complex
+    spectrum c_m = (1/N^3) sum f e^{-i k.x} of the integer wavevectors m,
+    k = 0 excluded, with the reference's REAL*4 |m| (RK = sqrt(REAL(|m|^2))):
+
+    * icic = 0 (:394-410): bin MP = max(INT(RK + 0.5), 1); log bin MPlog
+      (log_bins), whose power is summed only for ipol = 0;
+    * icic = 1 (:373-393): MP = max(INT(RK + 0.0001), 1), MP2 = MP + 1, weights
+      w1 = MP - RK + 1 and w2 = 1 - w1 on the power, the mode count and the
+      bin's k (MP w1, (MP + 1) w2); no log bins;
+    * ipol = 1: the power is weighted by P2 = 1.5 mz^2/RK^2 - 0.5 (z the line
+      of sight: MI of :371 is the first index after the transposition) and
+      multiplied by 2.5 at the end (:420).
+
+    Bins 1..N/2 (:440-456): k = <RK> 2 pi/box, P = box^3 sum(power)/count,
+    zero where the count is not positive.  Returns (k, P, count, klog, Plog,
+    countlog)."""
     f = np.asarray(field, np.float64)
     N = f.shape[0]
     ck = np.fft.fftn(f) / N ** 3
     pw = (ck.real ** 2 + ck.imag ** 2).ravel()
-    m = kgrid(N)
+    m = kgrid(N).astype(np.int64)
     kz, ky, kx = np.meshgrid(m, m, m, indexing="ij")
-    rk = np.sqrt(kx * kx + ky * ky + kz * kz).ravel()
-    sel = rk >= 0.5
-    mp = np.maximum((rk[sel] + 0.5).astype(np.int64), 1)
-    nb = N + 1
-    cnt = np.bincount(mp, minlength=nb)[: nb]
-    psum = np.bincount(mp, weights=pw[sel], minlength=nb)[: nb]
-    ksum = np.bincount(mp, weights=rk[sel], minlength=nb)[: nb]
+    n2 = (kx * kx + ky * ky + kz * kz).ravel()
+    # the first index of FI when the bins are filled is z: the third
+    # transform runs after the I <-> K transposition (:307-316)
+    mz2 = (kz * kz).ravel()
+    sel = n2 > 0
+    n2, mz2, pw = n2[sel], mz2[sel], pw[sel]
+    rk = np.sqrt(n2.astype(np.float32))
+    if ipol:
+        cosmu2 = mz2.astype(np.float32) / (rk * rk)
+        amp = pw * (np.float32(1.5) * cosmu2 - np.float32(0.5)).astype(np.float64)
+    else:
+        amp = pw
+    nb = N + 2
+    if icic:
+        mp = np.maximum((rk + np.float32(0.0001)).astype(np.int64), 1)
+        mp2 = np.minimum(mp + 1, N)
+        w1 = (mp.astype(np.float32) - rk + np.float32(1.0)).astype(np.float32)
+        w2 = (np.float32(1.0) - w1).astype(np.float64)
+        w1 = w1.astype(np.float64)
+        d1 = mp.astype(np.float64)
+        cnt = np.bincount(mp, weights=w1, minlength=nb) + np.bincount(mp2, weights=w2, minlength=nb)
+        psum = np.bincount(mp, weights=amp * w1, minlength=nb) + np.bincount(mp2, weights=amp * w2, minlength=nb)
+        ksum = np.bincount(mp, weights=d1 * w1, minlength=nb) + np.bincount(mp2, weights=(d1 + 1.0) * w2, minlength=nb)
+        lcnt = lpsum = lksum = np.zeros(nb)
+    else:
+        mp = np.maximum((rk + np.float32(0.5)).astype(np.int64), 1)
+        cnt = np.bincount(mp, minlength=nb).astype(np.float64)
+        psum = np.bincount(mp, weights=amp, minlength=nb)
+        ksum = np.bincount(mp, weights=rk.astype(np.float64), minlength=nb)
+        lb = np.minimum(log_bins(int(n2.max()))[n2], N)
+        lcnt = np.bincount(lb, minlength=nb).astype(np.float64)
+        lpsum = np.bincount(lb, weights=pw, minlength=nb) if not ipol else np.zeros(nb)
+        lksum = np.bincount(lb, weights=rk.astype(np.float64), minlength=nb)
+    if ipol:
+        psum = 2.5 * psum
     bins = np.arange(1, N // 2 + 1)
-    c = cnt[bins].astype(np.float64)
-    with np.errstate(invalid="ignore", divide="ignore"):
-        P = np.where(c > 0, box ** 3 * psum[bins] / c, 0.0)
-        k = np.where(c > 0, ksum[bins] / c * 2.0 * math.pi / box, 0.0)
-    return k, P, c
+
+    def finish(c, ps, ks):
+        c = c[bins]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            P = np.where(c > 0, box ** 3 * ps[bins] / c, 0.0)
+            k = np.where(c > 0, ks[bins] / c * 2.0 * math.pi / box, 0.0)
+        return k, P, np.where(c > 0, c, 0.0)
+
+    return finish(cnt, psum, ksum) + finish(lcnt, lpsum, lksum)
+
+
+def power_spectrum(field, box):
+    """The NGP monopole of POWERfft5 (power_spectrum_ex with ipol = icic = 0):
+    bin MP = max(INT(RK + 0.5), 1), P = box^3 <|c_m|^2>, k_bin = <RK> 2 pi/box.
+    Returns (k, P, count) for bins 1..N/2."""
+    return power_spectrum_ex(field, box)[:3]

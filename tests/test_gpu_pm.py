@@ -208,9 +208,23 @@ def test_pm_kernels_match_the_reference_golden_vectors():
         with M.Solver(cfg) as S:
             S.set_density(f["field"])
             k, Pk, cnt = S.power_spectrum(which=0)
+            ex = {(ip, ic): S.power_spectrum_ex(0, ip, ic) for ip, ic in ((0, 0), (0, 1), (1, 0))}
         assert np.array_equal(cnt, f["cnt_p0_c0"])
         assert np.max(np.abs(Pk / f["pk_p0_c0"] - 1.0)) < 1e-6
         assert np.max(np.abs(k / f["k_p0_c0"] - 1.0)) < 1e-6
+        # POWERfft5's other options and log bins (the reference stores the
+        # counts in REAL*4; its quadrupole is a difference of REAL*4 terms)
+        scale = float(np.max(f["pk_p0_c0"]))
+        for (ip, ic), (k2, P2, c2, kl, Pl, cl) in ex.items():
+            t = f"p{ip}_c{ic}"
+            f32 = lambda a: np.asarray(a).astype(np.float32).astype(np.float64)  # noqa: E731
+            assert np.array_equal(f32(c2), f["cnt_" + t]) and np.array_equal(f32(cl), f["cntlog_" + t]), t
+            for a, b in ((k2, f["k_" + t]), (kl, f["klog_" + t])):
+                m = b != 0
+                assert np.array_equal(a == 0, b == 0) and np.max(np.abs(a[m] / b[m] - 1.0), initial=0.0) < 1e-6, t
+            tol = 1e-5 * scale if ip else 1e-6 * np.abs(f["pk_" + t])
+            assert np.all(np.abs(P2 - f["pk_" + t]) <= tol), t
+            assert np.all(np.abs(Pl - f["pklog_" + t]) <= 1e-6 * np.abs(f["pklog_" + t])), t
 
 
 @pytest.mark.gpu

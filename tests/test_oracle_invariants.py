@@ -292,6 +292,27 @@ def test_power_spectrum_single_mode():
     assert np.isclose(k[0], 2 * np.pi / box * np.mean([1.0, 1.0, 1.0, 1.0, 1.0, 1.0]) * 1.0, rtol=0.3)
 
 
+def test_log_bins_follow_the_closed_form():
+    """POWERfft5's log bin of |m| = sqrt(n2) is INT(log10|m|/0.02) =
+    floor(25 log10 n2) (PMP2mod_power.f90:396), clamped below at 1.  The
+    reference's REAL*4 evaluation (the sum with 1000. keeps 2^-14 or 2^-13)
+    can move a value that lies within 1e-4 below an integer up to it;
+    elsewhere the bin is the exact floor.  Checked for every n2 up to 3 (512/2)^2: a wrong base, a
+    missing square root or a lost rounding step fails."""
+    n2max = 3 * 256 ** 2
+    got = O.log_bins(n2max)[1:]
+    n2 = np.arange(1, n2max + 1)
+    v = 25.0 * np.log10(n2.astype(np.float64))
+    near = np.abs(v - np.rint(v)) < 1e-4
+    exact = np.maximum(np.floor(v).astype(np.int64), 1)
+    assert np.array_equal(got[~near], exact[~near])
+    up = np.maximum(np.rint(v[near]).astype(np.int64), 1)
+    assert np.all((got[near] == up) | (got[near] == exact[near]))
+    assert np.any(got[near] != exact[near])  # some are rounded up, as the REAL*4 sum does
+    assert near.sum() < 50  # the rounded cases are the exception (n2 = 10^j among them)
+    assert got[9] == 25 and got[99] == 50  # n2 = 10, 100: log10|m| = 1/2, 1
+
+
 def test_central_gradient_of_sinusoid():
     N = 16
     x = np.arange(N)

@@ -13,7 +13,10 @@ driven by tests/golden/make_golden.py.  Bars:
   * POTENTfft5: the reference rounds FI to REAL*4 after each axis pass; the
     oracle transforms in FP64: 1e-6 of the field's maximum;
   * POWERfft5 (NGP bins): mode counts exact, P(k) and k to 1e-6 relative (the
-    reference takes |k| with a REAL*4 sqrt and squares REAL*4 amplitudes).
+    reference takes |k| with a REAL*4 sqrt and squares REAL*4 amplitudes);
+    with the CIC k-space bins and the log bins as well: counts equal after
+    the reference's REAL*4 storage of them; the quadrupole to 1e-5 of the
+    largest monopole bin (it is a difference of REAL*4-rounded terms).
 Each bar is orders of magnitude below what a wrong weight, index, sign,
 Green's function, normalisation or bin rule produces (checked below by
 planting such mistakes in the oracle's inputs).
@@ -125,3 +128,43 @@ def test_power_spectrum_matches_reference_powerfft5(N):
     rk = np.sqrt(kx * kx + ky * ky + kz * kz).ravel()
     floor_bins = np.bincount(np.maximum(rk[rk >= 0.5].astype(np.int64), 1), minlength=N + 1)[1: N // 2 + 1]
     assert not np.array_equal(floor_bins.astype(np.float64), f["cnt_p0_c0"])
+
+
+def _f32(x):
+    return np.asarray(x, np.float64).astype(np.float32).astype(np.float64)
+
+
+def _rel(a, b):
+    m = b != 0
+    return float(np.max(np.abs(a[m] / b[m] - 1.0))) if m.any() else 0.0
+
+
+@pytest.mark.parametrize("N", [16, 12])
+@pytest.mark.parametrize("ipol,icic", [(0, 0), (0, 1), (1, 0)])
+def test_power_spectrum_options_match_reference_powerfft5(N, ipol, icic):
+    """POWERfft5's NGP and CIC k-space bins, monopole and quadrupole, and the
+    log bins (PMP2mod_power.f90:366-456)."""
+    f = load(f"pm_power_{N}")
+    box = float(f["params"][1])
+    t = f"p{ipol}_c{icic}"
+    k, Pk, cnt, klog, plog, cntlog = O.power_spectrum_ex(f["field"], box, ipol, icic)
+    assert np.array_equal(_f32(cnt), f["cnt_" + t])
+    assert np.array_equal(_f32(cntlog), f["cntlog_" + t])
+    assert _rel(k, f["k_" + t]) < 1e-6 and _rel(klog, f["klog_" + t]) < 1e-6
+    assert np.array_equal(Pk == 0, f["pk_" + t] == 0) and np.array_equal(plog == 0, f["pklog_" + t] == 0)
+    if ipol == 0:
+        assert _rel(Pk, f["pk_" + t]) < 1e-6 and _rel(plog, f["pklog_" + t]) < 1e-6
+    else:
+        scale = float(np.max(f["pk_p0_c0"]))
+        assert np.max(np.abs(Pk - f["pk_" + t])) < 1e-5 * scale
+        assert not np.any(plog)  # the log bins take no quadrupole power (:398-402)
+
+
+def test_power_spectrum_planted_mistakes_fail_the_reference():
+    f = load("pm_power_16")
+    box = float(f["params"][1])
+    fld = f["field"].astype(np.float64)
+    scale = float(np.max(f["pk_p0_c0"]))
+    # the quadrupole about x (the original first axis) instead of z
+    _, Pq, *_ = O.power_spectrum_ex(np.transpose(fld, (2, 1, 0)), box, 1, 0)
+    assert np.max(np.abs(Pq - f["pk_p1_c0"])) > 1e-2 * scale
