@@ -538,6 +538,9 @@ void launch_df_correct(int frn, const float* dH, const float* rd, float* dfine, 
 bool launch_df_tail(int frn, int nlev, const TailLevelIn* lv, hipStream_t s)
 {
     if (nlev < 2 || nlev > TAIL_MAXL) return false;
+    long lds = 0;  // the LDS copies: d, xb, g of the first level, d, xb, g, rres, rd below
+    for (int i = 0; i < nlev; i++) lds += (i == 0 ? 3L : 5L) * lv[i].n * lv[i].n * lv[i].n;
+    if (lds > TAIL_LDS_FLOATS) return false;
     TailDesc T;
     T.nl = nlev - 1;
     for (int i = 0; i < nlev; i++) {

@@ -649,26 +649,74 @@ __device__ __forceinline__ void restrict_base_cell(const float* __restrict__ xf,
 // inner coarse-grid correction d_h += P(d_H - R d_h), kept only where the
 // corrected x stays positive (the delta-form image of the reference's guard);
 // z-marching like k_correct64, on e = d_H - R d_h.  Colour-1 sons only: every
-// inner correction is followed by a colour-0 pass (see son_corrected).  The
-// sons' old d and base weight are loaded by df_sons_load before any son of
-// the plane is stored (each son is read and written by its own thread only),
-// so all loads of a coarse plane are in flight together.
-template <int FRN, int S3>
+// inner correction is followed by a colour-0 pass (see son_corrected).
+// df_sons_load loads the sons' old d and raw base xb (the weight xb^{n+1} is
+// formed at use, so no arithmetic waits on a load in flight).
+template <int S3>
 __device__ __forceinline__ void df_sons_load(const float* __restrict__ df, const float* __restrict__ xbf, const Geo& gf,
-                                             int I, int J, int K, float* old, float* wb)
+                                             int I, int J, int K, float* old, float* xb)
 {
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         if (!son_corrected<S3, true>(t)) continue;
         const unsigned fi = son_index(gf, I, J, K, t & 1, t >> 1, S3);
         old[t] = elem(df, fi);
-        wb[t] = wpow<FRN>(elem(xbf, fi));
+        xb[t] = elem(xbf, fi);
     }
 }
 
-template <int S3>
-__device__ __forceinline__ void df_sons_apply(const float* ez, const float* ek, float* __restrict__ df, const Geo& gf,
-                                              int I, int J, int K, const float* old, const float* wb)
+// The raw coarse values of plane K that e = dH - rd over the 3x3 block needs.
+// SHFL: each row's centre is loaded and its x-neighbours come from the
+// adjacent lanes of the wave, which hold columns I-1 and I+1 of the same row
+// and plane run where lok / rok say so (else they are loaded too): a third of
+// the loads, same values.  Loading and forming e are separate steps so that
+// the loads of the next plane can be in flight while this one is stored.
+struct ERaw {
+    float h[9], r[9];
+};
+
+template <bool SHFL>
+__device__ __forceinline__ void e_load(const float* __restrict__ dH, const float* __restrict__ rd, const Block9& B,
+                                       int K, bool lok, bool rok, ERaw& w)
+{
+#pragma unroll
+    for (int y = 0; y < 3; y++) {
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+            if (SHFL && x == 0 && lok) continue;
+            if (SHFL && x == 2 && rok) continue;
+            const unsigned o = B.at(x + 3 * y, K);
+            w.h[x + 3 * y] = elem(dH, o);
+            w.r[x + 3 * y] = elem(rd, o);
+        }
+    }
+}
+
+template <bool SHFL>
+__device__ __forceinline__ void e_form(const ERaw& w, bool lok, bool rok, float* e)
+{
+#pragma unroll
+    for (int y = 0; y < 3; y++) {
+        const float c = w.h[1 + 3 * y] - w.r[1 + 3 * y];
+        e[3 * y + 1] = c;
+        if constexpr (SHFL) {
+            const float l = __shfl_up(c, 1u, 64);
+            const float r = __shfl_down(c, 1u, 64);
+            e[3 * y] = lok ? l : w.h[3 * y] - w.r[3 * y];
+            e[3 * y + 2] = rok ? r : w.h[3 * y + 2] - w.r[3 * y + 2];
+        } else {
+            e[3 * y] = w.h[3 * y] - w.r[3 * y];
+            e[3 * y + 2] = w.h[3 * y + 2] - w.r[3 * y + 2];
+        }
+    }
+}
+
+// new values of the corrected sons of fine plane 2K + S3 (d + P e, P the
+// trilinear weights 1, 3, 9, 27 / 64 over the coarse planes ez and ek) and
+// whether each keeps x > 0
+template <int FRN, int S3>
+__device__ __forceinline__ void df_sons_new(const float* ez, const float* ek, const float* old, const float* xb,
+                                            float* dn, bool* keep)
 {
     const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
     const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
@@ -684,47 +732,37 @@ __device__ __forceinline__ void df_sons_apply(const float* ez, const float* ek, 
             const int q = x + 3 * y;
             P += wt[ind] * ((ind & 4) ? ek[q] : ez[q]);
         }
-        const float dn = old[t] + P;
-        if (wb[t] + dn > 0.0f) elem(df, son_index(gf, I, J, K, s1, s2, S3)) = dn;
+        dn[t] = old[t] + P;
+        keep[t] = wpow<FRN>(xb[t]) + dn[t] > 0.0f;
     }
 }
 
-// e = dH - rd over the 3x3 coarse block of plane K.  SHFL: each row's centre
-// is loaded and its x-neighbours come from the adjacent lanes of the wave,
-// which hold columns I-1 and I+1 of the same row and plane run where lok /
-// rok say so (else they are loaded too): a third of the loads, same values.
+// e over the 3x3 coarse block of plane K in one step
 template <bool SHFL>
 __device__ __forceinline__ void e_block9(const float* __restrict__ dH, const float* __restrict__ rd, const Block9& B,
                                          int K, bool lok, bool rok, float* e)
 {
+    ERaw w;
+    e_load<SHFL>(dH, rd, B, K, lok, rok, w);
+    e_form<SHFL>(w, lok, rok, e);
+}
+
+template <int S3>
+__device__ __forceinline__ void df_sons_store(float* __restrict__ df, const Geo& gf, int I, int J, int K,
+                                              const float* dn, const bool* keep)
+{
 #pragma unroll
-    for (int y = 0; y < 3; y++) {
-        if constexpr (SHFL) {
-            const unsigned oc = B.at(1 + 3 * y, K);
-            const float c = elem(dH, oc) - elem(rd, oc);
-            float l = __shfl_up(c, 1u, 64);
-            float r = __shfl_down(c, 1u, 64);
-            if (!lok) {
-                const unsigned o = B.at(3 * y, K);
-                l = elem(dH, o) - elem(rd, o);
-            }
-            if (!rok) {
-                const unsigned o = B.at(2 + 3 * y, K);
-                r = elem(dH, o) - elem(rd, o);
-            }
-            e[3 * y] = l;
-            e[3 * y + 1] = c;
-            e[3 * y + 2] = r;
-        } else {
-#pragma unroll
-            for (int x = 0; x < 3; x++) {
-                const unsigned o = B.at(x + 3 * y, K);
-                e[x + 3 * y] = elem(dH, o) - elem(rd, o);
-            }
-        }
+    for (int t = 0; t < 4; t++) {
+        if (!son_corrected<S3, true>(t)) continue;
+        if (keep[t]) elem(df, son_index(gf, I, J, K, t & 1, t >> 1, S3)) = dn[t];
     }
 }
 
+// Software-pipelined over the column's coarse planes: plane K's sons are
+// computed, then every load of plane K+1 (its sons' old d and base weight,
+// coarse plane K+2) is issued, then plane K's sons are stored.  A son is read
+// and written by its own thread only and the planes' sons are distinct, so
+// the values are those of the plane-by-plane order.
 template <int FRN, bool SHFL = false>
 __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, const float* __restrict__ rd,
                                                float* __restrict__ df, const float* __restrict__ xbf, const Geo& gf,
@@ -733,18 +771,29 @@ __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, con
     const int I = cc.m, J = cc.j;
     Block9 B;
     B.init(gc, I, J);
-    float em[9], ek[9];  // planes K-1 and K
-    e_block9<SHFL>(dH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, em);
-    e_block9<SHFL>(dH, rd, B, cc.k0, lok, rok, ek);
+    ERaw wm, wk, wp;  // coarse planes K-1, K, K+1
+    float o0[4], w0[4], o1[4], w1[4];  // old d and raw xb of the sons of fine planes 2K, 2K+1
+    e_load<SHFL>(dH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, wm);
+    e_load<SHFL>(dH, rd, B, cc.k0, lok, rok, wk);
+    e_load<SHFL>(dH, rd, B, wrap_inc(cc.k0, gc.n), lok, rok, wp);
+    df_sons_load<0>(df, xbf, gf, I, J, cc.k0, o0, w0);
+    df_sons_load<1>(df, xbf, gf, I, J, cc.k0, o1, w1);
+    float em[9], ek[9], ep[9];
+    e_form<SHFL>(wm, lok, rok, em);
+    e_form<SHFL>(wk, lok, rok, ek);
     for (int K = cc.k0; K < cc.k1; K++) {
-        // every load of this plane (sons of fine planes 2K and 2K+1, coarse
-        // plane K+1), then the updates
-        float o0[4], w0[4], o1[4], w1[4], ep[9];
-        df_sons_load<FRN, 0>(df, xbf, gf, I, J, K, o0, w0);
-        df_sons_load<FRN, 1>(df, xbf, gf, I, J, K, o1, w1);
-        e_block9<SHFL>(dH, rd, B, wrap_inc(K, gc.n), lok, rok, ep);
-        df_sons_apply<0>(em, ek, df, gf, I, J, K, o0, w0);
-        df_sons_apply<1>(ep, ek, df, gf, I, J, K, o1, w1);
+        e_form<SHFL>(wp, lok, rok, ep);
+        float dn0[4], dn1[4];
+        bool k0[4], k1[4];
+        df_sons_new<FRN, 0>(em, ek, o0, w0, dn0, k0);
+        df_sons_new<FRN, 1>(ep, ek, o1, w1, dn1, k1);
+        if (K + 1 < cc.k1) {
+            df_sons_load<0>(df, xbf, gf, I, J, K + 1, o0, w0);
+            df_sons_load<1>(df, xbf, gf, I, J, K + 1, o1, w1);
+            e_load<SHFL>(dH, rd, B, wrap_inc(K + 1, gc.n), lok, rok, wp);
+        }
+        df_sons_store<0>(df, gf, I, J, K, dn0, k0);
+        df_sons_store<1>(df, gf, I, J, K, dn1, k1);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             em[q] = ek[q];
@@ -831,6 +880,9 @@ struct TailLevel {
 
 constexpr int TAIL_MAXL = 8;
 constexpr int TAIL_MAX_N = 16;  // levels with n <= 16 run in the tail
+// LDS for the tail's arrays: d, xb, g of its first level and d, xb, g, rres,
+// rd of each coarser one (15208 floats for 16^3 down to 2^3)
+constexpr int TAIL_LDS_FLOATS = 15232;
 
 struct TailDesc {
     int nl;  // levels below the first (L[nl] is the coarsest)
@@ -870,44 +922,83 @@ __device__ __forceinline__ void coarse_ijk(const Geo& gc, int t, int& I, int& J,
     K = (int)((unsigned)t / (n * n));
 }
 
-template <int FRN>
-__device__ void df_tail_vcycle(const TailDesc& T)
+// The tail runs out of LDS: the first level's d, xb and g and every coarser
+// level's xb are copied in, the cycle works on the LDS copies (the arithmetic
+// and its order are those of the level kernels), and the first level's d is
+// copied back; coarser levels are internal to the cycle.  The launcher checks
+// that the levels fit in TAIL_LDS_FLOATS.
+struct TailArrays {
+    float *d, *xb, *f, *rres, *rd;
+};
+
+// level i's arrays in the LDS block: d, xb, g (and rres, rd below the first
+// level), levels one after the other
+__device__ __forceinline__ TailArrays tail_lds_level(const TailDesc& G, int i, float* lds)
 {
+    int off = 0;
+    for (int q = 0; q < i; q++) off += (q == 0 ? 3 : 5) * G.L[q].g.n * G.L[q].g.n * G.L[q].g.n;
+    const int cells = G.L[i].g.n * G.L[i].g.n * G.L[i].g.n;
+    float* p = lds + off;
+    return TailArrays{p, p + cells, p + 2 * cells, i > 0 ? p + 3 * cells : nullptr, i > 0 ? p + 4 * cells : nullptr};
+}
+
+template <int FRN>
+__device__ void df_tail_vcycle(const TailDesc& G)
+{
+    __shared__ float lds[TAIL_LDS_FLOATS];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int i = 0; i < T.nl; i++) {
-        const TailLevel &L = T.L[i], &C = T.L[i + 1];
+    for (int i = 0; i <= G.nl; i++) {
+        const TailArrays A = tail_lds_level(G, i, lds);
+        const int cells = G.L[i].g.n * G.L[i].g.n * G.L[i].g.n;
+        for (int t = tid; t < cells; t += nthr) A.xb[t] = G.L[i].xb[t];
+        if (i == 0)
+            for (int t = tid; t < cells; t += nthr) {
+                A.d[t] = G.L[0].d[t];
+                A.f[t] = G.L[0].f[t];
+            }
+    }
+    __syncthreads();
+    for (int i = 0; i < G.nl; i++) {
+        const TailLevel &L = G.L[i], &C = G.L[i + 1];
+        const TailArrays l = tail_lds_level(G, i, lds), c = tail_lds_level(G, i + 1, lds);
         for (int it = 0; it < 4; it++) {
             for (int t = tid; t < (int)L.g.half; t += nthr)
-                tail_update(DfSmoothOp<FRN>{L.d, L.xb, L.f, L.dc}, L.g, it & 1, t);
+                tail_update(DfSmoothOp<FRN>{l.d, l.xb, l.f, L.dc}, L.g, it & 1, t);
             __syncthreads();
         }
         const int ncc = C.g.n * C.g.n * C.g.n;
         for (int t = tid; t < ncc; t += nthr) {
             int I, J, K;
             coarse_ijk(C.g, t, I, J, K);
-            df_restrict_residual_col<FRN>(L.d, L.xb, L.f, C.rres, C.rd, C.d, L.g, C.g, L.dc, ZCol{I, J, K, K + 1});
+            df_restrict_residual_col<FRN>(l.d, l.xb, l.f, c.rres, c.rd, c.d, L.g, C.g, L.dc, ZCol{I, J, K, K + 1});
         }
         __syncthreads();
-        for (int t = tid; t < ncc; t += nthr) tail_cell_op(DfResOp<FRN>{C.rd, C.xb, C.rres, C.f, C.dc}, C.g, t);
+        for (int t = tid; t < ncc; t += nthr) tail_cell_op(DfResOp<FRN>{c.rd, c.xb, c.rres, c.f, C.dc}, C.g, t);
         __syncthreads();
     }
-    const TailLevel& B = T.L[T.nl];
-    df_coarsest_block<FRN>(B.d, B.xb, B.f, B.g, 8, B.dc);
-    for (int i = T.nl - 1; i >= 0; i--) {
-        const TailLevel &L = T.L[i], &C = T.L[i + 1];
+    {
+        const TailArrays b = tail_lds_level(G, G.nl, lds);
+        df_coarsest_block<FRN>(b.d, b.xb, b.f, G.L[G.nl].g, 8, G.L[G.nl].dc);
+    }
+    for (int i = G.nl - 1; i >= 0; i--) {
+        const TailLevel &L = G.L[i], &C = G.L[i + 1];
+        const TailArrays l = tail_lds_level(G, i, lds), c = tail_lds_level(G, i + 1, lds);
         const int ncc = C.g.n * C.g.n * C.g.n;
         for (int t = tid; t < ncc; t += nthr) {
             int I, J, K;
             coarse_ijk(C.g, t, I, J, K);
-            df_correct_col<FRN>(C.d, C.rd, L.d, L.xb, L.g, C.g, ZCol{I, J, K, K + 1});
+            df_correct_col<FRN>(c.d, c.rd, l.d, l.xb, L.g, C.g, ZCol{I, J, K, K + 1});
         }
         __syncthreads();
         for (int it = 0; it < 4; it++) {
             for (int t = tid; t < (int)L.g.half; t += nthr)
-                tail_update(DfSmoothOp<FRN>{L.d, L.xb, L.f, L.dc}, L.g, it & 1, t);
+                tail_update(DfSmoothOp<FRN>{l.d, l.xb, l.f, L.dc}, L.g, it & 1, t);
             __syncthreads();
         }
     }
+    const TailArrays a0 = tail_lds_level(G, 0, lds);
+    const int cells0 = G.L[0].g.n * G.L[0].g.n * G.L[0].g.n;
+    for (int t = tid; t < cells0; t += nthr) G.L[0].d[t] = a0.d[t];
 }
 
 }  // namespace mgs
