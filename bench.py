@@ -383,6 +383,10 @@ def serve_run(tag, cfgs, kws, dens, max_active, rng):
         "max_batch": st["max_batch"],
         "failed_or_unconverged": len(bad),
         "alone_vs_mix_max_abs_diff": diff,
+        # the saturated leg's worker time (admission runs beside steps while
+        # requests are active, so it overlaps step_ms)
+        "saturated_worker_ms": {k: sat[k] for k in ("step_ms", "admit_ms", "finish_ms")},
+        "saturated_steps": sat["steps"],
     }
 
 

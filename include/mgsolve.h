@@ -342,8 +342,10 @@ typedef struct mgs_server_stats {
     double solves_per_s;     /* completed / (last completion - first submission) */
     int32_t max_batch;       /* most requests advanced by one batched step */
     int32_t reserved;
-    double step_ms;          /* worker time in batched steps (launch to results on the host) */
-    double admit_ms;         /* worker time admitting requests (density upload included) */
+    double step_ms;          /* worker time in batched steps (launch to results on the host; admission
+                                made while a step runs is inside this span) */
+    double admit_ms;         /* worker time admitting requests (density upload included; while requests
+                                are active it runs beside the step on a second stream) */
     double finish_ms;        /* worker time applying the stopping rule and freeing slots */
     int64_t spilled;         /* unclaimed results copied to the host to admit others */
 } mgs_server_stats;

@@ -573,7 +573,8 @@ struct Request {
 struct Slot {
     Request* r = nullptr;
     Hier* h = nullptr;
-    bool fresh = true;  // outer residual of the start field not yet taken
+    bool pending = false;  // admitted while a step ran: joins the next step
+    bool fresh = true;     // outer residual of the start field not yet taken
     int it = 0;         // outer residuals taken so far
     double res0 = 0.0;
     double res1 = 1.0e5, res2 = 1.0e10;  // the two residuals before the last (mode-0 rule)
@@ -588,7 +589,8 @@ using namespace mgs::serve;
 struct mgs_server {
     mgs_server_opts o{};
     int device = 0;
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr;   // the batched steps
+    hipStream_t cs = nullptr;  // admission uploads and spills, beside the steps
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -616,6 +618,8 @@ struct mgs_server {
     // pinned copy-out staging: idle buffers by size, bytes pinned in all (under mu)
     std::map<size_t, std::vector<void*>> pins;
     size_t pinned = 0;
+    // idle copy-out streams of the waiters (under mu), kept until destroy
+    std::vector<hipStream_t> wstreams;
     // statistics
     std::vector<double> lat;
     int64_t submitted = 0, rejected = 0, steps = 0, spilled = 0;
@@ -824,30 +828,31 @@ void fill_desc(SlotDesc& sd, const Hier& h, const mgs_params& p)
 }
 
 // upload the density of a freshly admitted request (nDGP: and restrict it to
-// every level, as cache_densities) and start from u = u_bg (nDGP: phi = 0)
+// every level, as cache_densities) and start from u = u_bg (nDGP: phi = 0).
+// On the admission stream: the hierarchy is idle (from the pool or new), so
+// this runs beside a step of the active requests.
 int start_request(mgs_server* S, Slot& sl)
 {
     Request* r = sl.r;
     Hier* h = sl.h;
+    hipStream_t cs = S->cs;
     const int N = r->p.ngrid;
     const size_t n3 = (size_t)N * N * N;
     if (r->dtype == MGS_F32) {
-        if (hipMemcpyAsync(h->d[0], r->delta, n3 * 4, hipMemcpyHostToDevice, S->s) != hipSuccess)
-            return MGS_E_HIP;
-        launch_pack_f64_from_f32(h->d[0], h->rho, N, S->s);
+        if (hipMemcpyAsync(h->d[0], r->delta, n3 * 4, hipMemcpyHostToDevice, cs) != hipSuccess) return MGS_E_HIP;
+        launch_pack_f64_from_f32(h->d[0], h->rho, N, cs);
     } else {
-        if (hipMemcpyAsync(h->u, r->delta, n3 * 8, hipMemcpyHostToDevice, S->s) != hipSuccess)
-            return MGS_E_HIP;
-        launch_pack_f64_from_f64(h->u, h->rho, N, S->s);
+        if (hipMemcpyAsync(h->u, r->delta, n3 * 8, hipMemcpyHostToDevice, cs) != hipSuccess) return MGS_E_HIP;
+        launch_pack_f64_from_f64(h->u, h->rho, N, cs);
     }
     double ubg = 0.0;
     if (h->model == MGS_MODEL_DGP) {
-        for (int l = 1; l <= h->lc; l++) launch_restrict8_64(h->rhol[l - 1], h->rhol[l], nullptr, N >> l, S->s);
+        for (int l = 1; l <= h->lc; l++) launch_restrict8_64(h->rhol[l - 1], h->rhol[l], nullptr, N >> l, cs);
     } else {
         fr_background(r->p, &ubg);
     }
-    launch_fill_f64(h->u, n3, ubg, S->s);
-    if (hipStreamSynchronize(S->s) != hipSuccess) return MGS_E_HIP;
+    launch_fill_f64(h->u, n3, ubg, cs);
+    if (hipStreamSynchronize(cs) != hipSuccess) return MGS_E_HIP;
     if (!r->verify) {  // a verified request's density stays for the FP64 solve in wait
         give_stage(S, r->delta, r->delta_bytes);
         r->delta = nullptr;
@@ -909,9 +914,10 @@ bool spill_oldest(mgs_server* S)
     const int N = h->N;
     const size_t n3 = (size_t)N * N * N;
     r->u.resize(n3);
-    launch_unpack_f64_to_f64(h->u, h->rho, N, 1.0, S->s);
-    const bool ok = hipMemcpyAsync(r->u.data(), h->rho, n3 * 8, hipMemcpyDeviceToHost, S->s) == hipSuccess &&
-                    hipStreamSynchronize(S->s) == hipSuccess;
+    // the held hierarchy is out of every step: the admission stream
+    launch_unpack_f64_to_f64(h->u, h->rho, N, 1.0, S->cs);
+    const bool ok = hipMemcpyAsync(r->u.data(), h->rho, n3 * 8, hipMemcpyDeviceToHost, S->cs) == hipSuccess &&
+                    hipStreamSynchronize(S->cs) == hipSuccess;
     {
         std::lock_guard<std::mutex> lk(S->mu);
         if (!ok) r->res.status = MGS_E_HIP;
@@ -923,8 +929,10 @@ bool spill_oldest(mgs_server* S)
     return true;
 }
 
-// move waiting requests into free slots, FIFO, while their hierarchies fit
-int admit(mgs_server* S)
+// move waiting requests into free slots, FIFO, while their hierarchies fit;
+// during_step: a step of the active requests is running (the new slots join
+// the next one)
+int admit(mgs_server* S, bool during_step)
 {
     int admitted = 0;
     for (;;) {
@@ -971,6 +979,7 @@ int admit(mgs_server* S)
         Slot& sl = S->slots[free_slot];
         sl.r = r;
         sl.h = h;
+        sl.pending = during_step;
         sl.fresh = true;
         sl.it = 0;
         sl.res0 = 0.0;
@@ -1122,9 +1131,25 @@ int run_step(mgs_server* S, int dfr, int ddgp)
     return hipGraphLaunch(it->second, S->s) == hipSuccess ? 0 : MGS_E_HIP;
 }
 
+// One iteration: launch a step of the active requests, admit waiting
+// requests while it runs (their uploads go on the admission stream, beside
+// the step; they join the next step), wait for the step's norms, apply the
+// stopping rule.  With no request active, admission runs alone first.
 void worker_main(mgs_server* S)
 {
     (void)hipSetDevice(S->device);
+    auto count_active = [S] {
+        int n = 0;
+        for (int i = 0; i < S->o.max_active; i++) n += S->slots[i].r ? 1 : 0;
+        return n;
+    };
+    auto timed_admit = [S](bool during_step) {
+        const Clock::time_point ta = Clock::now();
+        admit(S, during_step);
+        const double dt = ms_between(ta, Clock::now());
+        std::lock_guard<std::mutex> lk(S->mu);  // statistics are read under mu
+        S->admit_ms += dt;
+    };
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
@@ -1135,15 +1160,8 @@ void worker_main(mgs_server* S)
             });
             if (S->stop) break;
         }
-        const Clock::time_point ta = Clock::now();
-        admit(S);
-        {
-            const double dt = ms_between(ta, Clock::now());
-            std::lock_guard<std::mutex> lk(S->mu);  // statistics are read under mu
-            S->admit_ms += dt;
-        }
-        int nact = 0;
-        for (int i = 0; i < S->o.max_active; i++) nact += S->slots[i].r ? 1 : 0;
+        if (!count_active()) timed_admit(false);
+        const int nact = count_active();
         if (!nact) {
             // the head of the queue waits for a hierarchy a waiter is copying
             // out (or was failed by admit()): sleep until one is returned
@@ -1163,9 +1181,12 @@ void worker_main(mgs_server* S)
             rc = MGS_E_HIP;
         S->desc_dirty = false;
         if (!rc) rc = run_step(S, dfr, ddgp);
-        if (!rc && (hipMemcpyAsync(S->hsum, S->dsum, sizeof(double) * MAXS, hipMemcpyDeviceToHost, S->s) != hipSuccess ||
-                    hipStreamSynchronize(S->s) != hipSuccess))
+        if (!rc && hipMemcpyAsync(S->hsum, S->dsum, sizeof(double) * MAXS, hipMemcpyDeviceToHost, S->s) != hipSuccess)
             rc = MGS_E_HIP;
+        // hS and hP are not touched again before the synchronization below
+        // (admission only marks descriptors dirty)
+        if (!rc) timed_admit(true);
+        if (hipStreamSynchronize(S->s) != hipSuccess) rc = rc ? rc : MGS_E_HIP;
         const Clock::time_point tf = Clock::now();
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -1177,6 +1198,10 @@ void worker_main(mgs_server* S)
         for (int i = 0; i < S->o.max_active; i++) {
             Slot& sl = S->slots[i];
             if (!sl.r) continue;
+            if (sl.pending) {  // admitted during this step: no norm of its own yet
+                sl.pending = false;
+                continue;
+            }
             if (rc) {
                 finish_request(S, sl, rc);
                 continue;
@@ -1222,9 +1247,6 @@ void worker_main(mgs_server* S)
     }
 }
 
-// a result held in HBM into the caller's host array: unpacked into the
-// reference order in the requested type and scale (into the hierarchy's
-// level-0 scratch) and copied, on a stream of the calling thread
 // Pinned staging for copy-outs, pooled by size and kept until the server is
 // destroyed (a buffer per claim would cost a pinning per result); at most
 // PIN_MAX bytes in all, beyond which a claim copies through the runtime's
@@ -1280,15 +1302,39 @@ void copy_host(void* dst, const void* src, size_t bytes)
     for (auto& x : th) x.join();
 }
 
+// A waiter's copy-out stream: an idle one of the server's, or a new
+// non-blocking one; returned to the server after the copy (a stream per
+// claim would cost a creation and a destruction each).
+hipStream_t take_wstream(mgs_server* S)
+{
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!S->wstreams.empty()) {
+            hipStream_t ws = S->wstreams.back();
+            S->wstreams.pop_back();
+            return ws;
+        }
+    }
+    hipStream_t ws = nullptr;
+    if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    return ws;
+}
+
+void give_wstream(mgs_server* S, hipStream_t ws)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->wstreams.push_back(ws);
+}
+
 // a result held in HBM into the caller's host array: unpacked into the
 // reference order in the requested type and scale (into the hierarchy's
 // level-0 scratch), copied to pinned staging and from there into the array,
-// on a stream of the calling thread
+// on a copy-out stream of the server's
 int copy_out(mgs_server* S, Hier* h, void* u, int32_t dtype, double scale)
 {
     if (hipSetDevice(S->device) != hipSuccess) return MGS_E_HIP;
-    hipStream_t ws = nullptr;
-    if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) return MGS_E_HIP;
+    hipStream_t ws = take_wstream(S);
+    if (!ws) return MGS_E_HIP;
     const int N = h->N;
     const size_t n3 = (size_t)N * N * N;
     const size_t bytes = n3 * (dtype == MGS_F32 ? sizeof(float) : sizeof(double));
@@ -1303,7 +1349,7 @@ int copy_out(mgs_server* S, Hier* h, void* u, int32_t dtype, double scale)
     void* pin = bytes >= PIN_MIN ? take_pin(S, bytes) : nullptr;
     const hipError_t e = hipMemcpyAsync(pin ? pin : u, dev, bytes, hipMemcpyDeviceToHost, ws);
     const hipError_t es = hipStreamSynchronize(ws);
-    (void)hipStreamDestroy(ws);
+    give_wstream(S, ws);
     const bool ok = e == hipSuccess && es == hipSuccess;
     if (pin) {
         if (ok) copy_host(u, pin, bytes);
@@ -1381,6 +1427,7 @@ int mgs_server_create(const mgs_server_opts* opts, mgs_server** out)
     // client's, or a waiter's FP64 comparison solve) synchronizes with the
     // worker's stream, which is being captured into graphs between steps
     if (hipStreamCreateWithFlags(&S->s, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&S->cs, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&S->dS, sizeof(SlotDesc) * MAXS) != hipSuccess ||
         hipMalloc((void**)&S->dP, sizeof(Plan)) != hipSuccess ||
         hipMalloc((void**)&S->dsum, sizeof(double) * MAXS) != hipSuccess ||
@@ -1413,6 +1460,7 @@ int mgs_server_destroy(mgs_server* S)
         S->cv_done.wait(lk, [&] { return S->waiters == 0; });
     }
     if (S->s) (void)hipStreamSynchronize(S->s);
+    if (S->cs) (void)hipStreamSynchronize(S->cs);
     S->held.clear();  // their hierarchies are freed with the requests below
     for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
     for (int i = 0; i < MAXS; i++)
@@ -1442,7 +1490,9 @@ int mgs_server_destroy(mgs_server* S)
     if (S->hS) (void)hipHostFree(S->hS);
     if (S->hP) (void)hipHostFree(S->hP);
     if (S->hsum) (void)hipHostFree(S->hsum);
+    for (hipStream_t ws : S->wstreams) (void)hipStreamDestroy(ws);
     if (S->s) (void)hipStreamDestroy(S->s);
+    if (S->cs) (void)hipStreamDestroy(S->cs);
     delete S;
     return 0;
 }
