@@ -375,10 +375,11 @@ __global__ __launch_bounds__(1024) void k_dgpm_coarsest(float* __restrict__ e, c
     dgpm_coarsest_block(e, pb, rho, rhs, tmp, g, sweeps, c);
 }
 
-// The same correction z-marching over coarse columns, as k_df_correct: two
-// planes of e = e_H - R e_h in registers (x-neighbours from the adjacent
-// lanes), all 8 sons of a coarse cell, every load of a coarse plane issued
-// before its sons are stored.  Sums in dgpm_correct_cell's order.
+// The same correction z-marching over coarse columns, as k_df_correct: planes
+// of e = e_H - R e_h in registers (x-neighbours from the adjacent lanes), all
+// 8 sons of a coarse cell, software-pipelined (plane K's sons computed, the
+// loads of plane K+1 issued, then plane K's sons stored).  Sums in
+// dgpm_correct_cell's order.
 template <int S3>
 __device__ __forceinline__ void dgpm_sons_load(const float* __restrict__ ef, const Geo& gf, int I, int J, int K,
                                                float* old)
@@ -388,8 +389,7 @@ __device__ __forceinline__ void dgpm_sons_load(const float* __restrict__ ef, con
 }
 
 template <int S3>
-__device__ __forceinline__ void dgpm_sons_apply(const float* ez, const float* ek, float* __restrict__ ef,
-                                                const Geo& gf, int I, int J, int K, const float* old)
+__device__ __forceinline__ void dgpm_sons_new(const float* ez, const float* ek, const float* old, float* nv)
 {
     const float a = 1.0f / 64.0f, b = 3.0f * a, c9 = 9.0f * a, dd = 27.0f * a;
     const float wt[8] = {a, b, b, c9, b, c9, c9, dd};
@@ -403,8 +403,16 @@ __device__ __forceinline__ void dgpm_sons_apply(const float* ez, const float* ek
             const int y = (ind & 2) ? 1 : (s2 ? 2 : 0);
             P += wt[ind] * ((ind & 4) ? ek[x + 3 * y] : ez[x + 3 * y]);
         }
-        elem(ef, son_index(gf, I, J, K, s1, s2, S3)) = old[t] + P;
+        nv[t] = old[t] + P;
     }
+}
+
+template <int S3>
+__device__ __forceinline__ void dgpm_sons_store(float* __restrict__ ef, const Geo& gf, int I, int J, int K,
+                                                const float* nv)
+{
+#pragma unroll
+    for (int t = 0; t < 4; t++) elem(ef, son_index(gf, I, J, K, t & 1, t >> 1, S3)) = nv[t];
 }
 
 __global__ __launch_bounds__(256) void k_dgpm_zcorrect(const float* __restrict__ eH, const float* __restrict__ rd,
@@ -421,16 +429,28 @@ __global__ __launch_bounds__(256) void k_dgpm_zcorrect(const float* __restrict__
     const int I = cc.m, J = cc.j;
     Block9 B;
     B.init(gc, I, J);
-    float em[9], ek[9];  // planes K-1 and K
-    e_block9<true>(eH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, em);
-    e_block9<true>(eH, rd, B, cc.k0, lok, rok, ek);
+    ERaw wm, wk, wp;  // coarse planes K-1, K, K+1
+    float o0[4], o1[4];
+    e_load<true>(eH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, wm);
+    e_load<true>(eH, rd, B, cc.k0, lok, rok, wk);
+    e_load<true>(eH, rd, B, wrap_inc(cc.k0, gc.n), lok, rok, wp);
+    dgpm_sons_load<0>(ef, gf, I, J, cc.k0, o0);
+    dgpm_sons_load<1>(ef, gf, I, J, cc.k0, o1);
+    float em[9], ek[9], ep[9];
+    e_form<true>(wm, lok, rok, em);
+    e_form<true>(wk, lok, rok, ek);
     for (int K = cc.k0; K < cc.k1; K++) {
-        float o0[4], o1[4], ep[9];
-        dgpm_sons_load<0>(ef, gf, I, J, K, o0);
-        dgpm_sons_load<1>(ef, gf, I, J, K, o1);
-        e_block9<true>(eH, rd, B, wrap_inc(K, gc.n), lok, rok, ep);
-        dgpm_sons_apply<0>(em, ek, ef, gf, I, J, K, o0);
-        dgpm_sons_apply<1>(ep, ek, ef, gf, I, J, K, o1);
+        e_form<true>(wp, lok, rok, ep);
+        float n0[4], n1[4];
+        dgpm_sons_new<0>(em, ek, o0, n0);
+        dgpm_sons_new<1>(ep, ek, o1, n1);
+        if (K + 1 < cc.k1) {
+            dgpm_sons_load<0>(ef, gf, I, J, K + 1, o0);
+            dgpm_sons_load<1>(ef, gf, I, J, K + 1, o1);
+            e_load<true>(eH, rd, B, wrap_inc(K + 1, gc.n), lok, rok, wp);
+        }
+        dgpm_sons_store<0>(ef, gf, I, J, K, n0);
+        dgpm_sons_store<1>(ef, gf, I, J, K, n1);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             em[q] = ek[q];

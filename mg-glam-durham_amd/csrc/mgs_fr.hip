@@ -239,7 +239,12 @@ __global__ __launch_bounds__(256) void k_df_restrict_residual(const float* __res
 {
     ZCol cc;
     if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
-    df_restrict_residual_col<FRN>(d0, xb0, g0, rres1, rd1, d1, gf, gc, c, cc);
+    // lanes t-1 / t+1 hold columns I-1 / I+1 of the same row and plane run
+    // when they are in the same block row, in this wave and inside the level
+    const unsigned t = threadIdx.y * (unsigned)TX + threadIdx.x;
+    const bool lok = threadIdx.x > 0 && (t & 63u) != 0;
+    const bool rok = (int)threadIdx.x + 1 < TX && cc.m + 1 < gc.n && ((t + 1) & 63u) != 0;
+    df_restrict_residual_col<FRN, true>(d0, xb0, g0, rres1, rd1, d1, gf, gc, c, cc, lok, rok);
 }
 
 template <int FRN>

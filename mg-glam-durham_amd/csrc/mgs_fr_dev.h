@@ -565,11 +565,12 @@ struct DfResOp {
 // One coarse column (I, J) = (cc.m, cc.j) marching over its run of K, like
 // k_fr_restrict_residual1_64.  The children's increments it reads also give
 // R d, stored as the frozen copy rd1 and the coarse start value d1.
-template <int FRN>
+template <int FRN, bool SHFL = false>
 __device__ __forceinline__ void df_restrict_residual_col(const float* __restrict__ d0, const float* __restrict__ xb0,
                                                          const float* __restrict__ g0, float* __restrict__ rres1,
                                                          float* __restrict__ rd1, float* __restrict__ d1,
-                                                         const Geo& gf, const Geo& gc, const DfCoef& c, const ZCol& cc)
+                                                         const Geo& gf, const Geo& gc, const DfCoef& c, const ZCol& cc,
+                                                         bool lok = false, bool rok = false)
 {
     const int I = cc.m, J = cc.j, N = gf.n;
     Pts12 F;
@@ -589,7 +590,25 @@ __device__ __forceinline__ void df_restrict_residual_col(const float* __restrict
             p.d[q] = dc;
         }
 #pragma unroll
-        for (int q = 0; q < 8; q++) p.f[q] = elem(d0, F.at(4 + q, z));
+        for (int q = 0; q < 8; q++) {
+            // SHFL: the x-faces 2I-1 and 2I+2 are children of columns I-1
+            // and I+1, which the adjacent lanes load where lok / rok say so
+            if (SHFL && q < 2 && lok) continue;
+            if (SHFL && q >= 2 && q < 4 && rok) continue;
+            p.f[q] = elem(d0, F.at(4 + q, z));
+        }
+        if constexpr (SHFL) {
+            const float l1 = __shfl_up(p.d[1], 1u, 64), l3 = __shfl_up(p.d[3], 1u, 64);
+            const float r0 = __shfl_down(p.d[0], 1u, 64), r2 = __shfl_down(p.d[2], 1u, 64);
+            if (lok) {
+                p.f[0] = l1;
+                p.f[1] = l3;
+            }
+            if (rok) {
+                p.f[2] = r0;
+                p.f[3] = r2;
+            }
+        }
     };
     Pl p0, p1, p2;
     float dl[4];
@@ -735,16 +754,6 @@ __device__ __forceinline__ void df_sons_new(const float* ez, const float* ek, co
         dn[t] = old[t] + P;
         keep[t] = wpow<FRN>(xb[t]) + dn[t] > 0.0f;
     }
-}
-
-// e over the 3x3 coarse block of plane K in one step
-template <bool SHFL>
-__device__ __forceinline__ void e_block9(const float* __restrict__ dH, const float* __restrict__ rd, const Block9& B,
-                                         int K, bool lok, bool rok, float* e)
-{
-    ERaw w;
-    e_load<SHFL>(dH, rd, B, K, lok, rok, w);
-    e_form<SHFL>(w, lok, rok, e);
 }
 
 template <int S3>
