@@ -112,8 +112,8 @@ typedef struct mgs_solve_opts {
     int32_t inner_cycles;    /* mixed: inner FP32 V-cycles per outer FP64 iteration; 0: mode 1 picks each
                                 outer iteration's count (1 to 4) from the contraction measured so far,
                                 mode 0 runs 1 */
-    int32_t cycle;           /* plain FP64: 0 V_cycle, 1 F_cycle, 2 W_cycle (the reference's Vcycle/Fcycle/
-                                Wcycle switches, PMP2extradof.f90:86-88); the mixed form uses V-cycles */
+    int32_t cycle;           /* 0 V_cycle, 1 F_cycle, 2 W_cycle (the reference's Vcycle/Fcycle/Wcycle
+                                switches, PMP2extradof.f90:86-88); mixed: the inner cycle's kind */
     int32_t reserved;
 } mgs_solve_opts;
 
@@ -289,10 +289,15 @@ int mgs_solve_noalloc(const mgs_params* params, const mgs_solve_opts* opts, cons
  * advance together: every batched step (outer residual, each level's colour
  * passes, transfers, coarsest solves, update) is one persistent launch over
  * the tiles of all active requests, replayed from a captured hipGraph.
- * Requests join at the next outer iteration after admission and leave when
- * converged, so each request's result is bit-identical alone, in any mix,
- * and to mgs_solve with the same parameters and inner_cycles 1 (mode 1 for
- * exit_rule MGS_EXIT_REL_TOL, mode 0 for MGS_EXIT_REFERENCE). */
+ * Requests join at the next step after admission and leave when converged;
+ * each step runs one inner V-cycle of every admitted request, and a request
+ * takes its FP64 outer residual when its outer iteration's inner cycles are
+ * done.  f(R) requests with MGS_EXIT_REL_TOL choose their inner cycles per
+ * outer iteration as mgs_solve's mode 1 with inner_cycles 0 does; the others
+ * run one.  Each request's result is therefore bit-identical alone, in any
+ * mix, and to mgs_solve with the same parameters: mode 1 and inner_cycles 0
+ * for f(R) with MGS_EXIT_REL_TOL, mode 1 and inner_cycles 1 for nDGP with
+ * MGS_EXIT_REL_TOL, mode 0 (one inner cycle) for MGS_EXIT_REFERENCE. */
 typedef struct mgs_server mgs_server;
 
 typedef struct mgs_server_opts {

@@ -550,14 +550,13 @@ static int cycle_plain(mgs_handle* h, int l, int kind)
 // ---------------------------------------------------------------------------
 // mixed precision: FP32 delta-form FAS V-cycle (inner) and FP64 outer loop
 // ---------------------------------------------------------------------------
-// One inner V-cycle at level l: V(sweeps0) on level 0, V(2,2) below.  On
+// One inner V-cycle at level l: V(sweeps0) on level 0, V(sweeps1) below.  On
 // level 0, `first` means colour 0 of d was set by the fused outer residual
 // (relaxed once from zero: that pass is skipped, and colour 1, not written
 // there, is set by the colour-1 pass that follows) and `last` fuses the final
 // colour-1 pass with the outer update of u.
 static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = false)
 {
-    const int Nsmth = 2;
     const int frn = h->p.fr_n;
     LevelF& v = h->M[l];
     const DfCoef dc = df_coefs(h->p, l);
@@ -582,7 +581,7 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
         if (ok) return 0;
     }
     const double s = 4.0;
-    const int npre = l == 0 ? h->sweeps0[0] : Nsmth;
+    const int npre = l == 0 ? h->sweeps0[0] : h->sweeps1[0];
     for (int it = (l == 0 && first) ? 1 : 0; it < 2 * npre; it++) {
         prof_begin(h, cls, 2.0 * n3d(v.n) * s);
         launch_df_smooth(frn, v.d, v.xb, v.f, v.n, (it & 1) ? 1 : 0, dc, h->s);
@@ -607,7 +606,7 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
     prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * s);
     launch_df_correct(frn, c.d, c.rd, v.d, v.xb, c.n, h->s);
     prof_end(h);
-    const int nps = l == 0 ? h->sweeps0[1] : Nsmth;
+    const int nps = l == 0 ? h->sweeps0[1] : h->sweeps1[1];
     const int npost = (l == 0 && last) ? 2 * nps - 1 : 2 * nps;
     for (int it = 0; it < npost; it++) {
         prof_begin(h, cls, 2.0 * n3d(v.n) * s);
@@ -764,6 +763,114 @@ static int vcycle_inner_dgp(mgs_handle* h, int l)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// mixed precision: inner F- and W-cycles (mgs_solve_opts.cycle 1 and 2) on
+// the delta-form problem, in the order of the reference's F_cycle and
+// W_cycle (PMP2extradof.f90:563-858, 867-1157): smoothing, a coarse-grid
+// correction by an F- (W-) cycle, smoothing, a correction by a V- (W-)
+// cycle, smoothing; the coarsest level only its solve.  f(R) smoothing
+// blocks: the V-cycle's pre-sweeps first and its post-sweeps after each
+// correction (level 0: sweeps0, the first colour-0 pass skipped after the
+// outer residual and the last colour-1 pass fused with the update; below:
+// sweeps1); nDGP: two pairs each, as its V-cycle.  The levels are launched
+// one by one (the captured graph and the single-workgroup tail follow the
+// V-cycle's shape); corrections below a V-correction use the V-cycle itself.
+// ---------------------------------------------------------------------------
+static int fw_inner(mgs_handle* h, int l, int kind, bool first, bool last);
+
+// `pairs` colour-0/colour-1 pairs of the f(R) inner smoother on level l
+static void df_smooth_pairs(mgs_handle* h, int l, int pairs, bool skip_first, bool fuse_last)
+{
+    const int frn = h->p.fr_n;
+    LevelF& v = h->M[l];
+    const DfCoef dc = df_coefs(h->p, l);
+    const int cls = l == 0 ? MGS_KC_SMOOTH : MGS_KC_COARSE;
+    const int n = 2 * pairs - (fuse_last ? 1 : 0);
+    for (int it = skip_first ? 1 : 0; it < n; it++) {
+        prof_begin(h, cls, 2.0 * n3d(v.n) * 4.0);
+        launch_df_smooth(frn, v.d, v.xb, v.f, v.n, (it & 1) ? 1 : 0, dc, h->s);
+        prof_end(h);
+    }
+    if (fuse_last) {
+        prof_begin(h, MGS_KC_UPDATE, 24.0 * n3d(v.n));
+        launch_df_smooth_update(frn, v.d, v.xb, v.f, h->L[0].u, v.n, dc, fr_coefs(h->p, 0).ubg, h->s);
+        prof_end(h);
+    }
+}
+
+// f(R) coarse-grid correction of level l by a `kind` cycle on level l+1
+static int df_cgc(mgs_handle* h, int l, int kind)
+{
+    const int frn = h->p.fr_n;
+    LevelF& v = h->M[l];
+    LevelF& c = h->M[l + 1];
+    const int cls = l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE;
+    prof_begin(h, cls, (3.0 * n3d(v.n) + 3.0 * n3d(c.n)) * 4.0);
+    launch_df_restrict_residual(frn, v.d, v.xb, v.f, c.rres, c.rd, c.d, v.n, df_coefs(h->p, l), h->s);
+    prof_end(h);
+    prof_begin(h, MGS_KC_COARSE, 4.0 * n3d(c.n) * 4.0);
+    launch_df_rhs(frn, c.rd, c.xb, c.rres, c.f, c.n, df_coefs(h->p, l + 1), h->s);
+    prof_end(h);
+    const int rc = kind == 0 ? vcycle_inner(h, l + 1) : fw_inner(h, l + 1, kind, false, false);
+    if (rc) return rc;
+    prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * 4.0);
+    launch_df_correct(frn, c.d, c.rd, v.d, v.xb, c.n, h->s);
+    prof_end(h);
+    return 0;
+}
+
+static int fw_inner(mgs_handle* h, int l, int kind, bool first, bool last)
+{
+    if (l >= h->lc) return vcycle_inner(h, l);  // the coarsest solve
+    const int pre = l == 0 ? h->sweeps0[0] : h->sweeps1[0];
+    const int post = l == 0 ? h->sweeps0[1] : h->sweeps1[1];
+    int rc;
+    df_smooth_pairs(h, l, pre, l == 0 && first, false);
+    if ((rc = df_cgc(h, l, kind == 1 ? 1 : 2))) return rc;
+    df_smooth_pairs(h, l, post, false, false);
+    if ((rc = df_cgc(h, l, kind == 1 ? 0 : 2))) return rc;
+    df_smooth_pairs(h, l, post, false, l == 0 && last);
+    return 0;
+}
+
+static int fw_inner_dgp(mgs_handle* h, int l, int kind);
+
+// nDGP coarse-grid correction of level l by a `kind` cycle on level l+1
+// (vcycle_inner_dgp's transfers)
+static int dgp_cgc(mgs_handle* h, int l, int kind)
+{
+    LevelF& v = h->M[l];
+    LevelD& V = h->L[l];
+    LevelF& c = h->M[l + 1];
+    prof_begin(h, l == 0 ? MGS_KC_RESTRICT : MGS_KC_COARSE, 6.0 * n3d(v.n) * 4.0);
+    launch_dgpm_residual(v.d, v.xb, V.rho, v.rho, v.f, v.res, v.n, dgp_coefs(h->p, l), h->s);
+    launch_restrict8_32(v.res, c.rres, nullptr, c.n, h->s);
+    launch_restrict8_32(v.d, c.rd, c.d, c.n, h->s);
+    prof_end(h);
+    prof_begin(h, MGS_KC_COARSE, 5.0 * n3d(c.n) * 4.0);
+    launch_dgpm_residual(c.rd, c.xb, h->L[l + 1].rho, c.rho, c.rres, c.f, c.n, dgp_coefs(h->p, l + 1), h->s);
+    prof_end(h);
+    const int rc = kind == 0 ? vcycle_inner_dgp(h, l + 1) : fw_inner_dgp(h, l + 1, kind);
+    if (rc) return rc;
+    prof_begin(h, l == 0 ? MGS_KC_CORRECT : MGS_KC_COARSE, (3.0 * n3d(v.n) + 2.0 * n3d(c.n)) * 4.0);
+    launch_dgpm_correct(c.d, c.rd, v.d, c.n, h->s);
+    prof_end(h);
+    return 0;
+}
+
+static int fw_inner_dgp(mgs_handle* h, int l, int kind)
+{
+    if (l >= h->lc) return vcycle_inner_dgp(h, l);  // the coarsest solve
+    const int cls = l == 0 ? MGS_KC_SMOOTH : MGS_KC_COARSE;
+    int rc;
+    dgpm_smooth4(h, l, cls);
+    if ((rc = dgp_cgc(h, l, kind == 1 ? 1 : 2))) return rc;
+    dgpm_smooth4(h, l, cls);
+    if ((rc = dgp_cgc(h, l, kind == 1 ? 0 : 2))) return rc;
+    dgpm_smooth4(h, l, cls);
+    return 0;
+}
+
 static int outer_residual_dgp(mgs_handle* h, double* rms)
 {
     LevelD& v = h->L[0];
@@ -787,16 +894,21 @@ static int outer_residual_dgp(mgs_handle* h, double* rms)
 // the first outer iteration runs the cap.  Fewer outer iterations save their
 // FP64 residual, update and base restriction, about half an inner cycle each.
 // *reaches: the count is expected to reach the target (not capped).
-static int auto_inner(const mgs_handle* h, double res, double target, double res_prev, int m_prev, bool* reaches)
+int auto_inner_count(int inner_max, double res, double target, double res_prev, int m_prev, bool* reaches)
 {
     *reaches = false;
-    if (m_prev <= 0 || !(res_prev > 0.0) || !(res > 0.0) || !(target > 0.0)) return h->inner_max;
+    if (m_prev <= 0 || !(res_prev > 0.0) || !(res > 0.0) || !(target > 0.0)) return inner_max;
     const double rho = std::pow(res / res_prev, 1.0 / m_prev);
-    if (!(rho > 0.0) || !(rho < 0.9)) return h->inner_max;
+    if (!(rho > 0.0) || !(rho < 0.9)) return inner_max;
     const double m = std::ceil(std::log(0.7 * target / res) / std::log(rho));
-    if (m > (double)h->inner_max) return h->inner_max;
+    if (m > (double)inner_max) return inner_max;
     *reaches = true;
     return m >= 1.0 ? (int)m : 1;
+}
+
+static int auto_inner(const mgs_handle* h, double res, double target, double res_prev, int m_prev, bool* reaches)
+{
+    return auto_inner_count(h->inner_max, res, target, res_prev, m_prev, reaches);
 }
 
 // FP64 defect correction around the FP32 inner FAS (f(R) and nDGP)
@@ -862,12 +974,14 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
             // d = 0 with its first colour-0 pass came from the outer residual;
             // the last inner pass also applies u += u_bg (x(d) - xb)
             for (int ic = 0; ic < inner; ic++) {
-                vcycle_inner(h, 0, ic == 0, ic == inner - 1);
+                if ((rc = o.cycle == 0 ? vcycle_inner(h, 0, ic == 0, ic == inner - 1)
+                                       : fw_inner(h, 0, o.cycle, ic == 0, ic == inner - 1)))
+                    return rc;
                 cycles++;
             }
         } else {
             for (int ic = 0; ic < inner; ic++) {
-                vcycle_inner_dgp(h, 0);
+                if ((rc = o.cycle == 0 ? vcycle_inner_dgp(h, 0) : fw_inner_dgp(h, 0, o.cycle))) return rc;
                 cycles++;
             }
             prof_begin(h, MGS_KC_UPDATE, 20.0 * n3d(h->L[0].n));
@@ -902,6 +1016,11 @@ int init_handle(mgs_handle* h, const mgs_params* params, bool use_arena, unsigne
         if (w && std::sscanf(w, "%d,%d", &a, &b) == 2 && a >= 1 && b >= 1 && a <= 8 && b <= 8) {
             h->sweeps0[0] = a;
             h->sweeps0[1] = b;
+        }
+        const char* w1 = std::getenv("MGS_COARSE_SWEEPS");
+        if (w1 && std::sscanf(w1, "%d,%d", &a, &b) == 2 && a >= 1 && b >= 1 && a <= 8 && b <= 8) {
+            h->sweeps1[0] = a;
+            h->sweeps1[1] = b;
         }
         const char* im = std::getenv("MGS_INNER_MAX");
         if (im && std::atoi(im) >= 1 && std::atoi(im) <= 64) h->inner_max = std::atoi(im);
@@ -1333,8 +1452,6 @@ int mgs_solve(mgs_handle* h, const mgs_solve_opts* opts, mgs_solve_report* repor
     std::memset(report, 0, sizeof(*report));
     if (opts->max_cycles < 1) return set_err(h, MGS_E_ARG, "max_cycles must be >= 1");
     if (opts->cycle < 0 || opts->cycle > 2) return set_err(h, MGS_E_ARG, "cycle must be 0 (V), 1 (F) or 2 (W)");
-    if (opts->cycle != 0 && h->p.precision == MGS_PREC_MIXED)
-        return set_err(h, MGS_E_ARG, "the mixed-precision solve uses V-cycles (cycle = 0)");
     // the solve's wall time on the stream; destroyed on every return path
     struct EventPair {
         hipEvent_t a = nullptr, b = nullptr;

@@ -112,6 +112,7 @@ struct mgs_handle {
     // automatic inner-cycle cap (tuning builds: MGS_INNER_SWEEPS "a,b" and
     // MGS_INNER_MAX override them for mgs_solve only)
     int sweeps0[2] = {mgs::MIXED_PRE0, mgs::MIXED_POST0};
+    int sweeps1[2] = {2, 2};  // the inner cycle's pre / post pairs below level 0 (above the tail)
     int inner_max = mgs::MIXED_INNER_MAX;
     // profiling
     bool prof = false;
@@ -144,6 +145,11 @@ int levelmax_of(int N);
 int levelmin_of(int N);
 // the parameter checks of mgs_create (0 or an MGS_E_* code with a reason)
 int validate_params(const mgs_params* p, std::string* why);
+// mixed mode 1, automatic inner count: the inner cycles of the next outer
+// iteration from its start residual res, the target, and the previous outer
+// iteration's start residual and count (m_prev 0: none yet, inner_max);
+// *reaches: the count is expected to reach the target
+int auto_inner_count(int inner_max, double res, double target, double res_prev, int m_prev, bool* reaches);
 // mode-0 stopping rule after outer/cycle `iter` with residual res and the two
 // before it (res1, res2); sets *conv when res <= res_conv
 bool reference_exit(const mgs_solve_opts& o, int iter, double res, double res1, double res2, bool* conv);

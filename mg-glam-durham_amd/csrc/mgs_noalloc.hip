@@ -70,8 +70,6 @@ extern "C" int mgs_solve_noalloc(const mgs_params* params, const mgs_solve_opts*
     if (rc) return set_err(nullptr, rc, why);
     if (opts->max_cycles < 1) return set_err(nullptr, MGS_E_ARG, "max_cycles must be >= 1");
     if (opts->cycle < 0 || opts->cycle > 2) return set_err(nullptr, MGS_E_ARG, "cycle must be 0 (V), 1 (F) or 2 (W)");
-    if (opts->cycle != 0 && params->precision == MGS_PREC_MIXED)
-        return set_err(nullptr, MGS_E_ARG, "the mixed-precision solve uses V-cycles (cycle = 0)");
     std::memset(report, 0, sizeof(*report));
 
     // bytes this hierarchy needs (a counting pass touches no device memory)

@@ -48,7 +48,7 @@ namespace serve {
 
 constexpr int MAXS = 64;    // slots
 constexpr int MAXL = 11;    // levels of a hierarchy (ngrid <= 1024)
-constexpr int GRID = 2048;  // blocks of a batched stencil/transfer launch
+constexpr int GRID = 8192;  // blocks of a batched stencil/transfer launch (tiles are walked, XCD-contiguous)
 constexpr int SWEEPS = 8;   // coarsest-level sweeps (as solve_mixed)
 
 struct Shape {
@@ -112,7 +112,23 @@ struct Plan {
     int nupd;
     int upd_slot[MAXS];
     unsigned upd_pre[MAXS + 1];  // level-0 cells (phi += e)
+    // f(R) requests by where their inner cycle stands in its outer
+    // iteration (level-0 stencil tiles): not the first cycle (their level-0
+    // pre-smoothing starts with a colour-0 pass), the last one (its final
+    // colour-1 pass applies the outer update), not the last one (a plain
+    // final colour-1 pass)
+    int nnf, nlast, nnl;
+    int nf_slot[MAXS], last_slot[MAXS], nl_slot[MAXS];
+    unsigned nf_pre[MAXS + 1], last_pre[MAXS + 1], nl_pre[MAXS + 1];
+    // f(R) requests in the first cycle of an outer iteration: the base xb
+    // restricted from level l to l+1 (level-(l+1) cells in blocks of 256)
+    int nrb[MAXL];
+    int rb_slot[MAXL][MAXS];
+    unsigned rb_pre[MAXL][MAXS + 1];
 };
+
+// the level-0 lists of the f(R) inner cycle
+enum { LIST_NOT_FIRST = 0, LIST_LAST = 1, LIST_NOT_LAST = 2 };
 
 // ---------------------------------------------------------------------------
 // batched kernels
@@ -129,6 +145,23 @@ __device__ __forceinline__ int find_slot(const unsigned* pre, int n, unsigned t)
     }
     return lo;
 }
+
+// The tiles 0 .. total-1 of a batched launch, XCD-contiguous: the blocks of
+// XCD x (blockIdx.x % 8 == x, the hardware's round-robin placement of
+// blocks) take the x-th eighth of the tile list, in order.  Neighbouring
+// tiles (a plane run's halo planes, the rows beside a slab) then share an
+// XCD's L2, as map_block arranges for a single-request launch.  A tile's
+// values do not depend on which block runs it.
+struct XcdTiles {
+    unsigned t, end, stride;
+    __device__ __forceinline__ explicit XcdTiles(unsigned total)
+    {
+        const unsigned per = (total + 7u) / 8u, x = blockIdx.x & 7u;
+        t = x * per + (blockIdx.x >> 3);
+        end = (x + 1u) * per < total ? (x + 1u) * per : total;
+        stride = gridDim.x >> 3;
+    }
+};
 
 __device__ __forceinline__ bool tile_col(const Geo& g, int width, const Shape& z, unsigned lb, ZCol& c)
 {
@@ -171,7 +204,8 @@ __device__ __forceinline__ bool tile_xy(int width, unsigned rows, const Shape& z
 __global__ __launch_bounds__(256) void kb_outer_residual(const SlotDesc* __restrict__ S, const Plan* __restrict__ P)
 {
     const unsigned total = P->res_pre[P->nres];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
         const int i = find_slot(P->res_pre, P->nres, t);
         const SlotDesc& sd = S[P->res_slot[i]];
         const LevelDesc& L = sd.L[0];
@@ -232,17 +266,18 @@ __global__ __launch_bounds__(1024) void kb_sums(const SlotDesc* __restrict__ S, 
     }
 }
 
-// xb of level l+1 from level l
+// xb of level l+1 from level l, for the requests starting an outer iteration
 __global__ __launch_bounds__(256) void kb_restrict_base(const SlotDesc* __restrict__ S, const Plan* __restrict__ P,
                                                         int l)
 {
-    const unsigned total = P->cell_pre[l][P->ninner[l]];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_slot(P->cell_pre[l], P->ninner[l], t);
-        const SlotDesc& sd = S[P->inner_slot[l][i]];
+    const unsigned total = P->rb_pre[l][P->nrb[l]];
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
+        const int i = find_slot(P->rb_pre[l], P->nrb[l], t);
+        const SlotDesc& sd = S[P->rb_slot[l][i]];
         const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
         const unsigned nc = (unsigned)C.g.n;
-        const unsigned cell = (t - P->cell_pre[l][i]) * 256u + threadIdx.x;
+        const unsigned cell = (t - P->rb_pre[l][i]) * 256u + threadIdx.x;
         if (cell < nc * nc * nc) {
             const int I = (int)(cell % nc), J = (int)((cell / nc) % nc), K = (int)(cell / (nc * nc));
             FRN_SWITCH(sd.frn, restrict_base_cell<FRN>(F.xb, C.xb, F.g, C.g, I, J, K));
@@ -255,7 +290,8 @@ __global__ __launch_bounds__(256) void kb_smooth(const SlotDesc* __restrict__ S,
                                                  int colour)
 {
     const unsigned total = P->z_pre[l][P->ninner[l]];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
         const int i = find_slot(P->z_pre[l], P->ninner[l], t);
         const SlotDesc& sd = S[P->inner_slot[l][i]];
         const LevelDesc& L = sd.L[l];
@@ -266,16 +302,58 @@ __global__ __launch_bounds__(256) void kb_smooth(const SlotDesc* __restrict__ S,
     }
 }
 
-// the last colour-1 pass on level 0 fused with the outer update of u
-__global__ __launch_bounds__(256) void kb_smooth_update(const SlotDesc* __restrict__ S, const Plan* __restrict__ P)
+// one level-0 list of the f(R) inner cycle (LIST_*)
+__device__ __forceinline__ void level0_list(const Plan* __restrict__ P, int list, int& n, const int*& slot,
+                                            const unsigned*& pre)
 {
-    const unsigned total = P->z_pre[0][P->ninner[0]];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_slot(P->z_pre[0], P->ninner[0], t);
-        const SlotDesc& sd = S[P->inner_slot[0][i]];
+    if (list == LIST_NOT_FIRST) {
+        n = P->nnf;
+        slot = P->nf_slot;
+        pre = P->nf_pre;
+    } else if (list == LIST_LAST) {
+        n = P->nlast;
+        slot = P->last_slot;
+        pre = P->last_pre;
+    } else {
+        n = P->nnl;
+        slot = P->nl_slot;
+        pre = P->nl_pre;
+    }
+}
+
+// one colour pass of the inner smoother on level 0 for the requests of a list
+__global__ __launch_bounds__(256) void kb_smooth0(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int list,
+                                                  int colour)
+{
+    int n;
+    const int* slot;
+    const unsigned* pre;
+    level0_list(P, list, n, slot, pre);
+    const unsigned total = pre[n];
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
+        const int i = find_slot(pre, n, t);
+        const SlotDesc& sd = S[slot[i]];
         const LevelDesc& L = sd.L[0];
         ZCol c;
-        if (tile_col(L.g, L.g.h, L.z, t - P->z_pre[0][i], c)) {
+        if (tile_col(L.g, L.g.h, L.z, t - pre[i], c)) {
+            FRN_SWITCH(sd.frn, colour_pass_col(DfSmoothOp<FRN>{L.d, L.xb, L.f, L.dc}, L.g, colour, c));
+        }
+    }
+}
+
+// the last colour-1 pass on level 0 fused with the outer update of u, for
+// the requests in the last inner cycle of their outer iteration
+__global__ __launch_bounds__(256) void kb_smooth_update(const SlotDesc* __restrict__ S, const Plan* __restrict__ P)
+{
+    const unsigned total = P->last_pre[P->nlast];
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
+        const int i = find_slot(P->last_pre, P->nlast, t);
+        const SlotDesc& sd = S[P->last_slot[i]];
+        const LevelDesc& L = sd.L[0];
+        ZCol c;
+        if (tile_col(L.g, L.g.h, L.z, t - P->last_pre[i], c)) {
             FRN_SWITCH(sd.frn,
                        colour_pass_col(DfSmoothUpdateOp<FRN>{L.d, L.xb, L.f, sd.u, L.dc, sd.fc.ubg}, L.g, 1, c));
         }
@@ -288,7 +366,8 @@ __global__ __launch_bounds__(256) void kb_restrict_residual(const SlotDesc* __re
                                                             const Plan* __restrict__ P, int l)
 {
     const unsigned total = P->zc_pre[l][P->ninner[l]];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
         const int i = find_slot(P->zc_pre[l], P->ninner[l], t);
         const SlotDesc& sd = S[P->inner_slot[l][i]];
         const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
@@ -304,7 +383,8 @@ __global__ __launch_bounds__(256) void kb_restrict_residual(const SlotDesc* __re
 __global__ __launch_bounds__(256) void kb_rhs(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l)
 {
     const unsigned total = P->zn_pre[l][P->ninner[l]];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
         const int i = find_slot(P->zn_pre[l], P->ninner[l], t);
         const SlotDesc& sd = S[P->inner_slot[l][i]];
         const LevelDesc& C = sd.L[l + 1];
@@ -319,7 +399,8 @@ __global__ __launch_bounds__(256) void kb_rhs(const SlotDesc* __restrict__ S, co
 __global__ __launch_bounds__(256) void kb_correct(const SlotDesc* __restrict__ S, const Plan* __restrict__ P, int l)
 {
     const unsigned total = P->zc_pre[l][P->ninner[l]];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
         const int i = find_slot(P->zc_pre[l], P->ninner[l], t);
         const SlotDesc& sd = S[P->inner_slot[l][i]];
         const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
@@ -354,7 +435,8 @@ __device__ __forceinline__ void dgp_cells(const SlotDesc* __restrict__ S, const 
                                           const int* slots, Body body)
 {
     const unsigned total = pre[count];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    for (XcdTiles it(total); it.t < it.end; it.t += it.stride) {
+        const unsigned t = it.t;
         const int i = find_slot(pre, count, t);
         body(S[slots[i]], (t - pre[i]) * 256u + threadIdx.x);
     }
@@ -526,7 +608,8 @@ static size_t layout(Hier& h, int N, int model, char* base)
     const Geo g0 = make_geo(N);
     // nDGP: the per-cell tiling's partial count bounds the z-marching one
     const size_t nb = dgp ? (size_t)make_launch(g0.h, 2L * N * N).NB : (size_t)make_zlaunch_both(g0).NB;
-    h.partials = (double*)take((nb + 64) * 8);
+    // the partials, launch_sum_partials' chunk sums after them, the norm
+    h.partials = (double*)take((nb + 1024 + 64) * 8);
     if (dgp) {
         h.rhol[0] = h.rho;
         for (int l = 1; l <= h.lc; l++) h.rhol[l] = (double*)take((size_t)(N >> l) * (N >> l) * (N >> l) * 8);
@@ -578,6 +661,15 @@ struct Slot {
     int it = 0;         // outer residuals taken so far
     double res0 = 0.0;
     double res1 = 1.0e5, res2 = 1.0e10;  // the two residuals before the last (mode-0 rule)
+    // inner cycles of the current outer iteration and those done; f(R)
+    // requests with MGS_EXIT_REL_TOL choose the count as mgs_solve's mode 1
+    // with inner_cycles 0 (auto_inner_count from the previous outer
+    // iteration's start residual and count), others run one
+    int inner = 1, ic = 0;
+    double res_prev = 0.0;
+    int m_prev = 0;
+    bool outer_now = false;  // this step ends with the request's outer residual (set by build_plan)
+    bool starting = false;   // upload and start residual queued on the admission stream (finish_starts)
 };
 
 }  // namespace serve
@@ -604,6 +696,7 @@ struct mgs_server {
     SlotDesc *dS = nullptr, *hS = nullptr;
     Plan *dP = nullptr, *hP = nullptr;
     double *dsum = nullptr, *hsum = nullptr;
+    double* hstart = nullptr;  // pinned: each slot's start-residual sum, written by the admission stream
     std::map<int, hipGraphExec_t> graphs;  // by the batch's f(R) and nDGP depths
     bool desc_dirty = true;
     // done requests whose result is held in HBM, oldest first, and the
@@ -828,10 +921,11 @@ void fill_desc(SlotDesc& sd, const Hier& h, const mgs_params& p)
 }
 
 // upload the density of a freshly admitted request (nDGP: and restrict it to
-// every level, as cache_densities) and start from u = u_bg (nDGP: phi = 0).
-// On the admission stream: the hierarchy is idle (from the pool or new), so
-// this runs beside a step of the active requests.
-int start_request(mgs_server* S, Slot& sl)
+// every level, as cache_densities), start from u = u_bg (nDGP: phi = 0) and
+// take the start field's outer residual.  Queued on the admission stream
+// (the hierarchy is idle, from the pool or new, so this runs beside a step
+// of the active requests); finish_starts waits for it and applies the norm.
+int start_request(mgs_server* S, Slot& sl, int slot)
 {
     Request* r = sl.r;
     Hier* h = sl.h;
@@ -852,11 +946,20 @@ int start_request(mgs_server* S, Slot& sl)
         fr_background(r->p, &ubg);
     }
     launch_fill_f64(h->u, n3, ubg, cs);
-    if (hipStreamSynchronize(cs) != hipSuccess) return MGS_E_HIP;
-    if (!r->verify) {  // a verified request's density stays for the FP64 solve in wait
-        give_stage(S, r->delta, r->delta_bytes);
-        r->delta = nullptr;
-    }
+    // the start field's outer residual and its norm, as the single-request
+    // solve takes them (same tiles, same partial sums, launch_sum_partials'
+    // order); they also set the first inner problem's g, xb and d
+    int nb;
+    if (h->model == MGS_MODEL_DGP)
+        nb = launch_dgpm_outer(h->u, h->rho, h->f[0], h->xb[0], h->d[0], h->partials, N, dgp_coefs(r->p, 0), cs);
+    else
+        nb = launch_outer_residual(r->p.fr_n, h->u, h->rho, h->f[0], h->xb[0], h->d[0], h->partials, N,
+                                   fr_coefs(r->p, 0), df_coefs(r->p, 0), false, cs);
+    double* dsum = h->partials + nb + 1024;
+    launch_sum_partials(h->partials, nb, dsum, cs);
+    if (hipMemcpyAsync(&S->hstart[slot], dsum, sizeof(double), hipMemcpyDeviceToHost, cs) != hipSuccess)
+        return MGS_E_HIP;
+    sl.starting = true;
     return 0;
 }
 
@@ -929,6 +1032,61 @@ bool spill_oldest(mgs_server* S)
     return true;
 }
 
+// The outer residual of a request's outer iteration has been taken (sum:
+// its squared-residual sum): solve_mixed's stopping rule (mode 1 for
+// MGS_EXIT_REL_TOL, mode 0 for MGS_EXIT_REFERENCE), then the next outer
+// iteration's inner cycles.  True when the request finished (its slot is
+// free).
+bool take_outer(mgs_server* S, Slot& sl, double sum)
+{
+    Request* r = sl.r;
+    const int N = r->p.ngrid;
+    const double res = std::sqrt(sum / ((double)N * N * N));
+    sl.it++;
+    if (!std::isfinite(res)) {
+        finish_request(S, sl, MGS_E_DOMAIN);
+        return true;
+    }
+    bool stop = false, conv = false;
+    if (sl.it == 1) {
+        sl.res0 = res;
+        r->res.res0 = res;
+        if (res == 0.0 && r->exit_rule == MGS_EXIT_REL_TOL) stop = conv = true;
+    } else if (r->exit_rule == MGS_EXIT_REFERENCE) {
+        mgs_solve_opts o{};
+        o.max_cycles = r->max_outer;
+        o.res_conv = r->res_conv;
+        stop = reference_exit(o, sl.it - 1, res, sl.res1, sl.res2, &conv);
+        sl.res2 = sl.res1;
+        sl.res1 = res;
+    } else {
+        conv = res < r->rel_tol * sl.res0;
+        stop = conv || (sl.it - 1) >= r->max_outer;
+    }
+    r->res.res_final = res;
+    r->res.outer = sl.it - 1;
+    r->res.converged = conv ? 1 : 0;
+    if (sl.fresh) {
+        sl.fresh = false;
+        S->desc_dirty = true;  // joins the inner cycle: new plan
+    }
+    if (stop) {
+        finish_request(S, sl, 0);
+        return true;
+    }
+    // the next outer iteration's inner cycles (solve_mixed's order)
+    if (r->p.model == MGS_MODEL_FR && r->exit_rule == MGS_EXIT_REL_TOL) {
+        bool reaches = false;
+        sl.inner = auto_inner_count(MIXED_INNER_MAX, res, r->rel_tol * sl.res0, sl.res_prev, sl.m_prev, &reaches);
+        sl.res_prev = res;
+        sl.m_prev = sl.inner;
+    } else {
+        sl.inner = 1;
+    }
+    sl.ic = 0;
+    return false;
+}
+
 // move waiting requests into free slots, FIFO, while their hierarchies fit;
 // during_step: a step of the active requests is running (the new slots join
 // the next one)
@@ -981,12 +1139,18 @@ int admit(mgs_server* S, bool during_step)
         sl.h = h;
         sl.pending = during_step;
         sl.fresh = true;
+        sl.inner = 1;
+        sl.ic = 0;
+        sl.res_prev = 0.0;
+        sl.m_prev = 0;
         sl.it = 0;
         sl.res0 = 0.0;
         sl.res1 = 1.0e5;
         sl.res2 = 1.0e10;
-        const int rc = start_request(S, sl);
+        const int rc = start_request(S, sl, free_slot);
         if (rc) {
+            (void)hipStreamSynchronize(S->cs);
+            sl.starting = false;
             finish_request(S, sl, rc);
             continue;
         }
@@ -994,6 +1158,33 @@ int admit(mgs_server* S, bool during_step)
         admitted++;
     }
     return admitted;
+}
+
+// Wait for the admission stream and apply each new request's start residual
+// (its first outer iteration's norm: the request's first step is then
+// already an inner cycle); the host staging of the density goes back to the
+// pool.
+void finish_starts(mgs_server* S)
+{
+    bool any = false;
+    for (int i = 0; i < S->o.max_active; i++) any = any || S->slots[i].starting;
+    if (!any) return;
+    const bool ok = hipStreamSynchronize(S->cs) == hipSuccess;
+    for (int i = 0; i < S->o.max_active; i++) {
+        Slot& sl = S->slots[i];
+        if (!sl.starting) continue;
+        sl.starting = false;
+        Request* r = sl.r;
+        if (!r->verify) {  // a verified request's density stays for the FP64 solve in wait
+            give_stage(S, r->delta, r->delta_bytes);
+            r->delta = nullptr;
+        }
+        if (!ok) {
+            finish_request(S, sl, MGS_E_HIP);
+            continue;
+        }
+        take_outer(S, sl, S->hstart[i]);
+    }
 }
 
 static unsigned blocks256(size_t cells) { return (unsigned)((cells + 255) / 256); }
@@ -1005,14 +1196,21 @@ void build_plan(mgs_server* S, int* dfr, int* ddgp)
     Plan& P = *S->hP;
     std::memset(&P, 0, sizeof(P));
     *dfr = *ddgp = -1;
+    auto add = [](int& n, int* slot, unsigned* pre, int i, unsigned tiles) {
+        slot[n] = i;
+        pre[n + 1] = pre[n] + tiles;
+        n++;
+    };
     for (int i = 0; i < S->o.max_active; i++) {
         Slot& sl = S->slots[i];
+        sl.outer_now = false;
         if (!sl.r) continue;
         const SlotDesc& sd = S->hS[i];
         const bool dgp = sd.model == MGS_MODEL_DGP;
-        P.res_slot[P.nres] = i;
-        P.res_pre[P.nres + 1] = P.res_pre[P.nres] + (dgp ? sd.zo.NB : sd.L[0].zb.NB);
-        P.nres++;
+        // the outer residual: nDGP every step, f(R) after the last inner
+        // cycle of an outer iteration (and for a fresh request's start field)
+        sl.outer_now = dgp || sl.fresh || sl.ic == sl.inner - 1;
+        if (sl.outer_now) add(P.nres, P.res_slot, P.res_pre, i, dgp ? sd.zo.NB : sd.L[0].zb.NB);
         if (sl.fresh) continue;
         if (dgp) {
             *ddgp = std::max(*ddgp, sd.lc);
@@ -1031,6 +1229,17 @@ void build_plan(mgs_server* S, int* dfr, int* ddgp)
             continue;
         }
         *dfr = std::max(*dfr, sd.lc);
+        const unsigned z0 = sd.L[0].z.NB;
+        if (sl.ic > 0) add(P.nnf, P.nf_slot, P.nf_pre, i, z0);
+        if (sl.ic == sl.inner - 1)
+            add(P.nlast, P.last_slot, P.last_pre, i, z0);
+        else
+            add(P.nnl, P.nl_slot, P.nl_pre, i, z0);
+        if (sl.ic == 0)
+            for (int l = 0; l < sd.lc; l++) {
+                const unsigned nc = (unsigned)sd.L[l + 1].g.n;
+                add(P.nrb[l], P.rb_slot[l], P.rb_pre[l], i, (nc * nc * nc + 255u) / 256u);
+            }
         for (int l = 0; l < sd.lc; l++) {
             const int k = P.ninner[l];
             const LevelDesc& C = sd.L[l + 1];
@@ -1053,17 +1262,27 @@ void vcycle_batched(mgs_server* S, int l, int depth)
     const dim3 G(GRID), B(256);
     hipLaunchKernelGGL(kb_coarsest, dim3(MAXS), dim3(1024), 0, s, S->dS, S->dP, l);
     if (l >= depth) return;
-    // pre-smoothing (level 0: the first colour-0 pass came with the outer residual)
+    // pre-smoothing (level 0: in the first inner cycle of an outer iteration
+    // the first colour-0 pass came with the outer residual)
     const int npre = 2 * (l == 0 ? MIXED_PRE0 : 2);
-    for (int it = l == 0 ? 1 : 0; it < npre; it++) hipLaunchKernelGGL(kb_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
+    for (int it = 0; it < npre; it++) {
+        if (l == 0 && it == 0)
+            hipLaunchKernelGGL(kb_smooth0, G, B, 0, s, S->dS, S->dP, (int)LIST_NOT_FIRST, 0);
+        else
+            hipLaunchKernelGGL(kb_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
+    }
     hipLaunchKernelGGL(kb_restrict_residual, G, B, 0, s, S->dS, S->dP, l);
     hipLaunchKernelGGL(kb_rhs, G, B, 0, s, S->dS, S->dP, l);
     vcycle_batched(S, l + 1, depth);
     hipLaunchKernelGGL(kb_correct, G, B, 0, s, S->dS, S->dP, l);
-    // post-smoothing (level 0: the last colour-1 pass is fused with the update)
+    // post-smoothing (level 0: in the last inner cycle of an outer iteration
+    // the last colour-1 pass is fused with the update)
     const int npost = l == 0 ? 2 * MIXED_POST0 - 1 : 4;
     for (int it = 0; it < npost; it++) hipLaunchKernelGGL(kb_smooth, G, B, 0, s, S->dS, S->dP, l, it & 1);
-    if (l == 0) hipLaunchKernelGGL(kb_smooth_update, G, B, 0, s, S->dS, S->dP);
+    if (l == 0) {
+        hipLaunchKernelGGL(kb_smooth0, G, B, 0, s, S->dS, S->dP, (int)LIST_NOT_LAST, 1);
+        hipLaunchKernelGGL(kb_smooth_update, G, B, 0, s, S->dS, S->dP);
+    }
 }
 
 // the nDGP inner V-cycle of every non-fresh nDGP request, recursing to
@@ -1160,7 +1379,10 @@ void worker_main(mgs_server* S)
             });
             if (S->stop) break;
         }
-        if (!count_active()) timed_admit(false);
+        if (!count_active()) {
+            timed_admit(false);
+            finish_starts(S);
+        }
         const int nact = count_active();
         if (!nact) {
             // the head of the queue waits for a hierarchy a waiter is copying
@@ -1206,39 +1428,14 @@ void worker_main(mgs_server* S)
                 finish_request(S, sl, rc);
                 continue;
             }
-            Request* r = sl.r;
-            const int N = r->p.ngrid;
-            const double res = std::sqrt(S->hsum[i] / ((double)N * N * N));
-            sl.it++;
-            if (!std::isfinite(res)) {
-                finish_request(S, sl, MGS_E_DOMAIN);
+            if (!sl.outer_now) {  // an inner cycle inside its outer iteration: no norm
+                sl.ic++;
                 continue;
             }
-            bool stop = false, conv = false;
-            if (sl.it == 1) {
-                sl.res0 = res;
-                r->res.res0 = res;
-                if (res == 0.0 && r->exit_rule == MGS_EXIT_REL_TOL) stop = conv = true;
-            } else if (r->exit_rule == MGS_EXIT_REFERENCE) {
-                mgs_solve_opts o{};
-                o.max_cycles = r->max_outer;
-                o.res_conv = r->res_conv;
-                stop = reference_exit(o, sl.it - 1, res, sl.res1, sl.res2, &conv);
-                sl.res2 = sl.res1;
-                sl.res1 = res;
-            } else {
-                conv = res < r->rel_tol * sl.res0;
-                stop = conv || (sl.it - 1) >= r->max_outer;
-            }
-            r->res.res_final = res;
-            r->res.outer = sl.it - 1;
-            r->res.converged = conv ? 1 : 0;
-            if (sl.fresh) {
-                sl.fresh = false;
-                S->desc_dirty = true;  // joins the inner cycle: new plan
-            }
-            if (stop) finish_request(S, sl, 0);
+            take_outer(S, sl, S->hsum[i]);
         }
+        // the requests admitted during the step
+        finish_starts(S);
         {
             const double dt = ms_between(tf, Clock::now());
             std::lock_guard<std::mutex> lk(S->mu);
@@ -1433,7 +1630,8 @@ int mgs_server_create(const mgs_server_opts* opts, mgs_server** out)
         hipMalloc((void**)&S->dsum, sizeof(double) * MAXS) != hipSuccess ||
         hipHostMalloc((void**)&S->hS, sizeof(SlotDesc) * MAXS, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&S->hP, sizeof(Plan), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&S->hsum, sizeof(double) * MAXS, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&S->hsum, sizeof(double) * MAXS, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&S->hstart, sizeof(double) * MAXS, hipHostMallocDefault) != hipSuccess) {
         mgs_server_destroy(S);
         return srv_err(nullptr, MGS_E_NOMEM, "server buffers");
     }
@@ -1490,6 +1688,7 @@ int mgs_server_destroy(mgs_server* S)
     if (S->hS) (void)hipHostFree(S->hS);
     if (S->hP) (void)hipHostFree(S->hP);
     if (S->hsum) (void)hipHostFree(S->hsum);
+    if (S->hstart) (void)hipHostFree(S->hstart);
     for (hipStream_t ws : S->wstreams) (void)hipStreamDestroy(ws);
     if (S->s) (void)hipStreamDestroy(S->s);
     if (S->cs) (void)hipStreamDestroy(S->cs);

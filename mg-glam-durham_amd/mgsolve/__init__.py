@@ -331,8 +331,8 @@ class Solver:
 
     def solve(self, mode=1, max_cycles=60, res_conv=1.0e-8, rel_tol=1.0e-8, init_guess=True, inner_cycles=0,
               cycle=0):
-        """scalar_field: cycle 0 V_cycle, 1 F_cycle, 2 W_cycle (plain FP64).  Mixed precision: inner_cycles
-        FP32 V-cycles per outer FP64 iteration, 0 = chosen per outer iteration (mode 1; 1 in mode 0)."""
+        """scalar_field: cycle 0 V_cycle, 1 F_cycle, 2 W_cycle.  Mixed precision: inner_cycles FP32 cycles of
+        that kind per outer FP64 iteration, 0 = chosen per outer iteration (mode 1; 1 in mode 0)."""
         o = SolveOpts(mode, max_cycles, res_conv, rel_tol, 1 if init_guess else 0, inner_cycles, int(cycle), 0)
         r = SolveReport()
         self._chk(self.L.mgs_solve(self.h, C.byref(o), C.byref(r)), "mgs_solve")

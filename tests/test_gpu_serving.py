@@ -2,9 +2,11 @@
 
 Each request's result must be the same alone, in a mix with other requests
 of other sizes, parameters and models (f(R), nDGP), and from the
-single-request solve (mgs_solve, mixed precision, mode 1 or 0): the server
-runs the same per-request arithmetic, tiling and reduction order, so the bar
-is bitwise equality.
+single-request solve (mgs_solve, mixed precision: f(R) requests stopping at
+a relative tolerance as mode 1 with automatic inner-cycle counts, the others
+as mode 0 or mode 1 with one inner cycle per outer iteration): the server
+runs the same per-request arithmetic, tiling, reduction order and inner
+counts, so the bar is bitwise equality.
 Admission by HBM footprint is exercised with a budget that holds only one
 request at a time.  Error paths are checked without a GPU."""
 import time
@@ -42,7 +44,7 @@ def density(N, box, sigma, seed, f32):
 def standalone(c, delta, tol):
     with M.Solver(c) as S:
         S.set_density(delta)
-        rep = S.solve(mode=1, rel_tol=tol, max_cycles=60, init_guess=True, inner_cycles=1)
+        rep = S.solve(mode=1, rel_tol=tol, max_cycles=60, init_guess=True, inner_cycles=0)
         return S.get_field(), rep
 
 
@@ -129,7 +131,8 @@ def test_dgp_results_equal_alone_in_mix_and_standalone():
     """nDGP requests stop by the reference's rule (the DGP residual of a
     periodic box levels off, so the stagnation exits end the solve); in a
     batch with f(R) requests each result is bitwise that of the request
-    alone and of mgs_solve (mixed, mode 0, one inner cycle)."""
+    alone and of mgs_solve (mixed, mode 0, one inner cycle; the f(R)
+    requests of the batch as mode 1 with automatic inner counts)."""
     tol = 1e-8
     reqs = []
     for k, (N, branch, box, a, h0rc, sigma, seed) in enumerate(DGP_MIX):
@@ -160,7 +163,7 @@ def test_dgp_results_equal_alone_in_mix_and_standalone():
             if kind == "dgp":
                 rep = S.solve(mode=0, res_conv=1e-30, max_cycles=40, inner_cycles=1)
             else:
-                rep = S.solve(mode=1, rel_tol=tol, max_cycles=60, inner_cycles=1)
+                rep = S.solve(mode=1, rel_tol=tol, max_cycles=60, inner_cycles=0)
             u_ref = S.get_field()
         assert np.array_equal(u_ref, u_mix)
         assert rep["outer"] == r_mix["outer"] and rep["res_final"] == r_mix["res_final"]
