@@ -290,10 +290,12 @@ def lognormal_set(cfgs, sigmas, seed0):
 
 
 def serve_run(tag, cfgs, kws, dens, max_active, rng):
-    """All requests at once (saturated throughput), then Poisson arrivals at
-    0.8x that rate (latency); three requests re-solved alone and compared with
-    their in-mix results.  kws[i]: submit arguments of request i (its exit
-    rule); success: status 0 and, for the rel_tol rule, converged."""
+    """All requests offered at once (saturated throughput), then Poisson
+    arrivals at 0.8x that rate (latency); in both a thread submits while the
+    client claims the fields as they finish.  Four requests are re-solved
+    alone and compared with their in-mix results.  kws[i]: submit arguments
+    of request i (its exit rule); success: status 0 and, for the rel_tol
+    rule, converged."""
     import threading
 
     n_req = len(cfgs)
@@ -301,64 +303,64 @@ def serve_run(tag, cfgs, kws, dens, max_active, rng):
     def ok(o, kw):
         return o["status"] == 0 and (o["converged"] or kw.get("exit_rule") == M.EXIT_REFERENCE)
 
-    with M.Server(max_active=max_active) as srv:
-        # the client claims every field, in submission order, into one reused
-        # output array per grid size (as in the Poisson leg below)
-        outbuf = {N: np.empty((N, N, N), np.float32) for N in sorted({c.N for c in cfgs})}
-        ts = time.perf_counter()
-        tickets = [srv.submit(c, d, **kw) for c, d, kw in zip(cfgs, dens, kws)]
-        outs = [srv.wait(t, c.N, np.float32, out=outbuf[c.N])[1] for t, c in zip(tickets, cfgs)]
-        sat_s = time.perf_counter() - ts
-        sat = srv.stats()
-    rate = 0.8 * n_req / sat_s
-    log(f"[{tag}] saturated: {n_req / sat_s:.1f} solves/s, p50 {sat['p50_ms']:.1f} ms, batch <= {sat['max_batch']}")
-    bad = [o for o, kw in zip(outs, kws) if not ok(o, kw)]
-    gaps = rng.exponential(1.0 / rate, n_req)
-    fields = {}
-    keep = {0, 1, n_req // 2, n_req - 1}
-    with M.Server(max_active=max_active) as srv:
+    def stream(srv, gaps, keep):
+        """One thread submits (after gaps[i] seconds for request i; none: at
+        once) while this one claims the fields as they finish, into one
+        reused output array per grid size; returns the wall time, the submit
+        and delivery times, the result dicts and the kept fields."""
         tickets = [None] * n_req
         t_sub = [0.0] * n_req
         t_got = [0.0] * n_req
+        res = [None] * n_req
+        kept = {}
 
         def producer():
             start = time.perf_counter()
             at = 0.0
             for i in range(n_req):
-                at += gaps[i]
-                dt = start + at - time.perf_counter()
-                if dt > 0:
-                    time.sleep(dt)
+                if gaps is not None:
+                    at += gaps[i]
+                    dt = start + at - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
                 t_sub[i] = time.perf_counter()
                 tickets[i] = srv.submit(cfgs[i], dens[i], **kws[i])
 
+        outbuf = {N: np.empty((N, N, N), np.float32) for N in sorted({c.N for c in cfgs})}
         th = threading.Thread(target=producer)
         tp = time.perf_counter()
         th.start()
-        # the client claims results in the order they finish, into one
-        # reused output array per grid size
         pending = list(range(n_req))
-        outbuf = {}
         while pending:
             ready = [i for i in pending if tickets[i] is not None and srv.test(tickets[i])]
             if not ready:
                 time.sleep(0.0002)
                 continue
             for i in ready:
-                N = cfgs[i].N
-                if N not in outbuf:
-                    outbuf[N] = np.empty((N, N, N), np.float32)
-                u, o = srv.wait(tickets[i], N, np.float32, out=outbuf[N])
+                u, o = srv.wait(tickets[i], cfgs[i].N, np.float32, out=outbuf[cfgs[i].N])
                 t_got[i] = time.perf_counter()
+                res[i] = o
                 if i in keep:
-                    fields[i] = u.copy()
-                if not ok(o, kws[i]):
-                    bad.append(o)
+                    kept[i] = u.copy()
             done = set(ready)
             pending = [i for i in pending if i not in done]
         th.join()
-        pois_s = time.perf_counter() - tp
+        return time.perf_counter() - tp, t_sub, t_got, res, kept
+
+    # saturated: every request offered at once
+    with M.Server(max_active=max_active) as srv:
+        sat_s, _, _, outs, _ = stream(srv, None, set())
+        sat = srv.stats()
+    rate = 0.8 * n_req / sat_s
+    log(f"[{tag}] saturated: {n_req / sat_s:.1f} solves/s, p50 {sat['p50_ms']:.1f} ms, batch <= {sat['max_batch']}")
+    bad = [o for o, kw in zip(outs, kws) if not ok(o, kw)]
+    # Poisson arrivals at 0.8 x the saturated rate
+    gaps = rng.exponential(1.0 / rate, n_req)
+    keep = {0, 1, n_req // 2, n_req - 1}
+    with M.Server(max_active=max_active) as srv:
+        pois_s, t_sub, t_got, outs_p, fields = stream(srv, gaps, keep)
         st = srv.stats()
+    bad += [o for o, kw in zip(outs_p, kws) if not ok(o, kw)]
     # delivered: submit call -> the field in the caller's array (wait() copies
     # it out of HBM; the client claims results as they finish)
     dl = 1e3 * (np.array(t_got) - np.array(t_sub))

@@ -670,6 +670,7 @@ struct Slot {
     int m_prev = 0;
     bool outer_now = false;  // this step ends with the request's outer residual (set by build_plan)
     bool starting = false;   // upload and start residual queued on the admission stream (finish_starts)
+    void* upin = nullptr;    // pinned bounce buffer of the upload, returned by finish_starts
 };
 
 }  // namespace serve
@@ -920,6 +921,61 @@ void fill_desc(SlotDesc& sd, const Hier& h, const mgs_params& p)
     }
 }
 
+// Pinned staging for uploads and copy-outs, pooled by size and kept until the
+// server is destroyed (a buffer per transfer would cost a pinning each); at
+// most PIN_MAX bytes in all, beyond which a transfer goes through the
+// runtime's pageable path.
+constexpr size_t PIN_MAX = size_t(2) << 30;
+constexpr size_t PIN_MIN = size_t(1) << 20;  // smaller fields go direct
+
+void* take_pin(mgs_server* S, size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->pins.find(bytes);
+        if (it != S->pins.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+        if (S->pinned + bytes > PIN_MAX) return nullptr;
+        S->pinned += bytes;
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->pinned -= bytes;
+        return nullptr;
+    }
+    return p;
+}
+
+void give_pin(mgs_server* S, void* p, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->pins[bytes].push_back(p);
+}
+
+// host copy of a large field with a few threads (one slice each)
+void copy_host(void* dst, const void* src, size_t bytes)
+{
+    const int nt = bytes >= (size_t(16) << 20) ? 4 : 1;
+    if (nt == 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t slice = (bytes / nt + 4095) & ~size_t(4095);
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) {
+        const size_t o = slice * t;
+        if (o >= bytes) break;
+        const size_t len = std::min(slice, bytes - o);
+        th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, len); });
+    }
+    std::memcpy(dst, src, std::min(slice, bytes));
+    for (auto& x : th) x.join();
+}
+
 // upload the density of a freshly admitted request (nDGP: and restrict it to
 // every level, as cache_densities), start from u = u_bg (nDGP: phi = 0) and
 // take the start field's outer residual.  Queued on the admission stream
@@ -932,13 +988,22 @@ int start_request(mgs_server* S, Slot& sl, int slot)
     hipStream_t cs = S->cs;
     const int N = r->p.ngrid;
     const size_t n3 = (size_t)N * N * N;
-    if (r->dtype == MGS_F32) {
-        if (hipMemcpyAsync(h->d[0], r->delta, n3 * 4, hipMemcpyHostToDevice, cs) != hipSuccess) return MGS_E_HIP;
-        launch_pack_f64_from_f32(h->d[0], h->rho, N, cs);
-    } else {
-        if (hipMemcpyAsync(h->u, r->delta, n3 * 8, hipMemcpyHostToDevice, cs) != hipSuccess) return MGS_E_HIP;
-        launch_pack_f64_from_f64(h->u, h->rho, N, cs);
+    // through a pinned bounce buffer: a multi-threaded host copy into it, then
+    // a DMA that runs on while the worker goes on (the runtime's pageable
+    // path copies on one thread and returns only when it is done)
+    const size_t bytes = n3 * (r->dtype == MGS_F32 ? 4 : 8);
+    void* dst = r->dtype == MGS_F32 ? (void*)h->d[0] : (void*)h->u;
+    const void* src = r->delta;
+    sl.upin = bytes >= PIN_MIN ? take_pin(S, bytes) : nullptr;
+    if (sl.upin) {
+        copy_host(sl.upin, r->delta, bytes);
+        src = sl.upin;
     }
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, cs) != hipSuccess) return MGS_E_HIP;
+    if (r->dtype == MGS_F32)
+        launch_pack_f64_from_f32(h->d[0], h->rho, N, cs);
+    else
+        launch_pack_f64_from_f64(h->u, h->rho, N, cs);
     double ubg = 0.0;
     if (h->model == MGS_MODEL_DGP) {
         for (int l = 1; l <= h->lc; l++) launch_restrict8_64(h->rhol[l - 1], h->rhol[l], nullptr, N >> l, cs);
@@ -1151,6 +1216,8 @@ int admit(mgs_server* S, bool during_step)
         if (rc) {
             (void)hipStreamSynchronize(S->cs);
             sl.starting = false;
+            if (sl.upin) give_pin(S, sl.upin, r->delta_bytes);
+            sl.upin = nullptr;
             finish_request(S, sl, rc);
             continue;
         }
@@ -1175,6 +1242,10 @@ void finish_starts(mgs_server* S)
         if (!sl.starting) continue;
         sl.starting = false;
         Request* r = sl.r;
+        if (sl.upin) {
+            give_pin(S, sl.upin, r->delta_bytes);
+            sl.upin = nullptr;
+        }
         if (!r->verify) {  // a verified request's density stays for the FP64 solve in wait
             give_stage(S, r->delta, r->delta_bytes);
             r->delta = nullptr;
@@ -1444,61 +1515,6 @@ void worker_main(mgs_server* S)
     }
 }
 
-// Pinned staging for copy-outs, pooled by size and kept until the server is
-// destroyed (a buffer per claim would cost a pinning per result); at most
-// PIN_MAX bytes in all, beyond which a claim copies through the runtime's
-// pageable path.
-constexpr size_t PIN_MAX = size_t(2) << 30;
-constexpr size_t PIN_MIN = size_t(1) << 20;  // smaller fields go direct
-
-void* take_pin(mgs_server* S, size_t bytes)
-{
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        auto it = S->pins.find(bytes);
-        if (it != S->pins.end() && !it->second.empty()) {
-            void* p = it->second.back();
-            it->second.pop_back();
-            return p;
-        }
-        if (S->pinned + bytes > PIN_MAX) return nullptr;
-        S->pinned += bytes;
-    }
-    void* p = nullptr;
-    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->pinned -= bytes;
-        return nullptr;
-    }
-    return p;
-}
-
-void give_pin(mgs_server* S, void* p, size_t bytes)
-{
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->pins[bytes].push_back(p);
-}
-
-// host copy of a large field with a few threads (one slice each)
-void copy_host(void* dst, const void* src, size_t bytes)
-{
-    const int nt = bytes >= (size_t(16) << 20) ? 4 : 1;
-    if (nt == 1) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t slice = (bytes / nt + 4095) & ~size_t(4095);
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) {
-        const size_t o = slice * t;
-        if (o >= bytes) break;
-        const size_t len = std::min(slice, bytes - o);
-        th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, len); });
-    }
-    std::memcpy(dst, src, std::min(slice, bytes));
-    for (auto& x : th) x.join();
-}
-
 // A waiter's copy-out stream: an idle one of the server's, or a new
 // non-blocking one; returned to the server after the copy (a stream per
 // claim would cost a creation and a destruction each).
@@ -1661,11 +1677,13 @@ int mgs_server_destroy(mgs_server* S)
     if (S->cs) (void)hipStreamSynchronize(S->cs);
     S->held.clear();  // their hierarchies are freed with the requests below
     for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
-    for (int i = 0; i < MAXS; i++)
+    for (int i = 0; i < MAXS; i++) {
         if (S->slots[i].h) {
             (void)hipFree(S->slots[i].h->base);
             delete S->slots[i].h;
         }
+        if (S->slots[i].upin) (void)hipHostFree(S->slots[i].upin);
+    }
     for (auto& kv : S->pool)
         for (Hier* h : kv.second) {
             (void)hipFree(h->base);
