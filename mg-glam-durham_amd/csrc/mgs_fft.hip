@@ -210,7 +210,8 @@ __global__ __launch_bounds__(256) void k_fifth_force(const double* __restrict__ 
 // library's log10f, the function the compiled reference calls.  The
 // quadrupole's line of sight is z: the reference fills its bins after the
 // I <-> K transposition (:307-316), so its MI is the z wavenumber.
-template <int IPOL, int ICIC>
+// LOGB: also fill the log bins (NGP only; skipped when no log output is asked for)
+template <int IPOL, int ICIC, bool LOGB>
 __global__ __launch_bounds__(256) void k_bin_power(const hipfftDoubleComplex* __restrict__ c, int N, double norm,
                                                    const int* __restrict__ logbin, double* __restrict__ part, int nb)
 {
@@ -259,8 +260,8 @@ __global__ __launch_bounds__(256) void k_bin_power(const hipfftDoubleComplex* __
                 atomicAdd(&hist[nb + mp], w);
                 atomicAdd(&hist[2 * nb + mp], w * (double)rk);
             }
-            const int ml = logbin[n2];
-            if (ml < nb) {
+            const int ml = LOGB ? logbin[n2] : nb;
+            if (LOGB && ml < nb) {
                 if (!IPOL) atomicAdd(&hist[3 * nb + ml], w * amp);
                 atomicAdd(&hist[4 * nb + ml], w);
                 atomicAdd(&hist[5 * nb + ml], w * (double)rk);
@@ -507,7 +508,8 @@ int mgs_power_spectrum_ex(mgs_handle* h, int32_t which, int32_t ipol, int32_t ic
     const size_t n3 = (size_t)N * N * N;
     const Geo g = make_geo(N);
     const int n2max = 3 * (N / 2) * (N / 2);
-    if (!f->logbin) {
+    const bool logb = !icic && (klog || pklog || countlog);  // log bins asked for
+    if (logb && !f->logbin) {
         const std::vector<int> t = log_bin_table(n2max);
         if (hipMalloc((void**)&f->logbin, t.size() * sizeof(int)) != hipSuccess)
             return set_err(h, MGS_E_NOMEM, "log bin table");
@@ -533,11 +535,14 @@ int mgs_power_spectrum_ex(mgs_handle* h, int32_t which, int32_t ipol, int32_t ic
     const size_t lds = 6 * (size_t)nb * sizeof(double);
     const dim3 G(BIN_BLOCKS), B(256);
     if (icic)
-        hipLaunchKernelGGL((ipol ? k_bin_power<1, 1> : k_bin_power<0, 1>), G, B, lds, h->s, f->cplx, N, norm,
-                           (const int*)f->logbin, f->binpart, nb);
+        hipLaunchKernelGGL((ipol ? k_bin_power<1, 1, false> : k_bin_power<0, 1, false>), G, B, lds, h->s, f->cplx, N,
+                           norm, (const int*)f->logbin, f->binpart, nb);
+    else if (logb)
+        hipLaunchKernelGGL((ipol ? k_bin_power<1, 0, true> : k_bin_power<0, 0, true>), G, B, lds, h->s, f->cplx, N,
+                           norm, (const int*)f->logbin, f->binpart, nb);
     else
-        hipLaunchKernelGGL((ipol ? k_bin_power<1, 0> : k_bin_power<0, 0>), G, B, lds, h->s, f->cplx, N, norm,
-                           (const int*)f->logbin, f->binpart, nb);
+        hipLaunchKernelGGL((ipol ? k_bin_power<1, 0, false> : k_bin_power<0, 0, false>), G, B, lds, h->s, f->cplx, N,
+                           norm, (const int*)f->logbin, f->binpart, nb);
     hipLaunchKernelGGL(k_bin_reduce, dim3((6 * nb + 255) / 256), dim3(256), 0, h->s, (const double*)f->binpart,
                        BIN_BLOCKS, 6 * nb, f->bins);
     std::vector<double> hb(6 * (size_t)nb);

@@ -522,7 +522,16 @@ void launch_df_restrict_residual(int frn, const float* d0, const float* xb0, con
                                  float* rd1, float* d1, int N, const DfCoef& c0, hipStream_t s)
 {
     const Geo gf = make_geo(N), gc = make_geo(N / 2);
-    const ZLaunch L = make_zlaunch(gc.n, gc.n);
+    int kc = zrun_length(gc.n);
+#ifdef MGS_TUNING
+    // tuning builds: MGS_ZRUN_RR sets this kernel's coarse planes per thread
+    static const int rr = [] {
+        const char* e = std::getenv("MGS_ZRUN_RR");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (rr > 0) kc = rr < gc.n ? rr : gc.n;
+#endif
+    const ZLaunch L = make_zlaunch(gc.n, gc.n, kc);
     FRN_DISPATCH(frn, k_df_restrict_residual, d0, xb0, g0, rres1, rd1, d1, gf, gc, c0, ZARGS);
 }
 
