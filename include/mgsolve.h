@@ -80,7 +80,7 @@ extern "C" {
 #define MGS_ARR_RU 5         /* frozen restricted field R u           FI3 S3 */
 
 typedef struct mgs_params {
-    int32_t ngrid;           /* NGRID, cells per side at level 0 (even) */
+    int32_t ngrid;           /* NGRID, cells per side at level 0 (even, <= 1024) */
     int32_t model;           /* MGS_MODEL_FR or MGS_MODEL_DGP */
     int32_t fr_n;            /* Hu-Sawicki n in {0,1,2} */
     int32_t branch;          /* DGP: +1 normal (N_branch), -1 self-accelerating (S_branch) */

@@ -1074,6 +1074,11 @@ static int check_params(const mgs_params* p, std::string* why)
         *why = "ngrid must be even and >= 2";
         return MGS_E_ARG;
     }
+    if (p->ngrid > 1024) {
+        // the kernels index a level with 32-bit offsets (at most 2^30 cells)
+        *why = "ngrid must be <= 1024";
+        return MGS_E_ARG;
+    }
     if (p->model != MGS_MODEL_FR && p->model != MGS_MODEL_DGP) {
         *why = "Model currently unsupported (MG_model must be 1 = f(R) or 2 = DGP)";
         return MGS_E_ARG;

@@ -539,14 +539,18 @@ __device__ __forceinline__ void dgp_zsmooth64_run(const double* __restrict__ h0,
                                                   const double* __restrict__ f, const Geo& g, int colour,
                                                   const DgpCoef& c, const DCol& d, int k0)
 {
+    // (j + k0) & 1 = S0 ^ colour: wave-uniform
+    const DHalves<double> H = dhalves(h0, h1, (S0 ? 1 : 0) ^ (colour & 1));
+    const double* rhoc = rho + (size_t)colour * g.half;
+    const double* fc = COARSE ? f + (size_t)colour * g.half : nullptr;
     DWin2<double> w;
-    dwin2_init(w, h0, h1, d, k0, g.n);
+    dwin2_init(w, H, d, k0, g.n);
     DSide<double> sd;
-    dside_load(h0, h1, d, k0, g.n, S0, sd);
+    dside_load(H, d, k0, 0, g.n, S0, sd);
     const double dLphi = -6.0 / c.dx2;
     unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
-    double rh = rho[(unsigned)colour * (unsigned)g.half + off];
-    double fv = COARSE ? f[(unsigned)colour * (unsigned)g.half + off] : 0.0;
+    double rh = elem(rhoc, off);
+    double fv = COARSE ? elem(fc, off) : 0.0;
 #pragma unroll
     for (int t = 0; t < DGP_KC; t++) {
         const int k = k0 + t;
@@ -556,15 +560,15 @@ __device__ __forceinline__ void dgp_zsmooth64_run(const double* __restrict__ h0,
         double rhn = 0.0, fvn = 0.0;
         const unsigned offn = off + d.plane;
         if (t + 1 < DGP_KC) {
-            draw2_load(h0, h1, d, wrap_inc(k + 1, g.n), r);
-            dside_load(h0, h1, d, wrap_inc(k, g.n), g.n, !s, sdn);
-            rhn = rho[(unsigned)colour * (unsigned)g.half + offn];
-            if constexpr (COARSE) fvn = f[(unsigned)colour * (unsigned)g.half + offn];
+            draw2_load(H, d, wrap_inc(k + 1, g.n), t + 2, r);
+            dside_load(H, d, wrap_inc(k, g.n), t + 1, g.n, !s, sdn);
+            rhn = elem(rhoc, offn);
+            if constexpr (COARSE) fvn = elem(fc, offn);
         }
         const S19<double> u = s19_win2(w, s, sd, d);
         double L = dgp_L19(u, rh, c);
         if constexpr (COARSE) L = L - fv;
-        out[off] = u.c0 - L / dLphi;
+        elem(out, off) = u.c0 - L / dLphi;
         if (t + 1 < DGP_KC) {
             dwin2_shift(w);
             dwin2_set(w, 2, r);
@@ -598,17 +602,20 @@ __device__ __forceinline__ void dgpm_zsmooth_run(const float* __restrict__ e0, c
                                                  const float* __restrict__ rho32, const float* __restrict__ rhs,
                                                  const Geo& g, int colour, const DgpF& c, const DCol& d, int k0)
 {
-    const float* b0 = pb;
-    const float* b1 = pb + g.half;
+    // (j + k0) & 1 = S0 ^ colour: wave-uniform
+    const int par0 = (S0 ? 1 : 0) ^ (colour & 1);
+    const DHalves<float> He = dhalves(e0, e1, par0), Hb = dhalves(pb, pb + g.half, par0);
+    const float* rhoc = rho32 + (size_t)colour * g.half;
+    const float* rhsc = rhs + (size_t)colour * g.half;
     DWin2<float> we, wb;
-    dwin2_init(we, e0, e1, d, k0, g.n);
-    dwin2_init(wb, b0, b1, d, k0, g.n);
+    dwin2_init(we, He, d, k0, g.n);
+    dwin2_init(wb, Hb, d, k0, g.n);
     DSide<float> se, sb;
-    dside_load(e0, e1, d, k0, g.n, S0, se);
-    dside_load(b0, b1, d, k0, g.n, S0, sb);
+    dside_load(He, d, k0, 0, g.n, S0, se);
+    dside_load(Hb, d, k0, 0, g.n, S0, sb);
     unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
-    float rh = rho32[(unsigned)colour * (unsigned)g.half + off];
-    float fv = rhs[(unsigned)colour * (unsigned)g.half + off];
+    float rh = elem(rhoc, off);
+    float fv = elem(rhsc, off);
 #pragma unroll
     for (int t = 0; t < DGP_KC; t++) {
         const int k = k0 + t;
@@ -619,16 +626,16 @@ __device__ __forceinline__ void dgpm_zsmooth_run(const float* __restrict__ e0, c
         const unsigned offn = off + d.plane;
         if (t + 1 < DGP_KC) {
             const int kn = wrap_inc(k + 1, g.n);
-            draw2_load(e0, e1, d, kn, re);
-            draw2_load(b0, b1, d, kn, rb);
-            dside_load(e0, e1, d, wrap_inc(k, g.n), g.n, !s, sen);
-            dside_load(b0, b1, d, wrap_inc(k, g.n), g.n, !s, sbn);
-            rhn = rho32[(unsigned)colour * (unsigned)g.half + offn];
-            fvn = rhs[(unsigned)colour * (unsigned)g.half + offn];
+            draw2_load(He, d, kn, t + 2, re);
+            draw2_load(Hb, d, kn, t + 2, rb);
+            dside_load(He, d, wrap_inc(k, g.n), t + 1, g.n, !s, sen);
+            dside_load(Hb, d, wrap_inc(k, g.n), t + 1, g.n, !s, sbn);
+            rhn = elem(rhoc, offn);
+            fvn = elem(rhsc, offn);
         }
         const D2f de = d2_of(s19_win2(we, s, se, d)), db = d2_of(s19_win2(wb, s, sb, d));
         const float D = dgp_dop(db, de, rh, c) - fv;
-        out[off] = de.c0 + D / (6.0f * c.inv_dx2);
+        elem(out, off) = de.c0 + D / (6.0f * c.inv_dx2);
         if (t + 1 < DGP_KC) {
             dwin2_shift(we);
             dwin2_shift(wb);

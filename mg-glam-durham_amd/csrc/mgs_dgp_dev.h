@@ -341,23 +341,47 @@ struct DSide {
 __device__ __forceinline__ int side_plane(int q) { return q == 3 ? 0 : (q == 4 ? 2 : 1); }
 __device__ __forceinline__ int side_row(int q) { return q == 1 ? 0 : (q == 2 ? 2 : 1); }
 
+// The two colour halves of a field seen from the columns of one wave: the
+// value at x = 2m of row jr, plane K lies in p[0] when jr + K has the parity
+// par0 of j + k0 (the wave's row and first plane), else in p[1]; the value at
+// x = 2m+1 in the other.  A wave is 64 consecutive m of one row and plane
+// run, so par0 and both pointers are wave-uniform (scalar registers), and
+// with the plane offset dk = K - k0 known when the z-march is unrolled, the
+// half of every load is chosen at compile time: (jr + K) & 1 =
+// par0 ^ ((R + 1 + dk) & 1) for row jr = j + R - 1 (n even, so the wrap
+// keeps the parity).
 template <typename T>
-__device__ __forceinline__ void draw2_load(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d, int K,
-                                           DRaw2<T>& r)
+struct DHalves {
+    const T* p[2];
+};
+
+template <typename T>
+__device__ __forceinline__ DHalves<T> dhalves(const T* h0, const T* h1, int par0)
+{
+    DHalves<T> H;
+    H.p[0] = par0 ? h1 : h0;
+    H.p[1] = par0 ? h0 : h1;
+    return H;
+}
+
+// the pair at (m, rows j-1..j+1) of plane K (dk = K - k0 modulo 2)
+template <typename T>
+__device__ __forceinline__ void draw2_load(const DHalves<T>& H, const DCol& d, int K, int dk, DRaw2<T>& r)
 {
 #pragma unroll
     for (int R = 0; R < 3; R++) {
-        const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
-        const bool odd = ((d.jr[R] + K) & 1) != 0;
-        r.ev[R] = (odd ? h1 : h0)[base + d.m];
-        r.ov[R] = (odd ? h0 : h1)[base + d.m];
+        const unsigned idx = (unsigned)K * d.plane + d.rowoff[R] + d.m;
+        const int f = (R + 1 + dk) & 1;
+        r.ev[R] = elem(H.p[f], idx);
+        r.ov[R] = elem(H.p[f ^ 1], idx);
     }
 }
 
-// the five side values of the cell at plane k with parity s (edge lane only)
+// the five side values of the cell at plane k (dk = k - k0) with parity s
+// (edge lane only)
 template <typename T>
-__device__ __forceinline__ void dside_load(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d, int k,
-                                           int n, bool s, DSide<T>& sd)
+__device__ __forceinline__ void dside_load(const DHalves<T>& H, const DCol& d, int k, int dk, int n, bool s,
+                                           DSide<T>& sd)
 {
 #pragma unroll
     for (int q = 0; q < 5; q++) sd.v[q] = T(0);
@@ -367,9 +391,9 @@ __device__ __forceinline__ void dside_load(const T* __restrict__ h0, const T* __
             const int P = side_plane(q), R = side_row(q);
             const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
             const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
-            const bool odd = ((d.jr[R] + K) & 1) != 0;
+            const int f = (R + P + dk) & 1;
             // x = 2m+2 has the colour of x = 2m, x = 2m-1 that of 2m+1
-            sd.v[q] = s ? (odd ? h1 : h0)[base + d.mp] : (odd ? h0 : h1)[base + d.mm];
+            sd.v[q] = s ? elem(H.p[f], base + d.mp) : elem(H.p[f ^ 1], base + d.mm);
         }
     }
 }
@@ -378,19 +402,24 @@ __device__ __forceinline__ void dside_load(const T* __restrict__ h0, const T* __
 // left edge lane loads x = 2m-1 (the side of the cell at 2m), the right edge
 // lane x = 2m+2 (the side of the cell at 2m+1), into the same five slots
 template <typename T>
-__device__ __forceinline__ void dside_load_edges(const T* __restrict__ h0, const T* __restrict__ h1, const DCol& d,
-                                                 int k, int n, DSide<T>& sd)
+__device__ __forceinline__ void dside_load_edges(const DHalves<T>& H, const DCol& d, int k, int dk, int n,
+                                                 DSide<T>& sd)
 {
 #pragma unroll
     for (int q = 0; q < 5; q++) sd.v[q] = T(0);
-    if (d.lload || d.rload) {
+    if (d.lload) {
 #pragma unroll
         for (int q = 0; q < 5; q++) {
             const int P = side_plane(q), R = side_row(q);
             const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
-            const unsigned base = (unsigned)K * d.plane + d.rowoff[R];
-            const bool odd = ((d.jr[R] + K) & 1) != 0;
-            sd.v[q] = d.rload ? (odd ? h1 : h0)[base + d.mp] : (odd ? h0 : h1)[base + d.mm];
+            sd.v[q] = elem(H.p[((R + P + dk) & 1) ^ 1], (unsigned)K * d.plane + d.rowoff[R] + d.mm);
+        }
+    } else if (d.rload) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            const int P = side_plane(q), R = side_row(q);
+            const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
+            sd.v[q] = elem(H.p[(R + P + dk) & 1], (unsigned)K * d.plane + d.rowoff[R] + d.mp);
         }
     }
 }
@@ -418,31 +447,55 @@ __device__ __forceinline__ void dwin2_shift(DWin2<T>& w)
 }
 
 template <typename T>
-__device__ __forceinline__ void dwin2_init(DWin2<T>& w, const T* __restrict__ h0, const T* __restrict__ h1,
-                                           const DCol& d, int k0, int n)
+__device__ __forceinline__ void dwin2_init(DWin2<T>& w, const DHalves<T>& H, const DCol& d, int k0, int n)
 {
     DRaw2<T> r0, r1, r2;
-    draw2_load(h0, h1, d, wrap_dec(k0, n), r0);
-    draw2_load(h0, h1, d, k0, r1);
-    draw2_load(h0, h1, d, wrap_inc(k0, n), r2);
+    draw2_load(H, d, wrap_dec(k0, n), -1, r0);
+    draw2_load(H, d, k0, 0, r1);
+    draw2_load(H, d, wrap_inc(k0, n), 1, r2);
     dwin2_set(w, 0, r0);
     dwin2_set(w, 1, r1);
     dwin2_set(w, 2, r2);
 }
 
-// the 19 stencil values of the cell of parity s (all lanes execute the
-// shuffles)
+// the value of lane + 1 (FROM_RIGHT) or lane - 1 of the wave, by a DPP
+// whole-wave shift (wave_shl:1 / wave_shr:1); the last / first lane, which
+// has no such neighbour, keeps its own value
+template <bool FROM_RIGHT>
+__device__ __forceinline__ int wave_shift1(int v)
+{
+    if constexpr (FROM_RIGHT)
+        return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);
+    else
+        return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);
+}
+
+template <bool FROM_RIGHT>
+__device__ __forceinline__ float wave_shift1(float v)
+{
+    return __int_as_float(wave_shift1<FROM_RIGHT>(__float_as_int(v)));
+}
+
+template <bool FROM_RIGHT>
+__device__ __forceinline__ double wave_shift1(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)wave_shift1<FROM_RIGHT>((int)(unsigned)u);
+    const unsigned hi = (unsigned)wave_shift1<FROM_RIGHT>((int)(unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// the 19 stencil values of the cell of parity s (all lanes of the wave
+// execute the shifts)
 template <typename T>
 __device__ __forceinline__ S19<T> s19_win2(const DWin2<T>& w, bool s, const DSide<T>& sd, const DCol& d)
 {
-    const bool edge = s ? d.rload : d.lload;
-    const int lane = (int)((threadIdx.x + threadIdx.y * blockDim.x) & 63u);
-    const int src = s ? lane + 1 : lane - 1;  // out of the wave only on the edge lane, which keeps sd
+    const bool edge = s ? d.rload : d.lload;  // the lane whose neighbour is outside the wave keeps sd
     T v[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const int P = side_plane(q), R = side_row(q);
-        const T sh = __shfl(s ? w.e[P][R] : w.o[P][R], src, 64);
+        const T sh = s ? wave_shift1<true>(w.e[P][R]) : wave_shift1<false>(w.o[P][R]);
         v[q] = edge ? sd.v[q] : sh;
     }
     // x = i + dx at slot (P, R), i = 2m + s; q: the side slot of (P, R)
@@ -489,12 +542,15 @@ __device__ __forceinline__ double dgp_both_zcol(const OP& op, const Geo& g, int 
     constexpr int NF = OP::NF;
     const DCol d = dcol_of(g, m, j);
     const unsigned half = (unsigned)g.half;
+    const int par0 = __builtin_amdgcn_readfirstlane((j + k0) & 1);  // wave-uniform
+    DHalves<T> H[NF];
     DWin2<T> w[NF];
     DSide<T> sd[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-        dwin2_init(w[f], op.half(f, 0), op.half(f, 1), d, k0, g.n);
-        dside_load_edges(op.half(f, 0), op.half(f, 1), d, k0, g.n, sd[f]);
+        H[f] = dhalves(op.half(f, 0), op.half(f, 1), par0);
+        dwin2_init(w[f], H[f], d, k0, g.n);
+        dside_load_edges(H[f], d, k0, 0, g.n, sd[f]);
     }
     unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
     unsigned par = (unsigned)((j + k0) & 1);  // colour of x = 2m
@@ -510,20 +566,26 @@ __device__ __forceinline__ double dgp_both_zcol(const OP& op, const Geo& g, int 
         if (t + 1 < DGP_KC) {
 #pragma unroll
             for (int f = 0; f < NF; f++) {
-                draw2_load(op.half(f, 0), op.half(f, 1), d, wrap_inc(k + 1, g.n), r[f]);
-                dside_load_edges(op.half(f, 0), op.half(f, 1), d, wrap_inc(k, g.n), g.n, sdn[f]);
+                draw2_load(H[f], d, wrap_inc(k + 1, g.n), t + 2, r[f]);
+                dside_load_edges(H[f], d, wrap_inc(k, g.n), t + 1, g.n, sdn[f]);
             }
             n0 = op.load(parn * half + offn);
             n1 = op.load((1u - parn) * half + offn);
         }
-        S19<T> s0[NF], s1[NF];
+        // one cell's stencils at a time (fewer live values)
+        double r0, r1;
+        {
+            S19<T> s0[NF];
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-            s0[f] = s19_win2(w[f], false, sd[f], d);
-            s1[f] = s19_win2(w[f], true, sd[f], d);
+            for (int f = 0; f < NF; f++) s0[f] = s19_win2(w[f], false, sd[f], d);
+            r0 = op.cell(par * half + off, o0, s0);
         }
-        const double r0 = op.cell(par * half + off, o0, s0);
-        const double r1 = op.cell((1u - par) * half + off, o1, s1);
+        {
+            S19<T> s1[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) s1[f] = s19_win2(w[f], true, sd[f], d);
+            r1 = op.cell((1u - par) * half + off, o1, s1);
+        }
         acc += r0 * r0 + r1 * r1;
         if (t + 1 < DGP_KC) {
 #pragma unroll

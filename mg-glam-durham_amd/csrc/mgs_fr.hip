@@ -129,47 +129,54 @@ __global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T
 }
 
 // correct_solution (PMP2extradof.f90:1361-1578), one coarse column per thread;
-// ONLY1: colour-1 sons only (son_corrected)
-template <bool ONLY1>
+// ONLY1: colour-1 sons only (son_corrected); GUARD: keep a son's old value
+// unless the new one is > 0 (f(R)).  Fine plane 2K takes coarse
+// planes K-1 (beside) and K (centre), plane 2K+1 planes K+1 and K, so fine
+// planes 2K+1 and 2K+2 both take coarse planes K and K+1: the column marches
+// in that pair order with two coarse planes of uH and R u in registers (a
+// third, prefetched, would not fit four waves per SIMD).  Each son is
+// written once, from values nothing in the launch writes, so the order of
+// the updates does not change the results.
+template <bool ONLY1, bool GUARD>
 __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH, const double* __restrict__ ru,
-                                                   double* __restrict__ uf, Geo gf, Geo gc, int guard, int TX, int TY,
-                                                   int XB, int JB, int KC, unsigned NB)
+                                                   double* __restrict__ uf, Geo gf, Geo gc, int TX, int TY, int XB,
+                                                   int JB, int KC, unsigned NB)
 {
+    // coarse arrays: elem() offsets (a coarse level holds at most 2^27 cells)
     ZCol cc;
     if (!zcol(gc.n, gc.n, TX, TY, XB, JB, KC, NB, cc)) return;
     const int I = cc.m, J = cc.j;
     Block9 B;
     B.init(gc, I, J);
-    double hs[9], rs[9], hk[9], rk[9];  // planes K-1 and K
-    const int Km = wrap_dec(cc.k0, gc.n);
+    double ha[9], ra[9], hb[9], rb[9];  // coarse planes K and K+1
+    {
+        const int Km = wrap_dec(cc.k0, gc.n);
+        double o[4];
+        correct64_sons_load<0, ONLY1>(uf, gf, I, J, cc.k0, o);
 #pragma unroll
-    for (int q = 0; q < 9; q++) {
-        hs[q] = uH[B.at(q, Km)];
-        rs[q] = ru[B.at(q, Km)];
-        hk[q] = uH[B.at(q, cc.k0)];
-        rk[q] = ru[B.at(q, cc.k0)];
+        for (int q = 0; q < 9; q++) {
+            ha[q] = elem(uH, B.at(q, Km));
+            ra[q] = elem(ru, B.at(q, Km));
+            hb[q] = elem(uH, B.at(q, cc.k0));
+            rb[q] = elem(ru, B.at(q, cc.k0));
+        }
+        correct64_sons_apply<0, ONLY1, GUARD>(ha, ra, hb, rb, uf, gf, I, J, cc.k0, o);  // fine plane 2 k0
     }
     for (int K = cc.k0; K < cc.k1; K++) {
-        // every load of this plane (sons of fine planes 2K and 2K+1, coarse
-        // plane K+1), then the updates
-        double o0[4], o1[4], hp[9], rp[9];
-        correct64_sons_load<0, ONLY1>(uf, gf, I, J, K, o0);
+        const bool more = K + 1 < cc.k1;
+        double o1[4], o0[4];
         correct64_sons_load<1, ONLY1>(uf, gf, I, J, K, o1);
+        if (more) correct64_sons_load<0, ONLY1>(uf, gf, I, J, K + 1, o0);
         const int Kp = wrap_inc(K, gc.n);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            hp[q] = uH[B.at(q, Kp)];
-            rp[q] = ru[B.at(q, Kp)];
+            ha[q] = hb[q];
+            ra[q] = rb[q];
+            hb[q] = elem(uH, B.at(q, Kp));
+            rb[q] = elem(ru, B.at(q, Kp));
         }
-        correct64_sons_apply<0, ONLY1>(hs, rs, hk, rk, uf, gf, I, J, K, guard, o0);
-        correct64_sons_apply<1, ONLY1>(hp, rp, hk, rk, uf, gf, I, J, K, guard, o1);
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-            hs[q] = hk[q];
-            rs[q] = rk[q];
-            hk[q] = hp[q];
-            rk[q] = rp[q];
-        }
+        correct64_sons_apply<1, ONLY1, GUARD>(hb, rb, ha, ra, uf, gf, I, J, K, o1);  // fine plane 2K+1
+        if (more) correct64_sons_apply<0, ONLY1, GUARD>(ha, ra, hb, rb, uf, gf, I, J, K + 1, o0);  // 2K+2
     }
 }
 
@@ -449,10 +456,14 @@ void launch_correct64(const double* uH, const double* ru, double* ufine, int nc,
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
     const ZLaunch L = make_zlaunch(nc, nc);
-    if (only1)
-        hipLaunchKernelGGL(k_correct64<true>, L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, ZARGS);
+    if (only1 && guard)
+        hipLaunchKernelGGL((k_correct64<true, true>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);
+    else if (only1)
+        hipLaunchKernelGGL((k_correct64<true, false>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);
+    else if (guard)
+        hipLaunchKernelGGL((k_correct64<false, true>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);
     else
-        hipLaunchKernelGGL(k_correct64<false>, L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, guard, ZARGS);
+        hipLaunchKernelGGL((k_correct64<false, false>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);
 }
 
 // partials[0..count) -> out[0]; stage 1 writes its chunk sums after the

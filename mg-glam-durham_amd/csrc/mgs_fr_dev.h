@@ -257,27 +257,29 @@ struct Pts12 {
 
 // Coarse 3x3 block around column (I, J) of a coarse level: the colour
 // parity and in-plane offset of each (x, y) of the block, so that a plane K
-// of the block loads with one add per value
+// of the block loads with two adds per value
 struct Block9 {
-    unsigned par[9], row[9], plane, half;
+    // x offsets (m) of columns I-1, I, I+1 and row offsets of rows J-1, J, J+1;
+    // the colour of (x, y) is p0 ^ (x + y) & 1 (nc even: the wrap keeps parity)
+    unsigned xo[3], yo[3], p0, plane, half;
     __device__ __forceinline__ void init(const Geo& gc, int I, int J)
     {
         const int nc = gc.n;
-        const int Xs[3] = {wrap_dec(I, nc), I, wrap_inc(I, nc)};
-        const int Ys[3] = {wrap_dec(J, nc), J, wrap_inc(J, nc)};
-#pragma unroll
-        for (int y = 0; y < 3; y++)
-#pragma unroll
-            for (int x = 0; x < 3; x++) {
-                par[x + 3 * y] = (unsigned)((Xs[x] + Ys[y]) & 1);
-                row[x + 3 * y] = (unsigned)Ys[y] * (unsigned)gc.h + (unsigned)(Xs[x] >> 1);
-            }
+        xo[0] = (unsigned)(wrap_dec(I, nc) >> 1);
+        xo[1] = (unsigned)(I >> 1);
+        xo[2] = (unsigned)(wrap_inc(I, nc) >> 1);
+        yo[0] = (unsigned)wrap_dec(J, nc) * (unsigned)gc.h;
+        yo[1] = (unsigned)J * (unsigned)gc.h;
+        yo[2] = (unsigned)wrap_inc(J, nc) * (unsigned)gc.h;
+        p0 = (unsigned)((I + J) & 1);
         plane = (unsigned)gc.n * (unsigned)gc.h;
         half = (unsigned)gc.half;
     }
+    // offset of point q = x + 3 y of the 3x3 block in coarse plane K
     __device__ __forceinline__ unsigned at(int q, int K) const
     {
-        return ((par[q] + (unsigned)K) & 1u) * half + (unsigned)K * plane + row[q];
+        const int x = q % 3, y = q / 3;
+        return ((p0 + (unsigned)(x + y) + (unsigned)K) & 1u) * half + (unsigned)K * plane + yo[y] + xo[x];
     }
 };
 
@@ -289,13 +291,21 @@ __device__ __forceinline__ unsigned son_index(const Geo& gf, int I, int J, int K
            ((unsigned)(2 * K + s3) * (unsigned)gf.n + (unsigned)(2 * J + s2)) * (unsigned)gf.h + (unsigned)I;
 }
 
+// son (2I+s1, 2J+s2, 2K+s3) of a double level: its colour half (the colour
+// is known at compile time, so the half's base stays scalar) and an offset in
+// it, which fits a 32-bit byte offset (a half holds at most 2^29 cells)
+__device__ __forceinline__ double& son64(double* uf, const Geo& gf, int I, int J, int K, int s1, int s2, int s3)
+{
+    double* h = uf + (size_t)((s1 + s2 + s3) & 1) * gf.half;
+    return elem(h, ((unsigned)(2 * K + s3) * (unsigned)gf.n + (unsigned)(2 * J + s2)) * (unsigned)gf.h + (unsigned)I);
+}
+
 // correct_solution, PMP2extradof.f90:1361-1578: u_h += P(u_H - R u_h) with the
 // 27/9/3/1 (/64) weights; guard: keep the old value unless the new one is > 0.
-// A thread owns coarse column (I, J) and marches over K with two planes of
-// the 3x3 coarse block in registers: the sons in fine plane 2K use planes
-// K-1 and K, then plane K+1 replaces K-1 for the sons in plane 2K+1.  The sum
-// keeps the reference's order P = P + w u_H - w R u over the 8 contributors
-// of a son.
+// A thread owns coarse column (I, J); the sons in fine plane 2K take the 3x3
+// coarse block of planes K-1 (hz, rz) and K (hk, rk), those in plane 2K+1
+// planes K+1 and K.  The sum keeps the reference's order P = P + w u_H - w R u
+// over the 8 contributors of a son.
 //
 // ONLY1: correct the colour-1 sons only.  Inside the f(R) cycles a
 // correction is always followed by a colour-0 relaxation pass, whose new
@@ -313,18 +323,18 @@ __device__ __forceinline__ bool son_corrected(int t)
 // before any son of the coarse plane is stored: each son is read and written
 // by its own thread only)
 template <int S3, bool ONLY1>
-__device__ __forceinline__ void correct64_sons_load(const double* __restrict__ uf, const Geo& gf, int I, int J, int K,
+__device__ __forceinline__ void correct64_sons_load(double* __restrict__ uf, const Geo& gf, int I, int J, int K,
                                                     double* old)
 {
 #pragma unroll
     for (int t = 0; t < 4; t++)
-        if (son_corrected<S3, ONLY1>(t)) old[t] = uf[son_index(gf, I, J, K, t & 1, t >> 1, S3)];
+        if (son_corrected<S3, ONLY1>(t)) old[t] = son64(uf, gf, I, J, K, t & 1, t >> 1, S3);
 }
 
-template <int S3, bool ONLY1>
+template <int S3, bool ONLY1, bool GUARD>
 __device__ __forceinline__ void correct64_sons_apply(const double* hz, const double* rz, const double* hk,
                                                      const double* rk, double* __restrict__ uf, const Geo& gf, int I,
-                                                     int J, int K, int guard, const double* old)
+                                                     int J, int K, const double* old)
 {
     const double a = 1.0 / 64.0, b = 3.0 * a, c9 = 9.0 * a, d = 27.0 * a;
     const double wt[8] = {a, b, b, c9, b, c9, c9, d};
@@ -345,7 +355,7 @@ __device__ __forceinline__ void correct64_sons_apply(const double* hz, const dou
             P = P - wt[ind] * vr;
         }
         const double tmpt = old[t] + P;
-        if (!guard || tmpt > 0.0) uf[son_index(gf, I, J, K, s1, s2, S3)] = tmpt;
+        if (!GUARD || tmpt > 0.0) son64(uf, gf, I, J, K, s1, s2, S3) = tmpt;
     }
 }
 

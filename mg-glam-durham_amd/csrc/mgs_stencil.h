@@ -34,6 +34,18 @@ struct ZLaunch {
     unsigned NB;
 };
 
+#ifdef MGS_TUNING
+// tuning builds: the launch-shape overrides below apply to levels with n <= MGS_ZMAXN (default: all)
+inline int tuning_max_n()
+{
+    static const int v = [] {
+        const char* e = std::getenv("MGS_ZMAXN");
+        return e ? std::atoi(e) : 1 << 30;
+    }();
+    return v;
+}
+#endif
+
 // planes marched per thread; in tuning builds (-DMGS_TUNING) MGS_ZRUN overrides it
 inline int zrun_length(int n)
 {
@@ -42,11 +54,12 @@ inline int zrun_length(int n)
         const char* e = std::getenv("MGS_ZRUN");
         return e ? std::atoi(e) : 0;
     }();
-    if (forced > 0) return forced < n ? forced : n;
+    if (forced > 0 && n <= tuning_max_n()) return forced < n ? forced : n;
 #endif
     // measured at 512^3 (runs of 2, 4, 6, 8, 16, 32 planes): 4 is best; the
     // window's two prologue loads per run cost less than the lost
-    // parallelism of longer runs
+    // parallelism of longer runs.  At 256^3 (the 512^3 solve's level 1) runs
+    // of 2, 8 and 16 planes and 64-wide blocks are slower as well.
     return n >= 4 ? 4 : n;
 }
 
@@ -56,11 +69,12 @@ inline ZLaunch make_zlaunch(int width, int n, int kc)
 {
     // widest block row; in tuning builds MGS_ZTX overrides it
 #ifdef MGS_TUNING
-    static const int max_tx = [] {
+    static const int max_tx_env = [] {
         const char* e = std::getenv("MGS_ZTX");
         const int v = e ? std::atoi(e) : 0;
         return (v == 32 || v == 64 || v == 128) ? v : 256;
     }();
+    const int max_tx = n <= tuning_max_n() ? max_tx_env : 256;
 #else
     constexpr int max_tx = 256;
 #endif

@@ -54,7 +54,8 @@ def test_invalid_parameters_are_rejected_before_any_device_call():
     h = C.c_void_p()
     for bad in (M.Config(N=15), M.Config(N=16, model=4), M.Config(N=16, fr_n=3),
                 M.Config(N=16, precision=M.PREC_MIXED, fr0=1e-5),
-                M.Config(N=18, precision=M.PREC_MIXED), M.Config(N=16, box=-1.0)):
+                M.Config(N=18, precision=M.PREC_MIXED), M.Config(N=16, box=-1.0),
+                M.Config(N=1280)):  # beyond the 32-bit level indexing (n <= 1024)
         p = bad.c()
         rc = L.mgs_create(C.byref(p), C.byref(h))
         assert rc == -1, bad
