@@ -138,7 +138,7 @@ __device__ __forceinline__ bool zcol_at(int width, int n, int TX, int TY, int XB
                                         int ty, ZCol& c)
 {
     unsigned xb, jb, kb;
-    if (JB & ZSLAB) {  // x blocks fastest, then the slab's rows, then plane runs, then slabs
+    if (JB > 0 && (JB & ZSLAB)) {  // x blocks fastest, then the slab's rows, then plane runs, then slabs
         const unsigned JS = (unsigned)(JB & (ZSLAB - 1)) >> 3;
         const unsigned KB = (unsigned)((n + KC - 1) / KC);
         xb = XB == 1 ? 0u : lb % (unsigned)XB;
