@@ -400,26 +400,41 @@ __device__ __forceinline__ void dside_load(const DHalves<T>& H, const DCol& d, i
 
 // both sides at once, for passes that evaluate both cells of the pair: the
 // left edge lane loads x = 2m-1 (the side of the cell at 2m), the right edge
-// lane x = 2m+2 (the side of the cell at 2m+1), into the same five slots
+// lane x = 2m+2 (the side of the cell at 2m+1), into the same five slots.
+// Measured at 512^3: FP64 windows run faster with one branch per edge
+// (scalar half bases); the two-field FP32 window of the mixed residual needs
+// 129 registers that way (three waves per SIMD, 10% slower) and 118 with
+// one branch choosing the half per lane.
 template <typename T>
 __device__ __forceinline__ void dside_load_edges(const DHalves<T>& H, const DCol& d, int k, int dk, int n,
                                                  DSide<T>& sd)
 {
 #pragma unroll
     for (int q = 0; q < 5; q++) sd.v[q] = T(0);
-    if (d.lload) {
+    if constexpr (sizeof(T) == 8) {
+        if (d.lload) {
 #pragma unroll
-        for (int q = 0; q < 5; q++) {
-            const int P = side_plane(q), R = side_row(q);
-            const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
-            sd.v[q] = elem(H.p[((R + P + dk) & 1) ^ 1], (unsigned)K * d.plane + d.rowoff[R] + d.mm);
+            for (int q = 0; q < 5; q++) {
+                const int P = side_plane(q), R = side_row(q);
+                const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
+                sd.v[q] = elem(H.p[((R + P + dk) & 1) ^ 1], (unsigned)K * d.plane + d.rowoff[R] + d.mm);
+            }
+        } else if (d.rload) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                const int P = side_plane(q), R = side_row(q);
+                const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
+                sd.v[q] = elem(H.p[(R + P + dk) & 1], (unsigned)K * d.plane + d.rowoff[R] + d.mp);
+            }
         }
-    } else if (d.rload) {
+    } else if (d.lload || d.rload) {
+        const unsigned mo = d.rload ? d.mp : d.mm;
 #pragma unroll
         for (int q = 0; q < 5; q++) {
             const int P = side_plane(q), R = side_row(q);
             const int K = P == 0 ? wrap_dec(k, n) : (P == 2 ? wrap_inc(k, n) : k);
-            sd.v[q] = elem(H.p[(R + P + dk) & 1], (unsigned)K * d.plane + d.rowoff[R] + d.mp);
+            const int f = (R + P + dk) & 1;
+            sd.v[q] = elem(d.rload ? H.p[f] : H.p[f ^ 1], (unsigned)K * d.plane + d.rowoff[R] + mo);
         }
     }
 }
