@@ -541,8 +541,16 @@ void launch_df_restrict_residual(int frn, const float* d0, const float* xb0, con
         return e ? std::atoi(e) : 0;
     }();
     if (rr > 0) kc = rr < gc.n ? rr : gc.n;
-#endif
+    // and MGS_RR_TX (64 or 128) its block width in coarse columns
+    static const int rr_tx = [] {
+        const char* e = std::getenv("MGS_RR_TX");
+        const int v = e ? std::atoi(e) : 0;
+        return (v == 64 || v == 128) ? v : 0;
+    }();
+    const ZLaunch L = make_zlaunch(gc.n, gc.n, kc, rr_tx);
+#else
     const ZLaunch L = make_zlaunch(gc.n, gc.n, kc);
+#endif
     FRN_DISPATCH(frn, k_df_restrict_residual, d0, xb0, g0, rres1, rd1, d1, gf, gc, c0, ZARGS);
 }
 

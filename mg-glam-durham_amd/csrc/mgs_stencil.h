@@ -64,8 +64,9 @@ inline int zrun_length(int n)
 }
 
 // (TX x TY)-thread blocks over `width` columns x n rows, each thread marching
-// a run of KC planes; k-run-major block order
-inline ZLaunch make_zlaunch(int width, int n, int kc)
+// a run of KC planes; k-run-major block order.  tx_cap > 0 caps the block
+// width (tuning builds)
+inline ZLaunch make_zlaunch(int width, int n, int kc, int tx_cap = 0)
 {
     // widest block row; in tuning builds MGS_ZTX overrides it
 #ifdef MGS_TUNING
@@ -80,6 +81,7 @@ inline ZLaunch make_zlaunch(int width, int n, int kc)
 #endif
     ZLaunch L;
     L.TX = width < max_tx ? width : max_tx;
+    if (tx_cap > 0 && L.TX > tx_cap) L.TX = tx_cap;
     int ty = 256 / L.TX;
     if (ty > n) ty = n;
     L.TY = ty < 1 ? 1 : ty;
