@@ -8,14 +8,14 @@
 //
 // Deposit into the solver's level-0 density (colour-split FP64, the array
 // mgs_set_density fills), so a solve can follow without a host trip.  The
-// particles are binned by the 8^3-cell tile of their CIC base cell, in runs
+// particles are binned by the 16^3-cell tile of their CIC base cell, in runs
 // of consecutive particles of one tile (a counting sort of the runs: per-tile
 // counts whose atomics also rank the runs, an exclusive scan, a scatter of
 // (first particle, length) into tile order); one workgroup per tile then
-// accumulates its particles' 8 weights in a 9^3 FP64 buffer in LDS and adds
-// each touched cell to the grid with one hardware FP64 atomic, instead of
-// eight global atomics per particle.  FP64 accumulation makes the sum independent of the atomic order
-// to ~1e-16 (the reference adds into REAL*4 FI).
+// accumulates its particles' 8 weights in a 17^3 fixed-point buffer in LDS
+// and adds each touched cell to the grid once, instead of eight global
+// atomics per particle.  Integer accumulation makes the sum independent of
+// the order of the additions (the reference adds into REAL*4 FI).
 //
 // Move: one thread per particle gathers the 32 potential values of its 2x2x2
 // node block and their outer neighbours (PMP2main.f90:496-558), forms the
@@ -110,9 +110,9 @@ __device__ __forceinline__ int cell_of(double X, int N)
 
 // DENSIT's CIC loop, PMP2mod_density.f90:235-281 (0-based cells here), over
 // the particles binned by tile.
-constexpr int PM_TILE = 8;                                             // cells per tile side
+constexpr int PM_TILE = 16;                                            // cells per tile side
 constexpr int PM_TBUF = PM_TILE + 1;                                   // with the +1 CIC reach
-constexpr int PM_TCELLS = PM_TBUF * PM_TBUF * PM_TBUF;                 // 729
+constexpr int PM_TCELLS = PM_TBUF * PM_TBUF * PM_TBUF;                 // 4913 (38.4 KiB of LDS)
 
 // tile of 0-based base cell (i0, j0, k0); TT tiles per side
 __device__ __forceinline__ int tile_of(int i0, int j0, int k0, int TT)
@@ -249,19 +249,35 @@ constexpr double PM_UNFIX = 1.0 / 1099511627776.0;  // 2^-40
 __device__ __forceinline__ unsigned long long pm_fix(double v) { return __double2ull_rn(v * PM_FIX); }
 
 // one workgroup per tile: the CIC weights of the particles of its runs
-// accumulated in LDS over the tile's cells and their +1 reach, then one
-// atomic per touched cell into the level's density storage, which holds the
-// fixed-point sums until k_deposit_finish converts them
+// accumulated in LDS over the tile's cells and their +1 reach, then added
+// per touched cell into the level's density storage, which holds the
+// fixed-point sums until k_deposit_finish converts them.
+//
+// PHASED: the launch covers the tiles of one parity class (tile coordinates
+// (2a + px, 2b + py, 2c + pz), phase = px + 2 py + 4 pz) and adds with plain
+// loads and stores.  Two tiles of a class are at least two tiles apart, so
+// their 17^3 boxes are disjoint (with TT even also across the periodic wrap,
+// whose reach lands in tile 0's box only), and the eight classes run as
+// eight launches in order.  Otherwise every tile in one launch adds with
+// device-scope 64-bit atomics.  Integer sums make both bitwise the same.
+template <bool PHASED>
 __global__ __launch_bounds__(PM_BLOCK) void k_tile_deposit(const float* __restrict__ xs, const float* __restrict__ ys,
                                                            const float* __restrict__ zs,
                                                            const int64_t* __restrict__ runs,
                                                            const int* __restrict__ count,
-                                                           const int64_t* __restrict__ first, int ntiles, int TT,
-                                                           unsigned long long* __restrict__ sums, Geo g, double wpar)
+                                                           const int64_t* __restrict__ first, int nlaunch, int TT,
+                                                           int phase, unsigned long long* __restrict__ sums, Geo g,
+                                                           double wpar)
 {
     __shared__ unsigned long long acc[PM_TCELLS];
     const int N = g.n;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int bt = blockIdx.x; bt < nlaunch; bt += gridDim.x) {
+        int t = bt;
+        if constexpr (PHASED) {
+            const int H = TT >> 1;
+            const int a = bt % H, b2 = (bt / H) % H, c2 = bt / (H * H);
+            t = ((2 * c2 + ((phase >> 2) & 1)) * TT + 2 * b2 + ((phase >> 1) & 1)) * TT + 2 * a + (phase & 1);
+        }
         const int c = count[t];
         if (c == 0) continue;  // uniform over the block
         for (int i = threadIdx.x; i < PM_TCELLS; i += blockDim.x) acc[i] = 0ull;
@@ -297,7 +313,10 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tile_deposit(const float* __restri
             gi = gi >= N ? gi - N : gi;
             gj = gj >= N ? gj - N : gj;
             gk = gk >= N ? gk - N : gk;
-            atomicAdd(&sums[split_index(g, gi, gj, gk)], v);
+            if constexpr (PHASED)
+                sums[split_index(g, gi, gj, gk)] += v;
+            else
+                atomicAdd(&sums[split_index(g, gi, gj, gk)], v);
         }
         __syncthreads();
     }
@@ -559,9 +578,16 @@ int mgs_cic_deposit(mgs_handle* h)
         hipSuccess)
         return set_err(h, MGS_E_HIP, "deposit scan");
     hipLaunchKernelGGL(k_bin_scatter, dim3(gbin), dim3(PM_BLOCK), 0, h->s, np, p->key, p->rank, p->first, p->runs);
-    hipLaunchKernelGGL(k_tile_deposit, dim3((unsigned)p->ntiles), dim3(PM_BLOCK), 0, h->s, b, b + np, b + 2 * np,
-                       p->runs, p->count, p->first, p->ntiles, TT, reinterpret_cast<unsigned long long*>(h->L[0].rho),
-                       g, wpar_of(N, np));
+    unsigned long long* sums = reinterpret_cast<unsigned long long*>(h->L[0].rho);
+    if (TT % 2 == 0) {
+        const int per = p->ntiles / 8;  // tiles of one parity class
+        for (int ph = 0; ph < 8; ph++)
+            hipLaunchKernelGGL(k_tile_deposit<true>, dim3((unsigned)per), dim3(PM_BLOCK), 0, h->s, b, b + np,
+                               b + 2 * np, p->runs, p->count, p->first, per, TT, ph, sums, g, wpar_of(N, np));
+    } else {
+        hipLaunchKernelGGL(k_tile_deposit<false>, dim3((unsigned)p->ntiles), dim3(PM_BLOCK), 0, h->s, b, b + np,
+                           b + 2 * np, p->runs, p->count, p->first, p->ntiles, TT, 0, sums, g, wpar_of(N, np));
+    }
     {
         const size_t n3 = (size_t)N * N * N;
         const unsigned gfin = (unsigned)std::min<size_t>((n3 + 255) / 256, 65536);

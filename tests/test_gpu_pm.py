@@ -35,12 +35,15 @@ def fr_cfg(N, prec=M.PREC_FP64):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N", [20, 40])
+@pytest.mark.parametrize("N", [20, 40, 44])
 def test_binned_deposit_clustered_and_partial_tiles(N):
-    """The deposit bins particles by 8^3-cell tile: grids that are not a
-    multiple of 8 (a partial last tile on each axis), particles piled into a
+    """The deposit bins particles by 16^3-cell tile: grids that are not a
+    multiple of 16 (a partial last tile on each axis), particles piled into a
     few cells (long runs of one tile, LDS accumulation contended), positions
-    on both edges of [1, N+1) (the +1 reach wraps), and a shuffled order."""
+    on both edges of [1, N+1) (the +1 reach wraps), and a shuffled order.
+    N = 20 (2 tiles a side, the last partial) takes the eight parity-class
+    launches with plain adds, whose last tile's reach wraps into tile 0;
+    N = 40 and 44 (3 tiles) the one-launch path with atomics."""
     rng = np.random.default_rng(N)
     n = 30000
     x, y, z, _ = particles(N, n, N + 1)
