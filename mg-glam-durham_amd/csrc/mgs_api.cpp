@@ -215,10 +215,12 @@ static int alloc_hierarchy(mgs_handle* h)
     int rc;
     if ((rc = dalloc(h, &h->partials, h->npart))) return rc;
     if ((rc = dalloc(h, &h->dsum, 16))) return rc;
-    if (h->use_arena)
+    if (h->use_arena) {
         h->hsum = h->hsum_host;
-    else
+    } else {
         HIPCHK(h, hipHostMalloc((void**)&h->hsum, 16 * sizeof(double), hipHostMallocDefault));
+        HIPCHK(h, hipEventCreateWithFlags(&h->ev_norm, hipEventDisableTiming));
+    }
     return 0;
 }
 
@@ -458,7 +460,7 @@ static int cache_densities(mgs_handle* h)
 // on the first call, replayed afterwards.  The coarse levels of a V-cycle are
 // ~10 small launches per level; one graph launch replaces them.
 template <typename F>
-static int run_captured(mgs_handle* h, hipGraphExec_t* exec, F body)
+static int run_captured(mgs_handle* h, hipGraphExec_t* exec, F body, int kc = MGS_KC_COARSE)
 {
     if (!h->use_graphs) return body();
     if (!*exec) {
@@ -475,7 +477,7 @@ static int run_captured(mgs_handle* h, hipGraphExec_t* exec, F body)
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return set_err(h, MGS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
-    prof_begin(h, MGS_KC_COARSE, 0.0);
+    prof_begin(h, kc, 0.0);
     HIPCHK(h, hipGraphLaunch(*exec, h->s));
     prof_end(h);
     return 0;
@@ -485,8 +487,10 @@ void drop_graphs(mgs_handle* h)
 {
     if (h->g_plain) (void)hipGraphExecDestroy(h->g_plain);
     if (h->g_inner) (void)hipGraphExecDestroy(h->g_inner);
+    if (h->g_rbase) (void)hipGraphExecDestroy(h->g_rbase);
     h->g_plain = nullptr;
     h->g_inner = nullptr;
+    h->g_rbase = nullptr;
 }
 
 // V_cycle(ilevel), PMP2extradof.f90:352-555 (Nsmth = 2, :361)
@@ -625,6 +629,55 @@ static int vcycle_inner(mgs_handle* h, int l, bool first = false, bool last = fa
 // FP64 outer residual; also writes the inner problem's g, xb and colour 0 of
 // d after its first colour-0 pass (reads u, delta; writes g, xb, d: 26 B/cell)
 // norm_only: the residual norm without the next inner problem's data
+// The inner problem's base x-bar (f(R)) or phi_b (nDGP), written on level 0
+// by the outer residual, restricted to every level: one captured graph
+static int restrict_base_all(mgs_handle* h)
+{
+    const bool fr = h->p.model == MGS_MODEL_FR;
+    return run_captured(
+        h, &h->g_rbase,
+        [h, fr]() {
+            for (int l = 1; l <= h->lc; l++) {
+                prof_begin(h, l == 1 ? MGS_KC_RESTRICT : MGS_KC_COARSE,
+                           (n3d(h->M[l - 1].n) + n3d(h->M[l].n)) * 4.0);
+                if (fr)
+                    launch_restrict_base(h->p.fr_n, h->M[l - 1].xb, h->M[l].xb, h->M[l].n, h->s);
+                else
+                    launch_restrict8_32(h->M[l - 1].xb, h->M[l].xb, nullptr, h->M[l].n, h->s);
+                prof_end(h);
+            }
+            return 0;
+        },
+        MGS_KC_RESTRICT);
+}
+
+// The outer norm's partials summed and copied to the host; with_base, the
+// base restriction (which only needs the level-0 base just written) is
+// enqueued behind the copy and the host waits for the copy alone, so that
+// the restriction runs while the host reads the norm and picks the next
+// inner count (an inner cycle, queued next, runs after it in stream order)
+static int finish_outer_norm(mgs_handle* h, int nb, bool with_base, double* rms)
+{
+    launch_sum_partials(h->partials, nb, h->dsum, h->s);
+    HIPCHK(h, hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    if (with_base && h->ev_norm) {
+        HIPCHK(h, hipEventRecord(h->ev_norm, h->s));
+        int rc = restrict_base_all(h);
+        if (rc) return rc;
+        HIPCHK(h, hipEventSynchronize(h->ev_norm));
+    } else {
+        if (with_base) {
+            int rc = restrict_base_all(h);
+            if (rc) return rc;
+        }
+        HIPCHK(h, hipStreamSynchronize(h->s));
+    }
+    *rms = std::sqrt(h->hsum[0] / n3d(h->L[0].n));
+    return 0;
+}
+
+// f(R) outer residual; unless norm_only it also writes the inner problem's
+// data and restricts its base to every level
 static int outer_residual(mgs_handle* h, double* rms, bool norm_only = false)
 {
     LevelD& v = h->L[0];
@@ -633,11 +686,7 @@ static int outer_residual(mgs_handle* h, double* rms, bool norm_only = false)
     const int nb = launch_outer_residual(h->p.fr_n, v.u, v.rho, m.f, m.xb, m.d, h->partials, v.n, fr_coefs(h->p, 0),
                                          df_coefs(h->p, 0), norm_only, h->s);
     prof_end(h);
-    launch_sum_partials(h->partials, nb, h->dsum, h->s);
-    HIPCHK(h, hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s));
-    HIPCHK(h, hipStreamSynchronize(h->s));
-    *rms = std::sqrt(h->hsum[0] / n3d(v.n));
-    return 0;
+    return finish_outer_norm(h, nb, !norm_only, rms);
 }
 
 // the reference's exits, PMP2extradof.f90:107-122
@@ -878,11 +927,7 @@ static int outer_residual_dgp(mgs_handle* h, double* rms)
     prof_begin(h, MGS_KC_RESIDUAL, 28.0 * n3d(v.n));
     const int nb = launch_dgpm_outer(v.u, v.rho, m.f, m.xb, m.d, h->partials, v.n, dgp_coefs(h->p, 0), h->s);
     prof_end(h);
-    launch_sum_partials(h->partials, nb, h->dsum, h->s);
-    HIPCHK(h, hipMemcpyAsync(h->hsum, h->dsum, sizeof(double), hipMemcpyDeviceToHost, h->s));
-    HIPCHK(h, hipStreamSynchronize(h->s));
-    *rms = std::sqrt(h->hsum[0] / n3d(v.n));
-    return 0;
+    return finish_outer_norm(h, nb, true, rms);
 }
 
 // Mode 1 with inner_cycles 0: the inner cycles of the next outer iteration.
@@ -916,7 +961,6 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
 {
     int rc;
     const bool fr = h->p.model == MGS_MODEL_FR;
-    const int frn = h->p.fr_n;
     const bool automatic = o.inner_cycles == 0 && o.mode == 1;
     int inner = o.inner_cycles > 0 ? o.inner_cycles : 1;
     if (!fr && (rc = cache_densities(h))) return rc;
@@ -961,15 +1005,8 @@ static int solve_mixed(mgs_handle* h, const mgs_solve_opts& o, mgs_solve_report*
             m_prev = inner;
         }
         last_expected = fr && o.mode == 1 && (reaches || it >= o.max_cycles);
-        // inner problem: base restricted to every level, increment from zero
-        for (int l = 1; l <= h->lc; l++) {
-            prof_begin(h, l == 1 ? MGS_KC_RESTRICT : MGS_KC_COARSE, (n3d(h->M[l - 1].n) + n3d(h->M[l].n)) * 4.0);
-            if (fr)
-                launch_restrict_base(frn, h->M[l - 1].xb, h->M[l].xb, h->M[l].n, h->s);
-            else
-                launch_restrict8_32(h->M[l - 1].xb, h->M[l].xb, nullptr, h->M[l].n, h->s);
-            prof_end(h);
-        }
+        // inner problem: the base, restricted to every level with the outer
+        // residual, and the increment from zero
         if (fr) {
             // d = 0 with its first colour-0 pass came from the outer residual;
             // the last inner pass also applies u += u_bg (x(d) - xb)
@@ -1045,6 +1082,8 @@ void release_handle(mgs_handle* h)
     h->staging = nullptr;
     if (h->hsum && h->hsum != h->hsum_host) (void)hipHostFree(h->hsum);
     h->hsum = nullptr;
+    if (h->ev_norm) (void)hipEventDestroy(h->ev_norm);
+    h->ev_norm = nullptr;
     fft_release(h);
     pm_release(h);
 }

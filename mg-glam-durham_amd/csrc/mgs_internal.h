@@ -103,6 +103,11 @@ struct mgs_handle {
     // every cycle; rebuilt when the model scalars change
     hipGraphExec_t g_plain = nullptr;
     hipGraphExec_t g_inner = nullptr;
+    // mixed solve: the inner problem's base restricted to every level
+    hipGraphExec_t g_rbase = nullptr;
+    // recorded after the outer norm's copy to the host (not in arena mode):
+    // the host waits for the norm while the base restriction runs
+    hipEvent_t ev_norm = nullptr;
     bool use_graphs = true;
     // mixed f(R): small inner levels in one launch; tuning builds
     // (-DMGS_TUNING) read MGS_NO_TAIL=1 to turn it off
