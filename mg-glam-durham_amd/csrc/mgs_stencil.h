@@ -27,6 +27,12 @@ namespace mgs {
 // rows-fastest order over the whole grid (whose plane-run footprint exceeds
 // an XCD's L2) stay on chip.
 constexpr int ZSLAB = 1 << 20;
+// bits 21-22 of JB: log2 of (number of slabs / 8).  A slab is marched plane run
+// by plane run, so its rows times one run's planes must stay in an XCD's 4 MB
+// L2 for the next run to find its halo plane there: from n = 1024 (one XCD's
+// eighth of the rows holds about 6 MB per run of 4 planes) there are 16
+// slabs, each XCD marching two in turn.
+constexpr int ZSLAB_SHIFT = 21;
 
 struct ZLaunch {
     dim3 grid, block;
@@ -103,10 +109,11 @@ inline ZLaunch make_zlaunch(int width, int n, int kc, int tx_cap = 0)
     const unsigned per = (L.NB + 7) / 8;
     L.grid = dim3(per * 8);
     L.block = dim3(L.TX, L.TY);
+    const int slab_log = n >= 1024 ? 1 : 0;  // 8 << slab_log slabs
     if (kfast)
         L.JB = -L.JB;
-    else if (L.JB % 8 == 0 && kb > 1)
-        L.JB |= ZSLAB;  // row slabs per XCD (zcol_at)
+    else if (L.JB % (8 << slab_log) == 0 && kb > 1)
+        L.JB |= ZSLAB | (slab_log << ZSLAB_SHIFT);  // row slabs, one or more per XCD (zcol_at)
     return L;
 }
 
@@ -141,7 +148,7 @@ __device__ __forceinline__ bool zcol_at(int width, int n, int TX, int TY, int XB
 {
     unsigned xb, jb, kb;
     if (JB > 0 && (JB & ZSLAB)) {  // x blocks fastest, then the slab's rows, then plane runs, then slabs
-        const unsigned JS = (unsigned)(JB & (ZSLAB - 1)) >> 3;
+        const unsigned JS = (unsigned)(JB & (ZSLAB - 1)) >> (3 + ((JB >> ZSLAB_SHIFT) & 3));
         const unsigned KB = (unsigned)((n + KC - 1) / KC);
         xb = XB == 1 ? 0u : lb % (unsigned)XB;
         unsigned t = XB == 1 ? lb : lb / (unsigned)XB;
