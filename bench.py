@@ -625,7 +625,7 @@ def main():
         "fp64": fp64,
         "speedup_mixed_vs_fp64": (fp64["ms_median"] / ms_step) if fp64 else None,
         "roofline": {
-            "kernel": "level-0 smoother colour pass (mixed, FP32 delta form: k_colour_pass<DfSmoothOp<1>>)",
+            "kernel": "level-0 smoother colour pass (mixed, FP32 delta form: k_colour_pass<DfSmoothOp<1, true>>)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

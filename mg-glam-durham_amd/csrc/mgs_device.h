@@ -91,6 +91,32 @@ __device__ __forceinline__ T& elem(T* a, unsigned i)
     return *reinterpret_cast<T*>(reinterpret_cast<char*>(a) + i * (unsigned)sizeof(T));
 }
 
+// Loads and stores of values a pass reads or writes once (a cell's own
+// data, its result; never the field whose window neighbours re-read).  With
+// NT they are non-temporal: on levels whose arrays exceed the last-level
+// cache (NT_MIN_N) that keeps the L2 for the re-read field; on smaller
+// levels the cache holds the arrays from one pass to the next, and plain
+// accesses keep them there.
+constexpr int NT_MIN_N = 512;
+
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_once(const T& r)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(&r);
+    else
+        return r;
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ void st_once(T& r, T v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, &r);
+    else
+        r = v;
+}
+
 // element i of a whole level's array (up to 2^30 cells): elem() where the
 // byte offset fits 32 bits for any index (4-byte elements), a[i] otherwise
 template <typename T>

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_restrict8(const T* __restrict__ fine, T
 // third, prefetched, would not fit four waves per SIMD).  Each son is
 // written once, from values nothing in the launch writes, so the order of
 // the updates does not change the results.
-template <bool ONLY1, bool GUARD>
+template <bool ONLY1, bool GUARD, bool NT = false>
 __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH, const double* __restrict__ ru,
                                                    double* __restrict__ uf, Geo gf, Geo gc, int TX, int TY, int XB,
                                                    int JB, int KC, unsigned NB)
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH
     {
         const int Km = wrap_dec(cc.k0, gc.n);
         double o[4];
-        correct64_sons_load<0, ONLY1>(uf, gf, I, J, cc.k0, o);
+        correct64_sons_load<0, ONLY1, NT>(uf, gf, I, J, cc.k0, o);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             ha[q] = elem(uH, B.at(q, Km));
@@ -160,13 +160,13 @@ __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH
             hb[q] = elem(uH, B.at(q, cc.k0));
             rb[q] = elem(ru, B.at(q, cc.k0));
         }
-        correct64_sons_apply<0, ONLY1, GUARD>(ha, ra, hb, rb, uf, gf, I, J, cc.k0, o);  // fine plane 2 k0
+        correct64_sons_apply<0, ONLY1, GUARD, NT>(ha, ra, hb, rb, uf, gf, I, J, cc.k0, o);  // fine plane 2 k0
     }
     for (int K = cc.k0; K < cc.k1; K++) {
         const bool more = K + 1 < cc.k1;
         double o1[4], o0[4];
-        correct64_sons_load<1, ONLY1>(uf, gf, I, J, K, o1);
-        if (more) correct64_sons_load<0, ONLY1>(uf, gf, I, J, K + 1, o0);
+        correct64_sons_load<1, ONLY1, NT>(uf, gf, I, J, K, o1);
+        if (more) correct64_sons_load<0, ONLY1, NT>(uf, gf, I, J, K + 1, o0);
         const int Kp = wrap_inc(K, gc.n);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
@@ -175,8 +175,8 @@ __global__ __launch_bounds__(256) void k_correct64(const double* __restrict__ uH
             hb[q] = elem(uH, B.at(q, Kp));
             rb[q] = elem(ru, B.at(q, Kp));
         }
-        correct64_sons_apply<1, ONLY1, GUARD>(hb, rb, ha, ra, uf, gf, I, J, K, o1);  // fine plane 2K+1
-        if (more) correct64_sons_apply<0, ONLY1, GUARD>(ha, ra, hb, rb, uf, gf, I, J, K + 1, o0);  // 2K+2
+        correct64_sons_apply<1, ONLY1, GUARD, NT>(hb, rb, ha, ra, uf, gf, I, J, K, o1);  // fine plane 2K+1
+        if (more) correct64_sons_apply<0, ONLY1, GUARD, NT>(ha, ra, hb, rb, uf, gf, I, J, K + 1, o0);  // 2K+2
     }
 }
 
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_restrict_base(const float* __restrict__
     restrict_base_cell<FRN>(xf, xc, gf, gc, I, (int)(row % (unsigned)gc.n), (int)(row / (unsigned)gc.n));
 }
 
-template <int FRN>
+template <int FRN, bool NT = false>
 __global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH, const float* __restrict__ rd,
                                                     float* __restrict__ df, const float* __restrict__ xbf, Geo gf,
                                                     Geo gc, int TX, int TY, int XB, int JB, int KC, unsigned NB)
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_df_correct(const float* __restrict__ dH
     const unsigned t = threadIdx.y * (unsigned)TX + threadIdx.x;
     const bool lok = threadIdx.x > 0 && (t & 63u) != 0;
     const bool rok = (int)threadIdx.x + 1 < TX && cc.m + 1 < gc.n && ((t + 1) & 63u) != 0;
-    df_correct_col<FRN, true>(dH, rd, df, xbf, gf, gc, cc, lok, rok);
+    df_correct_col<FRN, true, NT>(dH, rd, df, xbf, gf, gc, cc, lok, rok);
 }
 
 template <int FRN>
@@ -306,6 +306,21 @@ __global__ __launch_bounds__(1024) void k_df_tail(TailDesc T)
             hipLaunchKernelGGL((KERNEL<1>), L.grid, L.block, 0, s, __VA_ARGS__);         \
         else                                                                             \
             hipLaunchKernelGGL((KERNEL<2>), L.grid, L.block, 0, s, __VA_ARGS__);         \
+    } while (0)
+
+// the same with the kernel's one-touch accesses non-temporal where nt
+#define FRN_DISPATCH_NT(frn, nt, KERNEL, ...)                                               \
+    do {                                                                                    \
+        if (nt) {                                                                           \
+            if ((frn) == 0)                                                                 \
+                hipLaunchKernelGGL((KERNEL<0, true>), L.grid, L.block, 0, s, __VA_ARGS__);  \
+            else if ((frn) == 1)                                                            \
+                hipLaunchKernelGGL((KERNEL<1, true>), L.grid, L.block, 0, s, __VA_ARGS__);  \
+            else                                                                            \
+                hipLaunchKernelGGL((KERNEL<2, true>), L.grid, L.block, 0, s, __VA_ARGS__);  \
+        } else {                                                                            \
+            FRN_DISPATCH(frn, KERNEL, __VA_ARGS__);                                         \
+        }                                                                                   \
     } while (0)
 
 template <class OP>
@@ -383,7 +398,14 @@ void launch_fr_smooth64(int frn, int level, double* u, const double* rho, const 
                         const FrCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-#define SM(FRN, CO, RR) colour_pass(FrSmoothOp<FRN, CO, RR>{u, rho, f, c}, g, colour, s)
+    const bool nt = n >= NT_MIN_N;
+#define SM(FRN, CO, RR)                                                             \
+    do {                                                                            \
+        if (nt)                                                                     \
+            colour_pass(FrSmoothOp<FRN, CO, RR, true>{u, rho, f, c}, g, colour, s); \
+        else                                                                        \
+            colour_pass(FrSmoothOp<FRN, CO, RR>{u, rho, f, c}, g, colour, s);       \
+    } while (0)
 #define SM2(FRN, CO)            \
     do {                        \
         if (c.reference_roots)  \
@@ -408,7 +430,10 @@ int launch_fr_residual64(int frn, int level, const double* u, const double* rho,
                          double* partials, int n, const FrCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-#define RS(FRN, CO) both_colours(FrResOp<FRN, CO>{u, rho, f, res, c}, partials, g, s)
+    const bool nt = n >= NT_MIN_N;
+#define RS(FRN, CO)                                                                      \
+    (nt ? both_colours(FrResOp<FRN, CO, true>{u, rho, f, res, c}, partials, g, s) \
+        : both_colours(FrResOp<FRN, CO>{u, rho, f, res, c}, partials, g, s))
     if (level == 0) {
         if (frn == 0) return RS(0, false);
         if (frn == 1) return RS(1, false);
@@ -446,9 +471,17 @@ void launch_fr_rhs64(int frn, const double* ru, const double* rho, const double*
                      const FrCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    if (frn == 0) both_colours(FrResOp<0, true>{ru, rho, rres, f, c}, nullptr, g, s);
-    else if (frn == 1) both_colours(FrResOp<1, true>{ru, rho, rres, f, c}, nullptr, g, s);
-    else both_colours(FrResOp<2, true>{ru, rho, rres, f, c}, nullptr, g, s);
+#define RHS(FRN)                                                                              \
+    do {                                                                                      \
+        if (n >= NT_MIN_N)                                                                    \
+            both_colours(FrResOp<FRN, true, true>{ru, rho, rres, f, c}, nullptr, g, s);       \
+        else                                                                                  \
+            both_colours(FrResOp<FRN, true>{ru, rho, rres, f, c}, nullptr, g, s);             \
+    } while (0)
+    if (frn == 0) RHS(0);
+    else if (frn == 1) RHS(1);
+    else RHS(2);
+#undef RHS
 }
 
 void launch_correct64(const double* uH, const double* ru, double* ufine, int nc, int guard, bool only1,
@@ -456,14 +489,22 @@ void launch_correct64(const double* uH, const double* ru, double* ufine, int nc,
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
     const ZLaunch L = make_zlaunch(nc, nc);
+#define C64(O, G)                                                                                          \
+    do {                                                                                                   \
+        if (2 * nc >= NT_MIN_N)                                                                            \
+            hipLaunchKernelGGL((k_correct64<O, G, true>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS); \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_correct64<O, G>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);   \
+    } while (0)
     if (only1 && guard)
-        hipLaunchKernelGGL((k_correct64<true, true>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);
+        C64(true, true);
     else if (only1)
-        hipLaunchKernelGGL((k_correct64<true, false>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);
+        C64(true, false);
     else if (guard)
-        hipLaunchKernelGGL((k_correct64<false, true>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);
+        C64(false, true);
     else
-        hipLaunchKernelGGL((k_correct64<false, false>), L.grid, L.block, 0, s, uH, ru, ufine, gf, gc, ZARGS);
+        C64(false, false);
+#undef C64
 }
 
 // partials[0..count) -> out[0]; stage 1 writes its chunk sums after the
@@ -486,22 +527,38 @@ int launch_outer_residual(int frn, const double* u, const double* delta, float* 
 {
     const Geo g = make_geo(n);
     const unsigned half = (unsigned)g.half;
-#define OUTER(FRN)                                                                                       \
-    return norm_only ? both_colours(FrOuterOp<FRN, true>{u, delta, g32, xb, d, c, dc, half}, partials, g, s) \
-                     : both_colours(FrOuterOp<FRN, false>{u, delta, g32, xb, d, c, dc, half}, partials, g, s)
+    const bool nt = n >= NT_MIN_N;
+#define OUTER1(FRN, NO)                                                                                   \
+    return nt ? both_colours(FrOuterOp<FRN, NO, true>{u, delta, g32, xb, d, c, dc, half}, partials, g, s) \
+              : both_colours(FrOuterOp<FRN, NO>{u, delta, g32, xb, d, c, dc, half}, partials, g, s)
+#define OUTER(FRN)        \
+    do {                  \
+        if (norm_only)    \
+            OUTER1(FRN, true); \
+        OUTER1(FRN, false);    \
+    } while (0)
     if (frn == 0) OUTER(0);
     if (frn == 1) OUTER(1);
     OUTER(2);
 #undef OUTER
+#undef OUTER1
 }
 
 void launch_df_smooth_update(int frn, const float* d, const float* xb, const float* rhs, double* u, int n,
                              const DfCoef& c, double ubg, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    if (frn == 0) colour_pass(DfSmoothUpdateOp<0>{d, xb, rhs, u, c, ubg}, g, 1, s);
-    else if (frn == 1) colour_pass(DfSmoothUpdateOp<1>{d, xb, rhs, u, c, ubg}, g, 1, s);
-    else colour_pass(DfSmoothUpdateOp<2>{d, xb, rhs, u, c, ubg}, g, 1, s);
+#define UPD(FRN)                                                                        \
+    do {                                                                                \
+        if (n >= NT_MIN_N)                                                              \
+            colour_pass(DfSmoothUpdateOp<FRN, true>{d, xb, rhs, u, c, ubg}, g, 1, s);   \
+        else                                                                            \
+            colour_pass(DfSmoothUpdateOp<FRN>{d, xb, rhs, u, c, ubg}, g, 1, s);         \
+    } while (0)
+    if (frn == 0) UPD(0);
+    else if (frn == 1) UPD(1);
+    else UPD(2);
+#undef UPD
 }
 
 void launch_restrict_base(int frn, const float* xb_f, float* xb_c, int nc, hipStream_t s)
@@ -515,18 +572,34 @@ void launch_df_smooth(int frn, float* d, const float* xb, const float* rhs, int 
                       hipStream_t s)
 {
     const Geo g = make_geo(n);
-    if (frn == 0) colour_pass(DfSmoothOp<0>{d, xb, rhs, c}, g, colour, s);
-    else if (frn == 1) colour_pass(DfSmoothOp<1>{d, xb, rhs, c}, g, colour, s);
-    else colour_pass(DfSmoothOp<2>{d, xb, rhs, c}, g, colour, s);
+#define SMD(FRN)                                                             \
+    do {                                                                     \
+        if (n >= NT_MIN_N)                                                   \
+            colour_pass(DfSmoothOp<FRN, true>{d, xb, rhs, c}, g, colour, s); \
+        else                                                                 \
+            colour_pass(DfSmoothOp<FRN>{d, xb, rhs, c}, g, colour, s);       \
+    } while (0)
+    if (frn == 0) SMD(0);
+    else if (frn == 1) SMD(1);
+    else SMD(2);
+#undef SMD
 }
 
 void launch_df_residual(int frn, const float* d, const float* xb, const float* rhs, float* res, int n,
                         const DfCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    if (frn == 0) both_colours(DfResOp<0>{d, xb, rhs, res, c}, nullptr, g, s);
-    else if (frn == 1) both_colours(DfResOp<1>{d, xb, rhs, res, c}, nullptr, g, s);
-    else both_colours(DfResOp<2>{d, xb, rhs, res, c}, nullptr, g, s);
+#define RSD(FRN)                                                                      \
+    do {                                                                              \
+        if (n >= NT_MIN_N)                                                            \
+            both_colours(DfResOp<FRN, true>{d, xb, rhs, res, c}, nullptr, g, s);      \
+        else                                                                          \
+            both_colours(DfResOp<FRN>{d, xb, rhs, res, c}, nullptr, g, s);            \
+    } while (0)
+    if (frn == 0) RSD(0);
+    else if (frn == 1) RSD(1);
+    else RSD(2);
+#undef RSD
 }
 
 void launch_df_restrict_residual(int frn, const float* d0, const float* xb0, const float* g0, float* rres1,
@@ -565,7 +638,7 @@ void launch_df_correct(int frn, const float* dH, const float* rd, float* dfine, 
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
     const ZLaunch L = make_zlaunch(nc, nc);
-    FRN_DISPATCH(frn, k_df_correct, dH, rd, dfine, xbfine, gf, gc, ZARGS);
+    FRN_DISPATCH_NT(frn, 2 * nc >= NT_MIN_N, k_df_correct, dH, rd, dfine, xbfine, gf, gc, ZARGS);
 }
 
 bool launch_df_tail(int frn, int nlev, const TailLevelIn* lv, hipStream_t s)

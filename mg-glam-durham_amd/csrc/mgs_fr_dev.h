@@ -151,7 +151,7 @@ __device__ __forceinline__ unsigned sidx(const Geo& g, int i, int j, int k)
 // relaxation_iterations_fR, one colour, PMP2MGsolver_fR.f90:20-301.
 // REFROOT: the reference's closed form evaluated literally; otherwise the
 // same root from the FP32 seed refined by FP64 Newton steps.
-template <int FRN, bool COARSE, bool REFROOT>
+template <int FRN, bool COARSE, bool REFROOT, bool NT = false>
 struct FrSmoothOp {
     using T = double;
     struct Own {
@@ -165,8 +165,8 @@ struct FrSmoothOp {
     __device__ __forceinline__ Own load(unsigned o, unsigned) const
     {
         Own w;
-        w.rho = rho[o];
-        w.f = COARSE ? f[o] : 0.0;
+        w.rho = ld_once<NT>(rho[o]);
+        w.f = COARSE ? ld_once<NT>(f[o]) : 0.0;
         return w;
     }
     __device__ __forceinline__ void update(unsigned o, unsigned, const Own& w, double, double xp, double xm,
@@ -180,11 +180,11 @@ struct FrSmoothOp {
         else
             P = c.fct1s * w.rho - c.Q2 + c.fct2s * w.f - S / 6.0;
         if constexpr (REFROOT) {
-            u[o] = fr_root64<FRN>(P, c);
+            st_once<NT>(u[o], fr_root64<FRN>(P, c));
         } else {
             float x = seed_root<FRN>((float)(P * c.inv_w), (float)(c.Q2 * c.inv_w));
             if (!(x > 0.0f) || !(x < 3.0e38f)) x = 1.0f;  // Newton from u_bg
-            u[o] = newton_root64<FRN>((double)x * c.ubg, P, c.Q);
+            st_once<NT>(u[o], newton_root64<FRN>((double)x * c.ubg, P, c.Q));
         }
     }
 };
@@ -192,7 +192,7 @@ struct FrSmoothOp {
 // residual of one cell, calculate_residual_fR, PMP2MGsolver_fR.f90:434-458;
 // COARSE subtracts f (:482-488).  Also the FAS right-hand side (:836-923)
 // with (u, f, out) = (R u, R r, f_l).
-template <int FRN, bool COARSE>
+template <int FRN, bool COARSE, bool NT = false>
 struct FrResOp {
     using T = double;
     struct Own {
@@ -207,8 +207,8 @@ struct FrResOp {
     __device__ __forceinline__ Own load(unsigned o) const
     {
         Own w;
-        w.rho = rho[o];
-        w.f = COARSE ? f[o] : 0.0;
+        w.rho = ld_once<NT>(rho[o]);
+        w.f = COARSE ? ld_once<NT>(f[o]) : 0.0;
         return w;
     }
     __device__ __forceinline__ double cell(unsigned o, const Own& w, double uc, double xp, double xm, double yp,
@@ -221,7 +221,7 @@ struct FrResOp {
         OP = OP + c.fct1r * w.rho;
         OP = OP + c.fct2r * (c.a_fR_rt / uc - 1.0);
         if constexpr (COARSE) OP = OP - w.f;
-        if (out) out[o] = OP;
+        if (out) st_once<NT>(out[o], OP);
         return OP;
     }
 };
@@ -322,16 +322,16 @@ __device__ __forceinline__ bool son_corrected(int t)
 // the old values of the sons of fine plane 2K+S3 that are corrected (loaded
 // before any son of the coarse plane is stored: each son is read and written
 // by its own thread only)
-template <int S3, bool ONLY1>
+template <int S3, bool ONLY1, bool NT = false>
 __device__ __forceinline__ void correct64_sons_load(double* __restrict__ uf, const Geo& gf, int I, int J, int K,
                                                     double* old)
 {
 #pragma unroll
     for (int t = 0; t < 4; t++)
-        if (son_corrected<S3, ONLY1>(t)) old[t] = son64(uf, gf, I, J, K, t & 1, t >> 1, S3);
+        if (son_corrected<S3, ONLY1>(t)) old[t] = ld_once<NT>(son64(uf, gf, I, J, K, t & 1, t >> 1, S3));
 }
 
-template <int S3, bool ONLY1, bool GUARD>
+template <int S3, bool ONLY1, bool GUARD, bool NT = false>
 __device__ __forceinline__ void correct64_sons_apply(const double* hz, const double* rz, const double* hk,
                                                      const double* rk, double* __restrict__ uf, const Geo& gf, int I,
                                                      int J, int K, const double* old)
@@ -355,7 +355,7 @@ __device__ __forceinline__ void correct64_sons_apply(const double* hz, const dou
             P = P - wt[ind] * vr;
         }
         const double tmpt = old[t] + P;
-        if (!GUARD || tmpt > 0.0) son64(uf, gf, I, J, K, s1, s2, S3) = tmpt;
+        if (!GUARD || tmpt > 0.0) st_once<NT>(son64(uf, gf, I, J, K, s1, s2, S3), tmpt);
     }
 }
 
@@ -427,7 +427,7 @@ __device__ __forceinline__ float df_solve(float S, float xb, float r, const DfCo
     return d;
 }
 
-template <int FRN>
+template <int FRN, bool NT = false>
 struct DfSmoothOp {
     using T = float;
     struct Own {
@@ -438,12 +438,15 @@ struct DfSmoothOp {
     const float* rhs;
     DfCoef c;
     __device__ float* field() const { return d; }
-    __device__ __forceinline__ Own load(unsigned o, unsigned) const { return Own{elem(xb, o), elem(rhs, o)}; }
+    __device__ __forceinline__ Own load(unsigned o, unsigned) const
+    {
+        return Own{ld_once<NT>(elem(xb, o)), ld_once<NT>(elem(rhs, o))};
+    }
     __device__ __forceinline__ void update(unsigned o, unsigned, const Own& w, float, float xp, float xm, float yp,
                                            float ym, float zp, float zm) const
     {
         const float S = xp + xm + yp + ym + zp + zm;
-        elem(d, o) = df_solve<FRN>(S, w.xb, w.rhs, c);
+        st_once<NT>(elem(d, o), df_solve<FRN>(S, w.xb, w.rhs, c));
     }
 };
 
@@ -463,7 +466,7 @@ __device__ __forceinline__ double outer_updated(double u, float d, float xb, dou
 // new colour-1 increment goes straight into u (d itself is not needed
 // afterwards) and every colour-0 cell, visited through the window, is
 // updated from its final increment
-template <int FRN>
+template <int FRN, bool NT = false>
 struct DfSmoothUpdateOp {
     using T = float;
     struct Own {
@@ -480,19 +483,19 @@ struct DfSmoothUpdateOp {
     __device__ __forceinline__ Own load(unsigned o, unsigned oo) const
     {
         Own w;
-        w.xb = elem(xb, o);
-        w.rhs = elem(rhs, o);
-        w.u = u[o];
-        w.xbo = elem(xb, oo);
-        w.uo = u[oo];
+        w.xb = ld_once<NT>(elem(xb, o));
+        w.rhs = ld_once<NT>(elem(rhs, o));
+        w.u = ld_once<NT>(u[o]);
+        w.xbo = ld_once<NT>(elem(xb, oo));
+        w.uo = ld_once<NT>(u[oo]);
         return w;
     }
     __device__ __forceinline__ void update(unsigned o, unsigned oo, const Own& w, float dother, float xp, float xm,
                                            float yp, float ym, float zp, float zm) const
     {
         const float S = xp + xm + yp + ym + zp + zm;
-        u[o] = outer_updated<FRN>(w.u, df_solve<FRN>(S, w.xb, w.rhs, c), w.xb, ubg);
-        u[oo] = outer_updated<FRN>(w.uo, dother, w.xbo, ubg);
+        st_once<NT>(u[o], outer_updated<FRN>(w.u, df_solve<FRN>(S, w.xb, w.rhs, c), w.xb, ubg));
+        st_once<NT>(u[oo], outer_updated<FRN>(w.uo, dother, w.xbo, ubg));
     }
 };
 
@@ -503,7 +506,7 @@ struct DfSmoothUpdateOp {
 // written: the inner cycle's next pass is the colour-1 pass, which sets it
 // from colour 0 without reading it.  NORM_ONLY: the same residual and
 // partial sums, without the inner problem's data (a solve's last residual).
-template <int FRN, bool NORM_ONLY = false>
+template <int FRN, bool NORM_ONLY = false, bool NT = false>
 struct FrOuterOp {
     using T = double;
     const double* u;
@@ -518,7 +521,7 @@ struct FrOuterOp {
         double delta;
     };
     __device__ const double* field() const { return u; }
-    __device__ __forceinline__ Own load(unsigned o) const { return Own{delta[o]}; }
+    __device__ __forceinline__ Own load(unsigned o) const { return Own{ld_once<NT>(delta[o])}; }
     __device__ __forceinline__ double cell(unsigned o, const Own& w, double uc, double xp, double xm, double yp,
                                            double ym, double zp, double zm) const
     {
@@ -534,9 +537,9 @@ struct FrOuterOp {
         OP = OP + c.fct2r * (c.a_fR_rt * ru - 1.0);
         if constexpr (!NORM_ONLY) {
             const float gv = (float)(-OP), xv = (float)(uc * c.inv_ubg);
-            elem(g, o) = gv;
-            elem(xb, o) = xv;
-            if (o < half) elem(d, o) = df_solve<FRN>(0.0f, xv, gv, dc);
+            st_once<NT>(elem(g, o), gv);
+            st_once<NT>(elem(xb, o), xv);
+            if (o < half) st_once<NT>(elem(d, o), df_solve<FRN>(0.0f, xv, gv, dc));
         }
         return OP;
     }
@@ -544,7 +547,7 @@ struct FrOuterOp {
 
 // D(d) - rhs at a cell; with (d, rhs, out) = (R d, R r, f) the inner FAS
 // right-hand side f = D(R d) - R r
-template <int FRN>
+template <int FRN, bool NT = false>
 struct DfResOp {
     using T = float;
     const float* d;
@@ -556,7 +559,7 @@ struct DfResOp {
         float xb, rhs;
     };
     __device__ const float* field() const { return d; }
-    __device__ __forceinline__ Own load(unsigned o) const { return Own{elem(xb, o), elem(rhs, o)}; }
+    __device__ __forceinline__ Own load(unsigned o) const { return Own{ld_once<NT>(elem(xb, o)), ld_once<NT>(elem(rhs, o))}; }
     __device__ __forceinline__ double cell(unsigned o, const Own& w, float dc, float xp, float xm, float yp, float ym,
                                            float zp, float zm) const
     {
@@ -564,7 +567,7 @@ struct DfResOp {
         const float xbc = w.xb;
         const float x = xroot<FRN>(fmaxf(wpow<FRN>(xbc) + dc, 1.0e-30f));
         const float r = c.A * (S - 6.0f * dc) + dlocal<FRN>(dc, x, xbc, c.C) - w.rhs;
-        elem(out, o) = r;
+        st_once<NT>(elem(out, o), r);
         return 0.0;
     }
 };
@@ -681,7 +684,7 @@ __device__ __forceinline__ void restrict_base_cell(const float* __restrict__ xf,
 // inner correction is followed by a colour-0 pass (see son_corrected).
 // df_sons_load loads the sons' old d and raw base xb (the weight xb^{n+1} is
 // formed at use, so no arithmetic waits on a load in flight).
-template <int S3>
+template <int S3, bool NT = false>
 __device__ __forceinline__ void df_sons_load(const float* __restrict__ df, const float* __restrict__ xbf, const Geo& gf,
                                              int I, int J, int K, float* old, float* xb)
 {
@@ -689,8 +692,8 @@ __device__ __forceinline__ void df_sons_load(const float* __restrict__ df, const
     for (int t = 0; t < 4; t++) {
         if (!son_corrected<S3, true>(t)) continue;
         const unsigned fi = son_index(gf, I, J, K, t & 1, t >> 1, S3);
-        old[t] = elem(df, fi);
-        xb[t] = elem(xbf, fi);
+        old[t] = ld_once<NT>(elem(df, fi));
+        xb[t] = ld_once<NT>(elem(xbf, fi));
     }
 }
 
@@ -766,14 +769,14 @@ __device__ __forceinline__ void df_sons_new(const float* ez, const float* ek, co
     }
 }
 
-template <int S3>
+template <int S3, bool NT = false>
 __device__ __forceinline__ void df_sons_store(float* __restrict__ df, const Geo& gf, int I, int J, int K,
                                               const float* dn, const bool* keep)
 {
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         if (!son_corrected<S3, true>(t)) continue;
-        if (keep[t]) elem(df, son_index(gf, I, J, K, t & 1, t >> 1, S3)) = dn[t];
+        if (keep[t]) st_once<NT>(elem(df, son_index(gf, I, J, K, t & 1, t >> 1, S3)), dn[t]);
     }
 }
 
@@ -782,7 +785,7 @@ __device__ __forceinline__ void df_sons_store(float* __restrict__ df, const Geo&
 // coarse plane K+2) is issued, then plane K's sons are stored.  A son is read
 // and written by its own thread only and the planes' sons are distinct, so
 // the values are those of the plane-by-plane order.
-template <int FRN, bool SHFL = false>
+template <int FRN, bool SHFL = false, bool NT = false>
 __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, const float* __restrict__ rd,
                                                float* __restrict__ df, const float* __restrict__ xbf, const Geo& gf,
                                                const Geo& gc, const ZCol& cc, bool lok = false, bool rok = false)
@@ -795,8 +798,8 @@ __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, con
     e_load<SHFL>(dH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, wm);
     e_load<SHFL>(dH, rd, B, cc.k0, lok, rok, wk);
     e_load<SHFL>(dH, rd, B, wrap_inc(cc.k0, gc.n), lok, rok, wp);
-    df_sons_load<0>(df, xbf, gf, I, J, cc.k0, o0, w0);
-    df_sons_load<1>(df, xbf, gf, I, J, cc.k0, o1, w1);
+    df_sons_load<0, NT>(df, xbf, gf, I, J, cc.k0, o0, w0);
+    df_sons_load<1, NT>(df, xbf, gf, I, J, cc.k0, o1, w1);
     float em[9], ek[9], ep[9];
     e_form<SHFL>(wm, lok, rok, em);
     e_form<SHFL>(wk, lok, rok, ek);
@@ -807,12 +810,12 @@ __device__ __forceinline__ void df_correct_col(const float* __restrict__ dH, con
         df_sons_new<FRN, 0>(em, ek, o0, w0, dn0, k0);
         df_sons_new<FRN, 1>(ep, ek, o1, w1, dn1, k1);
         if (K + 1 < cc.k1) {
-            df_sons_load<0>(df, xbf, gf, I, J, K + 1, o0, w0);
-            df_sons_load<1>(df, xbf, gf, I, J, K + 1, o1, w1);
+            df_sons_load<0, NT>(df, xbf, gf, I, J, K + 1, o0, w0);
+            df_sons_load<1, NT>(df, xbf, gf, I, J, K + 1, o1, w1);
             e_load<SHFL>(dH, rd, B, wrap_inc(K + 1, gc.n), lok, rok, wp);
         }
-        df_sons_store<0>(df, gf, I, J, K, dn0, k0);
-        df_sons_store<1>(df, gf, I, J, K, dn1, k1);
+        df_sons_store<0, NT>(df, gf, I, J, K, dn0, k0);
+        df_sons_store<1, NT>(df, gf, I, J, K, dn1, k1);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             em[q] = ek[q];
