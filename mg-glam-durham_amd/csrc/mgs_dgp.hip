@@ -265,7 +265,11 @@ int launch_dgp_residual64(int level, const double* u, const double* rho, const d
     const Geo g = make_geo(n);
     if (dgp_zmarch(n)) {
         const unsigned hf = (unsigned)g.half;
-        if (level == 0) return (int)launch_both_z(DgpRes64Op<false>{u, rho, f, res, c, hf}, partials, n, s);
+        if (level == 0) {
+            if (n >= NT_MIN_N) return (int)launch_both_z(DgpRes64Op<false, true>{u, rho, f, res, c, hf}, partials, n, s);
+            return (int)launch_both_z(DgpRes64Op<false>{u, rho, f, res, c, hf}, partials, n, s);
+        }
+        if (n >= NT_MIN_N) return (int)launch_both_z(DgpRes64Op<true, true>{u, rho, f, res, c, hf}, partials, n, s);
         return (int)launch_both_z(DgpRes64Op<true>{u, rho, f, res, c, hf}, partials, n, s);
     }
     const Launch L = make_launch(g.h, 2L * n * n);
@@ -380,12 +384,12 @@ __global__ __launch_bounds__(1024) void k_dgpm_coarsest(float* __restrict__ e, c
 // 8 sons of a coarse cell, software-pipelined (plane K's sons computed, the
 // loads of plane K+1 issued, then plane K's sons stored).  Sums in
 // dgpm_correct_cell's order.
-template <int S3>
+template <int S3, bool NT>
 __device__ __forceinline__ void dgpm_sons_load(const float* __restrict__ ef, const Geo& gf, int I, int J, int K,
                                                float* old)
 {
 #pragma unroll
-    for (int t = 0; t < 4; t++) old[t] = elem(ef, son_index(gf, I, J, K, t & 1, t >> 1, S3));
+    for (int t = 0; t < 4; t++) old[t] = ld_once<NT>(elem(ef, son_index(gf, I, J, K, t & 1, t >> 1, S3)));
 }
 
 template <int S3>
@@ -407,14 +411,15 @@ __device__ __forceinline__ void dgpm_sons_new(const float* ez, const float* ek, 
     }
 }
 
-template <int S3>
+template <int S3, bool NT>
 __device__ __forceinline__ void dgpm_sons_store(float* __restrict__ ef, const Geo& gf, int I, int J, int K,
                                                 const float* nv)
 {
 #pragma unroll
-    for (int t = 0; t < 4; t++) elem(ef, son_index(gf, I, J, K, t & 1, t >> 1, S3)) = nv[t];
+    for (int t = 0; t < 4; t++) st_once<NT>(elem(ef, son_index(gf, I, J, K, t & 1, t >> 1, S3)), nv[t]);
 }
 
+template <bool NT>
 __global__ __launch_bounds__(256) void k_dgpm_zcorrect(const float* __restrict__ eH, const float* __restrict__ rd,
                                                        float* __restrict__ ef, Geo gf, Geo gc, int TX, int TY, int XB,
                                                        int JB, int KC, unsigned NB)
@@ -434,8 +439,8 @@ __global__ __launch_bounds__(256) void k_dgpm_zcorrect(const float* __restrict__
     e_load<true>(eH, rd, B, wrap_dec(cc.k0, gc.n), lok, rok, wm);
     e_load<true>(eH, rd, B, cc.k0, lok, rok, wk);
     e_load<true>(eH, rd, B, wrap_inc(cc.k0, gc.n), lok, rok, wp);
-    dgpm_sons_load<0>(ef, gf, I, J, cc.k0, o0);
-    dgpm_sons_load<1>(ef, gf, I, J, cc.k0, o1);
+    dgpm_sons_load<0, NT>(ef, gf, I, J, cc.k0, o0);
+    dgpm_sons_load<1, NT>(ef, gf, I, J, cc.k0, o1);
     float em[9], ek[9], ep[9];
     e_form<true>(wm, lok, rok, em);
     e_form<true>(wk, lok, rok, ek);
@@ -445,12 +450,12 @@ __global__ __launch_bounds__(256) void k_dgpm_zcorrect(const float* __restrict__
         dgpm_sons_new<0>(em, ek, o0, n0);
         dgpm_sons_new<1>(ep, ek, o1, n1);
         if (K + 1 < cc.k1) {
-            dgpm_sons_load<0>(ef, gf, I, J, K + 1, o0);
-            dgpm_sons_load<1>(ef, gf, I, J, K + 1, o1);
+            dgpm_sons_load<0, NT>(ef, gf, I, J, K + 1, o0);
+            dgpm_sons_load<1, NT>(ef, gf, I, J, K + 1, o1);
             e_load<true>(eH, rd, B, wrap_inc(K + 1, gc.n), lok, rok, wp);
         }
-        dgpm_sons_store<0>(ef, gf, I, J, K, n0);
-        dgpm_sons_store<1>(ef, gf, I, J, K, n1);
+        dgpm_sons_store<0, NT>(ef, gf, I, J, K, n0);
+        dgpm_sons_store<1, NT>(ef, gf, I, J, K, n1);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             em[q] = ek[q];
@@ -460,17 +465,23 @@ __global__ __launch_bounds__(256) void k_dgpm_zcorrect(const float* __restrict__
 }
 
 // phi += e
+template <bool NT>
 __global__ __launch_bounds__(256) void k_dgpm_update(double* __restrict__ u, const float* __restrict__ e, size_t count)
 {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) u[i] = u[i] + (double)e[i];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+        st_once<NT>(u[i], ld_once<NT>(u[i]) + (double)ld_once<NT>(e[i]));
 }
 
 int launch_dgpm_outer(const double* u, const double* rho, float* g32, float* pb, float* e, double* partials, int n,
                       const DgpCoef& c, hipStream_t s)
 {
     const Geo g = make_geo(n);
-    if (dgp_zmarch(n)) return (int)launch_both_z(DgpmOuterOp{u, rho, g32, pb, e, c, (unsigned)g.half}, partials, n, s);
+    if (dgp_zmarch(n)) {
+        if (n >= NT_MIN_N)
+            return (int)launch_both_z(DgpmOuterOp<true>{u, rho, g32, pb, e, c, (unsigned)g.half}, partials, n, s);
+        return (int)launch_both_z(DgpmOuterOp<>{u, rho, g32, pb, e, c, (unsigned)g.half}, partials, n, s);
+    }
     const Launch L = make_launch(g.h, 2L * n * n);
     hipLaunchKernelGGL(k_dgpm_outer, L.grid, L.block, 0, s, u, rho, g32, pb, e, partials, g, c, L.TX, L.TY, L.XB,
                        L.NB);
@@ -492,7 +503,10 @@ void launch_dgpm_residual(const float* e, const float* pb, const double* rho, co
 {
     const Geo g = make_geo(n);
     if (dgp_zmarch(n) && rho32) {
-        launch_both_z(DgpmResOp{e, pb, rho32, rhs, out, dgpf(c), (unsigned)g.half}, nullptr, n, s);
+        if (n >= NT_MIN_N)
+            launch_both_z(DgpmResOp<true>{e, pb, rho32, rhs, out, dgpf(c), (unsigned)g.half}, nullptr, n, s);
+        else
+            launch_both_z(DgpmResOp<>{e, pb, rho32, rhs, out, dgpf(c), (unsigned)g.half}, nullptr, n, s);
         return;
     }
     const Launch L = make_launch(g.h, 2L * n * n);
@@ -514,7 +528,12 @@ void launch_dgpm_correct(const float* eH, const float* rd, float* ef, int nc, hi
 {
     const Geo gf = make_geo(2 * nc), gc = make_geo(nc);
     const ZLaunch L = make_zlaunch(nc, nc);
-    hipLaunchKernelGGL(k_dgpm_zcorrect, L.grid, L.block, 0, s, eH, rd, ef, gf, gc, L.TX, L.TY, L.XB, L.JB, L.KC, L.NB);
+    if (2 * nc >= NT_MIN_N)
+        hipLaunchKernelGGL(k_dgpm_zcorrect<true>, L.grid, L.block, 0, s, eH, rd, ef, gf, gc, L.TX, L.TY, L.XB, L.JB, L.KC,
+                           L.NB);
+    else
+        hipLaunchKernelGGL(k_dgpm_zcorrect<false>, L.grid, L.block, 0, s, eH, rd, ef, gf, gc, L.TX, L.TY, L.XB, L.JB,
+                           L.KC, L.NB);
 }
 
 void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s)
@@ -522,7 +541,10 @@ void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s)
     const size_t count = (size_t)n * n * n;
     size_t b = (count + 255) / 256;
     if (b > 65536) b = 65536;
-    hipLaunchKernelGGL(k_dgpm_update, dim3((unsigned)b), dim3(256), 0, s, u, e, count);
+    if (n >= NT_MIN_N)
+        hipLaunchKernelGGL(k_dgpm_update<true>, dim3((unsigned)b), dim3(256), 0, s, u, e, count);
+    else
+        hipLaunchKernelGGL(k_dgpm_update<false>, dim3((unsigned)b), dim3(256), 0, s, u, e, count);
 }
 
 // ===========================================================================
@@ -533,7 +555,7 @@ void launch_dgpm_update(double* u, const float* e, int n, hipStream_t s)
 // the column's first cell (the same for a whole wave); with the run unrolled
 // the parity of every step is a constant, so the stencil values are the
 // window's registers themselves.
-template <bool COARSE, bool S0>
+template <bool COARSE, bool S0, bool NT>
 __device__ __forceinline__ void dgp_zsmooth64_run(const double* __restrict__ h0, const double* __restrict__ h1,
                                                   double* __restrict__ out, const double* __restrict__ rho,
                                                   const double* __restrict__ f, const Geo& g, int colour,
@@ -549,8 +571,8 @@ __device__ __forceinline__ void dgp_zsmooth64_run(const double* __restrict__ h0,
     dside_load(H, d, k0, 0, g.n, S0, sd);
     const double dLphi = -6.0 / c.dx2;
     unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
-    double rh = elem(rhoc, off);
-    double fv = COARSE ? elem(fc, off) : 0.0;
+    double rh = ld_once<NT>(elem(rhoc, off));
+    double fv = COARSE ? ld_once<NT>(elem(fc, off)) : 0.0;
 #pragma unroll
     for (int t = 0; t < DGP_KC; t++) {
         const int k = k0 + t;
@@ -562,13 +584,13 @@ __device__ __forceinline__ void dgp_zsmooth64_run(const double* __restrict__ h0,
         if (t + 1 < DGP_KC) {
             draw2_load(H, d, wrap_inc(k + 1, g.n), t + 2, r);
             dside_load(H, d, wrap_inc(k, g.n), t + 1, g.n, !s, sdn);
-            rhn = elem(rhoc, offn);
-            if constexpr (COARSE) fvn = elem(fc, offn);
+            rhn = ld_once<NT>(elem(rhoc, offn));
+            if constexpr (COARSE) fvn = ld_once<NT>(elem(fc, offn));
         }
         const S19<double> u = s19_win2(w, s, sd, d);
         double L = dgp_L19(u, rh, c);
         if constexpr (COARSE) L = L - fv;
-        elem(out, off) = u.c0 - L / dLphi;
+        st_once<NT>(elem(out, off), u.c0 - L / dLphi);
         if (t + 1 < DGP_KC) {
             dwin2_shift(w);
             dwin2_set(w, 2, r);
@@ -580,7 +602,7 @@ __device__ __forceinline__ void dgp_zsmooth64_run(const double* __restrict__ h0,
     }
 }
 
-template <bool COARSE>
+template <bool COARSE, bool NT>
 __global__ __launch_bounds__(256) void k_dgp_zsmooth64(const double* __restrict__ h0, const double* __restrict__ h1,
                                                        double* __restrict__ out, const double* __restrict__ rho,
                                                        const double* __restrict__ f, Geo g, int colour, DgpCoef c,
@@ -590,13 +612,13 @@ __global__ __launch_bounds__(256) void k_dgp_zsmooth64(const double* __restrict_
     if (!zcol(g, TX, TY, XB, JB, DGP_KC, NB, cc)) return;
     const DCol d = dcol_of(g, cc.m, cc.j);
     if ((colour + cc.j + cc.k0) & 1)
-        dgp_zsmooth64_run<COARSE, true>(h0, h1, out, rho, f, g, colour, c, d, cc.k0);
+        dgp_zsmooth64_run<COARSE, true, NT>(h0, h1, out, rho, f, g, colour, c, d, cc.k0);
     else
-        dgp_zsmooth64_run<COARSE, false>(h0, h1, out, rho, f, g, colour, c, d, cc.k0);
+        dgp_zsmooth64_run<COARSE, false, NT>(h0, h1, out, rho, f, g, colour, c, d, cc.k0);
 }
 
 // one colour of the mixed inner smoother (as k_dgpm_smooth); S0 as above
-template <bool S0>
+template <bool S0, bool NT>
 __device__ __forceinline__ void dgpm_zsmooth_run(const float* __restrict__ e0, const float* __restrict__ e1,
                                                  float* __restrict__ out, const float* __restrict__ pb,
                                                  const float* __restrict__ rho32, const float* __restrict__ rhs,
@@ -614,8 +636,8 @@ __device__ __forceinline__ void dgpm_zsmooth_run(const float* __restrict__ e0, c
     dside_load(He, d, k0, 0, g.n, S0, se);
     dside_load(Hb, d, k0, 0, g.n, S0, sb);
     unsigned off = (unsigned)k0 * d.plane + d.rowoff[1] + d.m;
-    float rh = elem(rhoc, off);
-    float fv = elem(rhsc, off);
+    float rh = ld_once<NT>(elem(rhoc, off));
+    float fv = ld_once<NT>(elem(rhsc, off));
 #pragma unroll
     for (int t = 0; t < DGP_KC; t++) {
         const int k = k0 + t;
@@ -630,12 +652,12 @@ __device__ __forceinline__ void dgpm_zsmooth_run(const float* __restrict__ e0, c
             draw2_load(Hb, d, kn, t + 2, rb);
             dside_load(He, d, wrap_inc(k, g.n), t + 1, g.n, !s, sen);
             dside_load(Hb, d, wrap_inc(k, g.n), t + 1, g.n, !s, sbn);
-            rhn = elem(rhoc, offn);
-            fvn = elem(rhsc, offn);
+            rhn = ld_once<NT>(elem(rhoc, offn));
+            fvn = ld_once<NT>(elem(rhsc, offn));
         }
         const D2f de = d2_of(s19_win2(we, s, se, d)), db = d2_of(s19_win2(wb, s, sb, d));
         const float D = dgp_dop(db, de, rh, c) - fv;
-        elem(out, off) = de.c0 + D / (6.0f * c.inv_dx2);
+        st_once<NT>(elem(out, off), de.c0 + D / (6.0f * c.inv_dx2));
         if (t + 1 < DGP_KC) {
             dwin2_shift(we);
             dwin2_shift(wb);
@@ -650,6 +672,7 @@ __device__ __forceinline__ void dgpm_zsmooth_run(const float* __restrict__ e0, c
     }
 }
 
+template <bool NT>
 __global__ __launch_bounds__(256) void k_dgpm_zsmooth(const float* __restrict__ e0, const float* __restrict__ e1,
                                                       float* __restrict__ out, const float* __restrict__ pb,
                                                       const float* __restrict__ rho32, const float* __restrict__ rhs,
@@ -660,9 +683,9 @@ __global__ __launch_bounds__(256) void k_dgpm_zsmooth(const float* __restrict__ 
     if (!zcol(g, TX, TY, XB, JB, DGP_KC, NB, cc)) return;
     const DCol d = dcol_of(g, cc.m, cc.j);
     if ((colour + cc.j + cc.k0) & 1)
-        dgpm_zsmooth_run<true>(e0, e1, out, pb, rho32, rhs, g, colour, c, d, cc.k0);
+        dgpm_zsmooth_run<true, NT>(e0, e1, out, pb, rho32, rhs, g, colour, c, d, cc.k0);
     else
-        dgpm_zsmooth_run<false>(e0, e1, out, pb, rho32, rhs, g, colour, c, d, cc.k0);
+        dgpm_zsmooth_run<false, NT>(e0, e1, out, pb, rho32, rhs, g, colour, c, d, cc.k0);
 }
 
 void launch_dgp_zpass64(int level, const double* h0, const double* h1, double* out, const double* rho,
@@ -670,12 +693,18 @@ void launch_dgp_zpass64(int level, const double* h0, const double* h1, double* o
 {
     const Geo g = make_geo(n);
     const ZLaunch L = dgp_zlaunch(n);
-    if (level == 0)
-        hipLaunchKernelGGL((k_dgp_zsmooth64<false>), L.grid, L.block, 0, s, h0, h1, out, rho, f, g, colour, c, L.TX,
-                           L.TY, L.XB, L.JB, L.NB);
-    else
-        hipLaunchKernelGGL((k_dgp_zsmooth64<true>), L.grid, L.block, 0, s, h0, h1, out, rho, f, g, colour, c, L.TX,
-                           L.TY, L.XB, L.JB, L.NB);
+#define ZS64(CO, NT)                                                                                    \
+    hipLaunchKernelGGL((k_dgp_zsmooth64<CO, NT>), L.grid, L.block, 0, s, h0, h1, out, rho, f, g, colour, c, L.TX, \
+                       L.TY, L.XB, L.JB, L.NB)
+    const bool nt = n >= NT_MIN_N;
+    if (level == 0) {
+        if (nt) ZS64(false, true);
+        else ZS64(false, false);
+    } else {
+        if (nt) ZS64(true, true);
+        else ZS64(true, false);
+    }
+#undef ZS64
 }
 
 void launch_dgpm_zpass(const float* e0, const float* e1, float* out, const float* pb, const float* rho32,
@@ -683,8 +712,12 @@ void launch_dgpm_zpass(const float* e0, const float* e1, float* out, const float
 {
     const Geo g = make_geo(n);
     const ZLaunch L = dgp_zlaunch(n);
-    hipLaunchKernelGGL(k_dgpm_zsmooth, L.grid, L.block, 0, s, e0, e1, out, pb, rho32, rhs, g, colour, dgpf(c), L.TX,
-                       L.TY, L.XB, L.JB, L.NB);
+    if (n >= NT_MIN_N)
+        hipLaunchKernelGGL(k_dgpm_zsmooth<true>, L.grid, L.block, 0, s, e0, e1, out, pb, rho32, rhs, g, colour, dgpf(c),
+                           L.TX, L.TY, L.XB, L.JB, L.NB);
+    else
+        hipLaunchKernelGGL(k_dgpm_zsmooth<false>, L.grid, L.block, 0, s, e0, e1, out, pb, rho32, rhs, g, colour,
+                           dgpf(c), L.TX, L.TY, L.XB, L.JB, L.NB);
 }
 
 __global__ __launch_bounds__(256) void k_round_f32(const double* __restrict__ a, float* __restrict__ b, size_t n)

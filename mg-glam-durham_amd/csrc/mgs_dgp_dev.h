@@ -619,7 +619,7 @@ __device__ __forceinline__ double dgp_both_zcol(const OP& op, const Geo& g, int 
 }
 
 // FP64 residual L(phi) (- f on coarse levels), stored in res when non-null
-template <bool COARSE>
+template <bool COARSE, bool NT = false>
 struct DgpRes64Op {
     using T = double;
     static constexpr int NF = 1;
@@ -633,18 +633,22 @@ struct DgpRes64Op {
     DgpCoef c;
     unsigned hf;
     __device__ const double* half(int, int colour) const { return u + (colour ? hf : 0u); }
-    __device__ __forceinline__ Own load(unsigned i) const { return Own{rho[i], COARSE ? f[i] : 0.0}; }
+    __device__ __forceinline__ Own load(unsigned i) const
+    {
+        return Own{ld_once<NT>(rho[i]), COARSE ? ld_once<NT>(f[i]) : 0.0};
+    }
     __device__ __forceinline__ double cell(unsigned i, const Own& o, const S19<double>* s) const
     {
         double OP = dgp_L19(s[0], o.rho, c);
         if constexpr (COARSE) OP = OP - o.f;
-        if (res) res[i] = OP;
+        if (res) st_once<NT>(res[i], OP);
         return OP;
     }
 };
 
 // the mixed solve's FP64 outer residual r = L(phi), with the inner problem's
 // data g = -r, phi_b = phi (FP32) and e = 0 (as dgpm_outer_cell)
+template <bool NT = false>
 struct DgpmOuterOp {
     using T = double;
     static constexpr int NF = 1;
@@ -659,19 +663,20 @@ struct DgpmOuterOp {
     DgpCoef c;
     unsigned hf;
     __device__ const double* half(int, int colour) const { return u + (colour ? hf : 0u); }
-    __device__ __forceinline__ Own load(unsigned i) const { return Own{rho[i]}; }
+    __device__ __forceinline__ Own load(unsigned i) const { return Own{ld_once<NT>(rho[i])}; }
     __device__ __forceinline__ double cell(unsigned i, const Own& o, const S19<double>* s) const
     {
         const double OP = dgp_L19(s[0], o.rho, c);
-        g32[i] = (float)(-OP);
-        pb[i] = (float)s[0].c0;
-        e[i] = 0.0f;
+        st_once<NT>(g32[i], (float)(-OP));
+        st_once<NT>(pb[i], (float)s[0].c0);
+        st_once<NT>(e[i], 0.0f);
         return OP;
     }
 };
 
 // the mixed inner residual out = D(e) - rhs (as dgpm_residual_cell, which
 // rounds the FP64 density to FP32: rho32 holds those values); no norm
+template <bool NT = false>
 struct DgpmResOp {
     using T = float;
     static constexpr int NF = 2;  // e, phi_b
@@ -686,10 +691,10 @@ struct DgpmResOp {
     DgpF c;
     unsigned hf;
     __device__ const float* half(int f, int colour) const { return (f ? pb : e) + (colour ? hf : 0u); }
-    __device__ __forceinline__ Own load(unsigned i) const { return Own{rho32[i], rhs[i]}; }
+    __device__ __forceinline__ Own load(unsigned i) const { return Own{ld_once<NT>(rho32[i]), ld_once<NT>(rhs[i])}; }
     __device__ __forceinline__ double cell(unsigned i, const Own& o, const S19<float>* s) const
     {
-        out[i] = dgp_dop(d2_of(s[1]), d2_of(s[0]), o.rho, c) - o.rhs;
+        st_once<NT>(out[i], dgp_dop(d2_of(s[1]), d2_of(s[0]), o.rho, c) - o.rhs);
         return 0.0;
     }
 };

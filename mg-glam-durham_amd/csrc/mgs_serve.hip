@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void kb_outer_residual(const SlotDesc* __restr
         if (sd.model == MGS_MODEL_DGP && sd.zdgp) {
             ZCol c;
             if (tile_col(L.g, L.g.h, sd.zo, lb, c))
-                acc = dgp_both_zcol(DgpmOuterOp{sd.u, sd.rho, L.f, L.xb, L.d, sd.gc0, (unsigned)L.g.half}, L.g, c.m,
+                acc = dgp_both_zcol(DgpmOuterOp<>{sd.u, sd.rho, L.f, L.xb, L.d, sd.gc0, (unsigned)L.g.half}, L.g, c.m,
                                     c.j, c.k0);
         } else if (sd.model == MGS_MODEL_DGP) {
             int m;
