@@ -214,8 +214,8 @@ __global__ __launch_bounds__(256) void kb_outer_residual(const SlotDesc* __restr
         if (sd.model == MGS_MODEL_DGP && sd.zdgp) {
             ZCol c;
             if (tile_col(L.g, L.g.h, sd.zo, lb, c))
-                acc = dgp_both_zcol(DgpmOuterOp<>{sd.u, sd.rho, L.f, L.xb, L.d, sd.gc0, (unsigned)L.g.half}, L.g, c.m,
-                                    c.j, c.k0);
+                acc = dgp_both_zcol(DgpmOuterOp<true>{sd.u, sd.rho, L.f, L.xb, L.d, sd.gc0, (unsigned)L.g.half}, L.g,
+                                    c.m, c.j, c.k0);
         } else if (sd.model == MGS_MODEL_DGP) {
             int m;
             unsigned row;
@@ -224,9 +224,10 @@ __global__ __launch_bounds__(256) void kb_outer_residual(const SlotDesc* __restr
         } else {
             ZCol c;
             if (tile_col(L.g, L.g.h, L.zb, lb, c)) {
-                FRN_SWITCH(sd.frn, acc = both_colours_col(FrOuterOp<FRN>{sd.u, sd.rho, L.f, L.xb, L.d, sd.fc, L.dc,
-                                                                         (unsigned)L.g.half},
-                                                          L.g, c));
+                FRN_SWITCH(sd.frn,
+                           acc = both_colours_col(FrOuterOp<FRN, false, true>{sd.u, sd.rho, L.f, L.xb, L.d, sd.fc,
+                                                                              L.dc, (unsigned)L.g.half},
+                                                  L.g, c));
             }
         }
         const double bs = block_sum(acc);
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(256) void kb_smooth0(const SlotDesc* __restrict__ S
         const LevelDesc& L = sd.L[0];
         ZCol c;
         if (tile_col(L.g, L.g.h, L.z, t - pre[i], c)) {
-            FRN_SWITCH(sd.frn, colour_pass_col(DfSmoothOp<FRN>{L.d, L.xb, L.f, L.dc}, L.g, colour, c));
+            FRN_SWITCH(sd.frn, colour_pass_col(DfSmoothOp<FRN, true>{L.d, L.xb, L.f, L.dc}, L.g, colour, c));
         }
     }
 }
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(256) void kb_smooth_update(const SlotDesc* __restri
         ZCol c;
         if (tile_col(L.g, L.g.h, L.z, t - P->last_pre[i], c)) {
             FRN_SWITCH(sd.frn,
-                       colour_pass_col(DfSmoothUpdateOp<FRN>{L.d, L.xb, L.f, sd.u, L.dc, sd.fc.ubg}, L.g, 1, c));
+                       colour_pass_col(DfSmoothUpdateOp<FRN, true>{L.d, L.xb, L.f, sd.u, L.dc, sd.fc.ubg}, L.g, 1, c));
         }
     }
 }
@@ -406,7 +407,11 @@ __global__ __launch_bounds__(256) void kb_correct(const SlotDesc* __restrict__ S
         const LevelDesc &F = sd.L[l], &C = sd.L[l + 1];
         ZCol c;
         if (tile_col(C.g, C.g.n, C.zc, t - P->zc_pre[l][i], c)) {
-            FRN_SWITCH(sd.frn, df_correct_col<FRN>(C.d, C.rd, F.d, F.xb, F.g, C.g, c));
+            if (l == 0) {
+                FRN_SWITCH(sd.frn, (df_correct_col<FRN, false, true>(C.d, C.rd, F.d, F.xb, F.g, C.g, c)));
+            } else {
+                FRN_SWITCH(sd.frn, df_correct_col<FRN>(C.d, C.rd, F.d, F.xb, F.g, C.g, c));
+            }
         }
     }
 }
