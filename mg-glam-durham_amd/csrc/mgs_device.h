@@ -94,7 +94,8 @@ __device__ __forceinline__ T& elem(T* a, unsigned i)
 // Loads and stores of values a pass reads or writes once (a cell's own
 // data, its result; never the field whose window neighbours re-read).  With
 // NT they are non-temporal: on levels whose arrays exceed the last-level
-// cache (NT_MIN_N) that keeps the L2 for the re-read field; on smaller
+// cache (NT_MIN_N), and in batched kernels whose one pass streams the level
+// 0 of many requests, that keeps the L2 for the re-read field; on smaller
 // levels the cache holds the arrays from one pass to the next, and plain
 // accesses keep them there.
 constexpr int NT_MIN_N = 512;
